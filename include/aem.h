@@ -1,0 +1,154 @@
+/*
+ * aem.h -- C ABI of libaem, the MI355X (gfx950) FDTD time-stepping library.
+ *
+ * libaem replaces the compiled solver executable of the reference
+ * (main_fdtd/fdtd_solver.f90, run as "./FDTD inputs.txt" by
+ * main_fdtd/master.py:438-441).  The reference has no function-level API:
+ * its boundary is a process that reads a positional text deck
+ * (fdtd_solver.f90:726-1298) and writes data.dat / *_geometry.bin
+ * (fdtd_solver.f90:2355-2357, 8536-8645).  The entry points below split
+ * that process into the steps a host program (ctypes, cgo, JNI ...) binds:
+ *
+ *   aem_plan_from_inputs   parse + setup        (fdtd_solver.f90:719-4369)
+ *   aem_sim_create         device allocation and upload (the "!$acc data"
+ *                          region, fdtd_solver.f90:4380-4430)
+ *   aem_run                leapfrog steps       (fdtd_solver.f90:4440-8299)
+ *   aem_finalize           post-loop products   (fdtd_solver.f90:8340-8534)
+ *   aem_get_series         port V, incident, far-field, plane sums
+ *   aem_write_outputs      data.dat + _geometry.bin (fdtd_solver.f90:8536-8645)
+ *
+ * Conventions
+ *   - Every function returns an aem_status; 0 is success.  The library
+ *     never exits the process: deck errors that make the reference STOP
+ *     ("ERROR: ...") return AEM_ERR_INPUT with the same message in
+ *     aem_last_error().
+ *   - Handles are not thread-safe.  Each aem_sim owns one HIP stream and
+ *     all of its device buffers; the caller owns every host buffer it
+ *     passes in.
+ *   - No PyTorch types cross this boundary: plain pointers and sizes only.
+ */
+#ifndef AEM_H
+#define AEM_H
+
+#include <stdint.h>
+#include "aem_plan.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum aem_status {
+    AEM_OK = 0,
+    AEM_ERR_INPUT = 1,        /* malformed deck / reference STOP condition */
+    AEM_ERR_IO = 2,           /* file missing or unwritable */
+    AEM_ERR_DEVICE = 3,       /* HIP runtime error */
+    AEM_ERR_STATE = 4,        /* call out of order, bad handle or argument */
+    AEM_ERR_UNSUPPORTED = 5,  /* deck feature outside this library (kmax, SPICE, gridded ports, IGP) */
+    AEM_ERR_NUMERIC = 6       /* non-finite field detected by the watchdog */
+} aem_status;
+
+typedef enum aem_series {
+    AEM_SERIES_PORT_V = 0,        /* Voltage_out(port,:)  (f90:8239-8243); idx = port (0-based) */
+    AEM_SERIES_PORT_I = 1,        /* Current(port,1) per step: the gap loop integral (f90:4800-4857) */
+    AEM_SERIES_INCIDENT = 2,      /* incident(:) (f90:8366-8407) -- valid after aem_finalize */
+    AEM_SERIES_E_THETA = 3,       /* E_theta_out(angle,:) (f90:8529-8531) -- after aem_finalize */
+    AEM_SERIES_E_PHI = 4,         /* E_phi_out(angle,:) */
+    AEM_SERIES_REFLECTED = 5,     /* E_reflected(:) normalised (f90:8342-8361) -- after aem_finalize */
+    AEM_SERIES_TRANSMITTED = 6,   /* E_transmitted(:) */
+    AEM_SERIES_FF_PC = 7          /* incident at the far-field phase centre (f90:8636-8642) */
+} aem_series;
+
+typedef enum aem_field {
+    AEM_EX = 0, AEM_EY = 1, AEM_EZ = 2, AEM_HX = 3, AEM_HY = 4, AEM_HZ = 5
+} aem_field;
+
+typedef struct aem_plan aem_plan;   /* host-side setup result (no GPU needed) */
+typedef struct aem_sim  aem_sim;    /* device state of one simulation */
+
+/* Thread-local text of the last error (never NULL). */
+const char* aem_last_error(void);
+
+/* Library version string. */
+const char* aem_version(void);
+
+/* ---- setup (host only; replaces fdtd_solver.f90:719-4369) ---------------- */
+
+/* Parse an inputs.txt deck (grammar of fdtd_solver.f90:726-1298, documented
+ * in main_fdtd/master.py:51-295) and build the plan.  Relative file names in
+ * the deck (optional bulk geometry) resolve against the deck's directory.
+ * Replaces: the process entry "./FDTD inputs.txt" (fdtd_solver.f90:726-727). */
+aem_status aem_plan_from_inputs(const char* inputs_path, aem_plan** out);
+
+/* Same, from deck text held in memory; base_dir resolves relative files. */
+aem_status aem_plan_from_text(const char* deck_text, const char* base_dir, aem_plan** out);
+
+/* Read-only view of the plan (valid until aem_plan_destroy). */
+const aem_plan_view* aem_plan_view_get(const aem_plan* plan);
+
+/* Write <name>_geometry.bin exactly as fdtd_solver.f90:2355-2357 does. */
+aem_status aem_plan_write_geometry(const aem_plan* plan, const char* path);
+
+void aem_plan_destroy(aem_plan* plan);
+
+/* ---- device lifetime ------------------------------------------------------ */
+
+/* Allocate device state for a plan on the current HIP device and upload
+ * coefficients (the reference's "!$acc data copyin" region,
+ * fdtd_solver.f90:4380-4430).  The plan must outlive the sim. */
+aem_status aem_sim_create(const aem_plan* plan, aem_sim** out);
+
+/* Convenience: aem_plan_from_inputs + aem_sim_create (the sim owns the plan). */
+aem_status aem_create_from_inputs(const char* inputs_path, aem_sim** out);
+
+void aem_destroy(aem_sim* sim);
+
+/* ---- stepping (fdtd_solver.f90:4440-8299) --------------------------------- */
+
+/* Advance n_steps leapfrog steps (H phase, E phase, per-step outputs).  The
+ * total may not exceed the deck's time_steps (output arrays are sized by
+ * it).  use_graph!=0 replays a captured HIP graph of one step. */
+aem_status aem_run(aem_sim* sim, int64_t n_steps, int32_t use_graph);
+
+/* Steps completed so far. */
+int64_t aem_steps_done(const aem_sim* sim);
+
+/* Block until all queued device work of this sim is complete. */
+aem_status aem_sync(aem_sim* sim);
+
+/* ---- outputs ---------------------------------------------------------------- */
+
+/* Post-loop work (fdtd_solver.f90:8340-8534): plane-sum normalisation,
+ * incident wave (analytic or the clear 1-D line), NTFF combine and output
+ * window.  Call once after the last aem_run. */
+aem_status aem_finalize(aem_sim* sim);
+
+/* Copy one time series (length <= time_steps) to host memory. */
+aem_status aem_get_series(aem_sim* sim, int32_t kind, int32_t idx, float* out, int64_t n);
+
+/* Copy one field component (plan->ncell floats, common layout) to host. */
+aem_status aem_get_field(aem_sim* sim, int32_t comp, float* out, int64_t n);
+
+/* Overwrite one field component from host memory (ncell floats). */
+aem_status aem_set_field(aem_sim* sim, int32_t comp, const float* in, int64_t n);
+
+/* Raw NTFF accumulators U/W (fdtd_solver.f90:7643-7927) as FP64:
+ * which = 0..5 -> Ux,Uy,Uz,Wx,Wy,Wz; out has n_angles*len_ff entries
+ * ([angle][bin]). */
+aem_status aem_get_ntff_raw(aem_sim* sim, int32_t which, double* out, int64_t n);
+
+/* far field phase correction time (f90:8629-8634) per angle, after finalize. */
+aem_status aem_get_ff_time(aem_sim* sim, float* out, int64_t n);
+
+/* Write data.dat (positional format read by utility_scripts/post_processor.py)
+ * and <name>_geometry.bin.  path NULL -> the deck's own file name. */
+aem_status aem_write_outputs(aem_sim* sim, const char* path);
+
+/* Average device time per step of the last aem_run (ms), from HIP events
+ * recorded on the sim's stream around the timed steps. */
+float aem_last_run_ms(const aem_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AEM_H */

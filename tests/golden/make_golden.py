@@ -1,0 +1,78 @@
+"""Regenerate the golden fixtures from the reference solver.
+
+Runs ``oracle/_ref/fdtd_ref`` (the serial reference, compiled by
+``make -C oracle ref`` from main_fdtd/fdtd_solver.f90) on every deck under
+``tests/golden/decks/`` and stores its outputs as ``tests/golden/<deck>.npz``:
+the series of data.dat (incident, port voltages, far field, periodic-slab
+reflection/transmission, phase-centre incident, far-field time corrections,
+the time step) and, for decks that export a video slice, every 25th step of
+the six cell-centred slices.  Only outputs are stored, never reference code.
+
+Usage:  python tests/golden/make_golden.py [deck-name ...]
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from alpineem_amd.datfile import read_dat, read_fortran_record_file  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "fdtd_ref")
+VIDEO_STRIDE = 25
+
+
+def run_deck(name: str) -> None:
+    deck = os.path.join(HERE, "decks", name + ".txt")
+    with tempfile.TemporaryDirectory() as tmp:
+        shutil.copy(deck, os.path.join(tmp, "inputs.txt"))
+        subprocess.run([REF, "inputs.txt"], cwd=tmp, check=True, stdout=subprocess.DEVNULL)
+        d = read_dat(os.path.join(tmp, "data.dat"))
+        out = {"dt": np.float32(d.dt), "time_steps": np.int32(d.time_steps)}
+        if d.incident is not None:
+            out["incident"] = d.incident
+        if d.reflected is not None:
+            out["reflected"] = d.reflected
+            out["transmitted"] = d.transmitted
+        if d.e_theta:
+            out["e_theta"] = np.stack(d.e_theta)
+            out["e_phi"] = np.stack(d.e_phi)
+            out["ff_time"] = d.ff_time
+        if d.port_v:
+            out["port_v"] = np.stack(d.port_v)
+        if d.ff_pc is not None:
+            out["ff_pc"] = d.ff_pc
+        vid = os.path.join(tmp, "data_video_Ex.bin")
+        if os.path.exists(vid):
+            a1 = int(d.header["video size_a"])
+            a2 = int(d.header["video size_b"])
+            frames = []
+            for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+                v = read_fortran_record_file(os.path.join(tmp, f"data_video_{comp}.bin"))
+                v = v.reshape(d.time_steps, a2, a1)
+                frames.append(v[VIDEO_STRIDE - 1::VIDEO_STRIDE])
+            out["video"] = np.stack(frames)           # (6, T/stride, size_b, size_a)
+            out["video_stride"] = np.int32(VIDEO_STRIDE)
+        geo = np.fromfile(os.path.join(tmp, "data_geometry.bin"), np.float32)
+        out["geometry_sum"] = np.float64(geo.astype(np.float64).sum())
+        out["geometry_nnz"] = np.int64(np.count_nonzero(geo))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print(f"{name}: " + ", ".join(f"{k}{tuple(np.shape(v))}" for k, v in out.items()))
+
+
+def main(argv):
+    names = argv or sorted(f[:-4] for f in os.listdir(os.path.join(HERE, "decks")) if f.endswith(".txt"))
+    if not os.path.exists(REF):
+        sys.exit(f"{REF} missing: run 'make -C oracle ref' first")
+    for n in names:
+        run_deck(n)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
