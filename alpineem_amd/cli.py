@@ -1,0 +1,63 @@
+"""Command-line drop-in for the reference executable: ``python -m alpineem_amd.cli inputs.txt``.
+
+Mirrors ``./FDTD inputs.txt`` (main_fdtd/master.py:438-441): parses the deck,
+runs every time step on the GPU and writes ``data.dat`` and
+``<name>_geometry.bin`` next to the deck, printing the reference's progress
+and timing lines (fdtd_solver.f90:724, 4366-4368, 8313-8316).  Deck errors
+print the reference's ``ERROR: ...`` message and exit with status 1.
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+from ._lib import AemError
+from .solver import Plan, Simulation
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if not argv:
+        print("usage: python -m alpineem_amd.cli <inputs.txt>", file=sys.stderr)
+        return 2
+    deck = os.path.abspath(argv[0])
+    t0 = time.time()
+    print("Reading in the inputs file and setting up the simulation...")
+    try:
+        plan = Plan.from_inputs(deck)
+        sim = Simulation(plan)
+    except AemError as e:
+        print(str(e).split(": ", 1)[-1])
+        return 1
+    print(f"Setup is finished. Time to complete was {time.time() - t0:.3f} seconds.")
+    print("Starting main FDTD algorithm...")
+    T = plan.time_steps
+    t1 = time.time()
+    done = 0
+    while done < T:
+        chunk = min(100, T - done)
+        sim.run(chunk)
+        done += chunk
+        if done % 100 == 0:
+            print(f"{done} of {T} time steps")
+    el = time.time() - t1
+    v = plan.view
+    cells = (v.NX - 1) * (v.NY - 1) * (v.NZ - 1)
+    print(f"Finished main FDTD algorithm. Time to complete was {el:.3f} seconds.")
+    print(f"Average of {1e-6 * cells * T / el:.2f} Mcells per second.")
+    print("Post processing some data...")
+    t2 = time.time()
+    cwd = os.getcwd()
+    try:
+        os.chdir(os.path.dirname(deck))
+        sim.finalize()
+        sim.write_outputs()
+    finally:
+        os.chdir(cwd)
+    print(f"Finished post processing. Time to complete was {time.time() - t2:.3f} seconds.")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

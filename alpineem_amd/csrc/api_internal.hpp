@@ -12,6 +12,9 @@ namespace aem {
 
 void set_error(const std::string& m);
 
+// Plan behind a device simulation (defined with the device state).
+const Plan* sim_plan(const aem_sim* s);
+
 // Run f, mapping exceptions to status codes and the thread-local message.
 template <class F>
 aem_status guard(F&& f) {

@@ -1,0 +1,929 @@
+// kernels.hpp -- HIP kernels of the FDTD leapfrog step for gfx950 (MI355X).
+//
+// One step of the reference loop (main_fdtd/fdtd_solver.f90:4440-8299) is a
+// fixed sequence of launches on one stream:
+//   H: bulk -> sheets -> TF/SF (y,z,x faces) -> ports -> CPML (x,y,z slabs) -> PBC
+//   E: bulk -> sheets -> Drude -> TF/SF (y,z,x) -> ports -> CPML (x,y,z) -> PBC -> E_old
+//   outputs: plane sums, NTFF currents + retarded-time binning, phase centre,
+//            step counter
+// Every FP32 expression keeps the reference's left-to-right order and the
+// file is compiled with -ffp-contract=off, so the bulk, CPML, sheet, Drude
+// and port arithmetic is bit-identical to the serial reference.
+//
+// Layout: every 3-D array uses off(i,j,k) = (i-1) + PX*((j-1) + NY*(k-1)),
+// i,j,k 1-based (x fastest); plane = PX*NY.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aemk {
+
+struct Grid {
+    int NX, NY, NZ, PX;
+    long long plane;
+};
+
+__device__ __forceinline__ long long off(const Grid& g, int i, int j, int k) {
+    return (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (long long)(k - 1));
+}
+
+// ---------------------------------------------------------------------------
+// Bulk H (f90:4442-4484).  One thread per (i,j) column marching a chunk of
+// k; Ex(k+1), Ey(k+1) are carried in registers to the next plane.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, float* __restrict__ Hy,
+                                                float* __restrict__ Hz, const float* __restrict__ Ex,
+                                                const float* __restrict__ Ey, const float* __restrict__ Ez,
+                                                const float* __restrict__ dhx, const float* __restrict__ dhy,
+                                                const float* __restrict__ dhz, float DB, int kchunk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.NX - 1 || j > g.NY - 1) return;
+    const int k0 = blockIdx.z * kchunk + 1;
+    const int k1 = min(k0 + kchunk - 1, g.NZ - 1);
+    if (k0 > k1) return;
+    const float dx_i = dhx[i - 1], dy_j = dhy[j - 1];
+    long long o = off(g, i, j, k0);
+    float ey_k = Ey[o], ex_k = Ex[o];
+    for (int k = k0; k <= k1; ++k) {
+        const float ey_k1 = Ey[o + g.plane];
+        const float ex_k1 = Ex[o + g.plane];
+        const float ez = Ez[o];
+        const float ez_xp = Ez[o + 1];
+        const float ez_yp = Ez[o + g.PX];
+        const float ey_xp = Ey[o + 1];
+        const float ex_yp = Ex[o + g.PX];
+        const float dz_k = dhz[k - 1];
+        Hx[o] = Hx[o] + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
+        Hy[o] = Hy[o] + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
+        Hz[o] = Hz[o] + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
+        ey_k = ey_k1;
+        ex_k = ex_k1;
+        o += g.plane;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bulk E (f90:5667-5710) with the per-cell coefficient tuple index.
+// Component ranges: Ex i>=1,j>=2,k>=2; Ey i>=2,j>=1,k>=2; Ez i>=2,j>=2,k>=1.
+// ---------------------------------------------------------------------------
+template <typename IdxT>
+__global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, float* __restrict__ Ey,
+                                                float* __restrict__ Ez, const float* __restrict__ Hx,
+                                                const float* __restrict__ Hy, const float* __restrict__ Hz,
+                                                const IdxT* __restrict__ idx, const float* __restrict__ tup,
+                                                const float* __restrict__ dex, const float* __restrict__ dey,
+                                                const float* __restrict__ dez, int kchunk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.NX - 1 || j > g.NY - 1) return;
+    const int k0 = blockIdx.z * kchunk + 1;
+    const int k1 = min(k0 + kchunk - 1, g.NZ - 1);
+    if (k0 > k1) return;
+    const float dx_i = dex[i - 1], dy_j = dey[j - 1];
+    const bool ix = i >= 2, jy = j >= 2;
+    long long o = off(g, i, j, k0);
+    float hx_km = 0.f, hy_km = 0.f;
+    if (k0 >= 2) {
+        hx_km = Hx[o - g.plane];
+        hy_km = Hy[o - g.plane];
+    }
+    for (int k = k0; k <= k1; ++k) {
+        const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+        const int t = (int)idx[o] * 6;
+        const float dz_k = dez[k - 1];
+        if (jy && k >= 2) {
+            const float hz_ym = Hz[o - g.PX];
+            Ex[o] = tup[t + 0] * Ex[o] + tup[t + 1] * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
+        }
+        if (ix && k >= 2) {
+            const float hz_xm = Hz[o - 1];
+            Ey[o] = tup[t + 2] * Ey[o] + tup[t + 3] * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
+        }
+        if (ix && jy) {
+            const float hy_xm = Hy[o - 1];
+            const float hx_ym = Hx[o - g.PX];
+            Ez[o] = tup[t + 4] * Ez[o] + tup[t + 5] * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
+        }
+        hx_km = hx;
+        hy_km = hy;
+        o += g.plane;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CPML slabs (H: f90:5227-5375, E: f90:6522-6670).  One launch per normal
+// axis, in x, y, z order, which reproduces the reference's per-component add
+// order (Hx: y then z, Hy: x then z, Hz: x then y; same for E).
+// psi of face f / slot s is stored [k][j][i] restricted to the slab, with the
+// slab coordinate replacing the normal index (t = 0..nt-1).
+// ---------------------------------------------------------------------------
+struct Cpml {
+    int n[6];                  // npml per face
+    const float* b[6];         // bh (H launch) or be (E launch)
+    const float* c[6];         // ch or ce
+    float* psi[6][2];          // per face, two components
+};
+
+// H, slab normal to x.  Threads: (t, j, k); t enumerates low then high layers.
+__global__ void k_h_cpml_x(Grid g, Cpml p, float* Hy, float* Hz, const float* Ey, const float* Ez, float DB,
+                           float dx) {
+    const int nl = max(p.n[0] - 1, 0), nh = max(p.n[1] - 1, 0), nt = nl + nh;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)nt * (g.NY - 1) * (g.NZ - 1);
+    if (tid >= tot) return;
+    const int t = (int)(tid % nt);
+    const long long r = tid / nt;
+    const int j = (int)(r % (g.NY - 1)) + 1, k = (int)(r / (g.NY - 1)) + 1;
+    int i, f, q;
+    if (t < nl) { i = t + 1; f = 0; q = i; }
+    else { i = g.NX + 1 - p.n[1] + (t - nl); f = 1; q = g.NX - i; }
+    const int ntf = f == 0 ? nl : nh;
+    const long long po = (long long)(q - 1) + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+    const long long o = off(g, i, j, k);
+    const float b = p.b[f][q - 1], c = p.c[f][q - 1];
+    float* ps = p.psi[f][0];
+    ps[po] = b * ps[po] + c * (Ez[o + 1] - Ez[o]) / dx;
+    Hy[o] = Hy[o] + DB * ps[po];
+    ps = p.psi[f][1];
+    ps[po] = b * ps[po] + c * (Ey[o] - Ey[o + 1]) / dx;
+    Hz[o] = Hz[o] + DB * ps[po];
+}
+
+// H, slab normal to y.  Threads: (i, t, k).
+__global__ void k_h_cpml_y(Grid g, Cpml p, float* Hx, float* Hz, const float* Ex, const float* Ez, float DB,
+                           float dy) {
+    const int nl = max(p.n[2] - 1, 0), nh = max(p.n[3] - 1, 0), nt = nl + nh;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)(g.NX - 1) * nt * (g.NZ - 1);
+    if (tid >= tot) return;
+    const int i = (int)(tid % (g.NX - 1)) + 1;
+    const long long r = tid / (g.NX - 1);
+    const int t = (int)(r % nt), k = (int)(r / nt) + 1;
+    int j, f, q;
+    if (t < nl) { j = t + 1; f = 2; q = j; }
+    else { j = g.NY + 1 - p.n[3] + (t - nl); f = 3; q = g.NY - j; }
+    const int ntf = f == 2 ? nl : nh;
+    const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(q - 1) + (long long)ntf * (k - 1));
+    const long long o = off(g, i, j, k);
+    const float b = p.b[f][q - 1], c = p.c[f][q - 1];
+    float* ps = p.psi[f][0];
+    ps[po] = b * ps[po] + c * (Ez[o] - Ez[o + g.PX]) / dy;
+    Hx[o] = Hx[o] + DB * ps[po];
+    ps = p.psi[f][1];
+    ps[po] = b * ps[po] + c * (Ex[o + g.PX] - Ex[o]) / dy;
+    Hz[o] = Hz[o] + DB * ps[po];
+}
+
+// H, slab normal to z.  Threads: (i, j, t).
+__global__ void k_h_cpml_z(Grid g, Cpml p, float* Hx, float* Hy, const float* Ex, const float* Ey, float DB,
+                           float dz) {
+    const int nl = max(p.n[4] - 1, 0), nh = max(p.n[5] - 1, 0), nt = nl + nh;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)(g.NX - 1) * (g.NY - 1) * nt;
+    if (tid >= tot) return;
+    const int i = (int)(tid % (g.NX - 1)) + 1;
+    const long long r = tid / (g.NX - 1);
+    const int j = (int)(r % (g.NY - 1)) + 1, t = (int)(r / (g.NY - 1));
+    int k, f, q;
+    if (t < nl) { k = t + 1; f = 4; q = k; }
+    else { k = g.NZ + 1 - p.n[5] + (t - nl); f = 5; q = g.NZ - k; }
+    const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (q - 1));
+    const long long o = off(g, i, j, k);
+    const float b = p.b[f][q - 1], c = p.c[f][q - 1];
+    float* ps = p.psi[f][0];
+    ps[po] = b * ps[po] + c * (Ey[o + g.plane] - Ey[o]) / dz;
+    Hx[o] = Hx[o] + DB * ps[po];
+    ps = p.psi[f][1];
+    ps[po] = b * ps[po] + c * (Ex[o] - Ex[o + g.plane]) / dz;
+    Hy[o] = Hy[o] + DB * ps[po];
+}
+
+template <typename IdxT>
+__device__ __forceinline__ float gb_at(const IdxT* idx, const float* tup, long long o, int c) {
+    return tup[(int)idx[o] * 6 + 2 * c + 1];
+}
+
+// E, slab normal to x: Ey (j 1..NY-1, k 2..NZ-1) and Ez (j 2..NY-1, k 1..NZ-1).
+// Layers: low i = 2..n1 (q = i), high i = NX+1-n2..NX-1 (q = NX+1-i).
+template <typename IdxT>
+__global__ void k_e_cpml_x(Grid g, Cpml p, float* Ey, float* Ez, const float* Hy, const float* Hz,
+                           const IdxT* idx, const float* tup, float dx) {
+    const int nl = max(p.n[0] - 1, 0), nh = max(p.n[1] - 1, 0), nt = nl + nh;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)nt * (g.NY - 1) * (g.NZ - 1);
+    if (tid >= tot) return;
+    const int t = (int)(tid % nt);
+    const long long r = tid / nt;
+    const int j = (int)(r % (g.NY - 1)) + 1, k = (int)(r / (g.NY - 1)) + 1;
+    int i, f, q, ql;
+    if (t < nl) { i = t + 2; f = 0; q = i; ql = t; }
+    else { i = g.NX + 1 - p.n[1] + (t - nl); f = 1; q = g.NX + 1 - i; ql = q - 2; }
+    const int ntf = f == 0 ? nl : nh;
+    const long long po = (long long)ql + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+    const long long o = off(g, i, j, k);
+    const float b = p.b[f][q - 1], c = p.c[f][q - 1];
+    if (k >= 2) {
+        float* ps = p.psi[f][0];
+        ps[po] = b * ps[po] + c * (Hz[o - 1] - Hz[o]) / dx;
+        Ey[o] = Ey[o] + gb_at(idx, tup, o, 1) * ps[po];
+    }
+    if (j >= 2) {
+        float* ps = p.psi[f][1];
+        ps[po] = b * ps[po] + c * (Hy[o] - Hy[o - 1]) / dx;
+        Ez[o] = Ez[o] + gb_at(idx, tup, o, 2) * ps[po];
+    }
+}
+
+// E, slab normal to y: Ex (i 1..NX-1, k 2..NZ-1) and Ez (i 2..NX-1, k 1..NZ-1).
+template <typename IdxT>
+__global__ void k_e_cpml_y(Grid g, Cpml p, float* Ex, float* Ez, const float* Hx, const float* Hz,
+                           const IdxT* idx, const float* tup, float dy) {
+    const int nl = max(p.n[2] - 1, 0), nh = max(p.n[3] - 1, 0), nt = nl + nh;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)(g.NX - 1) * nt * (g.NZ - 1);
+    if (tid >= tot) return;
+    const int i = (int)(tid % (g.NX - 1)) + 1;
+    const long long r = tid / (g.NX - 1);
+    const int t = (int)(r % nt), k = (int)(r / nt) + 1;
+    int j, f, q, ql;
+    if (t < nl) { j = t + 2; f = 2; q = j; ql = t; }
+    else { j = g.NY + 1 - p.n[3] + (t - nl); f = 3; q = g.NY + 1 - j; ql = q - 2; }
+    const int ntf = f == 2 ? nl : nh;
+    const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)ql + (long long)ntf * (k - 1));
+    const long long o = off(g, i, j, k);
+    const float b = p.b[f][q - 1], c = p.c[f][q - 1];
+    if (k >= 2) {
+        float* ps = p.psi[f][0];
+        ps[po] = b * ps[po] + c * (Hz[o] - Hz[o - g.PX]) / dy;
+        Ex[o] = Ex[o] + gb_at(idx, tup, o, 0) * ps[po];
+    }
+    if (i >= 2) {
+        float* ps = p.psi[f][1];
+        ps[po] = b * ps[po] + c * (Hx[o - g.PX] - Hx[o]) / dy;
+        Ez[o] = Ez[o] + gb_at(idx, tup, o, 2) * ps[po];
+    }
+}
+
+// E, slab normal to z: Ex (i 1..NX-1, j 2..NY-1) and Ey (i 2..NX-1, j 1..NY-1).
+template <typename IdxT>
+__global__ void k_e_cpml_z(Grid g, Cpml p, float* Ex, float* Ey, const float* Hx, const float* Hy,
+                           const IdxT* idx, const float* tup, float dz) {
+    const int nl = max(p.n[4] - 1, 0), nh = max(p.n[5] - 1, 0), nt = nl + nh;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)(g.NX - 1) * (g.NY - 1) * nt;
+    if (tid >= tot) return;
+    const int i = (int)(tid % (g.NX - 1)) + 1;
+    const long long r = tid / (g.NX - 1);
+    const int j = (int)(r % (g.NY - 1)) + 1, t = (int)(r / (g.NY - 1));
+    int k, f, q, ql;
+    if (t < nl) { k = t + 2; f = 4; q = k; ql = t; }
+    else { k = g.NZ + 1 - p.n[5] + (t - nl); f = 5; q = g.NZ + 1 - k; ql = q - 2; }
+    const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * ql);
+    const long long o = off(g, i, j, k);
+    const float b = p.b[f][q - 1], c = p.c[f][q - 1];
+    if (j >= 2) {
+        float* ps = p.psi[f][0];
+        ps[po] = b * ps[po] + c * (Hy[o - g.plane] - Hy[o]) / dz;
+        Ex[o] = Ex[o] + gb_at(idx, tup, o, 0) * ps[po];
+    }
+    if (i >= 2) {
+        float* ps = p.psi[f][1];
+        ps[po] = b * ps[po] + c * (Hx[o] - Hx[o - g.plane]) / dz;
+        Ey[o] = Ey[o] + gb_at(idx, tup, o, 1) * ps[po];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TF/SF plane-wave faces (H f90:4627-4710, E f90:5950-6033) with the
+// analytic incident field Inc() (f90:8825-8842).
+// ---------------------------------------------------------------------------
+struct Tfsf {
+    int lo[3], hi[3], delay[3], pbc[3];
+    float WE[3], WH[3];
+    float s_th, c_th, s_ph, c_ph;
+    float kH[3], kE[3];
+    float dx, dy, dz, dt, inv_c, spread, t_spread;
+    int pulse_type;
+};
+
+__device__ __forceinline__ float inc_fn(const Tfsf& p, float w, float xf, float yf, float zf, float cf) {
+    const float tx = p.s_th * p.c_ph * (xf - (float)p.delay[0]) * p.dx;
+    const float ty = p.s_th * p.s_ph * (yf - (float)p.delay[1]) * p.dy;
+    const float tz = p.c_th * (zf - (float)p.delay[2]) * p.dz;
+    const float arg = (p.t_spread - (cf * p.dt) + p.inv_c * (tx + ty + tz)) / p.spread;
+    if (p.pulse_type == 1) return w * expf(-0.5f * (arg * arg));
+    return -1.0f * w * arg * expf(0.5f) * expf(-0.5f * (arg * arg));
+}
+
+// face: 0 = y faces, 1 = z faces, 2 = x faces.  (u,v) spans the face.
+__global__ void k_h_tfsf(Grid g, Tfsf p, int face, float* Hx, float* Hy, float* Hz, const int* step) {
+    const int n = *step;
+    const float cf = (float)n - 1.0f;
+    const int xl = p.lo[0], xh = p.hi[0], yl = p.lo[1], yh = p.hi[1], zl = p.lo[2], zh = p.hi[2];
+    const int px = p.pbc[0], py = p.pbc[1], pz = p.pbc[2];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    if (face == 0) {  // u = i, v = k
+        const int i = xl + a, k = zl + b;
+        if (k <= zh && i <= xh - 1 + px) {
+            Hz[off(g, i, yl - 1, k)] = Hz[off(g, i, yl - 1, k)] - p.kH[1] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)yl, (float)k, cf) * 1.0f;
+            Hz[off(g, i, yh, k)] = Hz[off(g, i, yh, k)] + p.kH[1] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)yh, (float)k, cf) * 1.0f;
+        }
+        if (k <= zh - 1 + pz && i <= xh) {
+            Hx[off(g, i, yl - 1, k)] = Hx[off(g, i, yl - 1, k)] + p.kH[1] * inc_fn(p, p.WE[2], (float)i, (float)yl, (float)k + 0.5f, cf) * 1.0f;
+            Hx[off(g, i, yh, k)] = Hx[off(g, i, yh, k)] - p.kH[1] * inc_fn(p, p.WE[2], (float)i, (float)yh, (float)k + 0.5f, cf) * 1.0f;
+        }
+    } else if (face == 1) {  // u = i, v = j
+        const int i = xl + a, j = yl + b;
+        if (j <= yh && i <= xh - 1 + px) {
+            Hy[off(g, i, j, zl - 1)] = Hy[off(g, i, j, zl - 1)] + p.kH[2] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)j, (float)zl, cf) * 1.0f;
+            Hy[off(g, i, j, zh)] = Hy[off(g, i, j, zh)] - p.kH[2] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)j, (float)zh, cf) * 1.0f;
+        }
+        if (j <= yh - 1 + py && i <= xh) {
+            Hx[off(g, i, j, zl - 1)] = Hx[off(g, i, j, zl - 1)] - p.kH[2] * inc_fn(p, p.WE[1], (float)i, (float)j + 0.5f, (float)zl, cf) * 1.0f;
+            Hx[off(g, i, j, zh)] = Hx[off(g, i, j, zh)] + p.kH[2] * inc_fn(p, p.WE[1], (float)i, (float)j + 0.5f, (float)zh, cf) * 1.0f;
+        }
+    } else {  // u = j, v = k
+        const int j = yl + a, k = zl + b;
+        if (k <= zh && j <= yh - 1 + py) {
+            Hz[off(g, xl - 1, j, k)] = Hz[off(g, xl - 1, j, k)] + p.kH[0] * inc_fn(p, p.WE[1], (float)xl, (float)j + 0.5f, (float)k, cf) * 1.0f;
+            Hz[off(g, xh, j, k)] = Hz[off(g, xh, j, k)] - p.kH[0] * inc_fn(p, p.WE[1], (float)xh, (float)j + 0.5f, (float)k, cf) * 1.0f;
+        }
+        if (k <= zh - 1 + pz && j <= yh) {
+            Hy[off(g, xl - 1, j, k)] = Hy[off(g, xl - 1, j, k)] - p.kH[0] * inc_fn(p, p.WE[2], (float)xl, (float)j, (float)k + 0.5f, cf) * 1.0f;
+            Hy[off(g, xh, j, k)] = Hy[off(g, xh, j, k)] + p.kH[0] * inc_fn(p, p.WE[2], (float)xh, (float)j, (float)k + 0.5f, cf) * 1.0f;
+        }
+    }
+}
+
+__global__ void k_e_tfsf(Grid g, Tfsf p, int face, float* Ex, float* Ey, float* Ez, const int* step) {
+    const int n = *step;
+    const float cf = (float)n - 0.5f;
+    const int xl = p.lo[0], xh = p.hi[0], yl = p.lo[1], yh = p.hi[1], zl = p.lo[2], zh = p.hi[2];
+    const int px = p.pbc[0], py = p.pbc[1], pz = p.pbc[2];
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    if (face == 0) {  // y faces: u = i (from xl), v = k (from zl)
+        const int i = xl + a, k = zl + b;
+        if (k <= zh - 1 + pz && i >= xl + px && i <= xh) {
+            Ez[off(g, i, yl, k)] = Ez[off(g, i, yl, k)] + p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yl - 0.5f, (float)k + 0.5f, cf) * 1.0f;
+            Ez[off(g, i, yh, k)] = Ez[off(g, i, yh, k)] - p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yh + 0.5f, (float)k + 0.5f, cf) * 1.0f;
+        }
+        if (k >= zl + pz && k <= zh && i <= xh - 1 + px) {
+            Ex[off(g, i, yl, k)] = Ex[off(g, i, yl, k)] - p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yl - 0.5f, (float)k, cf) * 1.0f;
+            Ex[off(g, i, yh, k)] = Ex[off(g, i, yh, k)] + p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yh + 0.5f, (float)k, cf) * 1.0f;
+        }
+    } else if (face == 1) {  // z faces: u = i, v = j
+        const int i = xl + a, j = yl + b;
+        if (j <= yh - 1 + py && i >= xl + px && i <= xh) {
+            Ey[off(g, i, j, zl)] = Ey[off(g, i, j, zl)] - p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zl - 0.5f, cf) * 1.0f;
+            Ey[off(g, i, j, zh)] = Ey[off(g, i, j, zh)] + p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zh + 0.5f, cf) * 1.0f;
+        }
+        if (j >= yl + py && j <= yh && i <= xh - 1 + px) {
+            Ex[off(g, i, j, zl)] = Ex[off(g, i, j, zl)] + p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zl - 0.5f, cf) * 1.0f;
+            Ex[off(g, i, j, zh)] = Ex[off(g, i, j, zh)] - p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zh + 0.5f, cf) * 1.0f;
+        }
+    } else {  // x faces: u = j, v = k
+        const int j = yl + a, k = zl + b;
+        if (k <= zh - 1 + pz && j >= yl + py && j <= yh) {
+            Ez[off(g, xl, j, k)] = Ez[off(g, xl, j, k)] - p.kE[0] * inc_fn(p, p.WH[1], (float)xl - 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f;
+            Ez[off(g, xh, j, k)] = Ez[off(g, xh, j, k)] + p.kE[0] * inc_fn(p, p.WH[1], (float)xh + 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f;
+        }
+        if (k >= zl + pz && k <= zh && j <= yh - 1 + py) {
+            Ey[off(g, xl, j, k)] = Ey[off(g, xl, j, k)] + p.kE[0] * inc_fn(p, p.WH[2], (float)xl - 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f;
+            Ey[off(g, xh, j, k)] = Ey[off(g, xh, j, k)] - p.kE[0] * inc_fn(p, p.WH[2], (float)xh + 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lumped ports (f90:4800-5013 H phase, 6120-6283 E phase).  One work-group
+// handles every port in order; the 1-D line updates are spread over threads.
+// ---------------------------------------------------------------------------
+struct PortDev {
+    int dir, loc[3], size[3];
+    float kI, kV, kS, dl, cur_num, cur_den, gap_div;
+    int excited;
+};
+
+struct Pulse {
+    float t_spread, spread, dt, inv_c, amp;
+    int pulse_type;
+};
+
+// E_or_V_1D() of f90:8844-8857
+__device__ __forceinline__ float e_or_v(const Pulse& p, float tc, float pos) {
+    const float x = ((p.t_spread - ((tc - 1.0f) * p.dt)) - p.inv_c * pos) / p.spread;
+    if (p.pulse_type == 1) return p.amp * expf(-0.5f * (x * x));
+    return p.amp * -1.0f * x * expf(0.5f) * expf(-0.5f * (x * x));
+}
+
+__global__ void __launch_bounds__(256) k_ports_h(Grid g, const PortDev* ports, int np, int M, float* V, float* I,
+                                                 float* Iout, int T, const float* Hx, const float* Hy,
+                                                 const float* Hz, float dx, float dy, float dz, Pulse pu,
+                                                 const int* step) {
+    const int n = *step;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < np; ++q) {
+            const PortDev pt = ports[q];
+            const int l1 = pt.loc[0], l2 = pt.loc[1], l3 = pt.loc[2];
+            const int d1 = pt.size[0] + l1 - 1, d2 = pt.size[1] + l2 - 1, d3 = pt.size[2] + l3 - 1;
+            float c = 0.0f;
+            if (pt.dir == 0) {
+                for (int ii = l1; ii <= d1; ++ii) {
+                    for (int jj = l2; jj <= d2; ++jj) c = c + dy * (Hy[off(g, ii, jj, l3 - 1)] - Hy[off(g, ii, jj, d3)]);
+                    for (int kk = l3; kk <= d3; ++kk) c = c + dz * (Hz[off(g, ii, l2, kk)] - Hz[off(g, ii, d2 - 1, kk)]);
+                }
+            } else if (pt.dir == 1) {
+                for (int jj = l2; jj <= d2; ++jj) {
+                    for (int ii = l1; ii <= d1; ++ii) c = c + dx * (Hx[off(g, ii, jj, l3)] - Hx[off(g, ii, jj, d3 - 1)]);
+                    for (int kk = l3; kk <= d3; ++kk) c = c + dz * (Hz[off(g, l1 - 1, jj, kk)] - Hz[off(g, d1, jj, kk)]);
+                }
+            } else {
+                for (int kk = l3; kk <= d3; ++kk) {
+                    for (int ii = l1; ii <= d1; ++ii) c = c + dx * (Hx[off(g, ii, l2 - 1, kk)] - Hx[off(g, ii, d2, kk)]);
+                    for (int jj = l2; jj <= d2; ++jj) c = c + dy * (Hy[off(g, l1, jj, kk)] - Hy[off(g, d1 - 1, jj, kk)]);
+                }
+            }
+            c = c * pt.cur_num / pt.cur_den;
+            I[(long long)q * M] = c;
+            Iout[(long long)q * T + (n - 1)] = c;
+        }
+    }
+    // 1-D line current, j = 2..M-1 (0-based j-1); reads V only
+    for (int q = 0; q < np; ++q) {
+        const float kI = ports[q].kI;
+        float* Iq = I + (long long)q * M;
+        const float* Vq = V + (long long)q * M;
+        for (int j = 2 + threadIdx.x; j <= M - 1; j += blockDim.x) Iq[j - 1] = Iq[j - 1] - kI * (Vq[j] - Vq[j - 1]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < np; ++q)
+            if (ports[q].excited) {
+                float* Iq = I + (long long)q * M;
+                Iq[1] = Iq[1] - ports[q].kI * e_or_v(pu, (float)n + 0.0f, 0.0f * ports[q].dl);
+            }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ports_e(Grid g, const PortDev* ports, int np, int M, float* V,
+                                                 const float* I, float* Vout, int T, float* Ex, float* Ey,
+                                                 float* Ez, Pulse pu, const int* step) {
+    const int n = *step;
+    for (int q = 0; q < np; ++q) {
+        const float kV = ports[q].kV;
+        float* Vq = V + (long long)q * M;
+        const float* Iq = I + (long long)q * M;
+        for (int j = 2 + threadIdx.x; j <= M - 1; j += blockDim.x) Vq[j - 1] = Vq[j - 1] - kV * (Iq[j - 1] - Iq[j - 2]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < np; ++q)
+            if (ports[q].excited) {
+                float* Vq = V + (long long)q * M;
+                Vq[1] = Vq[1] + ports[q].kS * e_or_v(pu, (float)n + 0.5f, 0.5f * ports[q].dl);
+            }
+        for (int q = 0; q < np; ++q) {
+            const PortDev pt = ports[q];
+            const float v2 = V[(long long)q * M + 1];
+            float* F = pt.dir == 0 ? Ex : (pt.dir == 1 ? Ey : Ez);
+            const float val = 1.0f * v2 / pt.gap_div;
+            for (int ii = pt.loc[0]; ii <= pt.loc[0] + pt.size[0] - 1; ++ii)
+                for (int jj = pt.loc[1]; jj <= pt.loc[1] + pt.size[1] - 1; ++jj)
+                    for (int kk = pt.loc[2]; kk <= pt.loc[2] + pt.size[2] - 1; ++kk) F[off(g, ii, jj, kk)] = val;
+            Vout[(long long)q * T + (n - 1)] = v2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thin sheets (H f90:4486-4554, E f90:5712-5780).  One launch per normal
+// axis handles every plane of that axis; (p,q) span the plane.
+// ---------------------------------------------------------------------------
+struct Sheets {
+    int np;                // planes on this axis
+    const int* pos;
+    const float* sig;
+    const float* ep;
+    float* es;             // special E of each plane
+    int P, Q;
+};
+
+__device__ __forceinline__ long long sh_off(const Sheets& s, int pl, int p, int q) {
+    return (long long)pl * s.P * s.Q + (p - 1) + (long long)s.P * (q - 1);
+}
+__device__ __forceinline__ bool sh_on(const Sheets& s, int pl, int p, int q) {
+    const long long o = sh_off(s, pl, p, q);
+    return s.sig[o] > 0 || s.ep[o] > 1;
+}
+
+__global__ void k_h_sheets(Grid g, Sheets s, int axis, float* Hx, float* Hy, float* Hz, const float* Ex,
+                           const float* Ey, const float* Ez, const float* dhx, const float* dhy, const float* dhz,
+                           float f) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int q = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    for (int pl = 0; pl < s.np; ++pl) {
+        const int c = s.pos[pl];
+        if (axis == 0) {           // (p,q) = (j,k), j,k in 1..N-1
+            const int i = c, j = p, k = q;
+            if (j > g.NY - 1 || k > g.NZ - 1 || !sh_on(s, pl, j, k)) continue;
+            const long long o = off(g, i, j, k);
+            Hy[o] = Hy[o] + f * ((Ex[o + g.plane] - Ex[o] + s.es[sh_off(s, pl, j, k)] - s.es[sh_off(s, pl, j, k + 1)]) * dhz[k - 1]);
+            Hz[o] = Hz[o] + f * ((Ex[o] - Ex[o + g.PX] + s.es[sh_off(s, pl, j + 1, k)] - s.es[sh_off(s, pl, j, k)]) * dhy[j - 1]);
+        } else if (axis == 1) {    // (p,q) = (i,k)
+            const int i = p, j = c, k = q;
+            if (i > g.NX - 1 || k > g.NZ - 1 || !sh_on(s, pl, i, k)) continue;
+            const long long o = off(g, i, j, k);
+            Hx[o] = Hx[o] + f * (Ey[o] - Ey[o + g.plane] + s.es[sh_off(s, pl, i, k + 1)] - s.es[sh_off(s, pl, i, k)]) * dhz[k - 1];
+            Hz[o] = Hz[o] + f * ((Ey[o + 1] - Ey[o] - s.es[sh_off(s, pl, i + 1, k)] + s.es[sh_off(s, pl, i, k)]) * dhx[i - 1]);
+        } else {                   // (p,q) = (i,j)
+            const int i = p, j = q, k = c;
+            if (i > g.NX - 1 || j > g.NY - 1 || !sh_on(s, pl, i, j)) continue;
+            const long long o = off(g, i, j, k);
+            Hx[o] = Hx[o] + f * (Ez[o + g.PX] - Ez[o] - s.es[sh_off(s, pl, i, j + 1)] + s.es[sh_off(s, pl, i, j)]) * dhy[j - 1];
+            Hy[o] = Hy[o] + f * ((Ez[o] - Ez[o + 1] + s.es[sh_off(s, pl, i + 1, j)] - s.es[sh_off(s, pl, i, j)]) * dhx[i - 1]);
+        }
+    }
+}
+
+__global__ void k_e_sheets(Grid g, Sheets s, int axis, const float* Hx, const float* Hy, const float* Hz,
+                           const float* dex, const float* dey, const float* dez, float dt, float ep0) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int q = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    for (int pl = 0; pl < s.np; ++pl) {
+        const int c = s.pos[pl];
+        int i, j, k;
+        float d1, d2;
+        if (axis == 0) { i = c; j = p; k = q; if (j > g.NY - 1 || k > g.NZ - 1) continue; }
+        else if (axis == 1) { i = p; j = c; k = q; if (i > g.NX - 1 || k > g.NZ - 1) continue; }
+        else { i = p; j = q; k = c; if (i > g.NX - 1 || j > g.NY - 1) continue; }
+        if (!sh_on(s, pl, p, q)) continue;
+        const long long so = sh_off(s, pl, p, q);
+        const float sg = s.sig[so], ep = s.ep[so];
+        const long long o = off(g, i, j, k);
+        float curl;
+        if (axis == 0) curl = (Hz[o] - Hz[o - g.PX]) * dey[j - 1] + (Hy[o - g.plane] - Hy[o]) * dez[k - 1];
+        else if (axis == 1) curl = (Hz[o - 1] - Hz[o]) * dex[i - 1] + (Hx[o] - Hx[o - g.plane]) * dez[k - 1];
+        else curl = (Hy[o] - Hy[o - 1]) * dex[i - 1] + (Hx[o - g.PX] - Hx[o]) * dey[j - 1];
+        (void)d1; (void)d2;
+        s.es[so] = ((1.0f - sg * dt / (2.0f * ep0 * ep)) / (1.0f + sg * dt / (2.0f * ep0 * ep))) * s.es[so] +
+                   ((dt / (ep0 * ep)) / (1.0f + sg * dt / (2.0f * ep0 * ep))) * curl;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Drude ADE on the plasma bounding box (f90:5782-5854) and the E_old save
+// (f90:6812-6826, restricted to the box).
+// ---------------------------------------------------------------------------
+struct DrudeBox {
+    int lo[3], hi[3];
+    long long nb;
+    const float* Ja;
+    const float* Jb;
+    float* J;
+    float* Eold;
+};
+
+template <typename IdxT>
+__global__ void k_drude(Grid g, DrudeBox b, int comp, int P, float* E, const IdxT* idx, const float* tup) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= b.nb) return;
+    const int bx = b.hi[0] - b.lo[0] + 1, by = b.hi[1] - b.lo[1] + 1;
+    const int i = b.lo[0] + (int)(tid % bx);
+    const int j = b.lo[1] + (int)((tid / bx) % by);
+    const int k = b.lo[2] + (int)(tid / ((long long)bx * by));
+    const long long o = off(g, i, j, k);
+    float t = 0.0f;
+    for (int q = 0; q < P; ++q) t = t + (1.0f + b.Ja[q * b.nb + tid]) * b.J[q * b.nb + tid];
+    const float e = E[o] - gb_at(idx, tup, o, comp) * t / 2.0f;
+    E[o] = e;
+    const float eo = b.Eold[tid];
+    for (int q = 0; q < P; ++q) b.J[q * b.nb + tid] = b.Ja[q * b.nb + tid] * b.J[q * b.nb + tid] + b.Jb[q * b.nb + tid] * (e + eo);
+}
+
+__global__ void k_save_eold(Grid g, DrudeBox b, const float* E) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= b.nb) return;
+    const int bx = b.hi[0] - b.lo[0] + 1, by = b.hi[1] - b.lo[1] + 1;
+    const int i = b.lo[0] + (int)(tid % bx);
+    const int j = b.lo[1] + (int)((tid / bx) % by);
+    const int k = b.lo[2] + (int)(tid / ((long long)bx * by));
+    b.Eold[tid] = E[off(g, i, j, k)];
+}
+
+// ---------------------------------------------------------------------------
+// Periodic boundary copies (H f90:5377-5433, E f90:6672-6741).  One launch per
+// axis, in x, y, z order; the special E of sheet planes is copied alongside.
+// ---------------------------------------------------------------------------
+__global__ void k_h_pbc(Grid g, int axis, float* Ha, float* Hb) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int v = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (axis == 0) {  // Hz, Hy: (NX-1,j,k) <- (1,j,k)
+        if (u > g.NY - 1 || v > g.NZ - 1) return;
+        Ha[off(g, g.NX - 1, u, v)] = Ha[off(g, 1, u, v)];
+        Hb[off(g, g.NX - 1, u, v)] = Hb[off(g, 1, u, v)];
+    } else if (axis == 1) {  // Hz, Hx
+        if (u > g.NX - 1 || v > g.NZ - 1) return;
+        Ha[off(g, u, g.NY - 1, v)] = Ha[off(g, u, 1, v)];
+        Hb[off(g, u, g.NY - 1, v)] = Hb[off(g, u, 1, v)];
+    } else {  // Hy, Hx
+        if (u > g.NX - 1 || v > g.NY - 1) return;
+        Ha[off(g, u, v, g.NZ - 1)] = Ha[off(g, u, v, 1)];
+        Hb[off(g, u, v, g.NZ - 1)] = Hb[off(g, u, v, 1)];
+    }
+}
+
+__global__ void k_e_pbc(Grid g, int axis, float* Ea, float* Eb) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int v = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (axis == 0) {  // Ez, Ey: (1,j,k) <- (NX-1,j,k)
+        if (u > g.NY - 1 || v > g.NZ - 1) return;
+        Ea[off(g, 1, u, v)] = Ea[off(g, g.NX - 1, u, v)];
+        Eb[off(g, 1, u, v)] = Eb[off(g, g.NX - 1, u, v)];
+    } else if (axis == 1) {  // Ez, Ex
+        if (u > g.NX - 1 || v > g.NZ - 1) return;
+        Ea[off(g, u, 1, v)] = Ea[off(g, u, g.NY - 1, v)];
+        Eb[off(g, u, 1, v)] = Eb[off(g, u, g.NY - 1, v)];
+    } else {  // Ey, Ex
+        if (u > g.NX - 1 || v > g.NY - 1) return;
+        Ea[off(g, u, v, 1)] = Ea[off(g, u, v, g.NZ - 1)];
+        Eb[off(g, u, v, 1)] = Eb[off(g, u, v, g.NZ - 1)];
+    }
+}
+
+// In-plane PBC copy of a special-E plane array: dst row/column <- src.
+// mode 0: p index 1 <- P_src (copy along p); mode 1: q index 1 <- Q_src.
+__global__ void k_sheet_pbc(Sheets s, int mode, int src, int n_other) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int pl = blockIdx.y;
+    if (u > n_other || pl >= s.np) return;
+    if (mode == 0) s.es[sh_off(s, pl, 1, u)] = s.es[sh_off(s, pl, src, u)];
+    else s.es[sh_off(s, pl, u, 1)] = s.es[sh_off(s, pl, u, src)];
+}
+
+// ---------------------------------------------------------------------------
+// Outputs
+// ---------------------------------------------------------------------------
+
+// Reflection/transmission plane sums (f90:7005-7103): one work-group, FP64
+// accumulation in a fixed order (deterministic), stored per step.
+__global__ void __launch_bounds__(256) k_plane_sums(Grid g, int axis, int lo_pos, int hi_pos, int u0, int u1, int v0,
+                                                    int v1, const float* Ea, const float* Eb, float c_a, float c_b1,
+                                                    float c_b2, int minus_b, double* refl, double* trans,
+                                                    const int* step) {
+    // axis 0: planes x=lo/hi, (u,v)=(j,k), Ea=Ez, Eb=Ey;  axis 1: (u,v)=(i,k), Ea=Ez, Eb=Ex;
+    // axis 2: (u,v)=(i,j), Ea=Ey, Eb=Ex
+    __shared__ double sr[256], st[256];
+    const int n = *step;
+    const int nu = u1 - u0 + 1, nv = v1 - v0 + 1;
+    double ar = 0.0, at = 0.0;
+    for (int t = threadIdx.x; t < nu * nv; t += blockDim.x) {
+        const int u = u0 + t % nu, v = v0 + t / nu;
+        long long ol, oh;
+        if (axis == 0) { ol = off(g, lo_pos, u, v); oh = off(g, hi_pos, u, v); }
+        else if (axis == 1) { ol = off(g, u, lo_pos, v); oh = off(g, u, hi_pos, v); }
+        else { ol = off(g, u, v, lo_pos); oh = off(g, u, v, hi_pos); }
+        // per-cell terms exactly as the reference forms them: (a*c_a) -/+ ((b*c_b1)*c_b2)
+        const float ra = Ea[ol] * c_a, rb = Eb[ol] * c_b1 * c_b2;
+        const float ta = Ea[oh] * c_a, tb = Eb[oh] * c_b1 * c_b2;
+        ar = ar + (double)ra;
+        at = at + (double)ta;
+        if (minus_b) { ar = ar - (double)rb; at = at - (double)tb; }
+        else { ar = ar + (double)rb; at = at + (double)tb; }
+    }
+    sr[threadIdx.x] = ar;
+    st[threadIdx.x] = at;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            st[threadIdx.x] += st[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        refl[n - 1] = sr[0];
+        trans[n - 1] = st[0];
+    }
+}
+
+// NTFF: face currents.  Faces 0..5 = x-low, x-high, y-low, y-high, z-low,
+// z-high; per face two magnetic (M) and two electric (J) components with
+// their previous-step values (f90:7512-7641).
+struct NtffFaces {
+    float* M[6][2];
+    float* Mo[6][2];
+    float* J[6][2];
+    float* Jo[6][2];
+    int fd0[6];            // row length of each face array
+    int lo[3], hi[3];
+    int pbc[3];
+};
+
+__global__ void k_ntff_currents(Grid g, NtffFaces f, const float* Ex, const float* Ey, const float* Ez,
+                                const float* Hx, const float* Hy, const float* Hz) {
+    const int face = blockIdx.z;
+    const int a = face / 2;
+    if (f.pbc[a]) return;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const int u = f.lo[b] + blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = f.lo[c] + blockIdx.y * blockDim.y + threadIdx.y;
+    if (u > f.hi[b] || v > f.hi[c]) return;
+    const bool low = (face & 1) == 0;
+    const int pos = low ? f.lo[a] : f.hi[a];
+    const float s = low ? 1.0f : -1.0f;      // sign flip between low and high faces
+    const long long fo = (long long)u + (long long)f.fd0[face] * v;
+    float m0, m1, j0, j1;
+    if (a == 0) {
+        const int i = pos, j = u, k = v;
+        const long long o = off(g, i, j, k);
+        m0 = (-1.0f * s) * (Ez[o] + Ez[o + g.PX] + Ez[o + 1] + Ez[o + 1 + g.PX]) / 4.0f * 1.0f;
+        m1 = s * (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f * 1.0f;
+        j0 = s * (Hz[o] + Hz[o + g.plane]) / 2.0f * 1.0f;
+        j1 = (-1.0f * s) * (Hy[o] + Hy[o + g.PX]) / 2.0f * 1.0f;
+    } else if (a == 1) {
+        const int i = u, j = pos, k = v;
+        const long long o = off(g, i, j, k);
+        m0 = s * (Ez[o] + Ez[o + 1] + Ez[o + g.PX] + Ez[o + 1 + g.PX]) / 4.0f * 1.0f;
+        m1 = (-1.0f * s) * (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f * 1.0f;
+        j0 = (-1.0f * s) * (Hz[o] + Hz[o + g.plane]) / 2.0f * 1.0f;
+        j1 = s * (Hx[o] + Hx[o + 1]) / 2.0f * 1.0f;
+    } else {
+        const int i = u, j = v, k = pos;
+        const long long o = off(g, i, j, k);
+        m0 = (-1.0f * s) * (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f * 1.0f;
+        m1 = s * (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f * 1.0f;
+        j0 = s * (Hy[o] + Hy[o + g.PX]) / 2.0f * 1.0f;
+        j1 = (-1.0f * s) * (Hx[o] + Hx[o + 1]) / 2.0f * 1.0f;
+    }
+    f.Mo[face][0][fo] = f.M[face][0][fo];
+    f.M[face][0][fo] = m0;
+    f.Mo[face][1][fo] = f.M[face][1][fo];
+    f.M[face][1][fo] = m1;
+    f.Jo[face][0][fo] = f.J[face][0][fo];
+    f.J[face][0][fo] = j0;
+    f.Jo[face][1][fo] = f.J[face][1][fo];
+    f.J[face][1][fo] = j1;
+}
+
+// NTFF retarded-time binning (f90:7643-7927).  Index and weight arithmetic is
+// the reference's FP32; accumulation is FP64 (atomic adds).
+struct NtffBins {
+    const float* ang;      // [n_angles][6]
+    int n_angles;
+    int ic, jc, kc;
+    float r_relay, dx, dy, dz, c_dt;
+    float K[3];
+    long long LL;          // bins per angle (len_ff + 2)
+    double* U[3];
+    double* W[3];
+};
+
+__global__ void k_ntff_bins(NtffFaces f, NtffBins bn, const int* step) {
+    const int n = *step;
+    const int face = blockIdx.z % 6;
+    const int kk = blockIdx.z / 6;
+    const int a = face / 2;
+    if (f.pbc[a]) return;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const int u = f.lo[b] + blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = f.lo[c] + blockIdx.y * blockDim.y + threadIdx.y;
+    if (u > f.hi[b] || v > f.hi[c]) return;
+    const bool low = (face & 1) == 0;
+    const int pos = low ? f.lo[a] : f.hi[a];
+    int i, j, k;
+    if (a == 0) { i = pos; j = u; k = v; }
+    else if (a == 1) { i = u; j = pos; k = v; }
+    else { i = u; j = v; k = pos; }
+    const float* an = bn.ang + 6 * kk;
+    const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
+    const float tfv = (bn.r_relay - ((float)(i - bn.ic) * bn.dx * s1 * c2 + (float)(j - bn.jc) * bn.dy * s1 * s2 +
+                                     (float)(k - bn.kc) * bn.dz * c1)) / bn.c_dt;
+    const long long fo = (long long)u + (long long)f.fd0[face] * v;
+    const double K = (double)bn.K[a];
+    // slots: (Ua,Ub) receive (M slot 0, slot 1) except on z faces where the
+    // reference orders y before x -- irrelevant for FP64 accumulation.
+    const int ua = a == 0 ? 1 : 0, ub = a == 2 ? 1 : 2;
+    const float dM0 = f.M[face][0][fo] - f.Mo[face][0][fo];
+    const float dM1 = f.M[face][1][fo] - f.Mo[face][1][fo];
+    const float dJ0 = f.J[face][0][fo] - f.Jo[face][0][fo];
+    const float dJ1 = f.J[face][1][fo] - f.Jo[face][1][fo];
+    float t = (float)n + 0.5f + tfv - 1.0f;
+    int ii = (int)t;
+    float w = t - (float)ii;
+    const float base = (1.0f - w) * bn.K[a], wk = w * bn.K[a];
+    (void)K;
+    double* Ua = bn.U[ua] + kk * bn.LL;
+    double* Ub = bn.U[ub] + kk * bn.LL;
+    atomicAdd(Ua + ii, (double)(base * dM0));
+    atomicAdd(Ua + ii + 1, (double)(wk * dM0));
+    atomicAdd(Ub + ii, (double)(base * dM1));
+    atomicAdd(Ub + ii + 1, (double)(wk * dM1));
+    t = (float)n + tfv - 1.0f;
+    ii = (int)t;
+    w = t - (float)ii;
+    const float base2 = (1.0f - w) * bn.K[a], wk2 = w * bn.K[a];
+    double* Wa = bn.W[ua] + kk * bn.LL;
+    double* Wb = bn.W[ub] + kk * bn.LL;
+    atomicAdd(Wa + ii, (double)(base2 * dJ0));
+    atomicAdd(Wa + ii + 1, (double)(wk2 * dJ0));
+    atomicAdd(Wb + ii, (double)(base2 * dJ1));
+    atomicAdd(Wb + ii + 1, (double)(wk2 * dJ1));
+}
+
+// Phase-centre samples (f90:8259-8290) and the step counter.
+__global__ void k_phase_centre(Grid g, int ic, int jc, int kc, int T, const float* Ex, const float* Ey,
+                               const float* Ez, const float* Hx, const float* Hy, const float* Hz, float* pc,
+                               const int* step) {
+    const int n = *step;
+    const long long o = off(g, ic, jc, kc);
+    float* P[6];
+    for (int q = 0; q < 6; ++q) P[q] = pc + (long long)q * (T + 1);
+    P[0][n - 1] = (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f;
+    P[1][n - 1] = (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f;
+    P[2][n - 1] = (Ez[o] + Ez[o + 1] + Ez[o + g.PX] + Ez[o + 1 + g.PX]) / 4.0f;
+    const float hx = (Hx[o] + Hx[o + 1]) / 4.0f, hy = (Hy[o] + Hy[o + g.PX]) / 4.0f,
+                hz = (Hz[o] + Hz[o + g.plane]) / 4.0f;
+    P[3][n - 1] = hx; P[4][n - 1] = hy; P[5][n - 1] = hz;
+    if (n < T) { P[3][n] = hx; P[4][n] = hy; P[5][n] = hz; }
+}
+
+__global__ void k_step_inc(int* step) { *step = *step + 1; }
+
+// ---------------------------------------------------------------------------
+// Post-loop products on the device
+// ---------------------------------------------------------------------------
+
+// Clear 1-D line of length 2M for the antenna incident wave (f90:8373-8407);
+// C_inc(0) (read one element before the array by the reference) is 0.
+__global__ void __launch_bounds__(1024) k_clear_line(int M, int T, float kI, float kV, float kS, float dl, Pulse pu,
+                                                     float* Vi, float* Ci, float* incident) {
+    const int M2 = 2 * M;
+    for (int n = 1; n <= T; ++n) {
+        for (int j = 1 + threadIdx.x; j <= M2 - 1; j += blockDim.x) Ci[j] = Ci[j] - kI * (Vi[j + 1] - Vi[j]);
+        __syncthreads();
+        if (threadIdx.x == 0) Ci[M] = Ci[M] - kI * e_or_v(pu, (float)n + 0.0f, 0.0f * dl);
+        __syncthreads();
+        for (int j = 1 + threadIdx.x; j <= M2 - 1; j += blockDim.x) Vi[j] = Vi[j] - kV * (Ci[j] - Ci[j - 1]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Vi[M] = Vi[M] + kS * e_or_v(pu, (float)n + 0.5f, 0.5f * dl);
+            incident[n - 1] = Vi[M];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_plane_incident(int T, Pulse pu, float dx, float* incident) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (n > T) return;
+    incident[n - 1] = e_or_v(pu, (float)n + 0.0f, 0.0f * dx);
+}
+
+// NTFF combine and output window (f90:8483-8534): one work-group per angle.
+__global__ void __launch_bounds__(256) k_ntff_combine(NtffBins bn, int L, int T, float eta, float dt, float* Eth,
+                                                      float* Eph, float* fftime, float* scratch) {
+    const int kk = blockIdx.x;
+    const float* an = bn.ang + 6 * kk;
+    const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
+    float* Et = scratch + (long long)kk * 2 * (L + 1);
+    float* Ep = Et + (L + 1);
+    __shared__ int first_t, first_p;
+    if (threadIdx.x == 0) { first_t = L; first_p = L; }
+    __syncthreads();
+    for (int j = 1 + threadIdx.x; j <= L; j += blockDim.x) {
+        const float wx = (float)bn.W[0][kk * bn.LL + j], wy = (float)bn.W[1][kk * bn.LL + j], wz = (float)bn.W[2][kk * bn.LL + j];
+        const float ux = (float)bn.U[0][kk * bn.LL + j], uy = (float)bn.U[1][kk * bn.LL + j], uz = (float)bn.U[2][kk * bn.LL + j];
+        const float wt = wx * c1 * c2 + wy * c1 * s2 - wz * s1;
+        const float wph = -1.0f * wx * s2 + wy * c2;
+        const float ut = ux * c1 * c2 + uy * c1 * s2 - uz * s1;
+        const float uph = -1.0f * ux * s2 + uy * c2;
+        const float et = -1.0f * eta * wt - uph;
+        const float ep = -1.0f * eta * wph + ut;
+        Et[j] = et;
+        Ep[j] = ep;
+        if (et != 0.0f) atomicMin(&first_t, j);
+        if (ep != 0.0f) atomicMin(&first_p, j);
+    }
+    __syncthreads();
+    int dot = min(first_p, first_t);
+    if (dot > L - T + 1) dot = L - T + 1;
+    for (int j = 1 + threadIdx.x; j <= T; j += blockDim.x) {
+        Eth[(long long)kk * T + j - 1] = Et[j + dot - 1];
+        Eph[(long long)kk * T + j - 1] = Ep[j + dot - 1];
+    }
+    if (threadIdx.x == 0) fftime[kk] = (bn.r_relay / 299792448.0f) - (float)(dot - 1) * dt;
+}
+
+// Periodic-slab normalisation (f90:8342-8361) to FP32.
+__global__ void k_plane_norm(int T, const double* refl, const double* trans, float den, float* rf, float* tf) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= T) return;
+    rf[n] = (float)refl[n] / den;
+    tf[n] = (float)trans[n] / den;
+}
+
+}  // namespace aemk

@@ -750,6 +750,9 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
             float vals[6] = {t, p, std::sin(t), std::cos(t), std::sin(p), std::cos(p)};
             P->ang.insert(P->ang.end(), vals, vals + 6);
         }
+        if (d.n_angles > 0 && d.pw_amp > 0 &&
+            (d.ic < 1 || d.ic > NX - 1 || d.jc < 1 || d.jc > NY - 1 || d.kc < 1 || d.kc > NZ - 1))
+            throw Error(1, "ERROR: far-field phase centre lies outside the grid");
         if (d.n_angles > 0)
             for (int a = 0; a < 3; ++a)
                 if (!d.pbc[a] && (v.ff_lo[a] < 1 || v.ff_hi[a] + 1 > N3[a] || v.ff_lo[a] > v.ff_hi[a]))

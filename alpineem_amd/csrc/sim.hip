@@ -1,0 +1,627 @@
+// sim.hip -- device state and step orchestration of libaem (include/aem.h).
+//
+// aem_sim_create uploads a plan (the reference's "!$acc data" region,
+// fdtd_solver.f90:4380-4430); aem_run enqueues the leapfrog step
+// (fdtd_solver.f90:4440-8299) on the sim's stream, optionally replaying one
+// captured hipGraph per step; aem_finalize runs the post-loop products
+// (fdtd_solver.f90:8340-8534) on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aem.h"
+#include "api_internal.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+using namespace aemk;
+
+namespace {
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw aem::Error(AEM_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+template <class T>
+T* dalloc(size_t n, std::vector<void*>& owned) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    owned.push_back(p);
+    return static_cast<T*>(p);
+}
+
+inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+struct aem_sim {
+    const aem::Plan* plan = nullptr;
+    aem::Plan* owned_plan = nullptr;
+    hipStream_t st = nullptr;
+    std::vector<void*> owned;
+    Grid g{};
+    int* d_step = nullptr;
+    int64_t steps_done = 0;
+    float* F[6] = {};
+    void* d_idx = nullptr;
+    float* d_tup = nullptr;
+    float* d_den[6] = {};
+    Cpml cpH{}, cpE{};
+    Sheets sh[3]{};
+    DrudeBox db[3]{};
+    PortDev* d_ports = nullptr;
+    float *d_V = nullptr, *d_I = nullptr, *d_Vout = nullptr, *d_Iout = nullptr;
+    NtffFaces nf{};
+    NtffBins nb{};
+    double *d_refl = nullptr, *d_trans = nullptr;
+    float* d_pc = nullptr;
+    float *d_incident = nullptr, *d_Eth = nullptr, *d_Eph = nullptr, *d_fftime = nullptr;
+    float *d_reflf = nullptr, *d_transf = nullptr, *d_ffpc = nullptr, *d_scratch = nullptr;
+    Tfsf tf{};
+    Pulse pu{};
+    int kchunk = 16;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    bool finalized = false;
+
+    ~aem_sim() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* p : owned) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+        delete owned_plan;
+    }
+
+    template <class T>
+    T* up(const T* host, size_t n) {
+        T* d = dalloc<T>(n, owned);
+        if (n && host) HIPCHK(hipMemcpyAsync(d, host, n * sizeof(T), hipMemcpyHostToDevice, st));
+        else HIPCHK(hipMemsetAsync(d, 0, (n ? n : 1) * sizeof(T), st));
+        return d;
+    }
+    template <class T>
+    T* zeros(size_t n) {
+        T* d = dalloc<T>(n, owned);
+        HIPCHK(hipMemsetAsync(d, 0, (n ? n : 1) * sizeof(T), st));
+        return d;
+    }
+
+    void create();
+    void enqueue_step(hipEvent_t* mark = nullptr);
+    template <class IdxT>
+    void enqueue_step_t(hipEvent_t* mark);
+};
+
+void aem_sim::create() {
+    const aem_plan_view& v = plan->v;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    g.NX = v.NX; g.NY = v.NY; g.NZ = v.NZ; g.PX = v.PX; g.plane = v.plane;
+    const size_t nc = (size_t)v.ncell;
+    for (int q = 0; q < 6; ++q) F[q] = zeros<float>(nc + v.plane);  // one spare plane keeps halo reads in bounds
+    d_step = zeros<int>(1);
+    {
+        int one = 1;
+        HIPCHK(hipMemcpyAsync(d_step, &one, sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    if (v.idx_bytes == 1) d_idx = up<uint8_t>(v.tuple_idx8, nc);
+    else d_idx = up<uint16_t>(v.tuple_idx16, nc);
+    d_tup = up<float>(v.tuples, (size_t)v.n_tuple * 6);
+    const int dl[6] = {v.NX - 1, v.NY - 1, v.NZ - 1, v.NX - 1, v.NY - 1, v.NZ - 1};
+    const float* dsrc[6] = {v.den_ex, v.den_ey, v.den_ez, v.den_hx, v.den_hy, v.den_hz};
+    for (int q = 0; q < 6; ++q) d_den[q] = up<float>(dsrc[q], std::max(dl[q], 1));
+
+    // CPML: coefficients and psi slabs
+    for (int f = 0; f < 6; ++f) {
+        const int n = v.npml[f];
+        const int a = f / 2;
+        cpH.n[f] = cpE.n[f] = n;
+        cpH.b[f] = up<float>(v.bh[f], std::max(n - 1, 0));
+        cpH.c[f] = up<float>(v.ch[f], std::max(n - 1, 0));
+        cpE.b[f] = up<float>(v.be[f], std::max(n, 0));
+        cpE.c[f] = up<float>(v.ce[f], std::max(n, 0));
+        const size_t nt = (size_t)std::max(n - 1, 0);
+        size_t trans;
+        if (a == 0) trans = (size_t)v.NY * v.NZ;
+        else if (a == 1) trans = (size_t)v.PX * v.NZ;
+        else trans = (size_t)v.PX * v.NY;
+        for (int s = 0; s < 2; ++s) {
+            cpH.psi[f][s] = zeros<float>(nt * trans);
+            cpE.psi[f][s] = zeros<float>(nt * trans);
+        }
+    }
+
+    // sheets
+    for (int a = 0; a < 3; ++a) {
+        Sheets& s = sh[a];
+        s.np = v.n_sheet_planes[a];
+        s.P = v.sheet_dim[a][0];
+        s.Q = v.sheet_dim[a][1];
+        const size_t n = (size_t)s.np * s.P * s.Q;
+        s.pos = up<int>(v.sheet_pos[a], s.np);
+        s.sig = up<float>(v.sheet_sig[a], n);
+        s.ep = up<float>(v.sheet_ep[a], n);
+        s.es = zeros<float>(n);
+    }
+
+    // Drude boxes
+    if (v.has_drude) {
+        for (int c = 0; c < 3; ++c) {
+            DrudeBox& b = db[c];
+            for (int q = 0; q < 3; ++q) { b.lo[q] = v.dbox[c][2 * q]; b.hi[q] = v.dbox[c][2 * q + 1]; }
+            long long bx = std::max(0, b.hi[0] - b.lo[0] + 1), by = std::max(0, b.hi[1] - b.lo[1] + 1),
+                      bz = std::max(0, b.hi[2] - b.lo[2] + 1);
+            b.nb = bx * by * bz;
+            b.Ja = up<float>(v.Ja[c], (size_t)b.nb * v.n_poles);
+            b.Jb = up<float>(v.Jb[c], (size_t)b.nb * v.n_poles);
+            b.J = zeros<float>((size_t)b.nb * v.n_poles);
+            b.Eold = zeros<float>((size_t)b.nb);
+        }
+    }
+
+    // ports
+    const int T = v.time_steps;
+    const int np = std::max(v.n_ports, 1);
+    std::vector<PortDev> pd(v.n_ports);
+    for (int q = 0; q < v.n_ports; ++q) {
+        const aem_port_view& p = v.ports[q];
+        PortDev& d = pd[q];
+        d.dir = p.dir;
+        for (int a = 0; a < 3; ++a) { d.loc[a] = p.loc[a]; d.size[a] = p.size[a]; }
+        d.kI = p.kI; d.kV = p.kV; d.kS = p.kS; d.dl = p.dl;
+        d.cur_num = p.cur_num; d.cur_den = p.cur_den; d.gap_div = p.gap_div;
+        d.excited = p.excited;
+    }
+    d_ports = up<PortDev>(pd.data(), pd.size());
+    d_V = zeros<float>((size_t)np * v.M);
+    d_I = zeros<float>((size_t)np * v.M);
+    d_Vout = zeros<float>((size_t)np * T);
+    d_Iout = zeros<float>((size_t)np * T);
+    pu.t_spread = v.t_spread; pu.spread = v.spread; pu.dt = v.dt; pu.inv_c = v.inv_c;
+    pu.amp = v.antenna_amp; pu.pulse_type = v.pulse_type;
+
+    // TF/SF
+    for (int a = 0; a < 3; ++a) {
+        tf.lo[a] = v.tf_lo[a]; tf.hi[a] = v.tf_hi[a]; tf.delay[a] = v.delay[a]; tf.pbc[a] = v.pbc[a];
+        tf.WE[a] = v.WE[a]; tf.WH[a] = v.WH[a]; tf.kH[a] = v.kHinc[a]; tf.kE[a] = v.kEinc[a];
+    }
+    tf.s_th = v.s_th; tf.c_th = v.c_th; tf.s_ph = v.s_ph; tf.c_ph = v.c_ph;
+    tf.dx = v.dx; tf.dy = v.dy; tf.dz = v.dz; tf.dt = v.dt; tf.inv_c = v.inv_c;
+    tf.spread = v.spread; tf.t_spread = v.t_spread; tf.pulse_type = v.pulse_type;
+
+    // NTFF
+    if (v.n_angles > 0) {
+        for (int a = 0; a < 3; ++a) { nf.lo[a] = v.ff_lo[a]; nf.hi[a] = v.ff_hi[a]; nf.pbc[a] = v.pbc[a]; }
+        for (int f = 0; f < 6; ++f) {
+            const int a = f / 2, b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+            nf.fd0[f] = v.ff_hi[b] + 2;
+            const size_t n = (size_t)(v.ff_hi[b] + 2) * (v.ff_hi[c] + 2);
+            for (int s = 0; s < 2; ++s) {
+                nf.M[f][s] = zeros<float>(n); nf.Mo[f][s] = zeros<float>(n);
+                nf.J[f][s] = zeros<float>(n); nf.Jo[f][s] = zeros<float>(n);
+            }
+        }
+        nb.ang = up<float>(v.ang, (size_t)v.n_angles * 6);
+        nb.n_angles = v.n_angles;
+        nb.ic = v.ic; nb.jc = v.jc; nb.kc = v.kc;
+        nb.r_relay = v.r_relay; nb.dx = v.dx; nb.dy = v.dy; nb.dz = v.dz;
+        nb.c_dt = aem::kC * v.dt;
+        for (int a = 0; a < 3; ++a) nb.K[a] = v.ff_K[a];
+        nb.LL = (long long)v.len_ff + 2;
+        for (int q = 0; q < 3; ++q) {
+            nb.U[q] = zeros<double>((size_t)v.n_angles * nb.LL);
+            nb.W[q] = zeros<double>((size_t)v.n_angles * nb.LL);
+        }
+        d_Eth = zeros<float>((size_t)v.n_angles * T);
+        d_Eph = zeros<float>((size_t)v.n_angles * T);
+        d_fftime = zeros<float>(v.n_angles);
+        d_scratch = zeros<float>((size_t)v.n_angles * 2 * (v.len_ff + 1));
+    }
+    d_refl = zeros<double>(T);
+    d_trans = zeros<double>(T);
+    d_reflf = zeros<float>(T);
+    d_transf = zeros<float>(T);
+    d_pc = zeros<float>((size_t)6 * (T + 1));
+    d_ffpc = zeros<float>(T);
+    d_incident = zeros<float>(T);
+
+    // z-chunk of the bulk kernels: enough work-groups to fill 256 CUs
+    const long long bxy = (long long)cdiv(v.NX - 1, 32) * cdiv(v.NY - 1, 8);
+    long long nch = std::max(1LL, std::min<long long>(v.NZ - 1, (4096 + bxy - 1) / bxy));
+    kchunk = (int)cdiv(v.NZ - 1, nch);
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+template <class IdxT>
+void aem_sim::enqueue_step_t(hipEvent_t* mark) {
+    const aem_plan_view& v = plan->v;
+    const IdxT* idx = static_cast<const IdxT*>(d_idx);
+    float *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
+    const dim3 bb(32, 8, 1);
+    const dim3 gb(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8), cdiv(v.NZ - 1, kchunk));
+    const int T = v.time_steps;
+
+    // ---- H phase ----
+    if (mark) HIPCHK(hipEventRecord(mark[0], st));
+    k_h_bulk<<<gb, bb, 0, st>>>(g, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], v.DB, kchunk);
+    if (mark) HIPCHK(hipEventRecord(mark[1], st));
+    const float dd[3] = {v.dx, v.dy, v.dz};
+    for (int a = 0; a < 3; ++a)
+        if (sh[a].np > 0) {
+            const float f = v.DB * (v.sheet_thickness / dd[a]);
+            const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
+            k_h_sheets<<<gs, bb, 0, st>>>(g, sh[a], a, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], f);
+        }
+    if (v.has_tfsf) {
+        const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
+        if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 0, Hx, Hy, Hz, d_step);
+        if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, st>>>(g, tf, 1, Hx, Hy, Hz, d_step);
+        if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 2, Hx, Hy, Hz, d_step);
+    }
+    if (v.n_ports > 0)
+        k_ports_h<<<1, 256, 0, st>>>(g, d_ports, v.n_ports, v.M, d_V, d_I, d_Iout, T, Hx, Hy, Hz, v.dx, v.dy, v.dz, pu, d_step);
+    {
+        const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
+        const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
+        const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
+        if (tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, st>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, v.dx);
+        if (ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, st>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, v.dy);
+        if (tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, st>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, v.dz);
+    }
+    if (v.pbc[0]) k_h_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 0, Hz, Hy);
+    if (v.pbc[1]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 1, Hz, Hx);
+    if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, st>>>(g, 2, Hy, Hx);
+
+    // ---- E phase ----
+    if (mark) HIPCHK(hipEventRecord(mark[2], st));
+    k_e_bulk<IdxT><<<gb, bb, 0, st>>>(g, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], d_den[1], d_den[2], kchunk);
+    if (mark) HIPCHK(hipEventRecord(mark[3], st));
+    for (int a = 0; a < 3; ++a)
+        if (sh[a].np > 0) {
+            const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
+            k_e_sheets<<<gs, bb, 0, st>>>(g, sh[a], a, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2], v.dt, aem::kEp0);
+        }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c)
+            if (db[c].nb > 0) k_drude<IdxT><<<cdiv(db[c].nb, 256), 256, 0, st>>>(g, db[c], c, v.n_poles, F[c], idx, d_tup);
+    if (v.has_tfsf) {
+        const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
+        if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 0, Ex, Ey, Ez, d_step);
+        if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, st>>>(g, tf, 1, Ex, Ey, Ez, d_step);
+        if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 2, Ex, Ey, Ez, d_step);
+    }
+    if (v.n_ports > 0)
+        k_ports_e<<<1, 256, 0, st>>>(g, d_ports, v.n_ports, v.M, d_V, d_I, d_Vout, T, Ex, Ey, Ez, pu, d_step);
+    {
+        const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
+        const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
+        const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
+        if (tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, st>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, v.dx);
+        if (ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, st>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, v.dy);
+        if (tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, st>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, v.dz);
+    }
+    if (v.pbc[0]) {
+        k_e_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 0, Ez, Ey);
+        if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[2].np), 256, 0, st>>>(sh[2], 0, v.NX - 1, v.NY - 1);
+        if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[1].np), 256, 0, st>>>(sh[1], 0, v.NX - 1, v.NZ - 1);
+    }
+    if (v.pbc[1]) {
+        k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 1, Ez, Ex);
+        if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[2].np), 256, 0, st>>>(sh[2], 1, v.NY - 1, v.NX - 1);
+        if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[0].np), 256, 0, st>>>(sh[0], 0, v.NY - 1, v.NZ - 1);
+    }
+    if (v.pbc[2]) {
+        k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, st>>>(g, 2, Ey, Ex);
+        if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[1].np), 256, 0, st>>>(sh[1], 1, v.NZ - 1, v.NX - 1);
+        if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[0].np), 256, 0, st>>>(sh[0], 1, v.NZ - 1, v.NY - 1);
+    }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c)
+            if (db[c].nb > 0) k_save_eold<<<cdiv(db[c].nb, 256), 256, 0, st>>>(g, db[c], F[c]);
+
+    // ---- per-step outputs ----
+    if (mark) HIPCHK(hipEventRecord(mark[4], st));
+    if (v.plane_sum_axis >= 0) {
+        const int ax = v.plane_sum_axis;
+        const int lo = v.tf_lo[ax], hi = v.tf_hi[ax];
+        if (ax == 0)
+            k_plane_sums<<<1, 256, 0, st>>>(g, 0, lo, hi, v.ff_lo[1], v.ff_hi[1], v.ff_lo[2], v.ff_hi[2], Ez, Ey, v.s_pol,
+                                            v.c_pol, v.c_ph, 1, d_refl, d_trans, d_step);
+        else if (ax == 1)
+            k_plane_sums<<<1, 256, 0, st>>>(g, 1, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[2], v.ff_hi[2], Ez, Ex, v.s_pol,
+                                            v.c_pol, v.s_ph, 0, d_refl, d_trans, d_step);
+        else
+            k_plane_sums<<<1, 256, 0, st>>>(g, 2, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[1], v.ff_hi[1], Ey, Ex, v.ps_c1,
+                                            v.ps_c2, 1.0f, 0, d_refl, d_trans, d_step);
+    }
+    if (v.n_angles > 0) {
+        int mu = 0, mv = 0;
+        for (int a = 0; a < 3; ++a) {
+            const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+            mu = std::max(mu, v.ff_hi[b] - v.ff_lo[b] + 1);
+            mv = std::max(mv, v.ff_hi[c] - v.ff_lo[c] + 1);
+        }
+        const dim3 gf(cdiv(mu, 32), cdiv(mv, 8), 6);
+        k_ntff_currents<<<gf, bb, 0, st>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
+        const dim3 gbins(cdiv(mu, 32), cdiv(mv, 8), 6 * v.n_angles);
+        k_ntff_bins<<<gbins, bb, 0, st>>>(nf, nb, d_step);
+    }
+    if (v.n_angles > 0 && v.plane_wave_amp > 0)
+        k_phase_centre<<<1, 1, 0, st>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
+    k_step_inc<<<1, 1, 0, st>>>(d_step);
+    if (mark) HIPCHK(hipEventRecord(mark[5], st));
+}
+
+void aem_sim::enqueue_step(hipEvent_t* mark) {
+    if (plan->v.idx_bytes == 1) enqueue_step_t<uint8_t>(mark);
+    else enqueue_step_t<uint16_t>(mark);
+}
+
+namespace aem {
+const Plan* sim_plan(const aem_sim* s) { return s ? s->plan : nullptr; }
+}  // namespace aem
+
+__global__ void k_ffpc(int T, const float* pc, float c2, float c1, float c3, float* out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= T) return;
+    out[n] = pc[n] * c2 + pc[(T + 1) + n] * c1 + pc[2 * (T + 1) + n] * c3;
+}
+
+extern "C" {
+
+aem_status aem_sim_create(const aem_plan* plan, aem_sim** out) {
+    if (!plan || !out) {
+        aem::set_error("aem_sim_create: null argument");
+        return AEM_ERR_STATE;
+    }
+    *out = nullptr;
+    aem_sim* s = new (std::nothrow) aem_sim();
+    if (!s) {
+        aem::set_error("host allocation failed");
+        return AEM_ERR_STATE;
+    }
+    s->plan = reinterpret_cast<const aem::Plan*>(plan);
+    aem_status stt = aem::guard([&]() { s->create(); });
+    if (stt != AEM_OK) {
+        delete s;
+        return stt;
+    }
+    *out = s;
+    return AEM_OK;
+}
+
+aem_status aem_create_from_inputs(const char* inputs_path, aem_sim** out) {
+    aem_plan* p = nullptr;
+    aem_status stt = aem_plan_from_inputs(inputs_path, &p);
+    if (stt != AEM_OK) return stt;
+    stt = aem_sim_create(p, out);
+    if (stt != AEM_OK) {
+        aem_plan_destroy(p);
+        return stt;
+    }
+    (*out)->owned_plan = reinterpret_cast<aem::Plan*>(p);
+    return AEM_OK;
+}
+
+void aem_destroy(aem_sim* sim) { delete sim; }
+
+aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
+    if (!s) {
+        aem::set_error("aem_run: null handle");
+        return AEM_ERR_STATE;
+    }
+    if (n_steps < 0 || s->steps_done + n_steps > s->plan->v.time_steps) {
+        aem::set_error("aem_run: step count exceeds the deck's time_steps");
+        return AEM_ERR_STATE;
+    }
+    if (s->finalized) {
+        aem::set_error("aem_run: simulation already finalized");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        if (n_steps == 0) return;
+        if (use_graph && !s->gexec) {
+            HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+            s->enqueue_step();
+            HIPCHK(hipStreamEndCapture(s->st, &s->graph));
+            HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipEventRecord(s->ev0, s->st));
+        for (int64_t q = 0; q < n_steps; ++q) {
+            if (use_graph) HIPCHK(hipGraphLaunch(s->gexec, s->st));
+            else s->enqueue_step();
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(s->ev1, s->st));
+        HIPCHK(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms / (float)n_steps;
+        s->steps_done += n_steps;
+    });
+}
+
+int64_t aem_steps_done(const aem_sim* s) { return s ? s->steps_done : -1; }
+
+float aem_last_run_ms(const aem_sim* s) { return s ? s->last_ms : 0.f; }
+
+aem_status aem_sync(aem_sim* s) {
+    if (!s) return AEM_ERR_STATE;
+    return aem::guard([&]() { HIPCHK(hipStreamSynchronize(s->st)); });
+}
+
+aem_status aem_finalize(aem_sim* s) {
+    if (!s) {
+        aem::set_error("aem_finalize: null handle");
+        return AEM_ERR_STATE;
+    }
+    if (s->finalized) return AEM_OK;
+    return aem::guard([&]() {
+        const aem_plan_view& v = s->plan->v;
+        const int T = v.time_steps;
+        if (v.plane_sum_axis >= 0) {
+            float den;
+            if (v.plane_sum_axis == 0) den = ((float)v.NY - 2.0f) * ((float)v.NZ - 2.0f);
+            else if (v.plane_sum_axis == 1) den = ((float)v.NX - 2.0f) * ((float)v.NZ - 2.0f);
+            else den = ((float)v.NY - 2.0f) * ((float)v.NX - 2.0f);
+            k_plane_norm<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->d_refl, s->d_trans, den, s->d_reflf, s->d_transf);
+        }
+        if (v.plane_wave_amp > 0) {
+            Pulse p = s->pu;
+            p.amp = v.plane_wave_amp;
+            k_plane_incident<<<cdiv(T, 256), 256, 0, s->st>>>(T, p, v.dx, s->d_incident);
+        }
+        if (v.antenna_amp > 0 && v.excitation_port > 0) {
+            const aem_port_view& pt = v.ports[v.excitation_port - 1];
+            float* Vi = s->zeros<float>((size_t)2 * v.M + 2);
+            float* Ci = s->zeros<float>((size_t)2 * v.M + 2);
+            k_clear_line<<<1, 1024, 0, s->st>>>(v.M, T, pt.kI, pt.kV, pt.kS, pt.dl, s->pu, Vi, Ci, s->d_incident);
+        }
+        if (v.n_angles > 0) {
+            const float eta = std::sqrt(aem::kMu0 / aem::kEp0);
+            k_ntff_combine<<<v.n_angles, 256, 0, s->st>>>(s->nb, v.len_ff, T, eta, v.dt, s->d_Eth, s->d_Eph, s->d_fftime,
+                                                          s->d_scratch);
+        }
+        if (v.n_angles > 0 && v.plane_wave_amp > 0)
+            k_ffpc<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->d_pc, v.ps_c2, v.ps_c1, v.s_pol * v.s_th, s->d_ffpc);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s->st));
+        s->finalized = true;
+    });
+}
+
+aem_status aem_get_series(aem_sim* s, int32_t kind, int32_t idx, float* out, int64_t n) {
+    if (!s || !out) {
+        aem::set_error("aem_get_series: null argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    const int T = v.time_steps;
+    if (n < 0 || n > T) {
+        aem::set_error("aem_get_series: length exceeds time_steps");
+        return AEM_ERR_STATE;
+    }
+    const float* src = nullptr;
+    switch (kind) {
+        case AEM_SERIES_PORT_V:
+        case AEM_SERIES_PORT_I:
+            if (idx < 0 || idx >= v.n_ports) { aem::set_error("port index out of range"); return AEM_ERR_STATE; }
+            src = (kind == AEM_SERIES_PORT_V ? s->d_Vout : s->d_Iout) + (size_t)idx * T;
+            break;
+        case AEM_SERIES_E_THETA:
+        case AEM_SERIES_E_PHI:
+            if (idx < 0 || idx >= v.n_angles) { aem::set_error("angle index out of range"); return AEM_ERR_STATE; }
+            if (!s->finalized) { aem::set_error("call aem_finalize first"); return AEM_ERR_STATE; }
+            src = (kind == AEM_SERIES_E_THETA ? s->d_Eth : s->d_Eph) + (size_t)idx * T;
+            break;
+        case AEM_SERIES_INCIDENT: src = s->d_incident; break;
+        case AEM_SERIES_REFLECTED: src = s->d_reflf; break;
+        case AEM_SERIES_TRANSMITTED: src = s->d_transf; break;
+        case AEM_SERIES_FF_PC: src = s->d_ffpc; break;
+        default: aem::set_error("unknown series kind"); return AEM_ERR_STATE;
+    }
+    if ((kind == AEM_SERIES_INCIDENT || kind == AEM_SERIES_REFLECTED || kind == AEM_SERIES_TRANSMITTED ||
+         kind == AEM_SERIES_FF_PC) && !s->finalized) {
+        aem::set_error("call aem_finalize first");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        HIPCHK(hipMemcpyAsync(out, src, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_get_field(aem_sim* s, int32_t comp, float* out, int64_t n) {
+    if (!s || !out || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
+        aem::set_error("aem_get_field: bad argument");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        HIPCHK(hipMemcpyAsync(out, s->F[comp], (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_set_field(aem_sim* s, int32_t comp, const float* in, int64_t n) {
+    if (!s || !in || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
+        aem::set_error("aem_set_field: bad argument");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        HIPCHK(hipMemcpyAsync(s->F[comp], in, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_get_ntff_raw(aem_sim* s, int32_t which, double* out, int64_t n) {
+    if (!s || !out || which < 0 || which > 5) {
+        aem::set_error("aem_get_ntff_raw: bad argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (v.n_angles <= 0 || n > (int64_t)v.n_angles * v.len_ff) {
+        aem::set_error("aem_get_ntff_raw: no far field or length too large");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        const double* src = which < 3 ? s->nb.U[which] : s->nb.W[which - 3];
+        std::vector<double> all((size_t)v.n_angles * s->nb.LL);
+        HIPCHK(hipMemcpyAsync(all.data(), src, all.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        for (int64_t q = 0; q < n; ++q) {
+            int64_t a = q / v.len_ff, b = q % v.len_ff;
+            out[q] = all[(size_t)(a * s->nb.LL + b + 1)];
+        }
+    });
+}
+
+aem_status aem_get_ff_time(aem_sim* s, float* out, int64_t n) {
+    if (!s || !out || n > s->plan->v.n_angles || !s->finalized) {
+        aem::set_error("aem_get_ff_time: bad argument or not finalized");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        HIPCHK(hipMemcpyAsync(out, s->d_fftime, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_time_phases(aem_sim* s, int64_t n_steps, float* ms, int32_t n_ms) {
+    if (!s || !ms || n_ms < 5 || n_steps <= 0 || s->steps_done + n_steps > s->plan->v.time_steps || s->finalized) {
+        aem::set_error("aem_time_phases: bad argument or not enough steps left");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        hipEvent_t ev[6];
+        for (int q = 0; q < 6; ++q) HIPCHK(hipEventCreate(&ev[q]));
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int64_t n = 0; n < n_steps; ++n) {
+            s->enqueue_step(ev);
+            HIPCHK(hipEventSynchronize(ev[5]));
+            for (int q = 0; q < 5; ++q) {
+                float t = 0.f;
+                HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+                acc[q] += t;
+            }
+        }
+        for (int q = 0; q < 6; ++q) (void)hipEventDestroy(ev[q]);
+        for (int q = 0; q < 5; ++q) ms[q] = (float)(acc[q] / (double)n_steps);
+        s->steps_done += n_steps;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+"""Host-side API over libaem: the drop-in for the reference's solver process.
+
+The reference runs ``./FDTD inputs.txt`` as a child process
+(``main_fdtd/master.py:438-441``) and reads results back from ``data.dat``.
+:class:`Plan` and :class:`Simulation` expose the same steps in-process:
+
+>>> plan = Plan.from_inputs("inputs.txt")        # parse + setup (no GPU needed)
+>>> sim = Simulation(plan)                        # device allocation
+>>> sim.run()                                     # all time steps on the GPU
+>>> sim.finalize(); sim.write_outputs()           # data.dat + _geometry.bin
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import FIELDS, SERIES, check, lib
+
+
+class Plan:
+    """Parsed deck and setup products (aem_plan_* in include/aem.h)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        self.view = _lib.PlanView.from_address(lib().aem_plan_view_get(handle))
+
+    @classmethod
+    def from_inputs(cls, path: str) -> "Plan":
+        h = C.c_void_p()
+        check(lib().aem_plan_from_inputs(os.fspath(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_text(cls, text: str, base_dir: str = ".") -> "Plan":
+        h = C.c_void_p()
+        check(lib().aem_plan_from_text(text.encode(), base_dir.encode(), C.byref(h)))
+        return cls(h)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def view_ptr(self) -> int:
+        return lib().aem_plan_view_get(self._h)
+
+    @property
+    def shape(self):
+        """(NZ, NY, PX) of every field array (x fastest)."""
+        v = self.view
+        return (v.NZ, v.NY, v.PX)
+
+    @property
+    def time_steps(self) -> int:
+        return self.view.time_steps
+
+    def write_geometry(self, path: str) -> None:
+        check(lib().aem_plan_write_geometry(self._h, os.fspath(path).encode()))
+
+    def close(self) -> None:
+        if self._h:
+            lib().aem_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """One simulation resident on the current HIP device."""
+
+    def __init__(self, plan: Plan):
+        self.plan = plan
+        h = C.c_void_p()
+        check(lib().aem_sim_create(plan.handle, C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_inputs(cls, path: str) -> "Simulation":
+        return cls(Plan.from_inputs(path))
+
+    # -- stepping --------------------------------------------------------------
+    def run(self, steps: Optional[int] = None, graph: bool = True) -> float:
+        """Advance ``steps`` (default: all remaining) leapfrog steps; returns
+        the device time per step in ms."""
+        if steps is None:
+            steps = self.plan.time_steps - self.steps_done
+        check(lib().aem_run(self._h, int(steps), 1 if graph else 0))
+        return float(lib().aem_last_run_ms(self._h))
+
+    def time_phases(self, steps: int) -> np.ndarray:
+        """Advance ``steps`` eagerly with HIP events between the step phases;
+        returns ms per step for [H bulk, H side, E bulk, E side, outputs]."""
+        ms = (C.c_float * 5)()
+        check(lib().aem_time_phases(self._h, int(steps), ms, 5))
+        return np.array(list(ms), np.float64)
+
+    @property
+    def steps_done(self) -> int:
+        return int(lib().aem_steps_done(self._h))
+
+    def finalize(self) -> None:
+        check(lib().aem_finalize(self._h))
+
+    # -- outputs -----------------------------------------------------------------
+    def series(self, kind: str, idx: int = 0, n: Optional[int] = None) -> np.ndarray:
+        n = self.plan.time_steps if n is None else n
+        out = np.zeros(n, np.float32)
+        check(lib().aem_get_series(self._h, SERIES[kind], idx, out.ctypes.data, n))
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        v = self.plan.view
+        out = np.zeros(v.ncell, np.float32)
+        check(lib().aem_get_field(self._h, FIELDS[name], out.ctypes.data, v.ncell))
+        return out.reshape(self.plan.shape)
+
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        v = self.plan.view
+        a = np.ascontiguousarray(values, np.float32).reshape(-1)
+        if a.size != v.ncell:
+            raise ValueError(f"expected {v.ncell} values, got {a.size}")
+        check(lib().aem_set_field(self._h, FIELDS[name], a.ctypes.data, a.size))
+
+    def ntff_raw(self, which: int) -> np.ndarray:
+        v = self.plan.view
+        n_ang = self._n_angles()
+        L = self._len_ff()
+        out = np.zeros(n_ang * L, np.float64)
+        check(lib().aem_get_ntff_raw(self._h, which, out.ctypes.data, out.size))
+        return out.reshape(n_ang, L)
+
+    def ff_time(self) -> np.ndarray:
+        n = self._n_angles()
+        out = np.zeros(n, np.float32)
+        check(lib().aem_get_ff_time(self._h, out.ctypes.data, n))
+        return out
+
+    def write_outputs(self, path: Optional[str] = None) -> None:
+        check(lib().aem_write_outputs(self._h, None if path is None else os.fspath(path).encode()))
+
+    def close(self) -> None:
+        if self._h:
+            lib().aem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _n_angles(self) -> int:
+        return int(self.plan.view.n_angles)
+
+    def _len_ff(self) -> int:
+        return int(self.plan.view.len_ff)

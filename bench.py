@@ -1,0 +1,150 @@
+"""Throughput benchmark of the FDTD leapfrog loop on one MI355X.
+
+Workload (the reference metric's configuration): a 400^3 FP32 grid, dx = 1 mm,
+dt = 0.99 x the 3-D Courant limit, 10-cell CPML on all faces, vacuum, and a
+z-directed 50 ohm lumped port at the centre driven by a Gaussian of spread
+40 dt and t_spread 160 dt.  A "step" is one full leapfrog step (H phase,
+E phase, CPML, port, per-step outputs).  Inputs are resident in HBM before
+the timed region; W warm-up steps are run first, then exactly K steps are
+timed (wall clock around a device-synchronised replay of K captured step
+graphs).  value = 400^3 x K / seconds, in Gcell-updates/s.
+
+The JSON line also carries
+  roofline      the dominant kernel's algorithmic HBM bytes per launch over its
+                average duration, timed with HIP events on the sim's stream;
+  cpu_baseline  the reference solver (oracle/_ref/fdtd_ref_omp, built from
+                the reference source by oracle/Makefile) on the same deck for a
+                bounded number of steps, read from its own Mcells/s line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+BYTES_H = 36          # read Hx,Hy,Hz,Ex,Ey,Ez + write Hx,Hy,Hz (FP32)
+BYTES_E = 37          # read Ex,Ey,Ez,Hx,Hy,Hz + 1-byte coefficient index, write Ex,Ey,Ez
+BYTES_STEP = BYTES_H + BYTES_E
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    """Time the reference OpenMP build (or, if it is absent, the single-thread
+    C oracle) on a bounded sample of the same workload."""
+    from alpineem_amd import decks
+    ref = os.path.join(ROOT, "oracle", "_ref", "fdtd_ref_omp")
+    cores = min(16, os.cpu_count() or 1)
+    deck = decks.roofline_deck(n, steps).text()
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as tmp:
+            open(os.path.join(tmp, "inputs.txt"), "w").write(deck)
+            env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+            r = subprocess.run([ref, "inputs.txt"], cwd=tmp, env=env, capture_output=True, text=True, timeout=600)
+            m = re.search(r"Average of\s+([0-9.Ee+-]+)\s+Mcells per second", r.stdout)
+            if r.returncode == 0 and m:
+                return {"value": round(float(m.group(1)) / 1000.0, 5), "unit": "Gcell/s", "cores": cores,
+                        "kind": "reference",
+                        "sample": f"reference OpenMP build, {n}^3 deck, {steps} steps, its own Mcells/s meter"}
+    # fallback: the C restatement, one thread, a few steps
+    from alpineem_amd.solver import Plan
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from tests.oracle_ffi import Oracle
+    plan = Plan.from_text(deck)
+    orc = Oracle(plan)
+    t0 = time.perf_counter()
+    orc.step(steps)
+    el = time.perf_counter() - t0
+    cells = n ** 3
+    return {"value": round(cells * steps / el / 1e9, 5), "unit": "Gcell/s", "cores": 1, "kind": "port",
+            "sample": f"C oracle, {n}^3 deck, {steps} steps, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=400)
+    ap.add_argument("--phase-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(z-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from alpineem_amd import decks
+    from alpineem_amd.solver import Plan, Simulation
+
+    n, K, W, P = args.n, args.steps, args.warmup, args.phase_steps
+    deck = decks.roofline_deck(n, W + K + P)
+    t0 = time.perf_counter()
+    plan = Plan.from_text(deck.text())
+    sim = Simulation(plan)
+    setup_s = time.perf_counter() - t0
+    print(f"[bench] setup {setup_s:.1f}s, grid {n}^3, tuples {plan.view.n_tuple}", file=sys.stderr, flush=True)
+
+    sim.run(W)                                    # warm-up (also captures the step graph)
+    t0 = time.perf_counter()
+    sim.run(K)                                    # returns after the device is synchronised
+    wall = time.perf_counter() - t0
+    cells = n ** 3
+    value = cells * K / wall / 1e9
+    ms_step = wall / K * 1e3
+    print(f"[bench] timed {K} steps: {ms_step:.4f} ms/step, {value:.2f} Gcell/s", file=sys.stderr, flush=True)
+
+    ph = sim.time_phases(P)                       # ms per step per phase, HIP events on the sim stream
+    names = ["h_bulk", "h_side", "e_bulk", "e_side", "outputs"]
+    print("[bench] phases (ms): " + ", ".join(f"{a}={b:.4f}" for a, b in zip(names, ph)), file=sys.stderr, flush=True)
+    interior = (n) ** 3
+    if ph[2] >= ph[0]:
+        dom, bpc, t_ms = "k_e_bulk", BYTES_E, ph[2]
+    else:
+        dom, bpc, t_ms = "k_h_bulk", BYTES_H, ph[0]
+    achieved = bpc * interior / (t_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_cell": bpc, "step_frac": round(value * BYTES_STEP / PEAK_HBM_GBS, 4)}
+    traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            roofline["traffic"] = json.load(open(traffic_file)).get(dom)
+        except Exception:
+            pass
+
+    cpu = None if args.no_cpu else cpu_baseline(n, args.cpu_steps)
+    line = {
+        "metric": "Gcell-updates/s at 400^3 FP32 (CPML, lumped port)",
+        "value": round(value, 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 vacuum, dx=1mm, dt=0.99 CFL, 10-cell CPML, z-directed 50 ohm lumped port, "
+                               f"Gaussian spread 40dt / t_spread 160dt", "cells": cells,
+                   "phase_ms": {a: round(float(b), 5) for a, b in zip(names, ph)}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

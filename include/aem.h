@@ -147,6 +147,13 @@ aem_status aem_write_outputs(aem_sim* sim, const char* path);
  * recorded on the sim's stream around the timed steps. */
 float aem_last_run_ms(const aem_sim* sim);
 
+/* Instrumentation with no counterpart in the reference (its only meter is
+ * the Mcells/s line, fdtd_solver.f90:8315): advance n_steps eagerly with HIP
+ * events on the sim's stream between the five phases of a step and return
+ * the average ms per phase in ms[0..4]: H bulk, H side kernels (sheets,
+ * TF/SF, ports, CPML, PBC), E bulk, E side kernels, per-step outputs. */
+aem_status aem_time_phases(aem_sim* sim, int64_t n_steps, float* ms, int32_t n_ms);
+
 #ifdef __cplusplus
 }
 #endif
