@@ -124,4 +124,5 @@ class PlanView(C.Structure):
                 ("plane_sum_axis", C.c_int32), ("ps_c1", C.c_float), ("ps_c2", C.c_float),
                 ("excitation", C.c_int32), ("min_steps", C.c_float),
                 ("video_on", C.c_int32), ("slice", C.c_int32), ("slice_location", C.c_int32),
-                ("vid_size1", C.c_int32), ("vid_size2", C.c_int32)]
+                ("vid_size1", C.c_int32), ("vid_size2", C.c_int32),
+                ("src_table", _FP), ("exp_half", C.c_float)]

@@ -305,16 +305,21 @@ struct Tfsf {
     float s_th, c_th, s_ph, c_ph;
     float kH[3], kE[3];
     float dx, dy, dz, dt, inv_c, spread, t_spread;
+    float exp_half;            // expf(0.5) as evaluated on the host
     int pulse_type;
 };
+
+// FP32 exp rounded from an FP64 evaluation: the correctly rounded value
+// except for arguments within ~1e-9 ulp of a rounding midpoint.
+__device__ __forceinline__ float exp_rn(float x) { return (float)exp((double)x); }
 
 __device__ __forceinline__ float inc_fn(const Tfsf& p, float w, float xf, float yf, float zf, float cf) {
     const float tx = p.s_th * p.c_ph * (xf - (float)p.delay[0]) * p.dx;
     const float ty = p.s_th * p.s_ph * (yf - (float)p.delay[1]) * p.dy;
     const float tz = p.c_th * (zf - (float)p.delay[2]) * p.dz;
     const float arg = (p.t_spread - (cf * p.dt) + p.inv_c * (tx + ty + tz)) / p.spread;
-    if (p.pulse_type == 1) return w * expf(-0.5f * (arg * arg));
-    return -1.0f * w * arg * expf(0.5f) * expf(-0.5f * (arg * arg));
+    if (p.pulse_type == 1) return w * exp_rn(-0.5f * (arg * arg));
+    return -1.0f * w * arg * p.exp_half * exp_rn(-0.5f * (arg * arg));
 }
 
 // face: 0 = y faces, 1 = z faces, 2 = x faces.  (u,v) spans the face.
@@ -408,17 +413,13 @@ struct PortDev {
     int excited;
 };
 
+// Time-only source waveforms E_or_V_1D() (f90:8844-8857), evaluated once per
+// step on the host: src[n-1] current source, src[T+n-1] voltage source,
+// src[2T+n-1] plane-wave incident field.
 struct Pulse {
-    float t_spread, spread, dt, inv_c, amp;
-    int pulse_type;
+    const float* src;
+    int T;
 };
-
-// E_or_V_1D() of f90:8844-8857
-__device__ __forceinline__ float e_or_v(const Pulse& p, float tc, float pos) {
-    const float x = ((p.t_spread - ((tc - 1.0f) * p.dt)) - p.inv_c * pos) / p.spread;
-    if (p.pulse_type == 1) return p.amp * expf(-0.5f * (x * x));
-    return p.amp * -1.0f * x * expf(0.5f) * expf(-0.5f * (x * x));
-}
 
 __global__ void __launch_bounds__(256) k_ports_h(Grid g, const PortDev* ports, int np, int M, float* V, float* I,
                                                  float* Iout, int T, const float* Hx, const float* Hy,
@@ -464,7 +465,7 @@ __global__ void __launch_bounds__(256) k_ports_h(Grid g, const PortDev* ports, i
         for (int q = 0; q < np; ++q)
             if (ports[q].excited) {
                 float* Iq = I + (long long)q * M;
-                Iq[1] = Iq[1] - ports[q].kI * e_or_v(pu, (float)n + 0.0f, 0.0f * ports[q].dl);
+                Iq[1] = Iq[1] - ports[q].kI * pu.src[n - 1];
             }
     }
 }
@@ -484,7 +485,7 @@ __global__ void __launch_bounds__(256) k_ports_e(Grid g, const PortDev* ports, i
         for (int q = 0; q < np; ++q)
             if (ports[q].excited) {
                 float* Vq = V + (long long)q * M;
-                Vq[1] = Vq[1] + ports[q].kS * e_or_v(pu, (float)n + 0.5f, 0.5f * ports[q].dl);
+                Vq[1] = Vq[1] + ports[q].kS * pu.src[pu.T + n - 1];
             }
         for (int q = 0; q < np; ++q) {
             const PortDev pt = ports[q];
@@ -859,28 +860,28 @@ __global__ void k_step_inc(int* step) { *step = *step + 1; }
 
 // Clear 1-D line of length 2M for the antenna incident wave (f90:8373-8407);
 // C_inc(0) (read one element before the array by the reference) is 0.
-__global__ void __launch_bounds__(1024) k_clear_line(int M, int T, float kI, float kV, float kS, float dl, Pulse pu,
-                                                     float* Vi, float* Ci, float* incident) {
+__global__ void __launch_bounds__(1024) k_clear_line(int M, int T, float kI, float kV, float kS, Pulse pu, float* Vi,
+                                                     float* Ci, float* incident) {
     const int M2 = 2 * M;
     for (int n = 1; n <= T; ++n) {
         for (int j = 1 + threadIdx.x; j <= M2 - 1; j += blockDim.x) Ci[j] = Ci[j] - kI * (Vi[j + 1] - Vi[j]);
         __syncthreads();
-        if (threadIdx.x == 0) Ci[M] = Ci[M] - kI * e_or_v(pu, (float)n + 0.0f, 0.0f * dl);
+        if (threadIdx.x == 0) Ci[M] = Ci[M] - kI * pu.src[n - 1];
         __syncthreads();
         for (int j = 1 + threadIdx.x; j <= M2 - 1; j += blockDim.x) Vi[j] = Vi[j] - kV * (Ci[j] - Ci[j - 1]);
         __syncthreads();
         if (threadIdx.x == 0) {
-            Vi[M] = Vi[M] + kS * e_or_v(pu, (float)n + 0.5f, 0.5f * dl);
+            Vi[M] = Vi[M] + kS * pu.src[T + n - 1];
             incident[n - 1] = Vi[M];
         }
         __syncthreads();
     }
 }
 
-__global__ void k_plane_incident(int T, Pulse pu, float dx, float* incident) {
+__global__ void k_plane_incident(int T, Pulse pu, float* incident) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (n > T) return;
-    incident[n - 1] = e_or_v(pu, (float)n + 0.0f, 0.0f * dx);
+    incident[n - 1] = pu.src[2 * T + n - 1];
 }
 
 // NTFF combine and output window (f90:8483-8534): one work-group per angle.

@@ -118,6 +118,7 @@ struct Plan {
     std::vector<float> Ja[3], Jb[3];
     std::vector<aem_port_view> ports;
     std::vector<float> ang;
+    std::vector<float> src;                       // [3][time_steps] source waveforms
 
     // rasterised geometry kept for the _geometry.bin writer (f90:2355-2357)
     std::vector<float> geom;                      // (4, nx, ny, nz) Fortran order

@@ -766,6 +766,28 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         v.ps_c1 = -1.0f * std::cos(pol) * std::cos(phi) - std::sin(pol) * std::cos(theta) * std::sin(phi);
         v.ps_c2 = std::cos(pol) * std::sin(phi) - std::sin(pol) * std::cos(theta) * std::cos(phi);
 
+        // ---- time-only source waveforms (f90:5001, 6162, 8368) -------------
+        // Evaluated here once so that the device uses the same FP32 values
+        // (and the same libm exp) as the reference's serial loop.
+        {
+            const int T = d.time_steps;
+            P->src.assign((size_t)3 * T, 0.0f);
+            if (d.exc_port > 0 && d.ant_amp > 0) {
+                const float dl = P->ports[d.exc_port - 1].dl;
+                for (int n = 1; n <= T; ++n) {
+                    P->src[n - 1] = e_or_v_1d(d.ant_amp, (float)n + 0.0f, 0.0f * dl, v.spread, v.t_spread,
+                                              d.pulse_type, dt);
+                    P->src[(size_t)T + n - 1] = e_or_v_1d(d.ant_amp, (float)n + 0.5f, 0.5f * dl, v.spread,
+                                                          v.t_spread, d.pulse_type, dt);
+                }
+            }
+            if (d.pw_amp > 0)
+                for (int n = 1; n <= T; ++n)
+                    P->src[(size_t)2 * T + n - 1] = e_or_v_1d(d.pw_amp, (float)n + 0.0f, 0.0f * dx, v.spread,
+                                                              v.t_spread, d.pulse_type, dt);
+            v.exp_half = std::exp(0.5f);
+        }
+
         // ---- wire up the view pointers ---------------------------------------
         v.den_ex = P->den[0].data(); v.den_ey = P->den[1].data(); v.den_ez = P->den[2].data();
         v.den_hx = P->den[3].data(); v.den_hy = P->den[4].data(); v.den_hz = P->den[5].data();
@@ -786,11 +808,19 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         }
         v.ports = P->ports.empty() ? nullptr : P->ports.data();
         v.ang = P->ang.empty() ? nullptr : P->ang.data();
+        v.src_table = P->src.data();
     } catch (...) {
         delete P;
         throw;
     }
     return P;
+}
+
+// E_or_V_1D of f90:8844-8857 (FP32, left-to-right).
+float e_or_v_1d(float amp, float tc, float pos, float spread, float t_spread, int ptype, float dt) {
+    const float x = ((t_spread - ((tc - 1.0f) * dt)) - 1.0f / kC * pos) / spread;
+    if (ptype == 1) return amp * std::exp(-0.5f * (x * x));
+    return amp * -1.0f * x * std::exp(0.5f) * std::exp(-0.5f * (x * x));
 }
 
 void write_geometry(const Plan& p, const std::string& path) {

@@ -191,8 +191,8 @@ void aem_sim::create() {
     d_I = zeros<float>((size_t)np * v.M);
     d_Vout = zeros<float>((size_t)np * T);
     d_Iout = zeros<float>((size_t)np * T);
-    pu.t_spread = v.t_spread; pu.spread = v.spread; pu.dt = v.dt; pu.inv_c = v.inv_c;
-    pu.amp = v.antenna_amp; pu.pulse_type = v.pulse_type;
+    pu.src = up<float>(v.src_table, (size_t)3 * T);
+    pu.T = T;
 
     // TF/SF
     for (int a = 0; a < 3; ++a) {
@@ -201,7 +201,7 @@ void aem_sim::create() {
     }
     tf.s_th = v.s_th; tf.c_th = v.c_th; tf.s_ph = v.s_ph; tf.c_ph = v.c_ph;
     tf.dx = v.dx; tf.dy = v.dy; tf.dz = v.dz; tf.dt = v.dt; tf.inv_c = v.inv_c;
-    tf.spread = v.spread; tf.t_spread = v.t_spread; tf.pulse_type = v.pulse_type;
+    tf.spread = v.spread; tf.t_spread = v.t_spread; tf.pulse_type = v.pulse_type; tf.exp_half = v.exp_half;
 
     // NTFF
     if (v.n_angles > 0) {
@@ -480,16 +480,13 @@ aem_status aem_finalize(aem_sim* s) {
             else den = ((float)v.NY - 2.0f) * ((float)v.NX - 2.0f);
             k_plane_norm<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->d_refl, s->d_trans, den, s->d_reflf, s->d_transf);
         }
-        if (v.plane_wave_amp > 0) {
-            Pulse p = s->pu;
-            p.amp = v.plane_wave_amp;
-            k_plane_incident<<<cdiv(T, 256), 256, 0, s->st>>>(T, p, v.dx, s->d_incident);
-        }
+        if (v.plane_wave_amp > 0)
+            k_plane_incident<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->pu, s->d_incident);
         if (v.antenna_amp > 0 && v.excitation_port > 0) {
             const aem_port_view& pt = v.ports[v.excitation_port - 1];
             float* Vi = s->zeros<float>((size_t)2 * v.M + 2);
             float* Ci = s->zeros<float>((size_t)2 * v.M + 2);
-            k_clear_line<<<1, 1024, 0, s->st>>>(v.M, T, pt.kI, pt.kV, pt.kS, pt.dl, s->pu, Vi, Ci, s->d_incident);
+            k_clear_line<<<1, 1024, 0, s->st>>>(v.M, T, pt.kI, pt.kV, pt.kS, s->pu, Vi, Ci, s->d_incident);
         }
         if (v.n_angles > 0) {
             const float eta = std::sqrt(aem::kMu0 / aem::kEp0);

@@ -147,6 +147,14 @@ typedef struct aem_plan_view {
     int32_t excitation;           /* 0 none, 1 plane wave, 2 antenna */
     float   min_steps;            /* f90:1811 */
     int32_t video_on, slice, slice_location, vid_size1, vid_size2;
+
+    /* ---- time-only source waveforms, evaluated once on the host --------
+     * src_table[0*T + n-1] = E_or_V_1D(antenna_amp, n+0.0, 0)      current source (f90:5001)
+     * src_table[1*T + n-1] = E_or_V_1D(antenna_amp, n+0.5, dl/2)   voltage source (f90:6162)
+     * src_table[2*T + n-1] = E_or_V_1D(plane_wave_amp, n+0.0, 0)   incident E (f90:8368)
+     * (dl of the excited port; zeros when the source is absent). */
+    const float *src_table;
+    float   exp_half;             /* expf(0.5) as the host evaluates it (pulse type 2) */
 } aem_plan_view;
 
 #ifdef __cplusplus

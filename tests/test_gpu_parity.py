@@ -51,9 +51,13 @@ def test_series_match_reference(runs, name):
     if "port_v" in g:
         checks += [(f"port_v[{q}]", sim.series("port_v", q), g["port_v"][q]) for q in range(v.n_ports)]
     if "e_theta" in g:
+        # one far-field vector (E_theta, E_phi) per angle: the cross-polar
+        # component alone is numerical noise (|E_phi| ~ 1e-7 |E_theta| for a
+        # z-dipole at phi=0) and has no meaningful relative error of its own
         for a in range(v.n_angles):
-            checks.append((f"e_theta[{a}]", sim.series("e_theta", a), g["e_theta"][a]))
-            checks.append((f"e_phi[{a}]", sim.series("e_phi", a), g["e_phi"][a]))
+            got = np.concatenate([sim.series("e_theta", a), sim.series("e_phi", a)])
+            ref = np.concatenate([g["e_theta"][a], g["e_phi"][a]])
+            checks.append((f"far[{a}]", got, ref))
     if "reflected" in g:
         checks.append(("reflected", sim.series("reflected"), g["reflected"]))
         checks.append(("transmitted", sim.series("transmitted"), g["transmitted"]))
