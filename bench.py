@@ -68,6 +68,23 @@ def cpu_baseline(n: int, steps: int) -> dict:
             "sample": f"C oracle, {n}^3 deck, {steps} steps, 1 thread"}
 
 
+def copy_probe(mib: int = 256, reps: int = 3) -> None:
+    """Device-to-device copies of a known size (``mib`` MiB), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0        # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy probe: {reps} x {mib} MiB device-to-device", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +94,7 @@ def main() -> int:
     ap.add_argument("--phase-steps", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="also run a known-size device copy (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -122,6 +140,8 @@ def main() -> int:
         except Exception:
             pass
 
+    if args.copy_probe:
+        copy_probe()
     cpu = None if args.no_cpu else cpu_baseline(n, args.cpu_steps)
     line = {
         "metric": "Gcell-updates/s at 400^3 FP32 (CPML, lumped port)",
