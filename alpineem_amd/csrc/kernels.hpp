@@ -29,14 +29,45 @@ __device__ __forceinline__ long long off(const Grid& g, int i, int j, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// Bulk H (f90:4442-4484).  One thread per (i,j) column marching a chunk of
-// k; Ex(k+1), Ey(k+1) are carried in registers to the next plane.
+// CPML state (H: f90:5227-5375, E: f90:6522-6670).  psi of face f / slot s is
+// stored restricted to its slab, with the slab coordinate t replacing the
+// normal index:  x faces [k][j][t],  y faces [k][t][i],  z faces [t][j][i].
 // ---------------------------------------------------------------------------
+struct Cpml {
+    int n[6];                  // npml per face
+    const float* b[6];         // bh (H) or be (E)
+    const float* c[6];         // ch or ce
+    float* psi[6][2];          // per face, two components
+};
+
+// Slab membership along one axis for H (layers 1..n1-1 and N+1-n2..N-1,
+// local index q = i or N-i) and for E (layers 2..n1 and N+1-n2..N-1, q = i or
+// N+1-i).  Returns the face (2a or 2a+1) or -1; q is the 1-based coefficient
+// index, t the 0-based slab storage index.
+__device__ __forceinline__ int slab_h(int i, int N, int n1, int n2, int& q) {
+    if (i <= n1 - 1) { q = i; return 0; }
+    if (i >= N + 1 - n2) { q = N - i; return 1; }
+    return -1;
+}
+__device__ __forceinline__ int slab_e(int i, int N, int n1, int n2, int& q, int& t) {
+    if (i >= 2 && i <= n1) { q = i; t = i - 2; return 0; }
+    if (i >= N + 1 - n2 && i <= N - 1) { q = N + 1 - i; t = q - 2; return 1; }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Bulk H (f90:4442-4484).  One thread per (i,j) column marching a chunk of
+// k; Ex(k+1), Ey(k+1) are carried in registers to the next plane.  With
+// FUSE the CPML corrections of the cell follow in the reference's per-cell
+// order (x, then y, then z slab), reusing the loaded neighbours.
+// ---------------------------------------------------------------------------
+template <bool FUSE>
 __global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, float* __restrict__ Hy,
                                                 float* __restrict__ Hz, const float* __restrict__ Ex,
                                                 const float* __restrict__ Ey, const float* __restrict__ Ez,
                                                 const float* __restrict__ dhx, const float* __restrict__ dhy,
-                                                const float* __restrict__ dhz, float DB, int kchunk) {
+                                                const float* __restrict__ dhz, float DB, int kchunk, Cpml cp,
+                                                float dx, float dy, float dz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > g.NX - 1 || j > g.NY - 1) return;
@@ -44,6 +75,9 @@ __global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, 
     const int k1 = min(k0 + kchunk - 1, g.NZ - 1);
     if (k0 > k1) return;
     const float dx_i = dhx[i - 1], dy_j = dhy[j - 1];
+    int qx = 0, qy = 0;
+    const int fx = FUSE ? slab_h(i, g.NX, cp.n[0], cp.n[1], qx) : -1;
+    const int fy = FUSE ? slab_h(j, g.NY, cp.n[2], cp.n[3], qy) : -1;
     long long o = off(g, i, j, k0);
     float ey_k = Ey[o], ex_k = Ex[o];
     for (int k = k0; k <= k1; ++k) {
@@ -55,9 +89,51 @@ __global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, 
         const float ey_xp = Ey[o + 1];
         const float ex_yp = Ex[o + g.PX];
         const float dz_k = dhz[k - 1];
-        Hx[o] = Hx[o] + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
-        Hy[o] = Hy[o] + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
-        Hz[o] = Hz[o] + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
+        float hx = Hx[o] + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
+        float hy = Hy[o] + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
+        float hz = Hz[o] + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
+        if (FUSE) {
+            if (fx >= 0) {                                   // psi_Hyx, psi_Hzx
+                const int f = fx;
+                const int ntf = cp.n[f] - 1;
+                const long long po = (long long)(qx - 1) + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+                const float b = cp.b[f][qx - 1], c = cp.c[f][qx - 1];
+                float p0 = b * cp.psi[f][0][po] + c * (ez_xp - ez) / dx;
+                cp.psi[f][0][po] = p0;
+                hy = hy + DB * p0;
+                float p1 = b * cp.psi[f][1][po] + c * (ey_k - ey_xp) / dx;
+                cp.psi[f][1][po] = p1;
+                hz = hz + DB * p1;
+            }
+            if (fy >= 0) {                                   // psi_Hxy, psi_Hzy
+                const int f = 2 + fy;
+                const int ntf = cp.n[f] - 1;
+                const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(qy - 1) + (long long)ntf * (k - 1));
+                const float b = cp.b[f][qy - 1], c = cp.c[f][qy - 1];
+                float p0 = b * cp.psi[f][0][po] + c * (ez - ez_yp) / dy;
+                cp.psi[f][0][po] = p0;
+                hx = hx + DB * p0;
+                float p1 = b * cp.psi[f][1][po] + c * (ex_yp - ex_k) / dy;
+                cp.psi[f][1][po] = p1;
+                hz = hz + DB * p1;
+            }
+            int qz;
+            const int fz = slab_h(k, g.NZ, cp.n[4], cp.n[5], qz);
+            if (fz >= 0) {                                   // psi_Hxz, psi_Hyz
+                const int f = 4 + fz;
+                const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (qz - 1));
+                const float b = cp.b[f][qz - 1], c = cp.c[f][qz - 1];
+                float p0 = b * cp.psi[f][0][po] + c * (ey_k1 - ey_k) / dz;
+                cp.psi[f][0][po] = p0;
+                hx = hx + DB * p0;
+                float p1 = b * cp.psi[f][1][po] + c * (ex_k - ex_k1) / dz;
+                cp.psi[f][1][po] = p1;
+                hy = hy + DB * p1;
+            }
+        }
+        Hx[o] = hx;
+        Hy[o] = hy;
+        Hz[o] = hz;
         ey_k = ey_k1;
         ex_k = ex_k1;
         o += g.plane;
@@ -67,14 +143,16 @@ __global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, 
 // ---------------------------------------------------------------------------
 // Bulk E (f90:5667-5710) with the per-cell coefficient tuple index.
 // Component ranges: Ex i>=1,j>=2,k>=2; Ey i>=2,j>=1,k>=2; Ez i>=2,j>=2,k>=1.
+// FUSE appends the CPML corrections as in k_h_bulk.
 // ---------------------------------------------------------------------------
-template <typename IdxT>
+template <typename IdxT, bool FUSE>
 __global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, float* __restrict__ Ey,
                                                 float* __restrict__ Ez, const float* __restrict__ Hx,
                                                 const float* __restrict__ Hy, const float* __restrict__ Hz,
                                                 const IdxT* __restrict__ idx, const float* __restrict__ tup,
                                                 const float* __restrict__ dex, const float* __restrict__ dey,
-                                                const float* __restrict__ dez, int kchunk) {
+                                                const float* __restrict__ dez, int kchunk, Cpml cp, float dx,
+                                                float dy, float dz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > g.NX - 1 || j > g.NY - 1) return;
@@ -83,6 +161,9 @@ __global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, 
     if (k0 > k1) return;
     const float dx_i = dex[i - 1], dy_j = dey[j - 1];
     const bool ix = i >= 2, jy = j >= 2;
+    int qx = 0, tx = 0, qy = 0, ty = 0;
+    const int fx = FUSE ? slab_e(i, g.NX, cp.n[0], cp.n[1], qx, tx) : -1;
+    const int fy = FUSE ? slab_e(j, g.NY, cp.n[2], cp.n[3], qy, ty) : -1;
     long long o = off(g, i, j, k0);
     float hx_km = 0.f, hy_km = 0.f;
     if (k0 >= 2) {
@@ -93,19 +174,69 @@ __global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, 
         const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
         const int t = (int)idx[o] * 6;
         const float dz_k = dez[k - 1];
-        if (jy && k >= 2) {
-            const float hz_ym = Hz[o - g.PX];
-            Ex[o] = tup[t + 0] * Ex[o] + tup[t + 1] * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
+        const bool ex_on = jy && k >= 2, ey_on = ix && k >= 2, ez_on = ix && jy;
+        const float hz_ym = jy ? Hz[o - g.PX] : 0.f;
+        const float hz_xm = ix ? Hz[o - 1] : 0.f;
+        const float hy_xm = ix ? Hy[o - 1] : 0.f;
+        const float hx_ym = jy ? Hx[o - g.PX] : 0.f;
+        float ex = 0.f, ey = 0.f, ez = 0.f;
+        if (ex_on) ex = tup[t + 0] * Ex[o] + tup[t + 1] * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
+        if (ey_on) ey = tup[t + 2] * Ey[o] + tup[t + 3] * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
+        if (ez_on) ez = tup[t + 4] * Ez[o] + tup[t + 5] * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
+        if (FUSE) {
+            if (fx >= 0) {                                   // psi_Eyx, psi_Ezx
+                const int f = fx;
+                const int ntf = cp.n[f] - 1;
+                const long long po = (long long)tx + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+                const float b = cp.b[f][qx - 1], c = cp.c[f][qx - 1];
+                if (k >= 2) {
+                    float p = b * cp.psi[f][0][po] + c * (hz_xm - hz) / dx;
+                    cp.psi[f][0][po] = p;
+                    ey = ey + tup[t + 3] * p;
+                }
+                if (jy) {
+                    float p = b * cp.psi[f][1][po] + c * (hy - hy_xm) / dx;
+                    cp.psi[f][1][po] = p;
+                    ez = ez + tup[t + 5] * p;
+                }
+            }
+            if (fy >= 0) {                                   // psi_Exy, psi_Ezy
+                const int f = 2 + fy;
+                const int ntf = cp.n[f] - 1;
+                const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)ty + (long long)ntf * (k - 1));
+                const float b = cp.b[f][qy - 1], c = cp.c[f][qy - 1];
+                if (k >= 2) {
+                    float p = b * cp.psi[f][0][po] + c * (hz - hz_ym) / dy;
+                    cp.psi[f][0][po] = p;
+                    ex = ex + tup[t + 1] * p;
+                }
+                if (ix) {
+                    float p = b * cp.psi[f][1][po] + c * (hx_ym - hx) / dy;
+                    cp.psi[f][1][po] = p;
+                    ez = ez + tup[t + 5] * p;
+                }
+            }
+            int qz, tz;
+            const int fz = slab_e(k, g.NZ, cp.n[4], cp.n[5], qz, tz);
+            if (fz >= 0) {                                   // psi_Exz, psi_Eyz
+                const int f = 4 + fz;
+                const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * tz);
+                const float b = cp.b[f][qz - 1], c = cp.c[f][qz - 1];
+                if (jy) {
+                    float p = b * cp.psi[f][0][po] + c * (hy_km - hy) / dz;
+                    cp.psi[f][0][po] = p;
+                    ex = ex + tup[t + 1] * p;
+                }
+                if (ix) {
+                    float p = b * cp.psi[f][1][po] + c * (hx - hx_km) / dz;
+                    cp.psi[f][1][po] = p;
+                    ey = ey + tup[t + 3] * p;
+                }
+            }
         }
-        if (ix && k >= 2) {
-            const float hz_xm = Hz[o - 1];
-            Ey[o] = tup[t + 2] * Ey[o] + tup[t + 3] * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
-        }
-        if (ix && jy) {
-            const float hy_xm = Hy[o - 1];
-            const float hx_ym = Hx[o - g.PX];
-            Ez[o] = tup[t + 4] * Ez[o] + tup[t + 5] * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
-        }
+        if (ex_on) Ex[o] = ex;
+        if (ey_on) Ey[o] = ey;
+        if (ez_on) Ez[o] = ez;
         hx_km = hx;
         hy_km = hy;
         o += g.plane;
@@ -113,18 +244,11 @@ __global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, 
 }
 
 // ---------------------------------------------------------------------------
-// CPML slabs (H: f90:5227-5375, E: f90:6522-6670).  One launch per normal
-// axis, in x, y, z order, which reproduces the reference's per-component add
-// order (Hx: y then z, Hy: x then z, Hz: x then y; same for E).
-// psi of face f / slot s is stored [k][j][i] restricted to the slab, with the
-// slab coordinate replacing the normal index (t = 0..nt-1).
+// Separate CPML slab kernels, used when a port, sheet, TF/SF face or Drude
+// box lies inside a slab (the reference applies those before the CPML
+// correction).  One launch per normal axis, in x, y, z order, which keeps
+// the per-component add order (Hx: y then z, Hy: x then z, Hz: x then y).
 // ---------------------------------------------------------------------------
-struct Cpml {
-    int n[6];                  // npml per face
-    const float* b[6];         // bh (H launch) or be (E launch)
-    const float* c[6];         // ch or ce
-    float* psi[6][2];          // per face, two components
-};
 
 // H, slab normal to x.  Threads: (t, j, k); t enumerates low then high layers.
 __global__ void k_h_cpml_x(Grid g, Cpml p, float* Hy, float* Hz, const float* Ey, const float* Ez, float DB,
@@ -779,8 +903,11 @@ struct NtffBins {
     float r_relay, dx, dy, dz, c_dt;
     float K[3];
     long long LL;          // bins per angle (len_ff + 2)
-    double* U[3];
+    double* U[3];          // FP64 accumulators (atomic binning)
     double* W[3];
+    float* Uf[3];          // FP32 accumulators (ordered binning, the reference's arithmetic)
+    float* Wf[3];
+    int exact;             // 1: Uf/Wf hold the result, 0: U/W
 };
 
 __global__ void k_ntff_bins(NtffFaces f, NtffBins bn, const int* step) {
@@ -833,6 +960,79 @@ __global__ void k_ntff_bins(NtffFaces f, NtffBins bn, const int* step) {
     atomicAdd(Wa + ii + 1, (double)(wk2 * dJ0));
     atomicAdd(Wb + ii, (double)(base2 * dJ1));
     atomicAdd(Wb + ii + 1, (double)(wk2 * dJ1));
+}
+
+// NTFF retarded-time binning with the serial reference's FP32 accumulation
+// order (f90:7643-7927).  One work-group per (angle, accumulator, bin
+// window); thread t owns bin lo+t and walks the step's contributions in the
+// reference's loop order -- face pairs x, y, z; (v outer, u inner); low then
+// high face -- adding those that target its bin.  A cell adds to a bin at
+// most once per accumulator, so each bin sees exactly the reference's
+// sequence of FP32 additions.
+struct NtffExact {
+    float* acc[6];          // Ux,Uy,Uz,Wx,Wy,Wz as FP32 [angle][LL]
+    const int* win;         // per angle: {lowest bin offset relative to n, window length} (host bound)
+};
+
+__global__ void __launch_bounds__(256) k_ntff_exact(NtffFaces f, NtffBins bn, NtffExact ex, const int* step) {
+    const int n = *step;
+    const int kk = blockIdx.x;
+    const int arr = blockIdx.y;               // 0..2 U, 3..5 W
+    const int comp = arr % 3;                 // x, y, z component of U or W
+    const bool isW = arr >= 3;
+    const int slot_in_win = blockIdx.z * blockDim.x + threadIdx.x;
+    const int bin = n + ex.win[2 * kk] + slot_in_win;
+    const bool own = slot_in_win < ex.win[2 * kk + 1] && bin >= 0 && bin < bn.LL;
+    float* A = ex.acc[arr] + kk * bn.LL;
+    float acc = own ? A[bin] : 0.0f;
+    const float* an = bn.ang + 6 * kk;
+    const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
+    __shared__ int r_ii[256];
+    __shared__ float r_lo[256], r_hi[256];
+    for (int a = 0; a < 3; ++a) {             // face pairs in the reference's order
+        if (a == comp || f.pbc[a]) continue;  // faces normal to a feed the other two components
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        // slot of this component on faces normal to a: (b, c) -> (0, 1)
+        const int slot = comp == b ? 0 : 1;
+        const int nu = f.hi[b] - f.lo[b] + 1, nv = f.hi[c] - f.lo[c] + 1;
+        const long long ncell = (long long)nu * nv * 2;
+        const float K = bn.K[a];
+        for (long long base = 0; base < ncell; base += blockDim.x) {
+            const long long cid = base + threadIdx.x;
+            if (cid < ncell) {
+                const int side = (int)(cid & 1);
+                const long long uv = cid >> 1;
+                const int u = f.lo[b] + (int)(uv % nu), v = f.lo[c] + (int)(uv / nu);
+                const int face = 2 * a + side;
+                const int pos = side ? f.hi[a] : f.lo[a];
+                int i, j, k;
+                if (a == 0) { i = pos; j = u; k = v; }
+                else if (a == 1) { i = u; j = pos; k = v; }
+                else { i = u; j = v; k = pos; }
+                const float tfv = (bn.r_relay - ((float)(i - bn.ic) * bn.dx * s1 * c2 + (float)(j - bn.jc) * bn.dy * s1 * s2 +
+                                                 (float)(k - bn.kc) * bn.dz * c1)) / bn.c_dt;
+                const long long fo = (long long)u + (long long)f.fd0[face] * v;
+                const float d = isW ? (f.J[face][slot][fo] - f.Jo[face][slot][fo])
+                                    : (f.M[face][slot][fo] - f.Mo[face][slot][fo]);
+                const float t = isW ? ((float)n + tfv - 1.0f) : ((float)n + 0.5f + tfv - 1.0f);
+                const int ii = (int)t;
+                const float w = t - (float)ii;
+                r_ii[threadIdx.x] = ii;
+                r_lo[threadIdx.x] = (1.0f - w) * K * d;
+                r_hi[threadIdx.x] = w * K * d;
+            }
+            __syncthreads();
+            const int m = (int)min((long long)blockDim.x, ncell - base);
+            if (own)
+                for (int q = 0; q < m; ++q) {
+                    const int ii = r_ii[q];
+                    if (ii == bin) acc = acc + r_lo[q];
+                    else if (ii + 1 == bin) acc = acc + r_hi[q];
+                }
+            __syncthreads();
+        }
+    }
+    if (own) A[bin] = acc;
 }
 
 // Phase-centre samples (f90:8259-8290) and the step counter.
@@ -896,8 +1096,15 @@ __global__ void __launch_bounds__(256) k_ntff_combine(NtffBins bn, int L, int T,
     if (threadIdx.x == 0) { first_t = L; first_p = L; }
     __syncthreads();
     for (int j = 1 + threadIdx.x; j <= L; j += blockDim.x) {
-        const float wx = (float)bn.W[0][kk * bn.LL + j], wy = (float)bn.W[1][kk * bn.LL + j], wz = (float)bn.W[2][kk * bn.LL + j];
-        const float ux = (float)bn.U[0][kk * bn.LL + j], uy = (float)bn.U[1][kk * bn.LL + j], uz = (float)bn.U[2][kk * bn.LL + j];
+        const long long q = kk * bn.LL + j;
+        float wx, wy, wz, ux, uy, uz;
+        if (bn.exact) {
+            wx = bn.Wf[0][q]; wy = bn.Wf[1][q]; wz = bn.Wf[2][q];
+            ux = bn.Uf[0][q]; uy = bn.Uf[1][q]; uz = bn.Uf[2][q];
+        } else {
+            wx = (float)bn.W[0][q]; wy = (float)bn.W[1][q]; wz = (float)bn.W[2][q];
+            ux = (float)bn.U[0][q]; uy = (float)bn.U[1][q]; uz = (float)bn.U[2][q];
+        }
         const float wt = wx * c1 * c2 + wy * c1 * s2 - wz * s1;
         const float wph = -1.0f * wx * s2 + wy * c2;
         const float ut = ux * c1 * c2 + uy * c1 * s2 - uz * s1;

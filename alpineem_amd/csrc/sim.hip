@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,6 +41,58 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+// The CPML correction can be folded into the bulk update when no cell of a
+// slab also receives a sheet, TF/SF, port or Drude update: the reference
+// applies those between the bulk update and the CPML correction
+// (fdtd_solver.f90:4486-5375, 5712-6670), so only then is the per-cell
+// order unchanged.
+bool cpml_fusable(const aem_plan_view& v) {
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = v.npml[2 * a] > 0 ? v.npml[2 * a] + 1 : 1;
+        hi[a] = v.npml[2 * a + 1] > 0 ? N[a] - v.npml[2 * a + 1] : N[a];
+    }
+    auto inside = [&](int a, int x) { return x >= lo[a] && x <= hi[a]; };
+    auto box_inside = [&](const int b[6]) {   // {xlo,xhi,ylo,yhi,zlo,zhi}
+        for (int a = 0; a < 3; ++a)
+            if (b[2 * a] <= b[2 * a + 1] && !(inside(a, b[2 * a]) && inside(a, b[2 * a + 1]))) return false;
+        return true;
+    };
+    for (int q = 0; q < v.n_ports; ++q) {
+        const aem_port_view& p = v.ports[q];
+        int b[6];
+        for (int a = 0; a < 3; ++a) { b[2 * a] = p.loc[a] - 1; b[2 * a + 1] = p.loc[a] + p.size[a]; }
+        if (!box_inside(b)) return false;
+    }
+    if (v.has_tfsf) {
+        int b[6];
+        for (int a = 0; a < 3; ++a) { b[2 * a] = v.tf_lo[a] - 1; b[2 * a + 1] = v.tf_hi[a] + 1; }
+        for (int a = 0; a < 3; ++a)
+            if (v.pbc[a]) { b[2 * a] = lo[a]; b[2 * a + 1] = hi[a]; }   // periodic axes carry no slab
+        if (!box_inside(b)) return false;
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < v.n_sheet_planes[a]; ++s) {
+            const int P = v.sheet_dim[a][0], Q = v.sheet_dim[a][1];
+            const float* sg = v.sheet_sig[a] + (size_t)s * P * Q;
+            const float* ep = v.sheet_ep[a] + (size_t)s * P * Q;
+            const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+            for (int q = 1; q <= Q; ++q)
+                for (int p = 1; p <= P; ++p) {
+                    const size_t k = (size_t)(p - 1) + (size_t)P * (q - 1);
+                    if (!(sg[k] > 0 || ep[k] > 1)) continue;
+                    if (!inside(a, v.sheet_pos[a][s]) || !inside(o1, p) || !inside(o1, p + 1) || !inside(o2, q) ||
+                        !inside(o2, q + 1))
+                        return false;
+                }
+        }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c)
+            if (!box_inside(v.dbox[c])) return false;
+    return true;
+}
+
 }  // namespace
 
 struct aem_sim {
@@ -61,6 +114,8 @@ struct aem_sim {
     float *d_V = nullptr, *d_I = nullptr, *d_Vout = nullptr, *d_Iout = nullptr;
     NtffFaces nf{};
     NtffBins nb{};
+    NtffExact nx_acc{};
+    int win_max = 0;
     double *d_refl = nullptr, *d_trans = nullptr;
     float* d_pc = nullptr;
     float *d_incident = nullptr, *d_Eth = nullptr, *d_Eph = nullptr, *d_fftime = nullptr;
@@ -68,6 +123,7 @@ struct aem_sim {
     Tfsf tf{};
     Pulse pu{};
     int kchunk = 16;
+    bool fuse_cpml = false;   // CPML corrections folded into the bulk kernels
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -222,9 +278,52 @@ void aem_sim::create() {
         nb.c_dt = aem::kC * v.dt;
         for (int a = 0; a < 3; ++a) nb.K[a] = v.ff_K[a];
         nb.LL = (long long)v.len_ff + 2;
-        for (int q = 0; q < 3; ++q) {
-            nb.U[q] = zeros<double>((size_t)v.n_angles * nb.LL);
-            nb.W[q] = zeros<double>((size_t)v.n_angles * nb.LL);
+        nb.exact = std::getenv("AEM_NTFF_ATOMIC") == nullptr ? 1 : 0;
+        if (nb.exact) {
+            for (int q = 0; q < 3; ++q) {
+                nb.Uf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
+                nb.Wf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
+                nx_acc.acc[q] = nb.Uf[q];
+                nx_acc.acc[3 + q] = nb.Wf[q];
+            }
+            // retarded-time offsets of every face cell, per angle, bound the
+            // window of bins one step can touch
+            std::vector<int> win(2 * (size_t)v.n_angles);
+            win_max = 0;
+            for (int kk = 0; kk < v.n_angles; ++kk) {
+                const float* an = v.ang + 6 * kk;
+                const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
+                float tmin = 1e30f, tmax = -1e30f;
+                for (int a = 0; a < 3; ++a) {
+                    if (v.pbc[a]) continue;
+                    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+                    for (int side = 0; side < 2; ++side) {
+                        const int pos = side ? v.ff_hi[a] : v.ff_lo[a];
+                        for (int vv = v.ff_lo[c]; vv <= v.ff_hi[c]; ++vv)
+                            for (int uu = v.ff_lo[b]; uu <= v.ff_hi[b]; ++uu) {
+                                int i, j, k;
+                                if (a == 0) { i = pos; j = uu; k = vv; }
+                                else if (a == 1) { i = uu; j = pos; k = vv; }
+                                else { i = uu; j = vv; k = pos; }
+                                const float tfv = (v.r_relay - ((float)(i - v.ic) * v.dx * s1 * c2 +
+                                                                (float)(j - v.jc) * v.dy * s1 * s2 +
+                                                                (float)(k - v.kc) * v.dz * c1)) / (aem::kC * v.dt);
+                                tmin = std::min(tmin, tfv);
+                                tmax = std::max(tmax, tfv);
+                            }
+                    }
+                }
+                const int lo = (int)std::floor(tmin) - 3, hi = (int)std::floor(tmax) + 3;
+                win[2 * kk] = lo;
+                win[2 * kk + 1] = std::max(hi - lo + 1, 0);
+                win_max = std::max(win_max, win[2 * kk + 1]);
+            }
+            nx_acc.win = up<int>(win.data(), win.size());
+        } else {
+            for (int q = 0; q < 3; ++q) {
+                nb.U[q] = zeros<double>((size_t)v.n_angles * nb.LL);
+                nb.W[q] = zeros<double>((size_t)v.n_angles * nb.LL);
+            }
         }
         d_Eth = zeros<float>((size_t)v.n_angles * T);
         d_Eph = zeros<float>((size_t)v.n_angles * T);
@@ -243,6 +342,7 @@ void aem_sim::create() {
     const long long bxy = (long long)cdiv(v.NX - 1, 32) * cdiv(v.NY - 1, 8);
     long long nch = std::max(1LL, std::min<long long>(v.NZ - 1, (4096 + bxy - 1) / bxy));
     kchunk = (int)cdiv(v.NZ - 1, nch);
+    fuse_cpml = cpml_fusable(v) && std::getenv("AEM_NO_CPML_FUSION") == nullptr;
     HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -257,7 +357,12 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
 
     // ---- H phase ----
     if (mark) HIPCHK(hipEventRecord(mark[0], st));
-    k_h_bulk<<<gb, bb, 0, st>>>(g, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], v.DB, kchunk);
+    if (fuse_cpml)
+        k_h_bulk<true><<<gb, bb, 0, st>>>(g, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], v.DB, kchunk, cpH,
+                                          v.dx, v.dy, v.dz);
+    else
+        k_h_bulk<false><<<gb, bb, 0, st>>>(g, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], v.DB, kchunk, cpH,
+                                           v.dx, v.dy, v.dz);
     if (mark) HIPCHK(hipEventRecord(mark[1], st));
     const float dd[3] = {v.dx, v.dy, v.dz};
     for (int a = 0; a < 3; ++a)
@@ -278,9 +383,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
         const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
-        if (tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, st>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, v.dx);
-        if (ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, st>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, v.dy);
-        if (tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, st>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, v.dz);
+        if (!fuse_cpml && tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, st>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, v.dx);
+        if (!fuse_cpml && ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, st>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, v.dy);
+        if (!fuse_cpml && tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, st>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, v.dz);
     }
     if (v.pbc[0]) k_h_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 0, Hz, Hy);
     if (v.pbc[1]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 1, Hz, Hx);
@@ -288,7 +393,12 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
 
     // ---- E phase ----
     if (mark) HIPCHK(hipEventRecord(mark[2], st));
-    k_e_bulk<IdxT><<<gb, bb, 0, st>>>(g, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], d_den[1], d_den[2], kchunk);
+    if (fuse_cpml)
+        k_e_bulk<IdxT, true><<<gb, bb, 0, st>>>(g, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], d_den[1], d_den[2],
+                                                kchunk, cpE, v.dx, v.dy, v.dz);
+    else
+        k_e_bulk<IdxT, false><<<gb, bb, 0, st>>>(g, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], d_den[1], d_den[2],
+                                                 kchunk, cpE, v.dx, v.dy, v.dz);
     if (mark) HIPCHK(hipEventRecord(mark[3], st));
     for (int a = 0; a < 3; ++a)
         if (sh[a].np > 0) {
@@ -310,9 +420,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
         const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
-        if (tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, st>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, v.dx);
-        if (ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, st>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, v.dy);
-        if (tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, st>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, v.dz);
+        if (!fuse_cpml && tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, st>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, v.dx);
+        if (!fuse_cpml && ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, st>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, v.dy);
+        if (!fuse_cpml && tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, st>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, v.dz);
     }
     if (v.pbc[0]) {
         k_e_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 0, Ez, Ey);
@@ -358,7 +468,12 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
         const dim3 gf(cdiv(mu, 32), cdiv(mv, 8), 6);
         k_ntff_currents<<<gf, bb, 0, st>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
         const dim3 gbins(cdiv(mu, 32), cdiv(mv, 8), 6 * v.n_angles);
-        k_ntff_bins<<<gbins, bb, 0, st>>>(nf, nb, d_step);
+        if (nb.exact) {
+            if (win_max > 0)
+                k_ntff_exact<<<dim3(v.n_angles, 6, cdiv(win_max, 256)), 256, 0, st>>>(nf, nb, nx_acc, d_step);
+        } else {
+            k_ntff_bins<<<gbins, bb, 0, st>>>(nf, nb, d_step);
+        }
     }
     if (v.n_angles > 0 && v.plane_wave_amp > 0)
         k_phase_centre<<<1, 1, 0, st>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
@@ -575,10 +690,18 @@ aem_status aem_get_ntff_raw(aem_sim* s, int32_t which, double* out, int64_t n) {
         return AEM_ERR_STATE;
     }
     return aem::guard([&]() {
-        const double* src = which < 3 ? s->nb.U[which] : s->nb.W[which - 3];
         std::vector<double> all((size_t)v.n_angles * s->nb.LL);
-        HIPCHK(hipMemcpyAsync(all.data(), src, all.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
-        HIPCHK(hipStreamSynchronize(s->st));
+        if (s->nb.exact) {
+            const float* src = which < 3 ? s->nb.Uf[which] : s->nb.Wf[which - 3];
+            std::vector<float> f32(all.size());
+            HIPCHK(hipMemcpyAsync(f32.data(), src, f32.size() * sizeof(float), hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+            for (size_t q = 0; q < all.size(); ++q) all[q] = f32[q];
+        } else {
+            const double* src = which < 3 ? s->nb.U[which] : s->nb.W[which - 3];
+            HIPCHK(hipMemcpyAsync(all.data(), src, all.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+        }
         for (int64_t q = 0; q < n; ++q) {
             int64_t a = q / v.len_ff, b = q % v.len_ff;
             out[q] = all[(size_t)(a * s->nb.LL + b + 1)];

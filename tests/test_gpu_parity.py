@@ -116,10 +116,56 @@ def test_fields_close_to_oracle(runs, name):
 
 @pytest.mark.parametrize("name", ["dipole36", "c1_mini", "antenna_x"])
 def test_antenna_decks_bit_exact(runs, name):
-    """No plane-wave source: the port voltage is the reference's, bit for bit."""
+    """No plane-wave source: port voltage, incident wave and far field (with
+    the reference's FP32 retarded-time binning order) are the reference's,
+    bit for bit."""
     plan, sim, g = runs[name]
+    np.testing.assert_array_equal(sim.series("incident"), g["incident"])
     for q in range(plan.view.n_ports):
         np.testing.assert_array_equal(sim.series("port_v", q), g["port_v"][q])
+    for a in range(plan.view.n_angles):
+        np.testing.assert_array_equal(sim.series("e_theta", a), g["e_theta"][a], err_msg=f"E_theta {a}")
+        np.testing.assert_array_equal(sim.series("e_phi", a), g["e_phi"][a], err_msg=f"E_phi {a}")
+
+
+def _with_env(var, fn):
+    old = os.environ.get(var)
+    os.environ[var] = "1"
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ[var]
+        else:
+            os.environ[var] = old
+
+
+@pytest.mark.parametrize("name", ["c1_mini", "drude_slab", "master_ports"])
+def test_cpml_fusion_matches_separate_kernels(golden_dir, name):
+    """Folding the CPML correction into the bulk kernels keeps the
+    reference's per-cell add order: fields equal the separate-kernel path."""
+    deck = os.path.join(golden_dir, "decks", name + ".txt")
+    fused = Simulation(Plan.from_inputs(deck))
+    plain = _with_env("AEM_NO_CPML_FUSION", lambda: Simulation(Plan.from_inputs(deck)))
+    fused.run(120)
+    plain.run(120)
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        np.testing.assert_array_equal(fused.field(comp), plain.field(comp), err_msg=comp)
+
+
+def test_ntff_fp64_atomic_mode_close(golden_dir):
+    """The FP64 atomic binning (opt-in) agrees with the ordered FP32 binning
+    within the reference's FP32 noise."""
+    deck = os.path.join(golden_dir, "decks", "c1_mini.txt")
+    a = Simulation(Plan.from_inputs(deck))
+    b = _with_env("AEM_NTFF_ATOMIC", lambda: Simulation(Plan.from_inputs(deck)))
+    for s in (a, b):
+        s.run()
+        s.finalize()
+    for ang in (1, 3, 5):
+        ea = np.concatenate([a.series("e_theta", ang), a.series("e_phi", ang)])
+        eb = np.concatenate([b.series("e_theta", ang), b.series("e_phi", ang)])
+        assert rel_l2(eb, ea) < 1e-5
 
 
 def test_graph_and_eager_paths_agree(golden_dir):
