@@ -44,6 +44,7 @@ SIGNATURES = [
     ("aem_write_outputs", C.c_int, [C.c_void_p, C.c_char_p]),
     ("aem_last_run_ms", C.c_float, [C.c_void_p]),
     ("aem_time_phases", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_float), C.c_int32]),
+    ("aem_core_cells", C.c_int64, [C.c_void_p]),
 ]
 
 _lib = None

@@ -56,28 +56,50 @@ __device__ __forceinline__ int slab_e(int i, int N, int n1, int n2, int& q, int&
 }
 
 // ---------------------------------------------------------------------------
-// Bulk H (f90:4442-4484).  One thread per (i,j) column marching a chunk of
-// k; Ex(k+1), Ey(k+1) are carried in registers to the next plane.  With
-// FUSE the CPML corrections of the cell follow in the reference's per-cell
+// Bulk launches cover boxes of cells.  With CPML fusion the domain is split
+// into disjoint boxes: the core (no slab cell; the CPML code is compiled
+// out), the x-edge block columns, the y slabs and the z slabs; each box's
+// kernel compiles in only the slab tests it needs (C = kCx|kCy|kCz), so the
+// core keeps the register budget of the plain update.  The x core is aligned
+// to 32-cell blocks so every wave row is one 128-byte line.
+// ---------------------------------------------------------------------------
+enum : int { kCx = 1, kCy = 2, kCz = 4 };
+
+struct Box {
+    int i0, j0, k0;   // first cell (1-based)
+    int ni, nj, nk;   // extents
+    int kc;           // planes marched by one thread
+};
+
+// Non-temporal access for the cell's own field values, which are read and
+// written exactly once per launch (the neighbour reads stay cached).
+template <class T>
+__device__ __forceinline__ T ld_once(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ __forceinline__ void st_once(T v, T* p) { __builtin_nontemporal_store(v, p); }
+
+// ---------------------------------------------------------------------------
+// Bulk H (f90:4442-4484).  One thread per (i,j) column marching b.kc planes;
+// Ex(k+1), Ey(k+1) are carried in registers to the next plane.  Slab cells
+// of the box then add their CPML corrections in the reference's per-cell
 // order (x, then y, then z slab), reusing the loaded neighbours.
 // ---------------------------------------------------------------------------
-template <bool FUSE>
-__global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, float* __restrict__ Hy,
+template <int C>
+__global__ void __launch_bounds__(256) k_h_bulk(Grid g, Box b, float* __restrict__ Hx, float* __restrict__ Hy,
                                                 float* __restrict__ Hz, const float* __restrict__ Ex,
                                                 const float* __restrict__ Ey, const float* __restrict__ Ez,
                                                 const float* __restrict__ dhx, const float* __restrict__ dhy,
-                                                const float* __restrict__ dhz, float DB, int kchunk, Cpml cp,
-                                                float dx, float dy, float dz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > g.NX - 1 || j > g.NY - 1) return;
-    const int k0 = blockIdx.z * kchunk + 1;
-    const int k1 = min(k0 + kchunk - 1, g.NZ - 1);
-    if (k0 > k1) return;
+                                                const float* __restrict__ dhz, float DB, Cpml cp, float dx, float dy,
+                                                float dz) {
+    const int i = b.i0 + (int)(blockIdx.x * 32 + threadIdx.x);
+    const int j = b.j0 + (int)(blockIdx.y * 8 + threadIdx.y);
+    if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
+    const int k0 = b.k0 + (int)blockIdx.z * b.kc;
+    const int k1 = min(k0 + b.kc, b.k0 + b.nk) - 1;
     const float dx_i = dhx[i - 1], dy_j = dhy[j - 1];
     int qx = 0, qy = 0;
-    const int fx = FUSE ? slab_h(i, g.NX, cp.n[0], cp.n[1], qx) : -1;
-    const int fy = FUSE ? slab_h(j, g.NY, cp.n[2], cp.n[3], qy) : -1;
+    const int fx = (C & kCx) ? slab_h(i, g.NX, cp.n[0], cp.n[1], qx) : -1;
+    const int fy = (C & kCy) ? slab_h(j, g.NY, cp.n[2], cp.n[3], qy) : -1;
     long long o = off(g, i, j, k0);
     float ey_k = Ey[o], ex_k = Ex[o];
     for (int k = k0; k <= k1; ++k) {
@@ -89,51 +111,51 @@ __global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, 
         const float ey_xp = Ey[o + 1];
         const float ex_yp = Ex[o + g.PX];
         const float dz_k = dhz[k - 1];
-        float hx = Hx[o] + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
-        float hy = Hy[o] + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
-        float hz = Hz[o] + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
-        if (FUSE) {
-            if (fx >= 0) {                                   // psi_Hyx, psi_Hzx
+        float hx = ld_once(&Hx[o]) + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
+        float hy = ld_once(&Hy[o]) + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
+        float hz = ld_once(&Hz[o]) + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
+        if (C) {
+            if ((C & kCx) && fx >= 0) {                      // psi_Hyx, psi_Hzx
                 const int f = fx;
                 const int ntf = cp.n[f] - 1;
                 const long long po = (long long)(qx - 1) + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
-                const float b = cp.b[f][qx - 1], c = cp.c[f][qx - 1];
-                float p0 = b * cp.psi[f][0][po] + c * (ez_xp - ez) / dx;
+                const float cb = cp.b[f][qx - 1], cc = cp.c[f][qx - 1];
+                float p0 = cb * cp.psi[f][0][po] + cc * (ez_xp - ez) / dx;
                 cp.psi[f][0][po] = p0;
                 hy = hy + DB * p0;
-                float p1 = b * cp.psi[f][1][po] + c * (ey_k - ey_xp) / dx;
+                float p1 = cb * cp.psi[f][1][po] + cc * (ey_k - ey_xp) / dx;
                 cp.psi[f][1][po] = p1;
                 hz = hz + DB * p1;
             }
-            if (fy >= 0) {                                   // psi_Hxy, psi_Hzy
+            if ((C & kCy) && fy >= 0) {                      // psi_Hxy, psi_Hzy
                 const int f = 2 + fy;
                 const int ntf = cp.n[f] - 1;
                 const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(qy - 1) + (long long)ntf * (k - 1));
-                const float b = cp.b[f][qy - 1], c = cp.c[f][qy - 1];
-                float p0 = b * cp.psi[f][0][po] + c * (ez - ez_yp) / dy;
+                const float cb = cp.b[f][qy - 1], cc = cp.c[f][qy - 1];
+                float p0 = cb * cp.psi[f][0][po] + cc * (ez - ez_yp) / dy;
                 cp.psi[f][0][po] = p0;
                 hx = hx + DB * p0;
-                float p1 = b * cp.psi[f][1][po] + c * (ex_yp - ex_k) / dy;
+                float p1 = cb * cp.psi[f][1][po] + cc * (ex_yp - ex_k) / dy;
                 cp.psi[f][1][po] = p1;
                 hz = hz + DB * p1;
             }
-            int qz;
-            const int fz = slab_h(k, g.NZ, cp.n[4], cp.n[5], qz);
-            if (fz >= 0) {                                   // psi_Hxz, psi_Hyz
+            int qz = 0;
+            const int fz = (C & kCz) ? slab_h(k, g.NZ, cp.n[4], cp.n[5], qz) : -1;
+            if ((C & kCz) && fz >= 0) {                      // psi_Hxz, psi_Hyz
                 const int f = 4 + fz;
                 const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (qz - 1));
-                const float b = cp.b[f][qz - 1], c = cp.c[f][qz - 1];
-                float p0 = b * cp.psi[f][0][po] + c * (ey_k1 - ey_k) / dz;
+                const float cb = cp.b[f][qz - 1], cc = cp.c[f][qz - 1];
+                float p0 = cb * cp.psi[f][0][po] + cc * (ey_k1 - ey_k) / dz;
                 cp.psi[f][0][po] = p0;
                 hx = hx + DB * p0;
-                float p1 = b * cp.psi[f][1][po] + c * (ex_k - ex_k1) / dz;
+                float p1 = cb * cp.psi[f][1][po] + cc * (ex_k - ex_k1) / dz;
                 cp.psi[f][1][po] = p1;
                 hy = hy + DB * p1;
             }
         }
-        Hx[o] = hx;
-        Hy[o] = hy;
-        Hz[o] = hz;
+        st_once(hx, &Hx[o]);
+        st_once(hy, &Hy[o]);
+        st_once(hz, &Hz[o]);
         ey_k = ey_k1;
         ex_k = ex_k1;
         o += g.plane;
@@ -143,27 +165,26 @@ __global__ void __launch_bounds__(256) k_h_bulk(Grid g, float* __restrict__ Hx, 
 // ---------------------------------------------------------------------------
 // Bulk E (f90:5667-5710) with the per-cell coefficient tuple index.
 // Component ranges: Ex i>=1,j>=2,k>=2; Ey i>=2,j>=1,k>=2; Ez i>=2,j>=2,k>=1.
-// FUSE appends the CPML corrections as in k_h_bulk.
+// Slab cells of the box append their CPML corrections as in k_h_bulk.
 // ---------------------------------------------------------------------------
-template <typename IdxT, bool FUSE>
-__global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, float* __restrict__ Ey,
+template <typename IdxT, int C>
+__global__ void __launch_bounds__(256) k_e_bulk(Grid g, Box b, float* __restrict__ Ex, float* __restrict__ Ey,
                                                 float* __restrict__ Ez, const float* __restrict__ Hx,
                                                 const float* __restrict__ Hy, const float* __restrict__ Hz,
                                                 const IdxT* __restrict__ idx, const float* __restrict__ tup,
                                                 const float* __restrict__ dex, const float* __restrict__ dey,
-                                                const float* __restrict__ dez, int kchunk, Cpml cp, float dx,
-                                                float dy, float dz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    if (i > g.NX - 1 || j > g.NY - 1) return;
-    const int k0 = blockIdx.z * kchunk + 1;
-    const int k1 = min(k0 + kchunk - 1, g.NZ - 1);
-    if (k0 > k1) return;
+                                                const float* __restrict__ dez, Cpml cp, float dx, float dy,
+                                                float dz) {
+    const int i = b.i0 + (int)(blockIdx.x * 32 + threadIdx.x);
+    const int j = b.j0 + (int)(blockIdx.y * 8 + threadIdx.y);
+    if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
+    const int k0 = b.k0 + (int)blockIdx.z * b.kc;
+    const int k1 = min(k0 + b.kc, b.k0 + b.nk) - 1;
     const float dx_i = dex[i - 1], dy_j = dey[j - 1];
     const bool ix = i >= 2, jy = j >= 2;
     int qx = 0, tx = 0, qy = 0, ty = 0;
-    const int fx = FUSE ? slab_e(i, g.NX, cp.n[0], cp.n[1], qx, tx) : -1;
-    const int fy = FUSE ? slab_e(j, g.NY, cp.n[2], cp.n[3], qy, ty) : -1;
+    const int fx = (C & kCx) ? slab_e(i, g.NX, cp.n[0], cp.n[1], qx, tx) : -1;
+    const int fy = (C & kCy) ? slab_e(j, g.NY, cp.n[2], cp.n[3], qy, ty) : -1;
     long long o = off(g, i, j, k0);
     float hx_km = 0.f, hy_km = 0.f;
     if (k0 >= 2) {
@@ -179,64 +200,66 @@ __global__ void __launch_bounds__(256) k_e_bulk(Grid g, float* __restrict__ Ex, 
         const float hz_xm = ix ? Hz[o - 1] : 0.f;
         const float hy_xm = ix ? Hy[o - 1] : 0.f;
         const float hx_ym = jy ? Hx[o - g.PX] : 0.f;
+        const float gax = tup[t + 0], gbx = tup[t + 1], gay = tup[t + 2], gby = tup[t + 3], gaz = tup[t + 4],
+                    gbz = tup[t + 5];
         float ex = 0.f, ey = 0.f, ez = 0.f;
-        if (ex_on) ex = tup[t + 0] * Ex[o] + tup[t + 1] * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
-        if (ey_on) ey = tup[t + 2] * Ey[o] + tup[t + 3] * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
-        if (ez_on) ez = tup[t + 4] * Ez[o] + tup[t + 5] * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
-        if (FUSE) {
-            if (fx >= 0) {                                   // psi_Eyx, psi_Ezx
+        if (ex_on) ex = gax * ld_once(&Ex[o]) + gbx * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
+        if (ey_on) ey = gay * ld_once(&Ey[o]) + gby * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
+        if (ez_on) ez = gaz * ld_once(&Ez[o]) + gbz * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
+        if (C) {
+            if ((C & kCx) && fx >= 0) {                      // psi_Eyx, psi_Ezx
                 const int f = fx;
                 const int ntf = cp.n[f] - 1;
                 const long long po = (long long)tx + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
-                const float b = cp.b[f][qx - 1], c = cp.c[f][qx - 1];
+                const float cb = cp.b[f][qx - 1], cc = cp.c[f][qx - 1];
                 if (k >= 2) {
-                    float p = b * cp.psi[f][0][po] + c * (hz_xm - hz) / dx;
+                    float p = cb * cp.psi[f][0][po] + cc * (hz_xm - hz) / dx;
                     cp.psi[f][0][po] = p;
-                    ey = ey + tup[t + 3] * p;
+                    ey = ey + gby * p;
                 }
                 if (jy) {
-                    float p = b * cp.psi[f][1][po] + c * (hy - hy_xm) / dx;
+                    float p = cb * cp.psi[f][1][po] + cc * (hy - hy_xm) / dx;
                     cp.psi[f][1][po] = p;
-                    ez = ez + tup[t + 5] * p;
+                    ez = ez + gbz * p;
                 }
             }
-            if (fy >= 0) {                                   // psi_Exy, psi_Ezy
+            if ((C & kCy) && fy >= 0) {                      // psi_Exy, psi_Ezy
                 const int f = 2 + fy;
                 const int ntf = cp.n[f] - 1;
                 const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)ty + (long long)ntf * (k - 1));
-                const float b = cp.b[f][qy - 1], c = cp.c[f][qy - 1];
+                const float cb = cp.b[f][qy - 1], cc = cp.c[f][qy - 1];
                 if (k >= 2) {
-                    float p = b * cp.psi[f][0][po] + c * (hz - hz_ym) / dy;
+                    float p = cb * cp.psi[f][0][po] + cc * (hz - hz_ym) / dy;
                     cp.psi[f][0][po] = p;
-                    ex = ex + tup[t + 1] * p;
+                    ex = ex + gbx * p;
                 }
                 if (ix) {
-                    float p = b * cp.psi[f][1][po] + c * (hx_ym - hx) / dy;
+                    float p = cb * cp.psi[f][1][po] + cc * (hx_ym - hx) / dy;
                     cp.psi[f][1][po] = p;
-                    ez = ez + tup[t + 5] * p;
+                    ez = ez + gbz * p;
                 }
             }
-            int qz, tz;
-            const int fz = slab_e(k, g.NZ, cp.n[4], cp.n[5], qz, tz);
-            if (fz >= 0) {                                   // psi_Exz, psi_Eyz
+            int qz = 0, tz = 0;
+            const int fz = (C & kCz) ? slab_e(k, g.NZ, cp.n[4], cp.n[5], qz, tz) : -1;
+            if ((C & kCz) && fz >= 0) {                      // psi_Exz, psi_Eyz
                 const int f = 4 + fz;
                 const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * tz);
-                const float b = cp.b[f][qz - 1], c = cp.c[f][qz - 1];
+                const float cb = cp.b[f][qz - 1], cc = cp.c[f][qz - 1];
                 if (jy) {
-                    float p = b * cp.psi[f][0][po] + c * (hy_km - hy) / dz;
+                    float p = cb * cp.psi[f][0][po] + cc * (hy_km - hy) / dz;
                     cp.psi[f][0][po] = p;
-                    ex = ex + tup[t + 1] * p;
+                    ex = ex + gbx * p;
                 }
                 if (ix) {
-                    float p = b * cp.psi[f][1][po] + c * (hx - hx_km) / dz;
+                    float p = cb * cp.psi[f][1][po] + cc * (hx - hx_km) / dz;
                     cp.psi[f][1][po] = p;
-                    ey = ey + tup[t + 3] * p;
+                    ey = ey + gby * p;
                 }
             }
         }
-        if (ex_on) Ex[o] = ex;
-        if (ey_on) Ey[o] = ey;
-        if (ez_on) Ez[o] = ez;
+        if (ex_on) st_once(ex, &Ex[o]);
+        if (ey_on) st_once(ey, &Ey[o]);
+        if (ez_on) st_once(ez, &Ez[o]);
         hx_km = hx;
         hy_km = hy;
         o += g.plane;

@@ -132,6 +132,11 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         if (d.is_mirror) throw Error(5, "infinite ground plane (IGP) decks are not supported by libaem");
         for (auto& p : d.ports)
             if (p.gridded) throw Error(5, "gridded ports are not supported by libaem");
+        // the non-SPICE reference build reads the count and ignores it
+        // (f90:1195-1286 are compiled only with use_spice_version); a deck that
+        // asks for SPICE ports is refused here rather than run without them
+        if (d.num_spice_ports > 0)
+            throw Error(5, "SPICE ports need the ngspice co-simulation build, which libaem does not provide");
         if (d.pulse_type != 1 && d.pulse_type != 2)
             throw Error(1, "ERROR: pulse type must be 1 (Gaussian) or 2 (differentiated Gaussian)");
 

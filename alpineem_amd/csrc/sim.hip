@@ -93,6 +93,57 @@ bool cpml_fusable(const aem_plan_view& v) {
     return true;
 }
 
+// One bulk launch: a box of cells and the slab tests compiled into its kernel.
+struct Region {
+    Box b;
+    int c;
+};
+
+constexpr int kPlanes = 4;   // z planes marched by one thread of a bulk kernel
+
+// Disjoint boxes covering the updated cells [1,NX-1]x[1,NY-1]x[1,NZ-1].
+// Without fusion: one box.  With fusion: the core outside every CPML slab
+// (x range aligned to 32-cell blocks), then the x-edge columns, the y slabs
+// and the z slabs.  H slab layers are 1..n-1 and N+1-n..N-1, E layers 2..n
+// and N+1-n..N-1, so one split serves both phases.
+std::vector<Region> bulk_regions(const aem_plan_view& v, bool fuse) {
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    auto box = [&](int a0, int a1, int b0, int b1, int c0, int c1, int c) {
+        return Region{Box{a0, b0, c0, a1 - a0 + 1, b1 - b0 + 1, c1 - c0 + 1, kPlanes}, c};
+    };
+    std::vector<Region> out;
+    if (!fuse) {
+        out.push_back(box(1, N[0] - 1, 1, N[1] - 1, 1, N[2] - 1, 0));
+        return out;
+    }
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = v.npml[2 * a] > 0 ? v.npml[2 * a] + 1 : 1;
+        hi[a] = v.npml[2 * a + 1] > 0 ? N[a] - v.npml[2 * a + 1] : N[a] - 1;
+    }
+    lo[0] = 1 + 32 * ((lo[0] - 1 + 31) / 32);
+    if (hi[0] < N[0] - 1) hi[0] = 32 * (hi[0] / 32);
+    std::vector<std::pair<int, int>> edge[3];
+    for (int a = 0; a < 3; ++a) {
+        if (lo[a] > hi[a]) {
+            edge[a].push_back({1, N[a] - 1});
+            continue;
+        }
+        if (lo[a] > 1) edge[a].push_back({1, lo[a] - 1});
+        if (hi[a] < N[a] - 1) edge[a].push_back({hi[a] + 1, N[a] - 1});
+    }
+    const bool cx = lo[0] <= hi[0], cy = lo[1] <= hi[1], cz = lo[2] <= hi[2];
+    if (cx && cy && cz) out.push_back(box(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 0));
+    if (cy && cz)
+        for (auto e : edge[0]) out.push_back(box(e.first, e.second, lo[1], hi[1], lo[2], hi[2], kCx));
+    if (cz)
+        for (auto e : edge[1]) out.push_back(box(1, N[0] - 1, e.first, e.second, lo[2], hi[2], kCx | kCy));
+    for (auto e : edge[2]) out.push_back(box(1, N[0] - 1, 1, N[1] - 1, e.first, e.second, kCx | kCy | kCz));
+    return out;
+}
+
+inline dim3 region_grid(const Box& b) { return dim3(cdiv(b.ni, 32), cdiv(b.nj, 8), cdiv(b.nk, b.kc)); }
+
 }  // namespace
 
 struct aem_sim {
@@ -122,8 +173,8 @@ struct aem_sim {
     float *d_reflf = nullptr, *d_transf = nullptr, *d_ffpc = nullptr, *d_scratch = nullptr;
     Tfsf tf{};
     Pulse pu{};
-    int kchunk = 16;
-    bool fuse_cpml = false;   // CPML corrections folded into the bulk kernels
+    bool fuse_cpml = false;        // CPML corrections folded into the bulk kernels
+    std::vector<Region> regions;   // bulk launches of each phase; [0] is the core when fused
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -338,11 +389,8 @@ void aem_sim::create() {
     d_ffpc = zeros<float>(T);
     d_incident = zeros<float>(T);
 
-    // z-chunk of the bulk kernels: enough work-groups to fill 256 CUs
-    const long long bxy = (long long)cdiv(v.NX - 1, 32) * cdiv(v.NY - 1, 8);
-    long long nch = std::max(1LL, std::min<long long>(v.NZ - 1, (4096 + bxy - 1) / bxy));
-    kchunk = (int)cdiv(v.NZ - 1, nch);
     fuse_cpml = cpml_fusable(v) && std::getenv("AEM_NO_CPML_FUSION") == nullptr;
+    regions = bulk_regions(v, fuse_cpml);
     HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -352,18 +400,37 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
     const IdxT* idx = static_cast<const IdxT*>(d_idx);
     float *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
     const dim3 bb(32, 8, 1);
-    const dim3 gb(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8), cdiv(v.NZ - 1, kchunk));
     const int T = v.time_steps;
+    auto h_bulk = [&](const Region& r) {
+        const dim3 gr = region_grid(r.b);
+#define AEM_H_BULK(C) k_h_bulk<C><<<gr, bb, 0, st>>>(g, r.b, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], \
+                                                     v.DB, cpH, v.dx, v.dy, v.dz)
+        switch (r.c) {
+            case 0: AEM_H_BULK(0); break;
+            case kCx: AEM_H_BULK(kCx); break;
+            case kCx | kCy: AEM_H_BULK(kCx | kCy); break;
+            default: AEM_H_BULK(kCx | kCy | kCz); break;
+        }
+#undef AEM_H_BULK
+    };
+    auto e_bulk = [&](const Region& r) {
+        const dim3 gr = region_grid(r.b);
+#define AEM_E_BULK(C) k_e_bulk<IdxT, C><<<gr, bb, 0, st>>>(g, r.b, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], \
+                                                           d_den[1], d_den[2], cpE, v.dx, v.dy, v.dz)
+        switch (r.c) {
+            case 0: AEM_E_BULK(0); break;
+            case kCx: AEM_E_BULK(kCx); break;
+            case kCx | kCy: AEM_E_BULK(kCx | kCy); break;
+            default: AEM_E_BULK(kCx | kCy | kCz); break;
+        }
+#undef AEM_E_BULK
+    };
 
-    // ---- H phase ----
+    // ---- H phase ----  (marks: [0,1] the core bulk launch, [1,2] the rest of H)
     if (mark) HIPCHK(hipEventRecord(mark[0], st));
-    if (fuse_cpml)
-        k_h_bulk<true><<<gb, bb, 0, st>>>(g, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], v.DB, kchunk, cpH,
-                                          v.dx, v.dy, v.dz);
-    else
-        k_h_bulk<false><<<gb, bb, 0, st>>>(g, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], v.DB, kchunk, cpH,
-                                           v.dx, v.dy, v.dz);
+    h_bulk(regions[0]);
     if (mark) HIPCHK(hipEventRecord(mark[1], st));
+    for (size_t q = 1; q < regions.size(); ++q) h_bulk(regions[q]);
     const float dd[3] = {v.dx, v.dy, v.dz};
     for (int a = 0; a < 3; ++a)
         if (sh[a].np > 0) {
@@ -391,15 +458,11 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
     if (v.pbc[1]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 1, Hz, Hx);
     if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, st>>>(g, 2, Hy, Hx);
 
-    // ---- E phase ----
+    // ---- E phase ----  (marks: [2,3] the core bulk launch, [3,4] the rest of E)
     if (mark) HIPCHK(hipEventRecord(mark[2], st));
-    if (fuse_cpml)
-        k_e_bulk<IdxT, true><<<gb, bb, 0, st>>>(g, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], d_den[1], d_den[2],
-                                                kchunk, cpE, v.dx, v.dy, v.dz);
-    else
-        k_e_bulk<IdxT, false><<<gb, bb, 0, st>>>(g, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], d_den[1], d_den[2],
-                                                 kchunk, cpE, v.dx, v.dy, v.dz);
+    e_bulk(regions[0]);
     if (mark) HIPCHK(hipEventRecord(mark[3], st));
+    for (size_t q = 1; q < regions.size(); ++q) e_bulk(regions[q]);
     for (int a = 0; a < 3; ++a)
         if (sh[a].np > 0) {
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
@@ -571,6 +634,12 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
 }
 
 int64_t aem_steps_done(const aem_sim* s) { return s ? s->steps_done : -1; }
+
+int64_t aem_core_cells(const aem_sim* s) {
+    if (!s || s->regions.empty()) return -1;
+    const Box& b = s->regions[0].b;
+    return (int64_t)b.ni * b.nj * b.nk;
+}
 
 float aem_last_run_ms(const aem_sim* s) { return s ? s->last_ms : 0.f; }
 

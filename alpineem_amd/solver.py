@@ -97,10 +97,17 @@ class Simulation:
 
     def time_phases(self, steps: int) -> np.ndarray:
         """Advance ``steps`` eagerly with HIP events between the step phases;
-        returns ms per step for [H bulk, H side, E bulk, E side, outputs]."""
+        returns ms per step for [H core bulk launch, rest of H, E core bulk
+        launch, rest of E, outputs]."""
         ms = (C.c_float * 5)()
         check(lib().aem_time_phases(self._h, int(steps), ms, 5))
         return np.array(list(ms), np.float64)
+
+    @property
+    def core_cells(self) -> int:
+        """Cells updated by the core bulk launch (phases 0 and 2 of
+        :meth:`time_phases`)."""
+        return int(lib().aem_core_cells(self._h))
 
     @property
     def steps_done(self) -> int:

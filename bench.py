@@ -122,17 +122,18 @@ def main() -> int:
     print(f"[bench] timed {K} steps: {ms_step:.4f} ms/step, {value:.2f} Gcell/s", file=sys.stderr, flush=True)
 
     ph = sim.time_phases(P)                       # ms per step per phase, HIP events on the sim stream
-    names = ["h_bulk", "h_side", "e_bulk", "e_side", "outputs"]
+    names = ["h_core", "h_rest", "e_core", "e_rest", "outputs"]
     print("[bench] phases (ms): " + ", ".join(f"{a}={b:.4f}" for a, b in zip(names, ph)), file=sys.stderr, flush=True)
-    interior = (n) ** 3
+    core = sim.core_cells                         # cells of the core bulk launch timed in phases 0 and 2
     if ph[2] >= ph[0]:
-        dom, bpc, t_ms = "k_e_bulk", BYTES_E, ph[2]
+        dom, bpc, t_ms = "k_e_bulk<uint8_t, 0>", BYTES_E, ph[2]
     else:
-        dom, bpc, t_ms = "k_h_bulk", BYTES_H, ph[0]
-    achieved = bpc * interior / (t_ms * 1e-3) / 1e9
+        dom, bpc, t_ms = "k_h_bulk<0>", BYTES_H, ph[0]
+    achieved = bpc * core / (t_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
-                "bytes_per_cell": bpc, "step_frac": round(value * BYTES_STEP / PEAK_HBM_GBS, 4)}
+                "bytes_per_cell": bpc, "cells_per_launch": core, "launch_ms": round(float(t_ms), 5),
+                "step_frac": round(value * BYTES_STEP / PEAK_HBM_GBS, 4)}
     traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(traffic_file):
         try:

@@ -150,9 +150,16 @@ float aem_last_run_ms(const aem_sim* sim);
 /* Instrumentation with no counterpart in the reference (its only meter is
  * the Mcells/s line, fdtd_solver.f90:8315): advance n_steps eagerly with HIP
  * events on the sim's stream between the five phases of a step and return
- * the average ms per phase in ms[0..4]: H bulk, H side kernels (sheets,
- * TF/SF, ports, CPML, PBC), E bulk, E side kernels, per-step outputs. */
+ * the average ms per phase in ms[0..4]: the core H bulk launch, the rest of
+ * H (slab boxes, sheets, TF/SF, ports, separate CPML, PBC), the core E bulk
+ * launch, the rest of E, per-step outputs. */
 aem_status aem_time_phases(aem_sim* sim, int64_t n_steps, float* ms, int32_t n_ms);
+
+/* Instrumentation, no counterpart in the reference: the number of cells of
+ * the core bulk launch timed as phases 0 and 2 by aem_time_phases (the box
+ * free of CPML slabs when the CPML correction is fused, else the whole
+ * updated grid).  -1 for a null handle. */
+int64_t aem_core_cells(const aem_sim* sim);
 
 #ifdef __cplusplus
 }
