@@ -56,12 +56,12 @@ __device__ __forceinline__ int slab_e(int i, int N, int n1, int n2, int& q, int&
 }
 
 // ---------------------------------------------------------------------------
-// Bulk launches cover boxes of cells.  With CPML fusion the domain is split
-// into disjoint boxes: the core (no slab cell; the CPML code is compiled
-// out), the x-edge block columns, the y slabs and the z slabs; each box's
-// kernel compiles in only the slab tests it needs (C = kCx|kCy|kCz), so the
-// core keeps the register budget of the plain update.  The x core is aligned
-// to 32-cell blocks so every wave row is one 128-byte line.
+// Bulk launches cover sets of boxes of cells.  With CPML fusion the domain is
+// split into a core box (every x, y and z outside the y/z slabs; only the
+// x-slab test is compiled in, which costs no occupancy) and one launch over
+// the y-slab and z-slab boxes with every slab test compiled in
+// (C = kCx|kCy|kCz).  A launch is a 1-D grid; each box owns a contiguous
+// range of work-groups, ordered x fastest, then y, then z chunks.
 // ---------------------------------------------------------------------------
 enum : int { kCx = 1, kCy = 2, kCz = 4 };
 
@@ -70,6 +70,35 @@ struct Box {
     int ni, nj, nk;   // extents
     int kc;           // planes marched by one thread
 };
+
+constexpr int kMaxBoxes = 4;
+
+struct BoxSet {
+    Box b[kMaxBoxes];
+    int first[kMaxBoxes + 1];   // first work-group of each box; first[n] = total
+    int n;
+};
+
+// Box of this work-group and its (x, y, z-chunk) block coordinates.  The
+// selection uses constant indices only, so the box table stays in kernel
+// arguments (scalar registers) instead of a private copy.
+__device__ __forceinline__ Box box_of(const BoxSet& s, int& bx, int& by, int& bz) {
+    const int g = (int)blockIdx.x;
+    Box b = s.b[0];
+    int f = s.first[0];
+#pragma unroll
+    for (int q = 1; q < kMaxBoxes; ++q)
+        if (q < s.n && g >= s.first[q]) {
+            b = s.b[q];
+            f = s.first[q];
+        }
+    const int nbx = (b.ni + 31) / 32, nby = (b.nj + 7) / 8;
+    const int l = g - f;
+    bx = l % nbx;
+    by = (l / nbx) % nby;
+    bz = l / (nbx * nby);
+    return b;
+}
 
 // Non-temporal access for the cell's own field values, which are read and
 // written exactly once per launch (the neighbour reads stay cached).
@@ -85,16 +114,18 @@ __device__ __forceinline__ void st_once(T v, T* p) { __builtin_nontemporal_store
 // order (x, then y, then z slab), reusing the loaded neighbours.
 // ---------------------------------------------------------------------------
 template <int C>
-__global__ void __launch_bounds__(256) k_h_bulk(Grid g, Box b, float* __restrict__ Hx, float* __restrict__ Hy,
+__global__ void __launch_bounds__(256) k_h_bulk(Grid g, BoxSet bs, float* __restrict__ Hx, float* __restrict__ Hy,
                                                 float* __restrict__ Hz, const float* __restrict__ Ex,
                                                 const float* __restrict__ Ey, const float* __restrict__ Ez,
                                                 const float* __restrict__ dhx, const float* __restrict__ dhy,
                                                 const float* __restrict__ dhz, float DB, Cpml cp, float dx, float dy,
                                                 float dz) {
-    const int i = b.i0 + (int)(blockIdx.x * 32 + threadIdx.x);
-    const int j = b.j0 + (int)(blockIdx.y * 8 + threadIdx.y);
+    int bx, by, bz;
+    const Box b = box_of(bs, bx, by, bz);
+    const int i = b.i0 + bx * 32 + (int)threadIdx.x;
+    const int j = b.j0 + by * 8 + (int)threadIdx.y;
     if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
-    const int k0 = b.k0 + (int)blockIdx.z * b.kc;
+    const int k0 = b.k0 + bz * b.kc;
     const int k1 = min(k0 + b.kc, b.k0 + b.nk) - 1;
     const float dx_i = dhx[i - 1], dy_j = dhy[j - 1];
     int qx = 0, qy = 0;
@@ -168,17 +199,19 @@ __global__ void __launch_bounds__(256) k_h_bulk(Grid g, Box b, float* __restrict
 // Slab cells of the box append their CPML corrections as in k_h_bulk.
 // ---------------------------------------------------------------------------
 template <typename IdxT, int C>
-__global__ void __launch_bounds__(256) k_e_bulk(Grid g, Box b, float* __restrict__ Ex, float* __restrict__ Ey,
+__global__ void __launch_bounds__(256) k_e_bulk(Grid g, BoxSet bs, float* __restrict__ Ex, float* __restrict__ Ey,
                                                 float* __restrict__ Ez, const float* __restrict__ Hx,
                                                 const float* __restrict__ Hy, const float* __restrict__ Hz,
                                                 const IdxT* __restrict__ idx, const float* __restrict__ tup,
                                                 const float* __restrict__ dex, const float* __restrict__ dey,
                                                 const float* __restrict__ dez, Cpml cp, float dx, float dy,
                                                 float dz) {
-    const int i = b.i0 + (int)(blockIdx.x * 32 + threadIdx.x);
-    const int j = b.j0 + (int)(blockIdx.y * 8 + threadIdx.y);
+    int bx, by, bz;
+    const Box b = box_of(bs, bx, by, bz);
+    const int i = b.i0 + bx * 32 + (int)threadIdx.x;
+    const int j = b.j0 + by * 8 + (int)threadIdx.y;
     if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
-    const int k0 = b.k0 + (int)blockIdx.z * b.kc;
+    const int k0 = b.k0 + bz * b.kc;
     const int k1 = min(k0 + b.kc, b.k0 + b.nk) - 1;
     const float dx_i = dex[i - 1], dy_j = dey[j - 1];
     const bool ix = i >= 2, jy = j >= 2;
@@ -191,21 +224,28 @@ __global__ void __launch_bounds__(256) k_e_bulk(Grid g, Box b, float* __restrict
         hx_km = Hx[o - g.plane];
         hy_km = Hy[o - g.plane];
     }
+    // i-1 / j-1 neighbours through clamped offsets: at i = 1 (j = 1) the load
+    // reads the cell itself, and every component that would use it (the
+    // outer PEC tangential E) is left unwritten, so all loads of an
+    // iteration issue together without branches.
+    const long long dxm = ix ? 1 : 0, dym = jy ? g.PX : 0;
+    int t_next = (int)idx[o];   // coefficient index, loaded one plane ahead
     for (int k = k0; k <= k1; ++k) {
         const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
-        const int t = (int)idx[o] * 6;
+        const int t = t_next * 6;
+        t_next = (int)idx[o + (k < k1 ? g.plane : 0)];
         const float dz_k = dez[k - 1];
         const bool ex_on = jy && k >= 2, ey_on = ix && k >= 2, ez_on = ix && jy;
-        const float hz_ym = jy ? Hz[o - g.PX] : 0.f;
-        const float hz_xm = ix ? Hz[o - 1] : 0.f;
-        const float hy_xm = ix ? Hy[o - 1] : 0.f;
-        const float hx_ym = jy ? Hx[o - g.PX] : 0.f;
+        const float hz_ym = Hz[o - dym];
+        const float hz_xm = Hz[o - dxm];
+        const float hy_xm = Hy[o - dxm];
+        const float hx_ym = Hx[o - dym];
+        const float ex0 = ld_once(&Ex[o]), ey0 = ld_once(&Ey[o]), ez0 = ld_once(&Ez[o]);
         const float gax = tup[t + 0], gbx = tup[t + 1], gay = tup[t + 2], gby = tup[t + 3], gaz = tup[t + 4],
                     gbz = tup[t + 5];
-        float ex = 0.f, ey = 0.f, ez = 0.f;
-        if (ex_on) ex = gax * ld_once(&Ex[o]) + gbx * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
-        if (ey_on) ey = gay * ld_once(&Ey[o]) + gby * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
-        if (ez_on) ez = gaz * ld_once(&Ez[o]) + gbz * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
+        float ex = gax * ex0 + gbx * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
+        float ey = gay * ey0 + gby * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
+        float ez = gaz * ez0 + gbz * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
         if (C) {
             if ((C & kCx) && fx >= 0) {                      // psi_Eyx, psi_Ezx
                 const int f = fx;
@@ -257,9 +297,11 @@ __global__ void __launch_bounds__(256) k_e_bulk(Grid g, Box b, float* __restrict
                 }
             }
         }
-        if (ex_on) st_once(ex, &Ex[o]);
-        if (ey_on) st_once(ey, &Ey[o]);
-        if (ez_on) st_once(ez, &Ez[o]);
+        // components outside the reference's update ranges get their own
+        // value back; storing every component keeps the loads unconditional
+        st_once(ex_on ? ex : ex0, &Ex[o]);
+        st_once(ey_on ? ey : ey0, &Ey[o]);
+        st_once(ez_on ? ez : ez0, &Ez[o]);
         hx_km = hx;
         hy_km = hy;
         o += g.plane;
@@ -1155,6 +1197,55 @@ __global__ void k_plane_norm(int T, const double* refl, const double* trans, flo
     if (n >= T) return;
     rf[n] = (float)refl[n] / den;
     tf[n] = (float)trans[n] / den;
+}
+
+// ---------------------------------------------------------------------------
+// Discrete Fourier transform of one output series at arbitrary frequencies,
+// in the convention of the reference's post-processor (numpy's FFT,
+// utility_scripts/post_processor.py:191-215):  X(f) = sum_n x[n] e^{-2 pi i f n dt},
+// n = 0..T-1.  Zero padding adds nothing to the sum, so X at f = k/(P dt)
+// equals bin k of the FFT of the series padded to P samples.
+//
+// FP64 throughout.  One work-group per frequency; thread t sums samples
+// t, t+256, ... with Kahan compensation, and the 256 partial sums are added
+// in a fixed tree order, so the result does not depend on scheduling.  The
+// phase f*dt*n is reduced to [0,1) before sincospi, which keeps it exact to
+// FP64 rounding for any series length.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_dft(const float* __restrict__ x, int n, const double* __restrict__ fdt,
+                                             double* __restrict__ out) {
+    __shared__ double sr[256], si[256];
+    const double a = fdt[blockIdx.x];
+    double re = 0.0, im = 0.0, cre = 0.0, cim = 0.0;
+    for (int t = threadIdx.x; t < n; t += 256) {
+        double p = a * (double)t;
+        p = p - floor(p);
+        double s, c;
+        sincospi(2.0 * p, &s, &c);
+        const double v = (double)x[t];
+        double y = v * c - cre;
+        double u = re + y;
+        cre = (u - re) - y;
+        re = u;
+        y = -(v * s) - cim;
+        u = im + y;
+        cim = (u - im) - y;
+        im = u;
+    }
+    sr[threadIdx.x] = re - cre;
+    si[threadIdx.x] = im - cim;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sr[threadIdx.x] += sr[threadIdx.x + w];
+            si[threadIdx.x] += si[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sr[0];
+        out[2 * blockIdx.x + 1] = si[0];
+    }
 }
 
 }  // namespace aemk

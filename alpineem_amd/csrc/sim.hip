@@ -93,38 +93,56 @@ bool cpml_fusable(const aem_plan_view& v) {
     return true;
 }
 
-// One bulk launch: a box of cells and the slab tests compiled into its kernel.
-struct Region {
-    Box b;
+// One bulk launch: a set of boxes and the slab tests compiled into its kernel.
+struct Launch {
+    BoxSet bs;
     int c;
+    unsigned blocks() const { return (unsigned)bs.first[bs.n]; }
+    long long cells() const {
+        long long n = 0;
+        for (int q = 0; q < bs.n; ++q) n += (long long)bs.b[q].ni * bs.b[q].nj * bs.b[q].nk;
+        return n;
+    }
 };
 
 constexpr int kPlanes = 4;   // z planes marched by one thread of a bulk kernel
 
-// Disjoint boxes covering the updated cells [1,NX-1]x[1,NY-1]x[1,NZ-1].
-// Without fusion: one box.  With fusion: the core outside every CPML slab
-// (x range aligned to 32-cell blocks), then the x-edge columns, the y slabs
-// and the z slabs.  H slab layers are 1..n-1 and N+1-n..N-1, E layers 2..n
-// and N+1-n..N-1, so one split serves both phases.
-std::vector<Region> bulk_regions(const aem_plan_view& v, bool fuse) {
+Launch make_launch(const std::vector<Box>& boxes, int c) {
+    Launch L{};
+    L.c = c;
+    L.bs.n = (int)boxes.size();
+    long long acc = 0;
+    for (int q = 0; q < L.bs.n; ++q) {
+        const Box& b = boxes[q];
+        L.bs.b[q] = b;
+        L.bs.first[q] = (int)acc;
+        acc += (long long)cdiv(b.ni, 32) * cdiv(b.nj, 8) * cdiv(b.nk, b.kc);
+    }
+    if (acc > 0x7fffffffLL) throw aem::Error(AEM_ERR_STATE, "grid too large for one bulk launch");
+    L.bs.first[L.bs.n] = (int)acc;
+    return L;
+}
+
+// Launches covering the updated cells [1,NX-1]x[1,NY-1]x[1,NZ-1] with
+// disjoint boxes.  Without fusion: one box.  With fusion: the core (all x,
+// y and z outside the y/z slabs) with the x-slab test, then one launch over
+// the y-slab and z-slab boxes.  H slab layers are 1..n-1 and N+1-n..N-1, E
+// layers 2..n and N+1-n..N-1, so one split serves both phases.
+std::vector<Launch> bulk_launches(const aem_plan_view& v, bool fuse) {
     const int N[3] = {v.NX, v.NY, v.NZ};
-    auto box = [&](int a0, int a1, int b0, int b1, int c0, int c1, int c) {
-        return Region{Box{a0, b0, c0, a1 - a0 + 1, b1 - b0 + 1, c1 - c0 + 1, kPlanes}, c};
+    auto box = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
+        return Box{a0, b0, c0, a1 - a0 + 1, b1 - b0 + 1, c1 - c0 + 1, kPlanes};
     };
-    std::vector<Region> out;
+    std::vector<Launch> out;
     if (!fuse) {
-        out.push_back(box(1, N[0] - 1, 1, N[1] - 1, 1, N[2] - 1, 0));
+        out.push_back(make_launch({box(1, N[0] - 1, 1, N[1] - 1, 1, N[2] - 1)}, 0));
         return out;
     }
     int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
+    std::vector<std::pair<int, int>> edge[3];
+    for (int a = 1; a < 3; ++a) {
         lo[a] = v.npml[2 * a] > 0 ? v.npml[2 * a] + 1 : 1;
         hi[a] = v.npml[2 * a + 1] > 0 ? N[a] - v.npml[2 * a + 1] : N[a] - 1;
-    }
-    lo[0] = 1 + 32 * ((lo[0] - 1 + 31) / 32);
-    if (hi[0] < N[0] - 1) hi[0] = 32 * (hi[0] / 32);
-    std::vector<std::pair<int, int>> edge[3];
-    for (int a = 0; a < 3; ++a) {
         if (lo[a] > hi[a]) {
             edge[a].push_back({1, N[a] - 1});
             continue;
@@ -132,17 +150,15 @@ std::vector<Region> bulk_regions(const aem_plan_view& v, bool fuse) {
         if (lo[a] > 1) edge[a].push_back({1, lo[a] - 1});
         if (hi[a] < N[a] - 1) edge[a].push_back({hi[a] + 1, N[a] - 1});
     }
-    const bool cx = lo[0] <= hi[0], cy = lo[1] <= hi[1], cz = lo[2] <= hi[2];
-    if (cx && cy && cz) out.push_back(box(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 0));
-    if (cy && cz)
-        for (auto e : edge[0]) out.push_back(box(e.first, e.second, lo[1], hi[1], lo[2], hi[2], kCx));
+    const bool cy = lo[1] <= hi[1], cz = lo[2] <= hi[2];
+    if (cy && cz) out.push_back(make_launch({box(1, N[0] - 1, lo[1], hi[1], lo[2], hi[2])}, kCx));
+    std::vector<Box> slabs;
     if (cz)
-        for (auto e : edge[1]) out.push_back(box(1, N[0] - 1, e.first, e.second, lo[2], hi[2], kCx | kCy));
-    for (auto e : edge[2]) out.push_back(box(1, N[0] - 1, 1, N[1] - 1, e.first, e.second, kCx | kCy | kCz));
+        for (auto e : edge[1]) slabs.push_back(box(1, N[0] - 1, e.first, e.second, lo[2], hi[2]));
+    for (auto e : edge[2]) slabs.push_back(box(1, N[0] - 1, 1, N[1] - 1, e.first, e.second));
+    if (!slabs.empty()) out.push_back(make_launch(slabs, kCx | kCy | kCz));
     return out;
 }
-
-inline dim3 region_grid(const Box& b) { return dim3(cdiv(b.ni, 32), cdiv(b.nj, 8), cdiv(b.nk, b.kc)); }
 
 }  // namespace
 
@@ -174,12 +190,14 @@ struct aem_sim {
     Tfsf tf{};
     Pulse pu{};
     bool fuse_cpml = false;        // CPML corrections folded into the bulk kernels
-    std::vector<Region> regions;   // bulk launches of each phase; [0] is the core when fused
+    std::vector<Launch> bulk;      // bulk launches of each phase; [0] is the core when fused
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     bool finalized = false;
+    double* d_dft = nullptr;   // aem_get_dft staging: f*dt, then (re,im) pairs
+    int dft_cap = 0;
 
     ~aem_sim() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -187,6 +205,7 @@ struct aem_sim {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (st) (void)hipStreamSynchronize(st);
+        if (d_dft) (void)hipFree(d_dft);
         for (void* p : owned) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
         delete owned_plan;
@@ -390,7 +409,7 @@ void aem_sim::create() {
     d_incident = zeros<float>(T);
 
     fuse_cpml = cpml_fusable(v) && std::getenv("AEM_NO_CPML_FUSION") == nullptr;
-    regions = bulk_regions(v, fuse_cpml);
+    bulk = bulk_launches(v, fuse_cpml);
     HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -401,26 +420,22 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
     float *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
     const dim3 bb(32, 8, 1);
     const int T = v.time_steps;
-    auto h_bulk = [&](const Region& r) {
-        const dim3 gr = region_grid(r.b);
-#define AEM_H_BULK(C) k_h_bulk<C><<<gr, bb, 0, st>>>(g, r.b, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], \
-                                                     v.DB, cpH, v.dx, v.dy, v.dz)
-        switch (r.c) {
+    auto h_bulk = [&](const Launch& L) {
+#define AEM_H_BULK(C) k_h_bulk<C><<<L.blocks(), bb, 0, st>>>(g, L.bs, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], \
+                                                             d_den[5], v.DB, cpH, v.dx, v.dy, v.dz)
+        switch (L.c) {
             case 0: AEM_H_BULK(0); break;
             case kCx: AEM_H_BULK(kCx); break;
-            case kCx | kCy: AEM_H_BULK(kCx | kCy); break;
             default: AEM_H_BULK(kCx | kCy | kCz); break;
         }
 #undef AEM_H_BULK
     };
-    auto e_bulk = [&](const Region& r) {
-        const dim3 gr = region_grid(r.b);
-#define AEM_E_BULK(C) k_e_bulk<IdxT, C><<<gr, bb, 0, st>>>(g, r.b, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, d_den[0], \
-                                                           d_den[1], d_den[2], cpE, v.dx, v.dy, v.dz)
-        switch (r.c) {
+    auto e_bulk = [&](const Launch& L) {
+#define AEM_E_BULK(C) k_e_bulk<IdxT, C><<<L.blocks(), bb, 0, st>>>(g, L.bs, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, \
+                                                                   d_den[0], d_den[1], d_den[2], cpE, v.dx, v.dy, v.dz)
+        switch (L.c) {
             case 0: AEM_E_BULK(0); break;
             case kCx: AEM_E_BULK(kCx); break;
-            case kCx | kCy: AEM_E_BULK(kCx | kCy); break;
             default: AEM_E_BULK(kCx | kCy | kCz); break;
         }
 #undef AEM_E_BULK
@@ -428,9 +443,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
 
     // ---- H phase ----  (marks: [0,1] the core bulk launch, [1,2] the rest of H)
     if (mark) HIPCHK(hipEventRecord(mark[0], st));
-    h_bulk(regions[0]);
+    h_bulk(bulk[0]);
     if (mark) HIPCHK(hipEventRecord(mark[1], st));
-    for (size_t q = 1; q < regions.size(); ++q) h_bulk(regions[q]);
+    for (size_t q = 1; q < bulk.size(); ++q) h_bulk(bulk[q]);
     const float dd[3] = {v.dx, v.dy, v.dz};
     for (int a = 0; a < 3; ++a)
         if (sh[a].np > 0) {
@@ -460,9 +475,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
 
     // ---- E phase ----  (marks: [2,3] the core bulk launch, [3,4] the rest of E)
     if (mark) HIPCHK(hipEventRecord(mark[2], st));
-    e_bulk(regions[0]);
+    e_bulk(bulk[0]);
     if (mark) HIPCHK(hipEventRecord(mark[3], st));
-    for (size_t q = 1; q < regions.size(); ++q) e_bulk(regions[q]);
+    for (size_t q = 1; q < bulk.size(); ++q) e_bulk(bulk[q]);
     for (int a = 0; a < 3; ++a)
         if (sh[a].np > 0) {
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
@@ -636,9 +651,8 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
 int64_t aem_steps_done(const aem_sim* s) { return s ? s->steps_done : -1; }
 
 int64_t aem_core_cells(const aem_sim* s) {
-    if (!s || s->regions.empty()) return -1;
-    const Box& b = s->regions[0].b;
-    return (int64_t)b.ni * b.nj * b.nk;
+    if (!s || s->bulk.empty()) return -1;
+    return (int64_t)s->bulk[0].cells();
 }
 
 float aem_last_run_ms(const aem_sim* s) { return s ? s->last_ms : 0.f; }
@@ -685,43 +699,85 @@ aem_status aem_finalize(aem_sim* s) {
     });
 }
 
+}  // extern "C"
+
+namespace {
+// Device address of an output series (T floats), or null with the error set.
+const float* series_src(const aem_sim* s, int32_t kind, int32_t idx) {
+    const aem_plan_view& v = s->plan->v;
+    const int T = v.time_steps;
+    switch (kind) {
+        case AEM_SERIES_PORT_V:
+        case AEM_SERIES_PORT_I:
+            if (idx < 0 || idx >= v.n_ports) { aem::set_error("port index out of range"); return nullptr; }
+            return (kind == AEM_SERIES_PORT_V ? s->d_Vout : s->d_Iout) + (size_t)idx * T;
+        case AEM_SERIES_E_THETA:
+        case AEM_SERIES_E_PHI:
+            if (idx < 0 || idx >= v.n_angles) { aem::set_error("angle index out of range"); return nullptr; }
+            if (!s->finalized) { aem::set_error("call aem_finalize first"); return nullptr; }
+            return (kind == AEM_SERIES_E_THETA ? s->d_Eth : s->d_Eph) + (size_t)idx * T;
+        case AEM_SERIES_INCIDENT:
+        case AEM_SERIES_REFLECTED:
+        case AEM_SERIES_TRANSMITTED:
+        case AEM_SERIES_FF_PC:
+            if (!s->finalized) { aem::set_error("call aem_finalize first"); return nullptr; }
+            return kind == AEM_SERIES_INCIDENT ? s->d_incident
+                   : kind == AEM_SERIES_REFLECTED ? s->d_reflf
+                   : kind == AEM_SERIES_TRANSMITTED ? s->d_transf
+                                                    : s->d_ffpc;
+        default: aem::set_error("unknown series kind"); return nullptr;
+    }
+}
+}  // namespace
+
+extern "C" {
+
 aem_status aem_get_series(aem_sim* s, int32_t kind, int32_t idx, float* out, int64_t n) {
     if (!s || !out) {
         aem::set_error("aem_get_series: null argument");
         return AEM_ERR_STATE;
     }
-    const aem_plan_view& v = s->plan->v;
-    const int T = v.time_steps;
+    const int T = s->plan->v.time_steps;
     if (n < 0 || n > T) {
         aem::set_error("aem_get_series: length exceeds time_steps");
         return AEM_ERR_STATE;
     }
-    const float* src = nullptr;
-    switch (kind) {
-        case AEM_SERIES_PORT_V:
-        case AEM_SERIES_PORT_I:
-            if (idx < 0 || idx >= v.n_ports) { aem::set_error("port index out of range"); return AEM_ERR_STATE; }
-            src = (kind == AEM_SERIES_PORT_V ? s->d_Vout : s->d_Iout) + (size_t)idx * T;
-            break;
-        case AEM_SERIES_E_THETA:
-        case AEM_SERIES_E_PHI:
-            if (idx < 0 || idx >= v.n_angles) { aem::set_error("angle index out of range"); return AEM_ERR_STATE; }
-            if (!s->finalized) { aem::set_error("call aem_finalize first"); return AEM_ERR_STATE; }
-            src = (kind == AEM_SERIES_E_THETA ? s->d_Eth : s->d_Eph) + (size_t)idx * T;
-            break;
-        case AEM_SERIES_INCIDENT: src = s->d_incident; break;
-        case AEM_SERIES_REFLECTED: src = s->d_reflf; break;
-        case AEM_SERIES_TRANSMITTED: src = s->d_transf; break;
-        case AEM_SERIES_FF_PC: src = s->d_ffpc; break;
-        default: aem::set_error("unknown series kind"); return AEM_ERR_STATE;
-    }
-    if ((kind == AEM_SERIES_INCIDENT || kind == AEM_SERIES_REFLECTED || kind == AEM_SERIES_TRANSMITTED ||
-         kind == AEM_SERIES_FF_PC) && !s->finalized) {
-        aem::set_error("call aem_finalize first");
-        return AEM_ERR_STATE;
-    }
+    const float* src = series_src(s, kind, idx);
+    if (!src) return AEM_ERR_STATE;
     return aem::guard([&]() {
         HIPCHK(hipMemcpyAsync(out, src, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_get_dft(aem_sim* s, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
+                       double* re_im) {
+    if (!s || !freqs_hz || !re_im || n_freq < 0) {
+        aem::set_error("aem_get_dft: bad argument");
+        return AEM_ERR_STATE;
+    }
+    const float* src = series_src(s, kind, idx);
+    if (!src) return AEM_ERR_STATE;
+    if (n_freq == 0) return AEM_OK;
+    const aem_plan_view& v = s->plan->v;
+    // port series hold the steps run so far; the post-loop series all T
+    const int n = (kind == AEM_SERIES_PORT_V || kind == AEM_SERIES_PORT_I) ? (int)s->steps_done : v.time_steps;
+    return aem::guard([&]() {
+        if (n_freq > s->dft_cap) {
+            if (s->d_dft) HIPCHK(hipFree(s->d_dft));
+            s->d_dft = nullptr;
+            s->dft_cap = 0;
+            HIPCHK(hipMalloc(&s->d_dft, (size_t)n_freq * 3 * sizeof(double)));
+            s->dft_cap = n_freq;
+        }
+        std::vector<double> fdt((size_t)n_freq);
+        for (int q = 0; q < n_freq; ++q) fdt[q] = freqs_hz[q] * (double)v.dt;
+        double* d_fdt = s->d_dft;
+        double* d_out = s->d_dft + n_freq;
+        HIPCHK(hipMemcpyAsync(d_fdt, fdt.data(), fdt.size() * sizeof(double), hipMemcpyHostToDevice, s->st));
+        k_dft<<<n_freq, 256, 0, s->st>>>(src, n, d_fdt, d_out);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(re_im, d_out, (size_t)n_freq * 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
     });
 }

@@ -123,6 +123,14 @@ class Simulation:
         check(lib().aem_get_series(self._h, SERIES[kind], idx, out.ctypes.data, n))
         return out
 
+    def dft(self, kind: str, freqs_hz, idx: int = 0) -> np.ndarray:
+        """Complex spectrum of an output series at ``freqs_hz``, computed on
+        the device in FP64 (numpy FFT sign and scaling; see aem_get_dft)."""
+        f = np.ascontiguousarray(freqs_hz, np.float64).reshape(-1)
+        out = np.zeros(2 * f.size, np.float64)
+        check(lib().aem_get_dft(self._h, SERIES[kind], idx, f.ctypes.data, f.size, out.ctypes.data))
+        return out[0::2] + 1j * out[1::2]
+
     def field(self, name: str) -> np.ndarray:
         v = self.plan.view
         out = np.zeros(v.ncell, np.float32)

@@ -139,6 +139,19 @@ aem_status aem_get_ntff_raw(aem_sim* sim, int32_t which, double* out, int64_t n)
 /* far field phase correction time (f90:8629-8634) per angle, after finalize. */
 aem_status aem_get_ff_time(aem_sim* sim, float* out, int64_t n);
 
+/* Discrete Fourier transform of an output series (same kinds and
+ * availability as aem_get_series) at n_freq arbitrary frequencies in Hz,
+ * computed on the device in FP64 with Kahan-compensated sums:
+ *   X(f) = sum_{n=0}^{N-1} x[n] exp(-2 pi i f n dt),
+ * the convention of numpy.fft.fft used by the reference's post-processor
+ * (utility_scripts/post_processor.py:191-215), where X at f = k/(P dt)
+ * equals FFT bin k of the series zero-padded to P samples.  N is the steps
+ * run so far for port series and time_steps for the post-loop series.
+ * re_im receives n_freq (re, im) pairs.  Replaces the post-processor's FFT
+ * step of the S-parameter, gain and RCS calculations (:191-266). */
+aem_status aem_get_dft(aem_sim* sim, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
+                       double* re_im);
+
 /* Write data.dat (positional format read by utility_scripts/post_processor.py)
  * and <name>_geometry.bin.  path NULL -> the deck's own file name. */
 aem_status aem_write_outputs(aem_sim* sim, const char* path);
