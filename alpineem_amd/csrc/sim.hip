@@ -116,7 +116,7 @@ Launch make_launch(const std::vector<Box>& boxes, int c) {
         const Box& b = boxes[q];
         L.bs.b[q] = b;
         L.bs.first[q] = (int)acc;
-        acc += (long long)cdiv(b.ni, 32) * cdiv(b.nj, 8) * cdiv(b.nk, b.kc);
+        acc += (long long)cdiv(b.ni, kTX) * cdiv(b.nj, kTY) * cdiv(b.nk, b.kc);
     }
     if (acc > 0x7fffffffLL) throw aem::Error(AEM_ERR_STATE, "grid too large for one bulk launch");
     L.bs.first[L.bs.n] = (int)acc;
@@ -419,9 +419,10 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
     const IdxT* idx = static_cast<const IdxT*>(d_idx);
     float *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
     const dim3 bb(32, 8, 1);
+    const dim3 bt(kTX, kTY, 1);   // bulk tile
     const int T = v.time_steps;
     auto h_bulk = [&](const Launch& L) {
-#define AEM_H_BULK(C) k_h_bulk<C><<<L.blocks(), bb, 0, st>>>(g, L.bs, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], \
+#define AEM_H_BULK(C) k_h_bulk<C><<<L.blocks(), bt, 0, st>>>(g, L.bs, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], \
                                                              d_den[5], v.DB, cpH, v.dx, v.dy, v.dz)
         switch (L.c) {
             case 0: AEM_H_BULK(0); break;
@@ -431,7 +432,7 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
 #undef AEM_H_BULK
     };
     auto e_bulk = [&](const Launch& L) {
-#define AEM_E_BULK(C) k_e_bulk<IdxT, C><<<L.blocks(), bb, 0, st>>>(g, L.bs, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, \
+#define AEM_E_BULK(C) k_e_bulk<IdxT, C><<<L.blocks(), bt, 0, st>>>(g, L.bs, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, \
                                                                    d_den[0], d_den[1], d_den[2], cpE, v.dx, v.dy, v.dz)
         switch (L.c) {
             case 0: AEM_E_BULK(0); break;
