@@ -19,7 +19,8 @@ from tests.oracle_ffi import Oracle
 
 pytestmark = pytest.mark.gpu
 
-DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini"]
+DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini",
+         "sheets_xyz"]
 TOL = 1e-5
 
 
@@ -89,7 +90,7 @@ def test_series_match_reference(runs, name):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("name", ["dipole36", "antenna_x", "c1_mini"])
+@pytest.mark.parametrize("name", ["dipole36", "antenna_x", "c1_mini", "sheets_xyz"])
 def test_fields_bit_exact_vs_oracle(runs, name):
     """Antenna decks (no TF/SF plane wave): every field value left in device
     memory equals the oracle's, which equals the serial reference's."""
@@ -114,7 +115,7 @@ def test_fields_close_to_oracle(runs, name):
         assert r <= 5e-5, (comp, r)
 
 
-@pytest.mark.parametrize("name", ["dipole36", "c1_mini", "antenna_x"])
+@pytest.mark.parametrize("name", ["dipole36", "c1_mini", "antenna_x", "sheets_xyz"])
 def test_antenna_decks_bit_exact(runs, name):
     """No plane-wave source: port voltage, incident wave and far field (with
     the reference's FP32 retarded-time binning order) are the reference's,

@@ -16,7 +16,8 @@ import pytest
 from alpineem_amd.solver import Plan
 from tests.oracle_ffi import Oracle
 
-DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini"]
+DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini",
+         "sheets_xyz"]
 
 
 def _run(golden_dir, name, fp64=False):
@@ -55,7 +56,7 @@ def test_oracle_bit_exact_vs_reference(golden_dir, name):
                                               err_msg=f"video comp {c} step {(f + 1) * stride}")
 
 
-@pytest.mark.parametrize("name", ["dipole36", "master_ports", "sphere_oblique"])
+@pytest.mark.parametrize("name", ["dipole36", "master_ports", "sphere_oblique", "sheets_xyz"])
 def test_geometry_file_matches_reference(golden_dir, name, tmp_path):
     plan, _, g = _run(golden_dir, name)
     out = tmp_path / "geo.bin"
