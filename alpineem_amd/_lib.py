@@ -30,6 +30,12 @@ SIGNATURES = [
     ("aem_plan_write_geometry", C.c_int, [C.c_void_p, C.c_char_p]),
     ("aem_plan_destroy", None, [C.c_void_p]),
     ("aem_sim_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("aem_sim_create_ex", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("aem_batch_create", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    ("aem_batch_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
+    ("aem_batch_last_run_ms", C.c_float, [C.c_void_p]),
+    ("aem_batch_size", C.c_int32, [C.c_void_p]),
+    ("aem_batch_destroy", None, [C.c_void_p]),
     ("aem_create_from_inputs", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     ("aem_destroy", None, [C.c_void_p]),
     ("aem_run", C.c_int, [C.c_void_p, C.c_int64, C.c_int32]),
@@ -79,6 +85,13 @@ def check(status: int) -> None:
 
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
+
+NTFF_BINNING = {"ordered": 0, "atomic": 1, "histogram": 2}
+
+
+class SimOptions(C.Structure):
+    """aem_sim_options (include/aem.h)."""
+    _fields_ = [("ntff_binning", C.c_int32), ("planes", C.c_int32), ("no_cpml_fusion", C.c_int32)]
 
 
 class PortView(C.Structure):

@@ -122,14 +122,12 @@ __device__ __forceinline__ void st_once(T v, T* p) { __builtin_nontemporal_store
 // order (x, then y, then z slab), reusing the loaded neighbours.
 // ---------------------------------------------------------------------------
 template <int C>
-__global__ void __launch_bounds__(kTX * kTY) k_h_bulk(Grid g, BoxSet bs, float* __restrict__ Hx, float* __restrict__ Hy,
-                                                float* __restrict__ Hz, const float* __restrict__ Ex,
-                                                const float* __restrict__ Ey, const float* __restrict__ Ez,
-                                                const float* __restrict__ dhx, const float* __restrict__ dhy,
-                                                const float* __restrict__ dhz, float DB, Cpml cp, float dx, float dy,
-                                                float dz) {
-    int bx, by, bz;
-    const Box b = box_of(bs, bx, by, bz);
+__device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx, int by, int bz,
+                                            float* __restrict__ Hx, float* __restrict__ Hy, float* __restrict__ Hz,
+                                            const float* __restrict__ Ex, const float* __restrict__ Ey,
+                                            const float* __restrict__ Ez, const float* __restrict__ dhx,
+                                            const float* __restrict__ dhy, const float* __restrict__ dhz, float DB,
+                                            const Cpml& cp, float dx, float dy, float dz) {
     const int i = b.i0 + bx * kTX + (int)threadIdx.x;
     const int j = b.j0 + by * kTY + (int)threadIdx.y;
     if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
@@ -207,15 +205,13 @@ __global__ void __launch_bounds__(kTX * kTY) k_h_bulk(Grid g, BoxSet bs, float* 
 // Slab cells of the box append their CPML corrections as in k_h_bulk.
 // ---------------------------------------------------------------------------
 template <typename IdxT, int C>
-__global__ void __launch_bounds__(kTX * kTY) k_e_bulk(Grid g, BoxSet bs, float* __restrict__ Ex, float* __restrict__ Ey,
-                                                float* __restrict__ Ez, const float* __restrict__ Hx,
-                                                const float* __restrict__ Hy, const float* __restrict__ Hz,
-                                                const IdxT* __restrict__ idx, const float* __restrict__ tup,
-                                                const float* __restrict__ dex, const float* __restrict__ dey,
-                                                const float* __restrict__ dez, Cpml cp, float dx, float dy,
-                                                float dz) {
-    int bx, by, bz;
-    const Box b = box_of(bs, bx, by, bz);
+__device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx, int by, int bz,
+                                            float* __restrict__ Ex, float* __restrict__ Ey, float* __restrict__ Ez,
+                                            const float* __restrict__ Hx, const float* __restrict__ Hy,
+                                            const float* __restrict__ Hz, const IdxT* __restrict__ idx,
+                                            const float* __restrict__ tup, const float* __restrict__ dex,
+                                            const float* __restrict__ dey, const float* __restrict__ dez,
+                                            const Cpml& cp, float dx, float dy, float dz) {
     const int i = b.i0 + bx * kTX + (int)threadIdx.x;
     const int j = b.j0 + by * kTY + (int)threadIdx.y;
     if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
@@ -314,6 +310,95 @@ __global__ void __launch_bounds__(kTX * kTY) k_e_bulk(Grid g, BoxSet bs, float* 
         hy_km = hy;
         o += g.plane;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Bulk kernels.  The single-simulation form takes its arguments by value and
+// a box set; the batched form runs many independent simulations in one
+// launch, each work-group finding its (member, box) in a table of entries
+// sorted by first work-group.  Both call the same body, so a member of a
+// batch computes exactly what it computes alone.
+// ---------------------------------------------------------------------------
+struct HArgs {
+    Grid g;
+    float *Hx, *Hy, *Hz;
+    const float *Ex, *Ey, *Ez;
+    const float *dhx, *dhy, *dhz;
+    float DB;
+    Cpml cp;
+    float dx, dy, dz;
+};
+
+struct EArgs {
+    Grid g;
+    float *Ex, *Ey, *Ez;
+    const float *Hx, *Hy, *Hz;
+    const void* idx;   // uint8_t or uint16_t per the kernel's IdxT
+    const float* tup;
+    const float *dex, *dey, *dez;
+    Cpml cp;
+    float dx, dy, dz;
+};
+
+struct BatchEntry {
+    int first;    // first work-group of this box in the launch
+    int member;   // index into the argument array
+    Box box;
+};
+
+// Entry of this work-group (binary search on `first`, uniform per group) and
+// its (x, y, z-chunk) block coordinates inside the entry's box.
+__device__ __forceinline__ BatchEntry entry_of(const BatchEntry* __restrict__ e, int n, int& bx, int& by, int& bz) {
+    const int g = (int)blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (e[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchEntry r = e[lo];
+    const int nbx = (r.box.ni + kTX - 1) / kTX, nby = (r.box.nj + kTY - 1) / kTY;
+    const int l = g - r.first;
+    bx = l % nbx;
+    by = (l / nbx) % nby;
+    bz = l / (nbx * nby);
+    return r;
+}
+
+template <int C>
+__global__ void __launch_bounds__(kTX * kTY) k_h_bulk(HArgs a, BoxSet bs) {
+    int bx, by, bz;
+    const Box b = box_of(bs, bx, by, bz);
+    h_bulk_body<C>(a.g, b, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy, a.dhz, a.DB, a.cp, a.dx,
+                   a.dy, a.dz);
+}
+
+template <int C>
+__global__ void __launch_bounds__(kTX * kTY) k_h_bulk_batch(const HArgs* __restrict__ args,
+                                                            const BatchEntry* __restrict__ ents, int n_ents) {
+    int bx, by, bz;
+    const BatchEntry e = entry_of(ents, n_ents, bx, by, bz);
+    const HArgs& a = args[e.member];
+    h_bulk_body<C>(a.g, e.box, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy, a.dhz, a.DB, a.cp,
+                   a.dx, a.dy, a.dz);
+}
+
+template <typename IdxT, int C>
+__global__ void __launch_bounds__(kTX * kTY) k_e_bulk(EArgs a, BoxSet bs) {
+    int bx, by, bz;
+    const Box b = box_of(bs, bx, by, bz);
+    e_bulk_body<IdxT, C>(a.g, b, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz, static_cast<const IdxT*>(a.idx),
+                         a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
+}
+
+template <typename IdxT, int C>
+__global__ void __launch_bounds__(kTX * kTY) k_e_bulk_batch(const EArgs* __restrict__ args,
+                                                            const BatchEntry* __restrict__ ents, int n_ents) {
+    int bx, by, bz;
+    const BatchEntry e = entry_of(ents, n_ents, bx, by, bz);
+    const EArgs& a = args[e.member];
+    e_bulk_body<IdxT, C>(a.g, e.box, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz,
+                         static_cast<const IdxT*>(a.idx), a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,10 +703,23 @@ struct Pulse {
     int T;
 };
 
-__global__ void __launch_bounds__(256) k_ports_h(Grid g, const PortDev* ports, int np, int M, float* V, float* I,
-                                                 float* Iout, int T, const float* Hx, const float* Hy,
-                                                 const float* Hz, float dx, float dy, float dz, Pulse pu,
-                                                 const int* step) {
+// Arguments of both port kernels; the batched forms take one per member and
+// run one work-group per member.
+struct PortArgs {
+    Grid g;
+    const PortDev* ports;
+    int np, M, T;
+    float *V, *I, *Vout, *Iout;
+    float *Ex, *Ey, *Ez;
+    const float *Hx, *Hy, *Hz;
+    float dx, dy, dz;
+    Pulse pu;
+    const int* step;
+};
+
+__device__ __forceinline__ void ports_h_body(const Grid& g, const PortDev* ports, int np, int M, float* V, float* I,
+                                             float* Iout, int T, const float* Hx, const float* Hy, const float* Hz,
+                                             float dx, float dy, float dz, const Pulse& pu, const int* step) {
     const int n = *step;
     if (threadIdx.x == 0) {
         for (int q = 0; q < np; ++q) {
@@ -667,9 +765,9 @@ __global__ void __launch_bounds__(256) k_ports_h(Grid g, const PortDev* ports, i
     }
 }
 
-__global__ void __launch_bounds__(256) k_ports_e(Grid g, const PortDev* ports, int np, int M, float* V,
-                                                 const float* I, float* Vout, int T, float* Ex, float* Ey,
-                                                 float* Ez, Pulse pu, const int* step) {
+__device__ __forceinline__ void ports_e_body(const Grid& g, const PortDev* ports, int np, int M, float* V,
+                                             const float* I, float* Vout, int T, float* Ex, float* Ey, float* Ez,
+                                             const Pulse& pu, const int* step) {
     const int n = *step;
     for (int q = 0; q < np; ++q) {
         const float kV = ports[q].kV;
@@ -695,6 +793,21 @@ __global__ void __launch_bounds__(256) k_ports_e(Grid g, const PortDev* ports, i
             Vout[(long long)q * T + (n - 1)] = v2;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_ports_h(PortArgs a) {
+    ports_h_body(a.g, a.ports, a.np, a.M, a.V, a.I, a.Iout, a.T, a.Hx, a.Hy, a.Hz, a.dx, a.dy, a.dz, a.pu, a.step);
+}
+__global__ void __launch_bounds__(256) k_ports_e(PortArgs a) {
+    ports_e_body(a.g, a.ports, a.np, a.M, a.V, a.I, a.Vout, a.T, a.Ex, a.Ey, a.Ez, a.pu, a.step);
+}
+__global__ void __launch_bounds__(256) k_ports_h_batch(const PortArgs* __restrict__ args) {
+    const PortArgs& a = args[blockIdx.x];
+    ports_h_body(a.g, a.ports, a.np, a.M, a.V, a.I, a.Iout, a.T, a.Hx, a.Hy, a.Hz, a.dx, a.dy, a.dz, a.pu, a.step);
+}
+__global__ void __launch_bounds__(256) k_ports_e_batch(const PortArgs* __restrict__ args) {
+    const PortArgs& a = args[blockIdx.x];
+    ports_e_body(a.g, a.ports, a.np, a.M, a.V, a.I, a.Vout, a.T, a.Ex, a.Ey, a.Ez, a.pu, a.step);
 }
 
 // ---------------------------------------------------------------------------
@@ -921,9 +1034,9 @@ struct NtffFaces {
     int pbc[3];
 };
 
-__global__ void k_ntff_currents(Grid g, NtffFaces f, const float* Ex, const float* Ey, const float* Ez,
-                                const float* Hx, const float* Hy, const float* Hz) {
-    const int face = blockIdx.z;
+__device__ __forceinline__ void ntff_currents_body(const Grid& g, const NtffFaces& f, int face, const float* Ex,
+                                                   const float* Ey, const float* Ez, const float* Hx,
+                                                   const float* Hy, const float* Hz) {
     const int a = face / 2;
     if (f.pbc[a]) return;
     const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
@@ -965,6 +1078,11 @@ __global__ void k_ntff_currents(Grid g, NtffFaces f, const float* Ex, const floa
     f.J[face][0][fo] = j0;
     f.Jo[face][1][fo] = f.J[face][1][fo];
     f.J[face][1][fo] = j1;
+}
+
+__global__ void k_ntff_currents(Grid g, NtffFaces f, const float* Ex, const float* Ey, const float* Ez,
+                                const float* Hx, const float* Hy, const float* Hz) {
+    ntff_currents_body(g, f, blockIdx.z, Ex, Ey, Ez, Hx, Hy, Hz);
 }
 
 // NTFF retarded-time binning (f90:7643-7927).  Index and weight arithmetic is
@@ -1106,6 +1224,100 @@ __global__ void __launch_bounds__(256) k_ntff_exact(NtffFaces f, NtffBins bn, Nt
         }
     }
     if (own) A[bin] = acc;
+}
+
+// NTFF retarded-time binning through an LDS histogram (f90:7643-7927 with
+// FP64 accumulation).  One work-group per (angle, accumulator): the step's
+// contributions all fall in a window of bins [n + win[0], n + win[0] +
+// win[1]) fixed by the NTFF box geometry, so the work-group adds them into an
+// FP64 histogram of that window in LDS and then adds the window into its own
+// accumulator row -- one global read-modify-write per bin instead of eight
+// global atomics per face cell.  Index and weight arithmetic per contribution
+// is the reference's FP32, as in k_ntff_bins; only the FP64 summation order
+// differs from the serial reference (agreement at FP32 rounding level).
+constexpr int kHistMax = 4096;   // bins of one window held in LDS (32 KB)
+
+__device__ __forceinline__ void ntff_hist_body(const NtffFaces& f, const NtffBins& bn, const int* win, int kk,
+                                               int arr, const int* step) {
+    __shared__ double h[kHistMax];
+    const int n = *step;
+    const int lo = n + win[2 * kk], len = min(win[2 * kk + 1], kHistMax);
+    for (int t = threadIdx.x; t < len; t += blockDim.x) h[t] = 0.0;
+    __syncthreads();
+    const int comp = arr % 3;
+    const bool isW = arr >= 3;
+    const float* an = bn.ang + 6 * kk;
+    const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
+    for (int a = 0; a < 3; ++a) {
+        if (a == comp || f.pbc[a]) continue;
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        const int slot = comp == b ? 0 : 1;
+        const int nu = f.hi[b] - f.lo[b] + 1, nv = f.hi[c] - f.lo[c] + 1;
+        const long long ncell = (long long)nu * nv * 2;
+        const float K = bn.K[a];
+        for (long long cid = threadIdx.x; cid < ncell; cid += blockDim.x) {
+            const int side = (int)(cid & 1);
+            const long long uv = cid >> 1;
+            const int u = f.lo[b] + (int)(uv % nu), v = f.lo[c] + (int)(uv / nu);
+            const int face = 2 * a + side;
+            const int pos = side ? f.hi[a] : f.lo[a];
+            int i, j, k;
+            if (a == 0) { i = pos; j = u; k = v; }
+            else if (a == 1) { i = u; j = pos; k = v; }
+            else { i = u; j = v; k = pos; }
+            const float tfv = (bn.r_relay - ((float)(i - bn.ic) * bn.dx * s1 * c2 + (float)(j - bn.jc) * bn.dy * s1 * s2 +
+                                             (float)(k - bn.kc) * bn.dz * c1)) / bn.c_dt;
+            const long long fo = (long long)u + (long long)f.fd0[face] * v;
+            const float d = isW ? (f.J[face][slot][fo] - f.Jo[face][slot][fo])
+                                : (f.M[face][slot][fo] - f.Mo[face][slot][fo]);
+            const float t = isW ? ((float)n + tfv - 1.0f) : ((float)n + 0.5f + tfv - 1.0f);
+            const int ii = (int)t;
+            const float w = t - (float)ii;
+            const int q = ii - lo;
+            if (q >= 0 && q + 1 < len) {
+                atomicAdd(&h[q], (double)((1.0f - w) * K * d));
+                atomicAdd(&h[q + 1], (double)(w * K * d));
+            }
+        }
+    }
+    __syncthreads();
+    double* A = (isW ? bn.W[comp] : bn.U[comp]) + (long long)kk * bn.LL;
+    for (int t = threadIdx.x; t < len; t += blockDim.x) {
+        const long long bin = (long long)lo + t;
+        if (bin >= 0 && bin < bn.LL) A[bin] += h[t];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ntff_hist(NtffFaces f, NtffBins bn, const int* win, const int* step) {
+    ntff_hist_body(f, bn, win, blockIdx.x, blockIdx.y, step);
+}
+
+// Per-member arguments of the batched far-field kernels.
+struct NtffArgs {
+    Grid g;
+    NtffFaces f;
+    NtffBins bn;
+    const int* win;
+    const float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;
+    const int* step;
+};
+
+// grid (x-blocks, y-blocks, 6 * members); blocks beyond a member's faces exit
+__global__ void k_ntff_currents_batch(const NtffArgs* __restrict__ args) {
+    const NtffArgs& a = args[blockIdx.z / 6];
+    ntff_currents_body(a.g, a.f, blockIdx.z % 6, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz);
+}
+
+// grid (max angles, 6, members)
+__global__ void __launch_bounds__(256) k_ntff_hist_batch(const NtffArgs* __restrict__ args) {
+    const NtffArgs& a = args[blockIdx.z];
+    if ((int)blockIdx.x >= a.bn.n_angles) return;
+    ntff_hist_body(a.f, a.bn, a.win, blockIdx.x, blockIdx.y, a.step);
+}
+
+__global__ void k_step_inc_batch(int* const* __restrict__ steps, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) *steps[m] = *steps[m] + 1;
 }
 
 // Phase-centre samples (f90:8259-8290) and the step counter.

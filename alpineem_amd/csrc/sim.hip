@@ -105,7 +105,24 @@ struct Launch {
     }
 };
 
-constexpr int kPlanes = 4;   // z planes marched by one thread of a bulk kernel
+constexpr int kPlanes = 4;   // default z planes marched by one thread of a bulk kernel
+
+// Parts of a step, in the order the reference applies them.
+enum : unsigned {
+    kPartHBulk = 1u << 0,    // bulk H (+ fused CPML)
+    kPartHPre = 1u << 1,     // H sheets, TF/SF
+    kPartHPort = 1u << 2,    // ports, H phase
+    kPartHPost = 1u << 3,    // separate CPML H, PBC
+    kPartEBulk = 1u << 4,    // bulk E (+ fused CPML)
+    kPartEPre = 1u << 5,     // E sheets, Drude, TF/SF
+    kPartEPort = 1u << 6,    // ports, E phase
+    kPartEPost = 1u << 7,    // separate CPML E, PBC, E_old
+    kPartOutPre = 1u << 8,   // plane sums
+    kPartNtff = 1u << 9,     // far-field currents and binning
+    kPartOutPost = 1u << 10, // phase-centre samples
+    kPartStep = 1u << 11,    // step counter
+    kPartAll = (1u << 12) - 1
+};
 
 Launch make_launch(const std::vector<Box>& boxes, int c) {
     Launch L{};
@@ -128,10 +145,10 @@ Launch make_launch(const std::vector<Box>& boxes, int c) {
 // y and z outside the y/z slabs) with the x-slab test, then one launch over
 // the y-slab and z-slab boxes.  H slab layers are 1..n-1 and N+1-n..N-1, E
 // layers 2..n and N+1-n..N-1, so one split serves both phases.
-std::vector<Launch> bulk_launches(const aem_plan_view& v, bool fuse) {
+std::vector<Launch> bulk_launches(const aem_plan_view& v, bool fuse, int planes) {
     const int N[3] = {v.NX, v.NY, v.NZ};
     auto box = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
-        return Box{a0, b0, c0, a1 - a0 + 1, b1 - b0 + 1, c1 - c0 + 1, kPlanes};
+        return Box{a0, b0, c0, a1 - a0 + 1, b1 - b0 + 1, c1 - c0 + 1, planes};
     };
     std::vector<Launch> out;
     if (!fuse) {
@@ -191,6 +208,14 @@ struct aem_sim {
     Pulse pu{};
     bool fuse_cpml = false;        // CPML corrections folded into the bulk kernels
     std::vector<Launch> bulk;      // bulk launches of each phase; [0] is the core when fused
+    HArgs ha{};                    // bulk kernel arguments (fields, tables, CPML state)
+    EArgs ea{};
+    PortArgs pa{};                 // port kernel arguments
+    NtffArgs na{};                 // far-field kernel arguments (batched form)
+    int ntff_mode = AEM_NTFF_ORDERED;
+    int ntff_mu = 0, ntff_mv = 0;  // largest face extents of the far-field box
+    int planes = kPlanes;
+    bool allow_fuse = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -225,14 +250,20 @@ struct aem_sim {
         return d;
     }
 
-    void create();
-    void enqueue_step(hipEvent_t* mark = nullptr);
+    void create(const aem_sim_options* opt);
+    void enqueue_step(hipEvent_t* mark = nullptr) { enqueue_parts(mark, st, kPartAll); }
+    void enqueue_parts(hipEvent_t* mark, hipStream_t s, unsigned parts);
     template <class IdxT>
-    void enqueue_step_t(hipEvent_t* mark);
+    void enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts);
 };
 
-void aem_sim::create() {
+void aem_sim::create(const aem_sim_options* opt) {
     const aem_plan_view& v = plan->v;
+    if (opt) {
+        ntff_mode = opt->ntff_binning;
+        if (opt->planes > 0) planes = opt->planes;
+        allow_fuse = opt->no_cpml_fusion == 0;
+    }
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
@@ -348,14 +379,8 @@ void aem_sim::create() {
         nb.c_dt = aem::kC * v.dt;
         for (int a = 0; a < 3; ++a) nb.K[a] = v.ff_K[a];
         nb.LL = (long long)v.len_ff + 2;
-        nb.exact = std::getenv("AEM_NTFF_ATOMIC") == nullptr ? 1 : 0;
-        if (nb.exact) {
-            for (int q = 0; q < 3; ++q) {
-                nb.Uf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
-                nb.Wf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
-                nx_acc.acc[q] = nb.Uf[q];
-                nx_acc.acc[3 + q] = nb.Wf[q];
-            }
+        nb.exact = ntff_mode == AEM_NTFF_ORDERED ? 1 : 0;
+        {
             // retarded-time offsets of every face cell, per angle, bound the
             // window of bins one step can touch
             std::vector<int> win(2 * (size_t)v.n_angles);
@@ -389,12 +414,23 @@ void aem_sim::create() {
                 win_max = std::max(win_max, win[2 * kk + 1]);
             }
             nx_acc.win = up<int>(win.data(), win.size());
+        }
+        if (nb.exact) {
+            for (int q = 0; q < 3; ++q) {
+                nb.Uf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
+                nb.Wf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
+                nx_acc.acc[q] = nb.Uf[q];
+                nx_acc.acc[3 + q] = nb.Wf[q];
+            }
         } else {
             for (int q = 0; q < 3; ++q) {
                 nb.U[q] = zeros<double>((size_t)v.n_angles * nb.LL);
                 nb.W[q] = zeros<double>((size_t)v.n_angles * nb.LL);
             }
         }
+        // the histogram keeps one window of bins in LDS; larger windows bin
+        // through global atomics
+        if (ntff_mode == AEM_NTFF_HISTOGRAM && win_max > kHistMax) ntff_mode = AEM_NTFF_ATOMIC;
         d_Eth = zeros<float>((size_t)v.n_angles * T);
         d_Eph = zeros<float>((size_t)v.n_angles * T);
         d_fftime = zeros<float>(v.n_angles);
@@ -408,13 +444,28 @@ void aem_sim::create() {
     d_ffpc = zeros<float>(T);
     d_incident = zeros<float>(T);
 
-    fuse_cpml = cpml_fusable(v) && std::getenv("AEM_NO_CPML_FUSION") == nullptr;
-    bulk = bulk_launches(v, fuse_cpml);
+    fuse_cpml = allow_fuse && cpml_fusable(v);
+    bulk = bulk_launches(v, fuse_cpml, planes);
+    ha = HArgs{g, F[3], F[4], F[5], F[0], F[1], F[2], d_den[3], d_den[4], d_den[5], v.DB, cpH, v.dx, v.dy, v.dz};
+    ea = EArgs{g, F[0], F[1], F[2], F[3], F[4], F[5], d_idx, d_tup, d_den[0], d_den[1], d_den[2], cpE,
+               v.dx, v.dy, v.dz};
+    pa = PortArgs{g, d_ports, v.n_ports, v.M, T, d_V, d_I, d_Vout, d_Iout, F[0], F[1], F[2], F[3], F[4], F[5],
+                  v.dx, v.dy, v.dz, pu, d_step};
+    for (int a = 0; a < 3; ++a) {
+        const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+        ntff_mu = std::max(ntff_mu, v.ff_hi[b] - v.ff_lo[b] + 1);
+        ntff_mv = std::max(ntff_mv, v.ff_hi[c] - v.ff_lo[c] + 1);
+    }
+    na = NtffArgs{g, nf, nb, nx_acc.win, F[0], F[1], F[2], F[3], F[4], F[5], d_step};
     HIPCHK(hipStreamSynchronize(st));
 }
 
+// Enqueue the selected parts of one step on stream s, in the reference's
+// order.  A simulation run alone enqueues every part; a batch replaces the
+// bulk, port, far-field and step-counter parts by batched launches and
+// enqueues the remaining parts per member between them.
 template <class IdxT>
-void aem_sim::enqueue_step_t(hipEvent_t* mark) {
+void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     const aem_plan_view& v = plan->v;
     const IdxT* idx = static_cast<const IdxT*>(d_idx);
     float *Ex = F[0], *Ey = F[1], *Ez = F[2], *Hx = F[3], *Hy = F[4], *Hz = F[5];
@@ -422,147 +473,136 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark) {
     const dim3 bt(kTX, kTY, 1);   // bulk tile
     const int T = v.time_steps;
     auto h_bulk = [&](const Launch& L) {
-#define AEM_H_BULK(C) k_h_bulk<C><<<L.blocks(), bt, 0, st>>>(g, L.bs, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], \
-                                                             d_den[5], v.DB, cpH, v.dx, v.dy, v.dz)
         switch (L.c) {
-            case 0: AEM_H_BULK(0); break;
-            case kCx: AEM_H_BULK(kCx); break;
-            default: AEM_H_BULK(kCx | kCy | kCz); break;
+            case 0: k_h_bulk<0><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
+            case kCx: k_h_bulk<kCx><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
+            default: k_h_bulk<kCx | kCy | kCz><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
         }
-#undef AEM_H_BULK
     };
     auto e_bulk = [&](const Launch& L) {
-#define AEM_E_BULK(C) k_e_bulk<IdxT, C><<<L.blocks(), bt, 0, st>>>(g, L.bs, Ex, Ey, Ez, Hx, Hy, Hz, idx, d_tup, \
-                                                                   d_den[0], d_den[1], d_den[2], cpE, v.dx, v.dy, v.dz)
         switch (L.c) {
-            case 0: AEM_E_BULK(0); break;
-            case kCx: AEM_E_BULK(kCx); break;
-            default: AEM_E_BULK(kCx | kCy | kCz); break;
+            case 0: k_e_bulk<IdxT, 0><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
+            case kCx: k_e_bulk<IdxT, kCx><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
+            default: k_e_bulk<IdxT, kCx | kCy | kCz><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
         }
-#undef AEM_E_BULK
     };
 
     // ---- H phase ----  (marks: [0,1] the core bulk launch, [1,2] the rest of H)
     if (mark) HIPCHK(hipEventRecord(mark[0], st));
-    h_bulk(bulk[0]);
+    if (parts & kPartHBulk) h_bulk(bulk[0]);
     if (mark) HIPCHK(hipEventRecord(mark[1], st));
-    for (size_t q = 1; q < bulk.size(); ++q) h_bulk(bulk[q]);
+    if (parts & kPartHBulk)
+        for (size_t q = 1; q < bulk.size(); ++q) h_bulk(bulk[q]);
     const float dd[3] = {v.dx, v.dy, v.dz};
     for (int a = 0; a < 3; ++a)
-        if (sh[a].np > 0) {
+        if ((parts & kPartHPre) && sh[a].np > 0) {
             const float f = v.DB * (v.sheet_thickness / dd[a]);
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
-            k_h_sheets<<<gs, bb, 0, st>>>(g, sh[a], a, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], f);
+            k_h_sheets<<<gs, bb, 0, s>>>(g, sh[a], a, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], f);
         }
-    if (v.has_tfsf) {
+    if ((parts & kPartHPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 0, Hx, Hy, Hz, d_step);
-        if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, st>>>(g, tf, 1, Hx, Hy, Hz, d_step);
-        if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 2, Hx, Hy, Hz, d_step);
+        if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Hx, Hy, Hz, d_step);
+        if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Hx, Hy, Hz, d_step);
+        if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Hx, Hy, Hz, d_step);
     }
-    if (v.n_ports > 0)
-        k_ports_h<<<1, 256, 0, st>>>(g, d_ports, v.n_ports, v.M, d_V, d_I, d_Iout, T, Hx, Hy, Hz, v.dx, v.dy, v.dz, pu, d_step);
-    {
+    if ((parts & kPartHPort) && v.n_ports > 0) k_ports_h<<<1, 256, 0, s>>>(pa);
+    if (parts & kPartHPost) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
         const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
-        if (!fuse_cpml && tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, st>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, v.dx);
-        if (!fuse_cpml && ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, st>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, v.dy);
-        if (!fuse_cpml && tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, st>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, v.dz);
+        if (!fuse_cpml && tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, s>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, v.dx);
+        if (!fuse_cpml && ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, s>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, v.dy);
+        if (!fuse_cpml && tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, s>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, v.dz);
+        if (v.pbc[0]) k_h_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 0, Hz, Hy);
+        if (v.pbc[1]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 1, Hz, Hx);
+        if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Hy, Hx);
     }
-    if (v.pbc[0]) k_h_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 0, Hz, Hy);
-    if (v.pbc[1]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 1, Hz, Hx);
-    if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, st>>>(g, 2, Hy, Hx);
 
     // ---- E phase ----  (marks: [2,3] the core bulk launch, [3,4] the rest of E)
     if (mark) HIPCHK(hipEventRecord(mark[2], st));
-    e_bulk(bulk[0]);
+    if (parts & kPartEBulk) e_bulk(bulk[0]);
     if (mark) HIPCHK(hipEventRecord(mark[3], st));
-    for (size_t q = 1; q < bulk.size(); ++q) e_bulk(bulk[q]);
+    if (parts & kPartEBulk)
+        for (size_t q = 1; q < bulk.size(); ++q) e_bulk(bulk[q]);
     for (int a = 0; a < 3; ++a)
-        if (sh[a].np > 0) {
+        if ((parts & kPartEPre) && sh[a].np > 0) {
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
-            k_e_sheets<<<gs, bb, 0, st>>>(g, sh[a], a, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2], v.dt, aem::kEp0);
+            k_e_sheets<<<gs, bb, 0, s>>>(g, sh[a], a, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2], v.dt, aem::kEp0);
         }
-    if (v.has_drude)
+    if ((parts & kPartEPre) && v.has_drude)
         for (int c = 0; c < 3; ++c)
-            if (db[c].nb > 0) k_drude<IdxT><<<cdiv(db[c].nb, 256), 256, 0, st>>>(g, db[c], c, v.n_poles, F[c], idx, d_tup);
-    if (v.has_tfsf) {
+            if (db[c].nb > 0) k_drude<IdxT><<<cdiv(db[c].nb, 256), 256, 0, s>>>(g, db[c], c, v.n_poles, F[c], idx, d_tup);
+    if ((parts & kPartEPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 0, Ex, Ey, Ez, d_step);
-        if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, st>>>(g, tf, 1, Ex, Ey, Ez, d_step);
-        if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, st>>>(g, tf, 2, Ex, Ey, Ez, d_step);
+        if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Ex, Ey, Ez, d_step);
+        if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Ex, Ey, Ez, d_step);
+        if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Ex, Ey, Ez, d_step);
     }
-    if (v.n_ports > 0)
-        k_ports_e<<<1, 256, 0, st>>>(g, d_ports, v.n_ports, v.M, d_V, d_I, d_Vout, T, Ex, Ey, Ez, pu, d_step);
-    {
+    if ((parts & kPartEPort) && v.n_ports > 0) k_ports_e<<<1, 256, 0, s>>>(pa);
+    if (parts & kPartEPost) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
         const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
-        if (!fuse_cpml && tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, st>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, v.dx);
-        if (!fuse_cpml && ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, st>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, v.dy);
-        if (!fuse_cpml && tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, st>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, v.dz);
+        if (!fuse_cpml && tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, s>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, v.dx);
+        if (!fuse_cpml && ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, s>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, v.dy);
+        if (!fuse_cpml && tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, s>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, v.dz);
+        if (v.pbc[0]) {
+            k_e_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 0, Ez, Ey);
+            if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[2].np), 256, 0, s>>>(sh[2], 0, v.NX - 1, v.NY - 1);
+            if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[1].np), 256, 0, s>>>(sh[1], 0, v.NX - 1, v.NZ - 1);
+        }
+        if (v.pbc[1]) {
+            k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 1, Ez, Ex);
+            if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[2].np), 256, 0, s>>>(sh[2], 1, v.NY - 1, v.NX - 1);
+            if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[0].np), 256, 0, s>>>(sh[0], 0, v.NY - 1, v.NZ - 1);
+        }
+        if (v.pbc[2]) {
+            k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Ey, Ex);
+            if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[1].np), 256, 0, s>>>(sh[1], 1, v.NZ - 1, v.NX - 1);
+            if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[0].np), 256, 0, s>>>(sh[0], 1, v.NZ - 1, v.NY - 1);
+        }
+        if (v.has_drude)
+            for (int c = 0; c < 3; ++c)
+                if (db[c].nb > 0) k_save_eold<<<cdiv(db[c].nb, 256), 256, 0, s>>>(g, db[c], F[c]);
     }
-    if (v.pbc[0]) {
-        k_e_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 0, Ez, Ey);
-        if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[2].np), 256, 0, st>>>(sh[2], 0, v.NX - 1, v.NY - 1);
-        if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[1].np), 256, 0, st>>>(sh[1], 0, v.NX - 1, v.NZ - 1);
-    }
-    if (v.pbc[1]) {
-        k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, st>>>(g, 1, Ez, Ex);
-        if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[2].np), 256, 0, st>>>(sh[2], 1, v.NY - 1, v.NX - 1);
-        if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[0].np), 256, 0, st>>>(sh[0], 0, v.NY - 1, v.NZ - 1);
-    }
-    if (v.pbc[2]) {
-        k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, st>>>(g, 2, Ey, Ex);
-        if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[1].np), 256, 0, st>>>(sh[1], 1, v.NZ - 1, v.NX - 1);
-        if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[0].np), 256, 0, st>>>(sh[0], 1, v.NZ - 1, v.NY - 1);
-    }
-    if (v.has_drude)
-        for (int c = 0; c < 3; ++c)
-            if (db[c].nb > 0) k_save_eold<<<cdiv(db[c].nb, 256), 256, 0, st>>>(g, db[c], F[c]);
 
     // ---- per-step outputs ----
     if (mark) HIPCHK(hipEventRecord(mark[4], st));
-    if (v.plane_sum_axis >= 0) {
+    if ((parts & kPartOutPre) && v.plane_sum_axis >= 0) {
         const int ax = v.plane_sum_axis;
         const int lo = v.tf_lo[ax], hi = v.tf_hi[ax];
         if (ax == 0)
-            k_plane_sums<<<1, 256, 0, st>>>(g, 0, lo, hi, v.ff_lo[1], v.ff_hi[1], v.ff_lo[2], v.ff_hi[2], Ez, Ey, v.s_pol,
+            k_plane_sums<<<1, 256, 0, s>>>(g, 0, lo, hi, v.ff_lo[1], v.ff_hi[1], v.ff_lo[2], v.ff_hi[2], Ez, Ey, v.s_pol,
                                             v.c_pol, v.c_ph, 1, d_refl, d_trans, d_step);
         else if (ax == 1)
-            k_plane_sums<<<1, 256, 0, st>>>(g, 1, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[2], v.ff_hi[2], Ez, Ex, v.s_pol,
+            k_plane_sums<<<1, 256, 0, s>>>(g, 1, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[2], v.ff_hi[2], Ez, Ex, v.s_pol,
                                             v.c_pol, v.s_ph, 0, d_refl, d_trans, d_step);
         else
-            k_plane_sums<<<1, 256, 0, st>>>(g, 2, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[1], v.ff_hi[1], Ey, Ex, v.ps_c1,
+            k_plane_sums<<<1, 256, 0, s>>>(g, 2, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[1], v.ff_hi[1], Ey, Ex, v.ps_c1,
                                             v.ps_c2, 1.0f, 0, d_refl, d_trans, d_step);
     }
-    if (v.n_angles > 0) {
-        int mu = 0, mv = 0;
-        for (int a = 0; a < 3; ++a) {
-            const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-            mu = std::max(mu, v.ff_hi[b] - v.ff_lo[b] + 1);
-            mv = std::max(mv, v.ff_hi[c] - v.ff_lo[c] + 1);
-        }
-        const dim3 gf(cdiv(mu, 32), cdiv(mv, 8), 6);
-        k_ntff_currents<<<gf, bb, 0, st>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
-        const dim3 gbins(cdiv(mu, 32), cdiv(mv, 8), 6 * v.n_angles);
-        if (nb.exact) {
+    if ((parts & kPartNtff) && v.n_angles > 0) {
+        const dim3 gf(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
+        k_ntff_currents<<<gf, bb, 0, s>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
+        if (ntff_mode == AEM_NTFF_ORDERED) {
             if (win_max > 0)
-                k_ntff_exact<<<dim3(v.n_angles, 6, cdiv(win_max, 256)), 256, 0, st>>>(nf, nb, nx_acc, d_step);
+                k_ntff_exact<<<dim3(v.n_angles, 6, cdiv(win_max, 256)), 256, 0, s>>>(nf, nb, nx_acc, d_step);
+        } else if (ntff_mode == AEM_NTFF_HISTOGRAM) {
+            k_ntff_hist<<<dim3(v.n_angles, 6), 256, 0, s>>>(nf, nb, nx_acc.win, d_step);
         } else {
-            k_ntff_bins<<<gbins, bb, 0, st>>>(nf, nb, d_step);
+            k_ntff_bins<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6 * v.n_angles), bb, 0, s>>>(nf, nb, d_step);
         }
     }
-    if (v.n_angles > 0 && v.plane_wave_amp > 0)
-        k_phase_centre<<<1, 1, 0, st>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
-    k_step_inc<<<1, 1, 0, st>>>(d_step);
+    if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
+        k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
+    if (parts & kPartStep) k_step_inc<<<1, 1, 0, s>>>(d_step);
     if (mark) HIPCHK(hipEventRecord(mark[5], st));
 }
 
-void aem_sim::enqueue_step(hipEvent_t* mark) {
-    if (plan->v.idx_bytes == 1) enqueue_step_t<uint8_t>(mark);
-    else enqueue_step_t<uint16_t>(mark);
+void aem_sim::enqueue_parts(hipEvent_t* mark, hipStream_t s, unsigned parts) {
+    if (plan->v.idx_bytes == 1) enqueue_step_t<uint8_t>(mark, s, parts);
+    else enqueue_step_t<uint16_t>(mark, s, parts);
 }
 
 namespace aem {
@@ -577,19 +617,26 @@ __global__ void k_ffpc(int T, const float* pc, float c2, float c1, float c3, flo
 
 extern "C" {
 
-aem_status aem_sim_create(const aem_plan* plan, aem_sim** out) {
+aem_status aem_sim_create(const aem_plan* plan, aem_sim** out) { return aem_sim_create_ex(plan, nullptr, out); }
+
+aem_status aem_sim_create_ex(const aem_plan* plan, const aem_sim_options* opt, aem_sim** out) {
     if (!plan || !out) {
         aem::set_error("aem_sim_create: null argument");
         return AEM_ERR_STATE;
     }
     *out = nullptr;
+    if (opt && (opt->ntff_binning < AEM_NTFF_ORDERED || opt->ntff_binning > AEM_NTFF_HISTOGRAM ||
+                opt->planes < 0 || opt->planes > 64)) {
+        aem::set_error("aem_sim_create_ex: ntff_binning must be an AEM_NTFF_* value and planes 0..64");
+        return AEM_ERR_STATE;
+    }
     aem_sim* s = new (std::nothrow) aem_sim();
     if (!s) {
         aem::set_error("host allocation failed");
         return AEM_ERR_STATE;
     }
     s->plan = reinterpret_cast<const aem::Plan*>(plan);
-    aem_status stt = aem::guard([&]() { s->create(); });
+    aem_status stt = aem::guard([&]() { s->create(opt); });
     if (stt != AEM_OK) {
         delete s;
         return stt;
@@ -869,5 +916,249 @@ aem_status aem_time_phases(aem_sim* s, int64_t n_steps, float* ms, int32_t n_ms)
         s->steps_done += n_steps;
     });
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched stepping of independent simulations (a GA population, an angle or
+// parameter sweep).  The reference runs such populations as separate solver
+// processes (optimization_scripts/GA.py:11-65, main_ga.py:34-72).  Here one
+// graph-captured step advances every member: the bulk updates of all members
+// run as one launch per kernel variant, the ports, far-field currents and
+// histogram binning and the step counters as one launch each, and any other
+// part a member needs (sheets, TF/SF, Drude, separate CPML, PBC, plane sums,
+// ordered or atomic far-field binning) is enqueued for that member between
+// them in the reference's order.  Members keep their own device state, so
+// series, fields and finalisation are read per member afterwards.
+// ===========================================================================
+
+struct aem_batch {
+    std::vector<aem_sim*> m;
+    hipStream_t st = nullptr;
+    std::vector<void*> owned;
+    struct Group {
+        int c = 0, idx_bytes = 1;
+        BatchEntry* ents = nullptr;
+        int n_ents = 0;
+        unsigned blocks = 0;
+    };
+    std::vector<Group> hg, eg;
+    HArgs* d_hargs = nullptr;
+    EArgs* d_eargs = nullptr;
+    PortArgs* d_pargs = nullptr;
+    int n_port = 0;
+    NtffArgs* d_nargs = nullptr;
+    int n_ntff = 0, max_angles = 0, ntff_mu = 0, ntff_mv = 0;
+    std::vector<int> ntff_alone;   // members whose far field bins outside the batch
+    int** d_steps = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+
+    ~aem_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* p : owned) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    template <class T>
+    T* upload(const std::vector<T>& h) {
+        T* d = dalloc<T>(h.size(), owned);
+        if (!h.empty()) HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        return d;
+    }
+
+    void create() {
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        const int n = (int)m.size();
+        std::vector<HArgs> ha(n);
+        std::vector<EArgs> ea(n);
+        for (int q = 0; q < n; ++q) {
+            ha[q] = m[q]->ha;
+            ea[q] = m[q]->ea;
+        }
+        d_hargs = upload(ha);
+        d_eargs = upload(ea);
+        // bulk groups: one launch per kernel variant (H: by slab tests; E: by
+        // slab tests and index width), entries in member order
+        auto build = [&](bool e_phase, std::vector<Group>& out) {
+            for (int c : {0, (int)kCx, (int)(kCx | kCy | kCz)})
+                for (int ib : {1, 2}) {
+                    if (!e_phase && ib == 2) continue;
+                    std::vector<BatchEntry> ents;
+                    long long blocks = 0;
+                    for (int q = 0; q < n; ++q) {
+                        if (e_phase && m[q]->plan->v.idx_bytes != ib) continue;
+                        for (const Launch& L : m[q]->bulk) {
+                            if (L.c != c) continue;
+                            for (int k = 0; k < L.bs.n; ++k) {
+                                const Box& b = L.bs.b[k];
+                                ents.push_back(BatchEntry{(int)blocks, q, b});
+                                blocks += (long long)cdiv(b.ni, kTX) * cdiv(b.nj, kTY) * cdiv(b.nk, b.kc);
+                            }
+                        }
+                    }
+                    if (ents.empty()) continue;
+                    if (blocks > 0x7fffffffLL) throw aem::Error(AEM_ERR_STATE, "batch too large for one bulk launch");
+                    Group gr;
+                    gr.c = c;
+                    gr.idx_bytes = ib;
+                    gr.ents = upload(ents);
+                    gr.n_ents = (int)ents.size();
+                    gr.blocks = (unsigned)blocks;
+                    out.push_back(gr);
+                }
+        };
+        build(false, hg);
+        build(true, eg);
+        std::vector<PortArgs> pa;
+        std::vector<NtffArgs> na;
+        std::vector<int*> steps(n);
+        for (int q = 0; q < n; ++q) {
+            aem_sim* s = m[q];
+            const aem_plan_view& v = s->plan->v;
+            if (v.n_ports > 0) pa.push_back(s->pa);
+            if (v.n_angles > 0) {
+                if (s->ntff_mode == AEM_NTFF_HISTOGRAM) {
+                    na.push_back(s->na);
+                    max_angles = std::max(max_angles, v.n_angles);
+                    ntff_mu = std::max(ntff_mu, s->ntff_mu);
+                    ntff_mv = std::max(ntff_mv, s->ntff_mv);
+                } else {
+                    ntff_alone.push_back(q);
+                }
+            }
+            steps[q] = s->d_step;
+        }
+        n_port = (int)pa.size();
+        n_ntff = (int)na.size();
+        d_pargs = upload(pa);
+        d_nargs = upload(na);
+        d_steps = upload(steps);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+
+    void enqueue(hipStream_t s) {
+        const dim3 bt(kTX, kTY, 1), bb(32, 8, 1);
+        const int n = (int)m.size();
+        auto each = [&](unsigned part) {
+            for (aem_sim* x : m) x->enqueue_parts(nullptr, s, part);
+        };
+        for (const Group& gr : hg) {
+            if (gr.c == 0) k_h_bulk_batch<0><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
+            else if (gr.c == kCx) k_h_bulk_batch<kCx><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
+            else k_h_bulk_batch<kCx | kCy | kCz><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
+        }
+        each(kPartHPre);
+        if (n_port) k_ports_h_batch<<<n_port, 256, 0, s>>>(d_pargs);
+        each(kPartHPost);
+        for (const Group& gr : eg) {
+            if (gr.idx_bytes == 1) {
+                if (gr.c == 0) k_e_bulk_batch<uint8_t, 0><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+                else if (gr.c == kCx) k_e_bulk_batch<uint8_t, kCx><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+                else k_e_bulk_batch<uint8_t, kCx | kCy | kCz><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+            } else {
+                if (gr.c == 0) k_e_bulk_batch<uint16_t, 0><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+                else if (gr.c == kCx) k_e_bulk_batch<uint16_t, kCx><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+                else k_e_bulk_batch<uint16_t, kCx | kCy | kCz><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+            }
+        }
+        each(kPartEPre);
+        if (n_port) k_ports_e_batch<<<n_port, 256, 0, s>>>(d_pargs);
+        each(kPartEPost);
+        each(kPartOutPre);
+        if (n_ntff) {
+            k_ntff_currents_batch<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6 * n_ntff), bb, 0, s>>>(d_nargs);
+            k_ntff_hist_batch<<<dim3(max_angles, 6, n_ntff), 256, 0, s>>>(d_nargs);
+        }
+        for (int q : ntff_alone) m[q]->enqueue_parts(nullptr, s, kPartNtff);
+        each(kPartOutPost);
+        k_step_inc_batch<<<cdiv(n, 256), 256, 0, s>>>(d_steps, n);
+    }
+};
+
+extern "C" {
+
+aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
+    if (!sims || !out || n <= 0) {
+        aem::set_error("aem_batch_create: need at least one simulation");
+        return AEM_ERR_STATE;
+    }
+    *out = nullptr;
+    for (int q = 0; q < n; ++q) {
+        if (!sims[q] || sims[q]->finalized) {
+            aem::set_error("aem_batch_create: null or finalized member");
+            return AEM_ERR_STATE;
+        }
+        for (int r = 0; r < q; ++r)
+            if (sims[r] == sims[q]) {
+                aem::set_error("aem_batch_create: a simulation appears twice");
+                return AEM_ERR_STATE;
+            }
+    }
+    aem_batch* b = new (std::nothrow) aem_batch();
+    if (!b) {
+        aem::set_error("host allocation failed");
+        return AEM_ERR_STATE;
+    }
+    b->m.assign(sims, sims + n);
+    aem_status stt = aem::guard([&]() {
+        for (aem_sim* s : b->m) HIPCHK(hipStreamSynchronize(s->st));
+        b->create();
+    });
+    if (stt != AEM_OK) {
+        delete b;
+        return stt;
+    }
+    *out = b;
+    return AEM_OK;
+}
+
+aem_status aem_batch_run(aem_batch* b, int64_t n_steps, int32_t use_graph) {
+    if (!b || n_steps < 0) {
+        aem::set_error("aem_batch_run: bad argument");
+        return AEM_ERR_STATE;
+    }
+    for (aem_sim* s : b->m)
+        if (s->finalized || s->steps_done + n_steps > s->plan->v.time_steps) {
+            aem::set_error("aem_batch_run: a member would exceed its time_steps or is finalized");
+            return AEM_ERR_STATE;
+        }
+    return aem::guard([&]() {
+        if (n_steps == 0) return;
+        for (aem_sim* s : b->m) HIPCHK(hipStreamSynchronize(s->st));
+        if (use_graph && !b->gexec) {
+            HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+            b->enqueue(b->st);
+            HIPCHK(hipStreamEndCapture(b->st, &b->graph));
+            HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipEventRecord(b->ev0, b->st));
+        for (int64_t q = 0; q < n_steps; ++q) {
+            if (use_graph) HIPCHK(hipGraphLaunch(b->gexec, b->st));
+            else b->enqueue(b->st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b->ev1, b->st));
+        HIPCHK(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->last_ms = ms / (float)n_steps;
+        for (aem_sim* s : b->m) s->steps_done += n_steps;
+    });
+}
+
+float aem_batch_last_run_ms(const aem_batch* b) { return b ? b->last_ms : 0.f; }
+
+int32_t aem_batch_size(const aem_batch* b) { return b ? (int32_t)b->m.size() : -1; }
+
+void aem_batch_destroy(aem_batch* b) { delete b; }
 
 }  // extern "C"

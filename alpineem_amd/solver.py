@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Optional
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -74,12 +74,18 @@ class Plan:
 
 
 class Simulation:
-    """One simulation resident on the current HIP device."""
+    """One simulation resident on the current HIP device.
 
-    def __init__(self, plan: Plan):
+    ``ntff`` selects the far-field binning ("ordered": the reference's FP32
+    sums bit for bit; "atomic" or "histogram": FP64 accumulation), ``planes``
+    the z planes each bulk thread marches, ``fuse_cpml=False`` forces the
+    separate CPML kernels (aem_sim_options in include/aem.h)."""
+
+    def __init__(self, plan: Plan, ntff: str = "ordered", planes: int = 0, fuse_cpml: bool = True):
         self.plan = plan
+        opt = _lib.SimOptions(_lib.NTFF_BINNING[ntff], int(planes), 0 if fuse_cpml else 1)
         h = C.c_void_p()
-        check(lib().aem_sim_create(plan.handle, C.byref(h)))
+        check(lib().aem_sim_create_ex(plan.handle, C.byref(opt), C.byref(h)))
         self._h = h
 
     @classmethod
@@ -177,3 +183,39 @@ class Simulation:
 
     def _len_ff(self) -> int:
         return int(self.plan.view.len_ff)
+
+
+class Batch:
+    """Independent simulations advanced together (aem_batch_* in include/aem.h).
+
+    Replaces the reference's one-process-per-member fan-out for populations
+    and sweeps (optimization_scripts/GA.py:11-65).  Members keep their own
+    state: after :meth:`run`, read, finalize and write each member as usual.
+    The batch keeps references to its members so they outlive it."""
+
+    def __init__(self, sims: Sequence[Simulation]):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        check(lib().aem_batch_create(arr, len(self.sims), C.byref(h)))
+        self._h = h
+
+    def run(self, steps: Optional[int] = None, graph: bool = True) -> float:
+        """Advance every member ``steps`` steps (default: what the first
+        member has left); returns the device time per batched step in ms."""
+        if steps is None:
+            s0 = self.sims[0]
+            steps = s0.plan.time_steps - s0.steps_done
+        check(lib().aem_batch_run(self._h, int(steps), 1 if graph else 0))
+        return float(lib().aem_batch_last_run_ms(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib().aem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -100,6 +100,32 @@ aem_status aem_sim_create(const aem_plan* plan, aem_sim** out);
 /* Convenience: aem_plan_from_inputs + aem_sim_create (the sim owns the plan). */
 aem_status aem_create_from_inputs(const char* inputs_path, aem_sim** out);
 
+/* Far-field retarded-time binning (f90:7643-7927):
+ *   ORDERED    FP32 sums in the serial reference's order -- its output bit
+ *              for bit (the default);
+ *   ATOMIC     FP64 accumulation through global atomics;
+ *   HISTOGRAM  FP64 accumulation through a per-step LDS histogram of the
+ *              bin window (fastest with many angles; required for batched
+ *              far-field work).
+ * ATOMIC and HISTOGRAM keep the reference's FP32 index and weight arithmetic
+ * and agree with it to FP32 rounding. */
+enum {
+    AEM_NTFF_ORDERED = 0,
+    AEM_NTFF_ATOMIC = 1,
+    AEM_NTFF_HISTOGRAM = 2
+};
+
+/* Execution options (no counterpart in the reference, whose build flags
+ * choose serial/OpenMP/OpenACC).  Zero-initialise, then set fields. */
+typedef struct aem_sim_options {
+    int32_t ntff_binning;   /* AEM_NTFF_* */
+    int32_t planes;         /* z planes marched per thread in the bulk kernels; 0 = default (4) */
+    int32_t no_cpml_fusion; /* 1: always run the separate CPML kernels after ports/TF/SF */
+} aem_sim_options;
+
+/* aem_sim_create with options (NULL = defaults). */
+aem_status aem_sim_create_ex(const aem_plan* plan, const aem_sim_options* options, aem_sim** out);
+
 void aem_destroy(aem_sim* sim);
 
 /* ---- stepping (fdtd_solver.f90:4440-8299) --------------------------------- */
@@ -151,6 +177,34 @@ aem_status aem_get_ff_time(aem_sim* sim, float* out, int64_t n);
  * step of the S-parameter, gain and RCS calculations (:191-266). */
 aem_status aem_get_dft(aem_sim* sim, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
                        double* re_im);
+
+/* ---- batched stepping of independent simulations -------------------------- */
+
+/* Group n created, unfinalized simulations for batched stepping.  Replaces
+ * the reference's process fan-out for populations and sweeps
+ * (optimization_scripts/GA.py:11-65, main_ga.py:34-72, run_parallel.sh:10):
+ * one graph-captured step advances every member, with the bulk updates of
+ * all members in one launch per kernel variant and the ports, far-field
+ * currents/histogram binning (members created with AEM_NTFF_HISTOGRAM) and
+ * step counters in one launch each.  Each member computes exactly what it
+ * computes when run alone.  Members keep their own state: read series and
+ * fields, finalize and destroy them individually, but do not aem_run a
+ * member or destroy it while the batch exists.  Members may differ in grid,
+ * materials, sources and time_steps. */
+typedef struct aem_batch aem_batch;
+aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out);
+
+/* Advance every member n_steps (each member's total may not exceed its
+ * time_steps).  use_graph!=0 replays a captured HIP graph of one batched
+ * step.  Returns after the device has finished. */
+aem_status aem_batch_run(aem_batch* batch, int64_t n_steps, int32_t use_graph);
+
+/* Device time per batched step of the last aem_batch_run (ms). */
+float aem_batch_last_run_ms(const aem_batch* batch);
+
+int32_t aem_batch_size(const aem_batch* batch);
+
+void aem_batch_destroy(aem_batch* batch);
 
 /* Write data.dat (positional format read by utility_scripts/post_processor.py)
  * and <name>_geometry.bin.  path NULL -> the deck's own file name. */

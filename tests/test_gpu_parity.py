@@ -129,44 +129,43 @@ def test_antenna_decks_bit_exact(runs, name):
         np.testing.assert_array_equal(sim.series("e_phi", a), g["e_phi"][a], err_msg=f"E_phi {a}")
 
 
-def _with_env(var, fn):
-    old = os.environ.get(var)
-    os.environ[var] = "1"
-    try:
-        return fn()
-    finally:
-        if old is None:
-            del os.environ[var]
-        else:
-            os.environ[var] = old
-
-
 @pytest.mark.parametrize("name", ["c1_mini", "drude_slab", "master_ports"])
-def test_cpml_fusion_matches_separate_kernels(golden_dir, name):
+@pytest.mark.parametrize("planes", [0, 1, 7])
+def test_cpml_fusion_matches_separate_kernels(golden_dir, name, planes):
     """Folding the CPML correction into the bulk kernels keeps the
-    reference's per-cell add order: fields equal the separate-kernel path."""
+    reference's per-cell add order: fields equal the separate-kernel path,
+    for any number of z planes per bulk thread."""
     deck = os.path.join(golden_dir, "decks", name + ".txt")
-    fused = Simulation(Plan.from_inputs(deck))
-    plain = _with_env("AEM_NO_CPML_FUSION", lambda: Simulation(Plan.from_inputs(deck)))
+    fused = Simulation(Plan.from_inputs(deck), planes=planes)
+    plain = Simulation(Plan.from_inputs(deck), fuse_cpml=False)
     fused.run(120)
     plain.run(120)
     for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
         np.testing.assert_array_equal(fused.field(comp), plain.field(comp), err_msg=comp)
 
 
-def test_ntff_fp64_atomic_mode_close(golden_dir):
-    """The FP64 atomic binning (opt-in) agrees with the ordered FP32 binning
-    within the reference's FP32 noise."""
-    deck = os.path.join(golden_dir, "decks", "c1_mini.txt")
-    a = Simulation(Plan.from_inputs(deck))
-    b = _with_env("AEM_NTFF_ATOMIC", lambda: Simulation(Plan.from_inputs(deck)))
+@pytest.mark.parametrize("mode", ["atomic", "histogram"])
+@pytest.mark.parametrize("name", ["c1_mini", "sphere_oblique"])
+def test_ntff_fp64_modes_close(golden_dir, name, mode):
+    """The FP64 binnings (atomic, LDS histogram) agree with the ordered FP32
+    binning -- the reference's output -- within its FP32 noise: 1e-5 rel-L2,
+    or the reference's own distance from FP64 arithmetic where that is
+    larger (cross-polar nulls)."""
+    deck = os.path.join(golden_dir, "decks", name + ".txt")
+    plan = Plan.from_inputs(deck)
+    a = Simulation(plan)
+    b = Simulation(Plan.from_inputs(deck), ntff=mode)
     for s in (a, b):
         s.run()
         s.finalize()
-    for ang in (1, 3, 5):
+    o64 = Oracle(plan, fp64=True)
+    o64.step(plan.time_steps)
+    o64.finalize()
+    for ang in range(plan.view.n_angles):
         ea = np.concatenate([a.series("e_theta", ang), a.series("e_phi", ang)])
         eb = np.concatenate([b.series("e_theta", ang), b.series("e_phi", ang)])
-        assert rel_l2(eb, ea) < 1e-5
+        e64 = np.concatenate([o64.series("e_theta", ang), o64.series("e_phi", ang)])
+        assert rel_l2(eb, ea) <= max(1e-5, rel_l2(ea, e64)), (ang, rel_l2(eb, ea))
 
 
 def test_graph_and_eager_paths_agree(golden_dir):
