@@ -225,11 +225,11 @@ struct aem_sim {
     int dft_cap = 0;
 
     ~aem_sim() {
+        if (st) (void)hipStreamSynchronize(st);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (st) (void)hipStreamSynchronize(st);
         if (d_dft) (void)hipFree(d_dft);
         for (void* p : owned) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
@@ -957,11 +957,11 @@ struct aem_batch {
     float last_ms = 0.f;
 
     ~aem_batch() {
+        if (st) (void)hipStreamSynchronize(st);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (st) (void)hipStreamSynchronize(st);
         for (void* p : owned) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }

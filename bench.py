@@ -88,6 +88,41 @@ def copy_probe(mib: int = 256, reps: int = 3) -> None:
 COPY_KIB = 256 * 1024      # --copy-probe: each copy reads and writes 256 MiB
 
 
+def population_bench(steps: int, count: int = 64) -> dict:
+    """The batched-population workload: ``count`` independent dipole antennas
+    with cube edges drawn from {80, 96, ..., 160} (seed 3), random arms and up
+    to four dielectric blocks, 10-cell CPML, 50 ohm port, 19 far-field
+    angles (decks.population).  Timed for ``steps`` steps: once as 64
+    sequential solo runs, once as one batch; same options (FP64 histogram
+    far-field binning) in both.  Only stepping is timed (each solo run and
+    the batch include their one graph capture); setup is not."""
+    from alpineem_amd import decks
+    from alpineem_amd.solver import Batch, Plan, Simulation
+
+    pop = decks.population(count=count, seed=3, steps=steps)
+    plans = [Plan.from_text(d.text()) for d in pop]
+    cells = sum(p.view.nx * p.view.ny * p.view.nz for p in plans)
+    seq = 0.0
+    for p in plans:
+        s = Simulation(p, ntff="histogram")
+        t0 = time.perf_counter()
+        s.run(steps)
+        seq += time.perf_counter() - t0
+        s.close()
+    sims = [Simulation(p, ntff="histogram") for p in plans]
+    b = Batch(sims)
+    t0 = time.perf_counter()
+    b.run(steps)
+    bat = time.perf_counter() - t0
+    res = {"members": count, "cells_per_step": cells, "steps": steps, "ntff": "histogram, 19 angles",
+           "sequential_gcells": round(cells * steps / seq / 1e9, 3), "batched_gcells": round(cells * steps / bat / 1e9, 3),
+           "speedup": round(seq / bat, 3), "sequential_s": round(seq, 3), "batched_s": round(bat, 3)}
+    b.close()
+    for s in sims:
+        s.close()
+    return res
+
+
 def _short_name(kernel: str) -> str:
     """'void aemk::k_e_bulk<unsigned char, 1>(aemk::Grid, ...)' -> 'k_e_bulk<unsigned char, 1>'."""
     s = kernel.split("(")[0].replace("void ", "").strip()
@@ -142,6 +177,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also run a known-size device copy (counter scaling)")
+    ap.add_argument("--population-steps", type=int, default=1000,
+                    help="steps of the 64-member batched-population comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -192,6 +229,13 @@ def main() -> int:
 
     if args.copy_probe:
         copy_probe()
+    sim.close()
+    pop = None
+    if args.population_steps > 0:
+        pop = population_bench(args.population_steps)
+        print(f"[bench] population: 64 members, {pop['cells_per_step'] / 1e6:.1f} Mcells per step, "
+              f"sequential {pop['sequential_gcells']} Gcell/s, batched {pop['batched_gcells']} Gcell/s, "
+              f"speedup {pop['speedup']}x", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(n, args.cpu_steps)
     line = {
         "metric": "Gcell-updates/s at 400^3 FP32 (CPML, lumped port)",
@@ -211,6 +255,7 @@ def main() -> int:
                    "phase_ms": {a: round(float(b), 5) for a, b in zip(names, ph)}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "population": pop,
     }
     print(json.dumps(line), flush=True)
     return 0
