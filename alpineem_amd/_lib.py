@@ -48,6 +48,13 @@ SIGNATURES = [
     ("aem_get_ntff_raw", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_get_ff_time", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_get_dft", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+    ("aem_objective_set", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_double, C.c_int32]),
+    ("aem_objective_get", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("aem_adjoint_record", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("aem_adjoint_eps_gradient", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    ("aem_adjoint_step", C.c_int, [C.c_void_p, C.c_int64]),
+    ("aem_adjoint_get_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    ("aem_adjoint_set_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_write_outputs", C.c_int, [C.c_void_p, C.c_char_p]),
     ("aem_last_run_ms", C.c_float, [C.c_void_p]),
     ("aem_time_phases", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_float), C.c_int32]),

@@ -1468,4 +1468,426 @@ __global__ void __launch_bounds__(256) k_dft(const float* __restrict__ x, int n,
     }
 }
 
+// ===========================================================================
+// Differentiation.  A frequency-domain far-field objective and the discrete
+// adjoint of the field update (bulk H/E and CPML; the update is linear in
+// the fields, so its adjoint is the transposed update run backwards).
+// ===========================================================================
+
+// Far-field objective F = r E_pol(theta, phi, f) (e^{jkr} dropped) by the
+// frequency-domain near-to-far-field transform of the surface currents that
+// k_ntff_currents samples (same faces, same averages):
+//   L = sum M e^{jk r^.r'} dS,  N = sum J e^{jk r^.r'} dS,
+//   E_theta r = -jk/(4 pi) (L_phi + eta N_theta),  E_phi r = jk/(4 pi) (L_theta - eta N_phi),
+// M sampled at t = n dt (E after step n), J at t = (n - 1/2) dt, r' the
+// face cell's offset from the phase centre.  Accumulated in FP64.
+struct Objective {
+    int lo[3], hi[3], pbc[3];
+    int ic, jc, kc;
+    double d[3], dt, omega, k;
+    double u[3];              // direction r^
+    double gM[3], gJ[3];      // real projections of M and J components
+    double pre_re, pre_im;    // complex prefactor
+    double* F;                // (re, im), device
+};
+
+// One surface term of a face cell: value = w * sum_q field[comp][o + offs[q]].
+struct FaceTerm {
+    int comp, n;
+    long long offs[4];
+    double w;
+};
+
+// The four current terms of a face cell (M slot 0/1, J slot 0/1) exactly as
+// k_ntff_currents forms them; slot s belongs to the component (b, c)[s].
+__device__ __forceinline__ void face_terms(const Grid& g, int a, bool low, FaceTerm t[4]) {
+    const double s = low ? 1.0 : -1.0;
+    const long long X = g.PX, P = g.plane;
+    if (a == 0) {
+        t[0] = FaceTerm{2, 4, {0, X, 1, 1 + X}, -s / 4};
+        t[1] = FaceTerm{1, 4, {0, 1, P, 1 + P}, s / 4};
+        t[2] = FaceTerm{5, 2, {0, P, 0, 0}, s / 2};
+        t[3] = FaceTerm{4, 2, {0, X, 0, 0}, -s / 2};
+    } else if (a == 1) {
+        t[0] = FaceTerm{2, 4, {0, 1, X, 1 + X}, s / 4};
+        t[1] = FaceTerm{0, 4, {0, X, P, X + P}, -s / 4};
+        t[2] = FaceTerm{5, 2, {0, P, 0, 0}, -s / 2};
+        t[3] = FaceTerm{3, 2, {0, 1, 0, 0}, s / 2};
+    } else {
+        t[0] = FaceTerm{1, 4, {0, 1, P, 1 + P}, -s / 4};
+        t[1] = FaceTerm{0, 4, {0, X, P, X + P}, s / 4};
+        t[2] = FaceTerm{4, 2, {0, X, 0, 0}, s / 2};
+        t[3] = FaceTerm{3, 2, {0, 1, 0, 0}, -s / 2};
+    }
+}
+
+// Complex weight of each face term at step n:  dF/d(term value).
+__device__ __forceinline__ void face_weights(const Objective& ob, int a, int i, int j, int k, int n, double wr[4],
+                                             double wi[4]) {
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const double dS = ob.d[b] * ob.d[c];
+    const double ph = ob.k * (ob.u[0] * (i - ob.ic) * ob.d[0] + ob.u[1] * (j - ob.jc) * ob.d[1] +
+                              ob.u[2] * (k - ob.kc) * ob.d[2]);
+    const double tE = n * ob.dt, tH = (n - 0.5) * ob.dt;
+    double sp, cp, sE, cE, sH, cH;
+    sincos(ph, &sp, &cp);
+    sincos(-ob.omega * tE, &sE, &cE);
+    sincos(-ob.omega * tH, &sH, &cH);
+    // g = pre * dS * e^{j ph}
+    const double gr = (ob.pre_re * cp - ob.pre_im * sp) * dS, gi = (ob.pre_re * sp + ob.pre_im * cp) * dS;
+    const double eMr = gr * cE - gi * sE, eMi = gr * sE + gi * cE;
+    const double eJr = gr * cH - gi * sH, eJi = gr * sH + gi * cH;
+    wr[0] = eMr * ob.gM[b]; wi[0] = eMi * ob.gM[b];
+    wr[1] = eMr * ob.gM[c]; wi[1] = eMi * ob.gM[c];
+    wr[2] = eJr * ob.gJ[b]; wi[2] = eJi * ob.gJ[b];
+    wr[3] = eJr * ob.gJ[c]; wi[3] = eJi * ob.gJ[c];
+}
+
+// Face cell (face, u, v) of the NTFF box -> (i, j, k); false outside.
+__device__ __forceinline__ bool face_cell(const Objective& ob, int face, int uu, int vv, int& i, int& j, int& k) {
+    const int a = face / 2;
+    if (ob.pbc[a]) return false;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const int u = ob.lo[b] + uu, v = ob.lo[c] + vv;
+    if (u > ob.hi[b] || v > ob.hi[c]) return false;
+    const int pos = (face & 1) ? ob.hi[a] : ob.lo[a];
+    if (a == 0) { i = pos; j = u; k = v; }
+    else if (a == 1) { i = u; j = pos; k = v; }
+    else { i = u; j = v; k = pos; }
+    return true;
+}
+
+// F += sum over face cells of this step's terms.  grid (x, y, 6 faces), block 256.
+__global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const float* Ex, const float* Ey,
+                                                   const float* Ez, const float* Hx, const float* Hy,
+                                                   const float* Hz, const int* step) {
+    const int face = blockIdx.z;
+    const int tid = threadIdx.x;
+    const float* Fl[6] = {Ex, Ey, Ez, Hx, Hy, Hz};
+    double re = 0.0, im = 0.0;
+    int i, j, k;
+    if (face_cell(ob, face, blockIdx.x * 32 + (tid & 31), blockIdx.y * 8 + (tid >> 5), i, j, k)) {
+        const int n = *step;
+        const int a = face / 2;
+        FaceTerm t[4];
+        face_terms(g, a, (face & 1) == 0, t);
+        double wr[4], wi[4];
+        face_weights(ob, a, i, j, k, n, wr, wi);
+        const long long o = off(g, i, j, k);
+        for (int q = 0; q < 4; ++q) {
+            double s = 0.0;
+            for (int r = 0; r < t[q].n; ++r) s += (double)Fl[t[q].comp][o + t[q].offs[r]];
+            const double val = t[q].w * s;
+            re += wr[q] * val;
+            im += wi[q] * val;
+        }
+    }
+    __shared__ double sr[256], si[256];
+    sr[tid] = re;
+    si[tid] = im;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sr[tid] += sr[tid + w];
+            si[tid] += si[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        atomicAdd(ob.F, sr[0]);
+        atomicAdd(ob.F + 1, si[0]);
+    }
+}
+
+// Adjoint source of J = |F|^2 at step n: dJ/d(term) = 2 Re(conj(F) w),
+// scattered onto the adjoint fields through each term's average.
+__global__ void __launch_bounds__(256) k_adj_source(Grid g, Objective ob, double Fr, double Fi, float* lEx,
+                                                    float* lEy, float* lEz, float* lHx, float* lHy, float* lHz,
+                                                    const int* step) {
+    const int face = blockIdx.z;
+    int i, j, k;
+    if (!face_cell(ob, face, blockIdx.x * 32 + (threadIdx.x & 31), blockIdx.y * 8 + (threadIdx.x >> 5), i, j, k))
+        return;
+    float* L[6] = {lEx, lEy, lEz, lHx, lHy, lHz};
+    const int n = *step;
+    const int a = face / 2;
+    FaceTerm t[4];
+    face_terms(g, a, (face & 1) == 0, t);
+    double wr[4], wi[4];
+    face_weights(ob, a, i, j, k, n, wr, wi);
+    const long long o = off(g, i, j, k);
+    for (int q = 0; q < 4; ++q) {
+        const double dj = 2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w;
+        for (int r = 0; r < t[q].n; ++r) atomicAdd(&L[t[q].comp][o + t[q].offs[r]], (float)dj);
+    }
+}
+
+// Design-box record of one step (between the H and E phases): the E value
+// each design edge starts the step with and the curl term S of its update
+// E' = ga E + gb S, both in the bulk kernel's FP32 arithmetic.
+struct AdjRec {
+    int lo[3];        // first edge (1-based) of the record box
+    int n[3];         // extents
+    long long nedge;  // n[0] * n[1] * n[2]
+    float* hist;      // [step][2 (E, S)][3 comps][nedge]
+};
+
+__global__ void k_adj_record(Grid g, AdjRec r, const float* Ex, const float* Ey, const float* Ez, const float* Hx,
+                             const float* Hy, const float* Hz, const float* dex, const float* dey, const float* dez,
+                             const int* step) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= r.nedge) return;
+    const int i = r.lo[0] + (int)(e % r.n[0]);
+    const long long rr = e / r.n[0];
+    const int j = r.lo[1] + (int)(rr % r.n[1]), k = r.lo[2] + (int)(rr / r.n[1]);
+    const long long o = off(g, i, j, k);
+    const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
+    const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+    const float s0 = (hz - Hz[o - g.PX]) * dy_j + (Hy[o - g.plane] - hy) * dz_k;
+    const float s1 = (Hz[o - 1] - hz) * dx_i + (hx - Hx[o - g.plane]) * dz_k;
+    const float s2 = (hy - Hy[o - 1]) * dx_i + (Hx[o - g.PX] - hx) * dy_j;
+    float* h = r.hist + (long long)(*step - 1) * 6 * r.nedge;
+    h[e] = Ex[o];
+    h[r.nedge + e] = Ey[o];
+    h[2 * r.nedge + e] = Ez[o];
+    h[3 * r.nedge + e] = s0;
+    h[4 * r.nedge + e] = s1;
+    h[5 * r.nedge + e] = s2;
+}
+
+// dJ/dga and dJ/dgb of every design edge from step n: lambda_E' * E and lambda_E' * S.
+__global__ void k_adj_grad(Grid g, AdjRec r, const float* lEx, const float* lEy, const float* lEz, double* dga,
+                           double* dgb, const int* step) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= r.nedge) return;
+    const int i = r.lo[0] + (int)(e % r.n[0]);
+    const long long rr = e / r.n[0];
+    const int j = r.lo[1] + (int)(rr % r.n[1]), k = r.lo[2] + (int)(rr / r.n[1]);
+    const long long o = off(g, i, j, k);
+    const float* h = r.hist + (long long)(*step - 1) * 6 * r.nedge;
+    const float lam[3] = {lEx[o], lEy[o], lEz[o]};
+    for (int c = 0; c < 3; ++c) {
+        dga[c * r.nedge + e] += (double)lam[c] * (double)h[c * r.nedge + e];
+        dgb[c * r.nedge + e] += (double)lam[c] * (double)h[(3 + c) * r.nedge + e];
+    }
+}
+
+// Is E component c at (i,j,k) updated by the bulk loop (outer PEC tangential
+// E is not)?  Ex: j,k >= 2;  Ey: i,k >= 2;  Ez: i,j >= 2; all <= N-1.
+__device__ __forceinline__ bool e_upd(const Grid& g, int c, int i, int j, int k) {
+    if (i < 1 || j < 1 || k < 1 || i > g.NX - 1 || j > g.NY - 1 || k > g.NZ - 1) return false;
+    if (c == 0) return j >= 2 && k >= 2;
+    if (c == 1) return i >= 2 && k >= 2;
+    return i >= 2 && j >= 2;
+}
+__device__ __forceinline__ bool h_in(const Grid& g, int i, int j, int k) {
+    return i >= 1 && j >= 1 && k >= 1 && i <= g.NX - 1 && j <= g.NY - 1 && k <= g.NZ - 1;
+}
+
+// mu_c = gb_c * lambda_E_c on updated components, 0 elsewhere.
+template <typename IdxT>
+__device__ __forceinline__ float adj_mu(const Grid& g, const float* lE, const IdxT* idx, const float* tup, int c,
+                                        int i, int j, int k) {
+    if (!e_upd(g, c, i, j, k)) return 0.0f;
+    const long long o = off(g, i, j, k);
+    return tup[(int)idx[o] * 6 + 2 * c + 1] * lE[o];
+}
+
+// lambda_H += (transposed E-update curl) applied to mu.  One thread per H cell.
+template <typename IdxT>
+__global__ void k_adj_h_gather(Grid g, float* lHx, float* lHy, float* lHz, const float* lEx, const float* lEy,
+                               const float* lEz, const IdxT* idx, const float* tup, const float* dex,
+                               const float* dey, const float* dez) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.NX - 1 || j > g.NY - 1) return;
+    const long long o = off(g, i, j, k);
+    auto de = [](const float* d, int n, int q) { return q <= n - 1 ? d[q - 1] : 0.0f; };
+    const float mx = adj_mu(g, lEx, idx, tup, 0, i, j, k), mx_j = adj_mu(g, lEx, idx, tup, 0, i, j + 1, k),
+                mx_k = adj_mu(g, lEx, idx, tup, 0, i, j, k + 1);
+    const float my = adj_mu(g, lEy, idx, tup, 1, i, j, k), my_i = adj_mu(g, lEy, idx, tup, 1, i + 1, j, k),
+                my_k = adj_mu(g, lEy, idx, tup, 1, i, j, k + 1);
+    const float mz = adj_mu(g, lEz, idx, tup, 2, i, j, k), mz_i = adj_mu(g, lEz, idx, tup, 2, i + 1, j, k),
+                mz_j = adj_mu(g, lEz, idx, tup, 2, i, j + 1, k);
+    const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
+    const float dx_i1 = de(dex, g.NX, i + 1), dy_j1 = de(dey, g.NY, j + 1), dz_k1 = de(dez, g.NZ, k + 1);
+    lHx[o] += my * dz_k - my_k * dz_k1 + mz_j * dy_j1 - mz * dy_j;
+    lHy[o] += mx_k * dz_k1 - mx * dz_k + mz * dx_i - mz_i * dx_i1;
+    lHz[o] += mx * dy_j - mx_j * dy_j1 + my_i * dx_i1 - my * dx_i;
+}
+
+// lambda_E <- ga lambda_E + DB (transposed H-update curl) lambda_H on updated
+// components; 0 on the fixed outer tangential components.
+template <typename IdxT>
+__global__ void k_adj_e_gather(Grid g, float* lEx, float* lEy, float* lEz, const float* lHx, const float* lHy,
+                               const float* lHz, const IdxT* idx, const float* tup, const float* dhx,
+                               const float* dhy, const float* dhz, float DB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.NX || j > g.NY) return;
+    const long long o = off(g, i, j, k);
+    auto lh = [&](const float* L, int a, int b, int c) { return h_in(g, a, b, c) ? L[off(g, a, b, c)] : 0.0f; };
+    auto dh = [](const float* d, int n, int q) { return (q >= 1 && q <= n - 1) ? d[q - 1] : 0.0f; };
+    const int t = e_upd(g, 0, i, j, k) || e_upd(g, 1, i, j, k) || e_upd(g, 2, i, j, k) ? (int)idx[o] * 6 : 0;
+    if (e_upd(g, 0, i, j, k))
+        lEx[o] = tup[t] * lEx[o] + DB * (lh(lHy, i, j, k) * dh(dhz, g.NZ, k) - lh(lHy, i, j, k - 1) * dh(dhz, g.NZ, k - 1) +
+                                         lh(lHz, i, j - 1, k) * dh(dhy, g.NY, j - 1) - lh(lHz, i, j, k) * dh(dhy, g.NY, j));
+    else if (i <= g.NX && j <= g.NY && k <= g.NZ)
+        lEx[o] = 0.0f;
+    if (e_upd(g, 1, i, j, k))
+        lEy[o] = tup[t + 2] * lEy[o] + DB * (lh(lHx, i, j, k - 1) * dh(dhz, g.NZ, k - 1) - lh(lHx, i, j, k) * dh(dhz, g.NZ, k) +
+                                             lh(lHz, i, j, k) * dh(dhx, g.NX, i) - lh(lHz, i - 1, j, k) * dh(dhx, g.NX, i - 1));
+    else
+        lEy[o] = 0.0f;
+    if (e_upd(g, 2, i, j, k))
+        lEz[o] = tup[t + 4] * lEz[o] + DB * (lh(lHx, i, j, k) * dh(dhy, g.NY, j) - lh(lHx, i, j - 1, k) * dh(dhy, g.NY, j - 1) +
+                                             lh(lHy, i - 1, j, k) * dh(dhx, g.NX, i - 1) - lh(lHy, i, j, k) * dh(dhx, g.NX, i));
+    else
+        lEz[o] = 0.0f;
+}
+
+// Transposed CPML of the E phase, slab normal to axis `ax` (threads as in
+// k_e_cpml_*).  nu = lambda_psi' + gb lambda_E on the psi-updated cells;
+// lambda_psi <- b nu; lambda_H gets the transposed difference.
+template <typename IdxT>
+__global__ void k_adj_e_cpml(Grid g, Cpml p, Cpml lp, int ax, const float* lEx, const float* lEy,
+                             const float* lEz, float* lHx, float* lHy, float* lHz, const IdxT* idx,
+                             const float* tup, float dd) {
+    const int A = ax;
+    const int n1 = p.n[2 * A], n2 = p.n[2 * A + 1];
+    const int nl = max(n1 - 1, 0), nh = max(n2 - 1, 0), nt = nl + nh;
+    const int N[3] = {g.NX, g.NY, g.NZ};
+    const int o1 = A == 0 ? 1 : 0, o2 = A == 2 ? 1 : 2;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)nt * (N[o1] - 1) * (N[o2] - 1);
+    if (tid >= tot) return;
+    const int t = (int)(tid % nt);
+    const long long r = tid / nt;
+    const int u = (int)(r % (N[o1] - 1)) + 1, w = (int)(r / (N[o1] - 1)) + 1;
+    int pos, f, q, ql;
+    if (t < nl) { pos = t + 2; f = 2 * A; q = pos; ql = t; }
+    else { pos = N[A] + 1 - n2 + (t - nl); f = 2 * A + 1; q = N[A] + 1 - pos; ql = q - 2; }
+    int ijk[3];
+    ijk[A] = pos; ijk[o1] = u; ijk[o2] = w;
+    const int i = ijk[0], j = ijk[1], k = ijk[2];
+    const int ntf = f % 2 == 0 ? nl : nh;
+    long long po;
+    if (A == 0) po = (long long)ql + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+    else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * ((long long)ql + (long long)ntf * (k - 1));
+    else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * ql);
+    const long long o = off(g, i, j, k);
+    const long long step = A == 0 ? 1 : (A == 1 ? g.PX : g.plane);
+    const float cb = p.b[f][q - 1], cc = p.c[f][q - 1];
+    const int t6 = (int)idx[o] * 6;
+    if (A == 0) {
+        if (k >= 2) {   // psi_Eyx:  Ey += gby psi,  psi' = b psi + c (Hz(i-1) - Hz) / dx
+            const float nu = lp.psi[f][0][po] + tup[t6 + 3] * lEy[o];
+            lp.psi[f][0][po] = cb * nu;
+            atomicAdd(&lHz[o - step], cc * nu / dd);
+            atomicAdd(&lHz[o], -(cc * nu / dd));
+        }
+        if (j >= 2) {   // psi_Ezx:  Ez += gbz psi,  psi' = b psi + c (Hy - Hy(i-1)) / dx
+            const float nu = lp.psi[f][1][po] + tup[t6 + 5] * lEz[o];
+            lp.psi[f][1][po] = cb * nu;
+            atomicAdd(&lHy[o], cc * nu / dd);
+            atomicAdd(&lHy[o - step], -(cc * nu / dd));
+        }
+    } else if (A == 1) {
+        if (k >= 2) {   // psi_Exy:  Ex += gbx psi,  psi' = b psi + c (Hz - Hz(j-1)) / dy
+            const float nu = lp.psi[f][0][po] + tup[t6 + 1] * lEx[o];
+            lp.psi[f][0][po] = cb * nu;
+            atomicAdd(&lHz[o], cc * nu / dd);
+            atomicAdd(&lHz[o - step], -(cc * nu / dd));
+        }
+        if (i >= 2) {   // psi_Ezy:  Ez += gbz psi,  psi' = b psi + c (Hx(j-1) - Hx) / dy
+            const float nu = lp.psi[f][1][po] + tup[t6 + 5] * lEz[o];
+            lp.psi[f][1][po] = cb * nu;
+            atomicAdd(&lHx[o - step], cc * nu / dd);
+            atomicAdd(&lHx[o], -(cc * nu / dd));
+        }
+    } else {
+        if (j >= 2) {   // psi_Exz:  Ex += gbx psi,  psi' = b psi + c (Hy(k-1) - Hy) / dz
+            const float nu = lp.psi[f][0][po] + tup[t6 + 1] * lEx[o];
+            lp.psi[f][0][po] = cb * nu;
+            atomicAdd(&lHy[o - step], cc * nu / dd);
+            atomicAdd(&lHy[o], -(cc * nu / dd));
+        }
+        if (i >= 2) {   // psi_Eyz:  Ey += gby psi,  psi' = b psi + c (Hx - Hx(k-1)) / dz
+            const float nu = lp.psi[f][1][po] + tup[t6 + 3] * lEy[o];
+            lp.psi[f][1][po] = cb * nu;
+            atomicAdd(&lHx[o], cc * nu / dd);
+            atomicAdd(&lHx[o - step], -(cc * nu / dd));
+        }
+    }
+}
+
+// Transposed CPML of the H phase, slab normal to axis `ax` (threads as in
+// k_h_cpml_*).  nu = lambda_psi' + DB lambda_H; lambda_psi <- b nu.
+__global__ void k_adj_h_cpml(Grid g, Cpml p, Cpml lp, int ax, float* lEx, float* lEy, float* lEz,
+                             const float* lHx, const float* lHy, const float* lHz, float DB, float dd) {
+    const int A = ax;
+    const int n1 = p.n[2 * A], n2 = p.n[2 * A + 1];
+    const int nl = max(n1 - 1, 0), nh = max(n2 - 1, 0), nt = nl + nh;
+    const int N[3] = {g.NX, g.NY, g.NZ};
+    const int o1 = A == 0 ? 1 : 0, o2 = A == 2 ? 1 : 2;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)nt * (N[o1] - 1) * (N[o2] - 1);
+    if (tid >= tot) return;
+    const int t = (int)(tid % nt);
+    const long long r = tid / nt;
+    const int u = (int)(r % (N[o1] - 1)) + 1, w = (int)(r / (N[o1] - 1)) + 1;
+    int pos, f, q;
+    if (t < nl) { pos = t + 1; f = 2 * A; q = pos; }
+    else { pos = N[A] + 1 - n2 + (t - nl); f = 2 * A + 1; q = N[A] - pos; }
+    int ijk[3];
+    ijk[A] = pos; ijk[o1] = u; ijk[o2] = w;
+    const int i = ijk[0], j = ijk[1], k = ijk[2];
+    const int ntf = f % 2 == 0 ? nl : nh;
+    long long po;
+    if (A == 0) po = (long long)(q - 1) + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+    else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * ((long long)(q - 1) + (long long)ntf * (k - 1));
+    else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (q - 1));
+    const long long o = off(g, i, j, k);
+    const long long step = A == 0 ? 1 : (A == 1 ? g.PX : g.plane);
+    const float cb = p.b[f][q - 1], cc = p.c[f][q - 1];
+    if (A == 0) {
+        // psi_Hyx' = b psi + c (Ez(i+1) - Ez) / dx;  Hy += DB psi'
+        float nu = lp.psi[f][0][po] + DB * lHy[o];
+        lp.psi[f][0][po] = cb * nu;
+        atomicAdd(&lEz[o + step], cc * nu / dd);
+        atomicAdd(&lEz[o], -(cc * nu / dd));
+        // psi_Hzx' = b psi + c (Ey - Ey(i+1)) / dx;  Hz += DB psi'
+        nu = lp.psi[f][1][po] + DB * lHz[o];
+        lp.psi[f][1][po] = cb * nu;
+        atomicAdd(&lEy[o], cc * nu / dd);
+        atomicAdd(&lEy[o + step], -(cc * nu / dd));
+    } else if (A == 1) {
+        // psi_Hxy' = b psi + c (Ez - Ez(j+1)) / dy;  Hx += DB psi'
+        float nu = lp.psi[f][0][po] + DB * lHx[o];
+        lp.psi[f][0][po] = cb * nu;
+        atomicAdd(&lEz[o], cc * nu / dd);
+        atomicAdd(&lEz[o + step], -(cc * nu / dd));
+        // psi_Hzy' = b psi + c (Ex(j+1) - Ex) / dy;  Hz += DB psi'
+        nu = lp.psi[f][1][po] + DB * lHz[o];
+        lp.psi[f][1][po] = cb * nu;
+        atomicAdd(&lEx[o + step], cc * nu / dd);
+        atomicAdd(&lEx[o], -(cc * nu / dd));
+    } else {
+        // psi_Hxz' = b psi + c (Ey(k+1) - Ey) / dz;  Hx += DB psi'
+        float nu = lp.psi[f][0][po] + DB * lHx[o];
+        lp.psi[f][0][po] = cb * nu;
+        atomicAdd(&lEy[o + step], cc * nu / dd);
+        atomicAdd(&lEy[o], -(cc * nu / dd));
+        // psi_Hyz' = b psi + c (Ex - Ex(k+1)) / dz;  Hy += DB psi'
+        nu = lp.psi[f][1][po] + DB * lHy[o];
+        lp.psi[f][1][po] = cb * nu;
+        atomicAdd(&lEx[o], cc * nu / dd);
+        atomicAdd(&lEx[o + step], -(cc * nu / dd));
+    }
+}
+
+__global__ void k_step_dec(int* step) { *step = *step - 1; }
+
 }  // namespace aemk

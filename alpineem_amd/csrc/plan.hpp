@@ -122,6 +122,10 @@ struct Plan {
 
     // rasterised geometry kept for the _geometry.bin writer (f90:2355-2357)
     std::vector<float> geom;                      // (4, nx, ny, nz) Fortran order
+
+    // bulk material index of every cell (0 = background, m = deck.mats[m-1]),
+    // (i-1) + (NX-1)*((j-1) + (NY-1)*(k-1)); kept for permittivity gradients
+    std::vector<uint16_t> cell_mat;
 };
 
 Plan* build_plan(const std::string& deck_text, const std::string& base_dir);

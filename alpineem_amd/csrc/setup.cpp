@@ -644,6 +644,7 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
                     size_t off = (size_t)(i - 1) + (size_t)v.PX * ((size_t)(j - 1) + (size_t)NY * (k - 1));
                     cell_tuple[off] = add_tuple(t6);
                 }
+        P->cell_mat = cellm;
         const size_t ntup = tmap.size();
         if (ntup > 65536) throw Error(5, "more than 65536 distinct update-coefficient tuples");
         v.n_tuple = (int32_t)ntup;

@@ -121,7 +121,9 @@ enum : unsigned {
     kPartNtff = 1u << 9,     // far-field currents and binning
     kPartOutPost = 1u << 10, // phase-centre samples
     kPartStep = 1u << 11,    // step counter
-    kPartAll = (1u << 12) - 1
+    kPartRecord = 1u << 12,  // design-box record for the adjoint (between H and E)
+    kPartObjective = 1u << 13, // far-field objective accumulation
+    kPartAll = (1u << 14) - 1
 };
 
 Launch make_launch(const std::vector<Box>& boxes, int c) {
@@ -216,6 +218,18 @@ struct aem_sim {
     int ntff_mu = 0, ntff_mv = 0;  // largest face extents of the far-field box
     int planes = kPlanes;
     bool allow_fuse = true;
+    // differentiation
+    Objective obj{};
+    bool obj_on = false;
+    AdjRec rec{};
+    bool rec_on = false;
+    int rec_lo[3] = {0, 0, 0}, rec_hi[3] = {0, 0, 0};   // design cell box
+    float* lam[6] = {};            // adjoint fields (allocated at the first backward step)
+    Cpml lpE{}, lpH{};             // adjoint CPML states
+    int* d_bstep = nullptr;
+    void ensure_adjoint();
+    template <class IdxT>
+    void enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, double* dga, double* dgb);
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -519,6 +533,10 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Hy, Hx);
     }
 
+    if ((parts & kPartRecord) && rec_on)
+        k_adj_record<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2],
+                                                          d_step);
+
     // ---- E phase ----  (marks: [2,3] the core bulk launch, [3,4] the rest of E)
     if (mark) HIPCHK(hipEventRecord(mark[2], st));
     if (parts & kPartEBulk) e_bulk(bulk[0]);
@@ -594,6 +612,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             k_ntff_bins<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6 * v.n_angles), bb, 0, s>>>(nf, nb, d_step);
         }
     }
+    if ((parts & kPartObjective) && obj_on)
+        k_objective<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6), 256, 0, s>>>(g, obj, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
     if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
         k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
     if (parts & kPartStep) k_step_inc<<<1, 1, 0, s>>>(d_step);
@@ -1058,7 +1078,7 @@ struct aem_batch {
         }
         each(kPartHPre);
         if (n_port) k_ports_h_batch<<<n_port, 256, 0, s>>>(d_pargs);
-        each(kPartHPost);
+        each(kPartHPost | kPartRecord);
         for (const Group& gr : eg) {
             if (gr.idx_bytes == 1) {
                 if (gr.c == 0) k_e_bulk_batch<uint8_t, 0><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
@@ -1079,7 +1099,7 @@ struct aem_batch {
             k_ntff_hist_batch<<<dim3(max_angles, 6, n_ntff), 256, 0, s>>>(d_nargs);
         }
         for (int q : ntff_alone) m[q]->enqueue_parts(nullptr, s, kPartNtff);
-        each(kPartOutPost);
+        each(kPartObjective | kPartOutPost);
         k_step_inc_batch<<<cdiv(n, 256), 256, 0, s>>>(d_steps, n);
     }
 };
@@ -1160,5 +1180,332 @@ float aem_batch_last_run_ms(const aem_batch* b) { return b ? b->last_ms : 0.f; }
 int32_t aem_batch_size(const aem_batch* b) { return b ? (int32_t)b->m.size() : -1; }
 
 void aem_batch_destroy(aem_batch* b) { delete b; }
+
+}  // extern "C"
+
+// ===========================================================================
+// Differentiation (no counterpart in the reference solver; its adjoint
+// workflow, optimization_scripts/adjoint_optimization/adjoint_optimization.py
+// :76-134, runs a second forward simulation with an adjoint source and
+// correlates FFTs of field slices).  Here the objective is the far-field
+// amplitude of the frequency-domain near-to-far-field transform, and its
+// gradient is the exact discrete adjoint of this solver's update: the field
+// update is linear, so the adjoint state runs the transposed H/E/CPML update
+// backwards from the last step, and the only forward data it needs are the
+// values multiplying the design coefficients (E and the curl term S of every
+// design edge, recorded each step).  The design is the permittivity of the
+// deck's materials inside a box of cells.
+// ===========================================================================
+
+void aem_sim::ensure_adjoint() {
+    if (lam[0]) return;
+    const aem_plan_view& v = plan->v;
+    for (int q = 0; q < 6; ++q) lam[q] = zeros<float>((size_t)v.ncell + v.plane);
+    lpE = cpE;
+    lpH = cpH;
+    for (int f = 0; f < 6; ++f) {
+        const int a = f / 2;
+        const size_t nt = (size_t)std::max(v.npml[f] - 1, 0);
+        size_t trans;
+        if (a == 0) trans = (size_t)v.NY * v.NZ;
+        else if (a == 1) trans = (size_t)v.PX * v.NZ;
+        else trans = (size_t)v.PX * v.NY;
+        for (int s = 0; s < 2; ++s) {
+            lpE.psi[f][s] = zeros<float>(nt * trans);
+            lpH.psi[f][s] = zeros<float>(nt * trans);
+        }
+    }
+    d_bstep = zeros<int>(1);
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+// One step of the adjoint sweep (the adjoint state after step n -> after
+// step n-1).  With `objective`, first adds the source of J = |F|^2 at step n
+// and accumulates the design-edge derivatives.
+template <class IdxT>
+void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, double* dga, double* dgb) {
+    const aem_plan_view& v = plan->v;
+    const IdxT* idx = static_cast<const IdxT*>(d_idx);
+    float *lEx = lam[0], *lEy = lam[1], *lEz = lam[2], *lHx = lam[3], *lHy = lam[4], *lHz = lam[5];
+    const dim3 bb(32, 8, 1);
+    const float dd[3] = {v.dx, v.dy, v.dz};
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    if (objective) {
+        k_adj_source<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6), 256, 0, s>>>(g, obj, Fr, Fi, lEx, lEy, lEz, lHx,
+                                                                                  lHy, lHz, d_bstep);
+        if (rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
+    }
+    auto slab_threads = [&](int a) {
+        const int nt = std::max(v.npml[2 * a] - 1, 0) + std::max(v.npml[2 * a + 1] - 1, 0);
+        const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+        return (long long)nt * (N[o1] - 1) * (N[o2] - 1);
+    };
+    for (int a = 0; a < 3; ++a) {
+        const long long tot = slab_threads(a);
+        if (tot > 0)
+            k_adj_e_cpml<IdxT><<<cdiv(tot, 256), 256, 0, s>>>(g, cpE, lpE, a, lEx, lEy, lEz, lHx, lHy, lHz, idx,
+                                                              d_tup, dd[a]);
+    }
+    k_adj_h_gather<IdxT><<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8), v.NZ - 1), bb, 0, s>>>(
+        g, lHx, lHy, lHz, lEx, lEy, lEz, idx, d_tup, d_den[0], d_den[1], d_den[2]);
+    k_adj_e_gather<IdxT><<<dim3(cdiv(v.NX, 32), cdiv(v.NY, 8), v.NZ), bb, 0, s>>>(
+        g, lEx, lEy, lEz, lHx, lHy, lHz, idx, d_tup, d_den[3], d_den[4], d_den[5], v.DB);
+    for (int a = 0; a < 3; ++a) {
+        const long long tot = slab_threads(a);
+        if (tot > 0)
+            k_adj_h_cpml<<<cdiv(tot, 256), 256, 0, s>>>(g, cpH, lpH, a, lEx, lEy, lEz, lHx, lHy, lHz, v.DB, dd[a]);
+    }
+    k_step_dec<<<1, 1, 0, s>>>(d_bstep);
+}
+
+namespace {
+
+// Features whose update is not part of the transposed step.
+const char* adjoint_unsupported(const aem_plan_view& v) {
+    if (v.n_ports > 0) return "lumped ports";
+    if (v.n_sheet_planes[0] + v.n_sheet_planes[1] + v.n_sheet_planes[2] > 0) return "thin sheets";
+    if (v.has_drude) return "Drude media";
+    if (v.pbc[0] || v.pbc[1] || v.pbc[2]) return "periodic boundaries";
+    return nullptr;
+}
+
+// dJ/d eps_r of every material from the per-edge derivatives dJ/dga, dJ/dgb:
+// each edge's eps and sigma are the four-cell averages of the setup
+// (fdtd_solver.f90:2516-2527), its Ga/Gb the formulas of f90:2837-2853.
+// Only cells inside the design box [lo, hi] count: the result is the
+// derivative for a change of the material's permittivity in those cells.
+void eps_chain_rule(const aem::Plan& P, const AdjRec& r, const int lo[3], const int hi[3],
+                    const std::vector<double>& dga, const std::vector<double>& dgb, int n_mat, double* grad) {
+    const aem_plan_view& v = P.v;
+    const double dt = v.dt, e0 = aem::kEp0;
+    auto cidx = [&](int i, int j, int k) {
+        return (size_t)(i - 1) + (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (size_t)(k - 1));
+    };
+    for (int m = 0; m < n_mat; ++m) grad[m] = 0.0;
+    for (long long e = 0; e < r.nedge; ++e) {
+        const int i = r.lo[0] + (int)(e % r.n[0]);
+        const long long rr = e / r.n[0];
+        const int j = r.lo[1] + (int)(rr % r.n[1]), k = r.lo[2] + (int)(rr / r.n[1]);
+        for (int c = 0; c < 3; ++c) {
+            int cell[4][3];
+            if (c == 0) { int t[4][3] = {{i, j, k}, {i, j - 1, k}, {i, j, k - 1}, {i, j - 1, k - 1}}; std::memcpy(cell, t, sizeof t); }
+            else if (c == 1) { int t[4][3] = {{i, j, k}, {i - 1, j, k}, {i, j, k - 1}, {i - 1, j, k - 1}}; std::memcpy(cell, t, sizeof t); }
+            else { int t[4][3] = {{i, j, k}, {i - 1, j, k}, {i, j - 1, k}, {i - 1, j - 1, k}}; std::memcpy(cell, t, sizeof t); }
+            int m[4];
+            float ev[4], sv[4];
+            for (int q = 0; q < 4; ++q) {
+                m[q] = P.cell_mat[cidx(cell[q][0], cell[q][1], cell[q][2])];
+                ev[q] = m[q] ? P.deck.mats[m[q] - 1].eps[c] : 1.0f;
+                sv[q] = m[q] ? P.deck.mats[m[q] - 1].sig[c] : 0.0f;
+            }
+            const double eps = (((ev[0] + ev[1]) + ev[2]) + ev[3]) / 4.0f;
+            const double sig = (((sv[0] + sv[1]) + sv[2]) + sv[3]) / 4.0f;
+            const double al = sig * dt / (2.0 * e0 * eps), A = dt / (e0 * eps);
+            const double dgb_de = -A / (eps * (1.0 + al) * (1.0 + al));
+            const double dga_de = 2.0 * al / (eps * (1.0 + al) * (1.0 + al));
+            const double dj = dga[c * r.nedge + e] * dga_de + dgb[c * r.nedge + e] * dgb_de;
+            for (int q = 0; q < 4; ++q) {
+                bool in = m[q] >= 1 && m[q] <= n_mat;
+                for (int a = 0; a < 3; ++a) in = in && cell[q][a] >= lo[a] && cell[q][a] <= hi[a];
+                if (in) grad[m[q] - 1] += dj / 4.0;
+            }
+        }
+    }
+}
+
+bool fresh(const aem_sim* s) { return s->steps_done == 0 && !s->gexec; }
+
+}  // namespace
+
+extern "C" {
+
+aem_status aem_objective_set(aem_sim* s, float theta_deg, float phi_deg, double freq_hz, int32_t pol) {
+    if (!s || freq_hz <= 0 || (pol != 0 && pol != 1)) {
+        aem::set_error("aem_objective_set: bad argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (!fresh(s) || v.n_angles <= 0) {
+        aem::set_error("aem_objective_set: needs a deck with a far-field box and a sim that has not stepped");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        Objective& o = s->obj;
+        for (int a = 0; a < 3; ++a) { o.lo[a] = v.ff_lo[a]; o.hi[a] = v.ff_hi[a]; o.pbc[a] = v.pbc[a]; }
+        o.ic = v.ic; o.jc = v.jc; o.kc = v.kc;
+        o.d[0] = v.dx; o.d[1] = v.dy; o.d[2] = v.dz;
+        o.dt = v.dt;
+        o.omega = 2.0 * M_PI * freq_hz;
+        o.k = o.omega / 299792458.0;
+        const double th = theta_deg * M_PI / 180.0, ph = phi_deg * M_PI / 180.0;
+        const double st = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
+        o.u[0] = st * cp; o.u[1] = st * sp; o.u[2] = ct;
+        const double eta = std::sqrt((double)aem::kMu0 / (double)aem::kEp0);
+        const double th_v[3] = {ct * cp, ct * sp, -st}, ph_v[3] = {-sp, cp, 0.0};
+        for (int a = 0; a < 3; ++a) {
+            if (pol == 0) { o.gM[a] = ph_v[a]; o.gJ[a] = eta * th_v[a]; }
+            else { o.gM[a] = th_v[a]; o.gJ[a] = -eta * ph_v[a]; }
+        }
+        o.pre_re = 0.0;
+        o.pre_im = (pol == 0 ? -1.0 : 1.0) * o.k / (4.0 * M_PI);
+        if (!o.F) o.F = s->zeros<double>(2);
+        s->obj_on = true;
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_objective_get(aem_sim* s, double* re_im) {
+    if (!s || !re_im || !s->obj_on) {
+        aem::set_error("aem_objective_get: no objective set");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        HIPCHK(hipMemcpyAsync(re_im, s->obj.F, 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_adjoint_record(aem_sim* s, const int32_t* lo, const int32_t* hi) {
+    if (!s || !lo || !hi) {
+        aem::set_error("aem_adjoint_record: null argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (!fresh(s)) {
+        aem::set_error("aem_adjoint_record: call before the first step");
+        return AEM_ERR_STATE;
+    }
+    if (const char* why = adjoint_unsupported(v)) {
+        aem::set_error(std::string("aem_adjoint_record: the adjoint step does not include ") + why);
+        return AEM_ERR_UNSUPPORTED;
+    }
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    for (int a = 0; a < 3; ++a) {
+        // the design edges [lo, hi+1] must lie in the interior, outside every CPML slab
+        const int lo_ok = std::max(2, v.npml[2 * a] + 1), hi_ok = N[a] - 1 - std::max(v.npml[2 * a + 1], 0);
+        if (lo[a] > hi[a] || lo[a] < lo_ok || hi[a] + 1 > hi_ok) {
+            aem::set_error("aem_adjoint_record: design box must lie inside the grid, outside the CPML slabs");
+            return AEM_ERR_STATE;
+        }
+    }
+    return aem::guard([&]() {
+        AdjRec& r = s->rec;
+        for (int a = 0; a < 3; ++a) {
+            s->rec_lo[a] = lo[a];
+            s->rec_hi[a] = hi[a];
+            r.lo[a] = lo[a];
+            r.n[a] = hi[a] - lo[a] + 2;
+        }
+        r.nedge = (long long)r.n[0] * r.n[1] * r.n[2];
+        r.hist = s->zeros<float>((size_t)v.time_steps * 6 * r.nedge);
+        s->rec_on = true;
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_adjoint_eps_gradient(aem_sim* s, int32_t n_mat, double* grad) {
+    if (!s || !grad || n_mat < 0) {
+        aem::set_error("aem_adjoint_eps_gradient: bad argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (!s->obj_on || !s->rec_on || s->steps_done != v.time_steps) {
+        aem::set_error("aem_adjoint_eps_gradient: needs an objective, a design record and a completed run");
+        return AEM_ERR_STATE;
+    }
+    if (n_mat > (int)s->plan->deck.mats.size()) {
+        aem::set_error("aem_adjoint_eps_gradient: more materials requested than the deck defines");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        s->ensure_adjoint();
+        double F[2];
+        HIPCHK(hipMemcpy(F, s->obj.F, sizeof F, hipMemcpyDeviceToHost));
+        const long long ne = s->rec.nedge;
+        std::vector<void*> tmp;
+        double* dga = dalloc<double>(3 * ne, tmp);
+        double* dgb = dalloc<double>(3 * ne, tmp);
+        HIPCHK(hipMemsetAsync(dga, 0, 3 * ne * sizeof(double), s->st));
+        HIPCHK(hipMemsetAsync(dgb, 0, 3 * ne * sizeof(double), s->st));
+        for (int q = 0; q < 6; ++q) HIPCHK(hipMemsetAsync(s->lam[q], 0, ((size_t)v.ncell + v.plane) * 4, s->st));
+        for (int f = 0; f < 6; ++f) {
+            const int a = f / 2;
+            const size_t nt = (size_t)std::max(v.npml[f] - 1, 0);
+            const size_t trans = a == 0 ? (size_t)v.NY * v.NZ : (a == 1 ? (size_t)v.PX * v.NZ : (size_t)v.PX * v.NY);
+            for (int q = 0; q < 2; ++q) {
+                if (nt * trans) HIPCHK(hipMemsetAsync(s->lpE.psi[f][q], 0, nt * trans * 4, s->st));
+                if (nt * trans) HIPCHK(hipMemsetAsync(s->lpH.psi[f][q], 0, nt * trans * 4, s->st));
+            }
+        }
+        const int T = v.time_steps;
+        HIPCHK(hipMemcpyAsync(s->d_bstep, &T, sizeof(int), hipMemcpyHostToDevice, s->st));
+        // one captured backward step, replayed T times
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t gx = nullptr;
+        HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+        if (v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, true, F[0], F[1], dga, dgb);
+        else s->enqueue_backward_t<uint16_t>(s->st, true, F[0], F[1], dga, dgb);
+        HIPCHK(hipStreamEndCapture(s->st, &gr));
+        HIPCHK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
+        HIPCHK(hipEventRecord(s->ev0, s->st));
+        for (int n = 0; n < T; ++n) HIPCHK(hipGraphLaunch(gx, s->st));
+        HIPCHK(hipEventRecord(s->ev1, s->st));
+        HIPCHK(hipEventSynchronize(s->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms / (float)T;
+        (void)hipGraphExecDestroy(gx);
+        (void)hipGraphDestroy(gr);
+        std::vector<double> ha(3 * ne), hb(3 * ne);
+        HIPCHK(hipMemcpy(ha.data(), dga, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hb.data(), dgb, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (void* p : tmp) (void)hipFree(p);
+        eps_chain_rule(*s->plan, s->rec, s->rec_lo, s->rec_hi, ha, hb, n_mat, grad);
+    });
+}
+
+aem_status aem_adjoint_step(aem_sim* s, int64_t n_steps) {
+    if (!s || n_steps < 0) {
+        aem::set_error("aem_adjoint_step: bad argument");
+        return AEM_ERR_STATE;
+    }
+    if (const char* why = adjoint_unsupported(s->plan->v)) {
+        aem::set_error(std::string("aem_adjoint_step: the adjoint step does not include ") + why);
+        return AEM_ERR_UNSUPPORTED;
+    }
+    return aem::guard([&]() {
+        s->ensure_adjoint();
+        for (int64_t n = 0; n < n_steps; ++n) {
+            if (s->plan->v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, false, 0, 0, nullptr, nullptr);
+            else s->enqueue_backward_t<uint16_t>(s->st, false, 0, 0, nullptr, nullptr);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_adjoint_get_field(aem_sim* s, int32_t comp, float* out, int64_t n) {
+    if (!s || !out || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
+        aem::set_error("aem_adjoint_get_field: bad argument");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        s->ensure_adjoint();
+        HIPCHK(hipMemcpyAsync(out, s->lam[comp], (size_t)n * 4, hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_adjoint_set_field(aem_sim* s, int32_t comp, const float* in, int64_t n) {
+    if (!s || !in || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
+        aem::set_error("aem_adjoint_set_field: bad argument");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        s->ensure_adjoint();
+        HIPCHK(hipMemcpyAsync(s->lam[comp], in, (size_t)n * 4, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
 
 }  // extern "C"

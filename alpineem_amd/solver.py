@@ -137,6 +137,45 @@ class Simulation:
         check(lib().aem_get_dft(self._h, SERIES[kind], idx, f.ctypes.data, f.size, out.ctypes.data))
         return out[0::2] + 1j * out[1::2]
 
+    # -- differentiation ---------------------------------------------------------
+    def set_objective(self, theta_deg: float, phi_deg: float, freq_hz: float, pol: str = "theta") -> None:
+        """Accumulate F = r E_pol(theta, phi, f) by the frequency-domain
+        near-to-far-field transform during the run (aem_objective_set)."""
+        check(lib().aem_objective_set(self._h, float(theta_deg), float(phi_deg), float(freq_hz),
+                                      0 if pol == "theta" else 1))
+
+    def objective(self) -> complex:
+        out = np.zeros(2, np.float64)
+        check(lib().aem_objective_get(self._h, out.ctypes.data))
+        return complex(out[0], out[1])
+
+    def record_design(self, lo, hi) -> None:
+        """Record the design box of cells [lo, hi] (1-based, inclusive) for
+        the adjoint gradient (aem_adjoint_record); call before stepping."""
+        a = np.ascontiguousarray(lo, np.int32)
+        b = np.ascontiguousarray(hi, np.int32)
+        check(lib().aem_adjoint_record(self._h, a.ctypes.data, b.ctypes.data))
+
+    def eps_gradient(self, n_materials: int) -> np.ndarray:
+        """d|F|^2 / d eps_r of deck materials 1..n_materials by the discrete
+        adjoint (aem_adjoint_eps_gradient); after the full run."""
+        out = np.zeros(n_materials, np.float64)
+        check(lib().aem_adjoint_eps_gradient(self._h, int(n_materials), out.ctypes.data))
+        return out
+
+    def adjoint_step(self, steps: int = 1) -> None:
+        check(lib().aem_adjoint_step(self._h, int(steps)))
+
+    def adjoint_field(self, name: str) -> np.ndarray:
+        v = self.plan.view
+        out = np.zeros(v.ncell, np.float32)
+        check(lib().aem_adjoint_get_field(self._h, FIELDS[name], out.ctypes.data, v.ncell))
+        return out.reshape(self.plan.shape)
+
+    def set_adjoint_field(self, name: str, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, np.float32).reshape(-1)
+        check(lib().aem_adjoint_set_field(self._h, FIELDS[name], a.ctypes.data, a.size))
+
     def field(self, name: str) -> np.ndarray:
         v = self.plan.view
         out = np.zeros(v.ncell, np.float32)

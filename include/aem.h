@@ -178,6 +178,45 @@ aem_status aem_get_ff_time(aem_sim* sim, float* out, int64_t n);
 aem_status aem_get_dft(aem_sim* sim, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
                        double* re_im);
 
+/* ---- differentiation --------------------------------------------------------
+ * Replaces the reference's adjoint workflow
+ * (optimization_scripts/adjoint_optimization/adjoint_optimization.py:55-134,
+ * a second forward run with an adjoint source and FFTs of field slices) by
+ * the exact discrete adjoint of this solver's update.  Supported decks: CPML
+ * or PEC boundaries, simple (dielectric/conductive) materials, plane-wave or
+ * no excitation; not lumped ports, thin sheets, Drude media or periodic
+ * boundaries (AEM_ERR_UNSUPPORTED). */
+
+/* Far-field objective F = r E_pol(theta, phi, f) e^{jkr}, pol 0 = theta,
+ * 1 = phi, accumulated during the run by the frequency-domain
+ * near-to-far-field transform of the surface currents of the deck's
+ * far-field box:  L = sum M e^{jk r.r'} dS, N = sum J e^{jk r.r'} dS with M
+ * sampled at n dt and J at (n - 1/2) dt, E_theta r = -jk/(4 pi)(L_phi +
+ * eta N_theta), E_phi r = jk/(4 pi)(L_theta - eta N_phi).  The deck needs a
+ * far-field box (at least one angle); call before the first step. */
+aem_status aem_objective_set(aem_sim* sim, float theta_deg, float phi_deg, double freq_hz, int32_t pol);
+aem_status aem_objective_get(aem_sim* sim, double* re_im);
+
+/* Record the design box of cells [lo, hi] (1-based, inclusive, in the
+ * interior outside the CPML slabs) during the run: every step stores E and
+ * the update's curl term on the box's edges (time_steps * 6 * edges floats).
+ * Call before the first step. */
+aem_status aem_adjoint_record(aem_sim* sim, const int32_t* lo, const int32_t* hi);
+
+/* After the full run: gradient of J = |F|^2 with respect to the relative
+ * permittivity (all three axes together) of the deck's materials 1..n_mat
+ * (declaration order), changed in the cells of the design box only, by the
+ * discrete adjoint (transposed update run backwards from the last step) and
+ * the setup's four-cell edge averaging (fdtd_solver.f90:2516-2527). */
+aem_status aem_adjoint_eps_gradient(aem_sim* sim, int32_t n_mat, double* grad_eps);
+
+/* Low-level adjoint access: apply the transposed step (no source) n_steps
+ * times to the adjoint state, and read/write its six field components
+ * (layout of aem_get_field).  The adjoint CPML state starts at zero. */
+aem_status aem_adjoint_step(aem_sim* sim, int64_t n_steps);
+aem_status aem_adjoint_get_field(aem_sim* sim, int32_t comp, float* out, int64_t n);
+aem_status aem_adjoint_set_field(aem_sim* sim, int32_t comp, const float* in, int64_t n);
+
 /* ---- batched stepping of independent simulations -------------------------- */
 
 /* Group n created, unfinalized simulations for batched stepping.  Replaces

@@ -1169,6 +1169,12 @@ int oracle_get_field(const oracle_state* s, int comp, float* out, int64_t n) {
     return 0;
 }
 
+int oracle_get_field64(const oracle_state* s, int comp, double* out, int64_t n) {
+    if (comp < 0 || comp > 5 || n > s->nc) return -1;
+    for (int64_t q = 0; q < n; ++q) out[q] = (double)s->F[comp][q];
+    return 0;
+}
+
 int oracle_set_field(oracle_state* s, int comp, const float* in, int64_t n) {
     if (comp < 0 || comp > 5 || n > s->nc) return -1;
     for (int64_t q = 0; q < n; ++q) s->F[comp][q] = (real)in[q];

@@ -33,6 +33,8 @@ int           oracle_steps_done(const oracle_state* s);
 int           oracle_finalize(oracle_state* s);
 /* copy a field component (ncell entries of the common layout) as float */
 int           oracle_get_field(const oracle_state* s, int comp, float* out, int64_t n);
+/* the same in double (the FP64 build's full precision) */
+int           oracle_get_field64(const oracle_state* s, int comp, double* out, int64_t n);
 /* overwrite a field component (ncell entries) -- for initial-value tests */
 int           oracle_set_field(oracle_state* s, int comp, const float* in, int64_t n);
 /* copy a series (same kinds as aem_series in include/aem.h) as float */

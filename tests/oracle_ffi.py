@@ -25,6 +25,7 @@ def load(fp64: bool = False) -> C.CDLL:
         L.oracle_steps_done.argtypes = [C.c_void_p]
         L.oracle_finalize.argtypes = [C.c_void_p]
         L.oracle_get_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
+        L.oracle_get_field64.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_set_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_get_series.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_get_ntff_raw.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
@@ -65,6 +66,13 @@ class Oracle:
         v = self.plan.view
         out = np.zeros(v.ncell, np.float32)
         self.L.oracle_get_field(self.h, FIELDS[name], out.ctypes.data, v.ncell)
+        return out.reshape(self.plan.shape)
+
+    def field64(self, name: str) -> np.ndarray:
+        """The component in double: full precision of the FP64 build."""
+        v = self.plan.view
+        out = np.zeros(v.ncell, np.float64)
+        self.L.oracle_get_field64(self.h, FIELDS[name], out.ctypes.data, v.ncell)
         return out.reshape(self.plan.shape)
 
     def set_field(self, name: str, values) -> None:
