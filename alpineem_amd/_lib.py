@@ -52,6 +52,9 @@ SIGNATURES = [
     ("aem_objective_get", C.c_int, [C.c_void_p, C.c_void_p]),
     ("aem_adjoint_record", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("aem_adjoint_eps_gradient", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    ("aem_design_box", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("aem_set_design_eps", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("aem_adjoint_design_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_adjoint_step", C.c_int, [C.c_void_p, C.c_int64]),
     ("aem_adjoint_get_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_adjoint_set_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),

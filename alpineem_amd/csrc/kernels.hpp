@@ -1888,6 +1888,77 @@ __global__ void k_adj_h_cpml(Grid g, Cpml p, Cpml lp, int ax, float* lEx, float*
     }
 }
 
+// Design region: the edges of [lo, lo+n-1] (cells [lo, lo+n-2] and their
+// far edges) carry per-edge Ga/Gb instead of a tuple.  The bulk kernel sees
+// the neutral tuple (Ga=1, Gb=0) there, which leaves E unchanged bit for
+// bit, and k_e_design applies the update in the bulk kernel's arithmetic.
+struct DesignDev {
+    int lo[3], n[3];
+    long long nedge;
+    const float* coef;   // [6][nedge]: ga_x, gb_x, ga_y, gb_y, ga_z, gb_z
+};
+
+template <typename IdxT>
+__global__ void k_set_idx(Grid g, DesignDev d, IdxT* idx, IdxT value) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.nedge) return;
+    const int i = d.lo[0] + (int)(e % d.n[0]);
+    const long long rr = e / d.n[0];
+    const int j = d.lo[1] + (int)(rr % d.n[1]), k = d.lo[2] + (int)(rr / d.n[1]);
+    idx[off(g, i, j, k)] = value;
+}
+
+__global__ void k_e_design(Grid g, DesignDev d, float* Ex, float* Ey, float* Ez, const float* Hx, const float* Hy,
+                           const float* Hz, const float* dex, const float* dey, const float* dez) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.nedge) return;
+    const int i = d.lo[0] + (int)(e % d.n[0]);
+    const long long rr = e / d.n[0];
+    const int j = d.lo[1] + (int)(rr % d.n[1]), k = d.lo[2] + (int)(rr / d.n[1]);
+    const long long o = off(g, i, j, k), n = d.nedge;
+    const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
+    const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+    const float hz_ym = Hz[o - g.PX], hz_xm = Hz[o - 1], hy_xm = Hy[o - 1], hx_ym = Hx[o - g.PX];
+    const float hx_km = Hx[o - g.plane], hy_km = Hy[o - g.plane];
+    const float* c = d.coef;
+    Ex[o] = c[e] * Ex[o] + c[n + e] * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
+    Ey[o] = c[2 * n + e] * Ey[o] + c[3 * n + e] * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
+    Ez[o] = c[4 * n + e] * Ez[o] + c[5 * n + e] * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
+}
+
+// Transposed k_e_design: lambda_H += S^T (gb lambda_E'), lambda_E <- ga lambda_E'.
+__global__ void k_adj_design(Grid g, DesignDev d, float* lEx, float* lEy, float* lEz, float* lHx, float* lHy,
+                             float* lHz, const float* dex, const float* dey, const float* dez) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.nedge) return;
+    const int i = d.lo[0] + (int)(e % d.n[0]);
+    const long long rr = e / d.n[0];
+    const int j = d.lo[1] + (int)(rr % d.n[1]), k = d.lo[2] + (int)(rr / d.n[1]);
+    const long long o = off(g, i, j, k), n = d.nedge;
+    const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
+    const float* c = d.coef;
+    const float lx = lEx[o], ly = lEy[o], lz = lEz[o];
+    const float mx = c[n + e] * lx, my = c[3 * n + e] * ly, mz = c[5 * n + e] * lz;
+    lEx[o] = c[e] * lx;
+    lEy[o] = c[2 * n + e] * ly;
+    lEz[o] = c[4 * n + e] * lz;
+    // Ex: (Hz - Hz(j-1)) dy + (Hy(k-1) - Hy) dz
+    atomicAdd(&lHz[o], mx * dy_j);
+    atomicAdd(&lHz[o - g.PX], -(mx * dy_j));
+    atomicAdd(&lHy[o - g.plane], mx * dz_k);
+    atomicAdd(&lHy[o], -(mx * dz_k));
+    // Ey: (Hz(i-1) - Hz) dx + (Hx - Hx(k-1)) dz
+    atomicAdd(&lHz[o - 1], my * dx_i);
+    atomicAdd(&lHz[o], -(my * dx_i));
+    atomicAdd(&lHx[o], my * dz_k);
+    atomicAdd(&lHx[o - g.plane], -(my * dz_k));
+    // Ez: (Hy - Hy(i-1)) dx + (Hx(j-1) - Hx) dy
+    atomicAdd(&lHy[o], mz * dx_i);
+    atomicAdd(&lHy[o - 1], -(mz * dx_i));
+    atomicAdd(&lHx[o - g.PX], mz * dy_j);
+    atomicAdd(&lHx[o], -(mz * dy_j));
+}
+
 __global__ void k_step_dec(int* step) { *step = *step - 1; }
 
 }  // namespace aemk

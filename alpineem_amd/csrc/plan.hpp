@@ -129,6 +129,13 @@ struct Plan {
 };
 
 Plan* build_plan(const std::string& deck_text, const std::string& base_dir);
+
+// Ga/Gb of every edge of the box [lo, hi+1] when the cells of [lo, hi] have
+// the per-axis relative permittivity eps3 ([3][cells of the box], x fastest)
+// and every other cell its deck material; the setup's four-cell averaging and
+// formulas for non-dispersive media without sheets (f90:2516-2527,
+// 2837-2853).  coef is [6][edges]: ga_x, gb_x, ga_y, gb_y, ga_z, gb_z.
+void design_coefficients(const Plan& P, const int lo[3], const int hi[3], const float* eps3, float* coef);
 void write_geometry(const Plan& p, const std::string& path);
 
 // FP32 pulse functions of the reference (fdtd_solver.f90:8825-8857).

@@ -829,6 +829,38 @@ float e_or_v_1d(float amp, float tc, float pos, float spread, float t_spread, in
     return amp * -1.0f * x * std::exp(0.5f) * std::exp(-0.5f * (x * x));
 }
 
+void design_coefficients(const Plan& P, const int lo[3], const int hi[3], const float* eps3, float* coef) {
+    const aem_plan_view& v = P.v;
+    const int n[3] = {hi[0] - lo[0] + 2, hi[1] - lo[1] + 2, hi[2] - lo[2] + 2};
+    const size_t nedge = (size_t)n[0] * n[1] * n[2];
+    const size_t ncb = (size_t)(n[0] - 1) * (n[1] - 1) * (n[2] - 1);
+    const float dt = v.dt;
+    auto cell_eps_sig = [&](int i, int j, int k, int c, float& e, float& sg) {
+        const int m = P.cell_mat[(size_t)(i - 1) + (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (k - 1))];
+        sg = m ? P.deck.mats[m - 1].sig[c] : 0.0f;
+        if (i >= lo[0] && i <= hi[0] && j >= lo[1] && j <= hi[1] && k >= lo[2] && k <= hi[2])
+            e = eps3[(size_t)c * ncb + (size_t)(i - lo[0]) +
+                     (size_t)(n[0] - 1) * ((size_t)(j - lo[1]) + (size_t)(n[1] - 1) * (k - lo[2]))];
+        else
+            e = m ? P.deck.mats[m - 1].eps[c] : 1.0f;
+    };
+    for (int k = lo[2]; k <= hi[2] + 1; ++k)
+        for (int j = lo[1]; j <= hi[1] + 1; ++j)
+            for (int i = lo[0]; i <= hi[0] + 1; ++i) {
+                const size_t e = (size_t)(i - lo[0]) + (size_t)n[0] * ((size_t)(j - lo[1]) + (size_t)n[1] * (k - lo[2]));
+                for (int c = 0; c < 3; ++c) {
+                    int q[4][3];
+                    if (c == 0) { int t[4][3] = {{i, j, k}, {i, j - 1, k}, {i, j, k - 1}, {i, j - 1, k - 1}}; std::memcpy(q, t, sizeof q); }
+                    else if (c == 1) { int t[4][3] = {{i, j, k}, {i - 1, j, k}, {i, j, k - 1}, {i - 1, j, k - 1}}; std::memcpy(q, t, sizeof q); }
+                    else { int t[4][3] = {{i, j, k}, {i - 1, j, k}, {i, j - 1, k}, {i - 1, j - 1, k}}; std::memcpy(q, t, sizeof q); }
+                    float ev[4], sv[4];
+                    for (int r = 0; r < 4; ++r) cell_eps_sig(q[r][0], q[r][1], q[r][2], c, ev[r], sv[r]);
+                    ga_gb(avg4(sv[0], sv[1], sv[2], sv[3]), avg4(ev[0], ev[1], ev[2], ev[3]), dt,
+                          coef[(2 * c) * nedge + e], coef[(2 * c + 1) * nedge + e]);
+                }
+            }
+}
+
 void write_geometry(const Plan& p, const std::string& path) {
     std::FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) throw Error(2, "cannot write '" + path + "'");

@@ -224,6 +224,11 @@ struct aem_sim {
     AdjRec rec{};
     bool rec_on = false;
     int rec_lo[3] = {0, 0, 0}, rec_hi[3] = {0, 0, 0};   // design cell box
+    bool des_on = false;           // per-cell permittivity in the design box
+    DesignDev des{};
+    std::vector<float> des_eps;    // [3][cells of the box], x fastest
+    float* d_des_coef = nullptr;
+    void upload_design();
     float* lam[6] = {};            // adjoint fields (allocated at the first backward step)
     Cpml lpE{}, lpH{};             // adjoint CPML states
     int* d_bstep = nullptr;
@@ -543,6 +548,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     if (mark) HIPCHK(hipEventRecord(mark[3], st));
     if (parts & kPartEBulk)
         for (size_t q = 1; q < bulk.size(); ++q) e_bulk(bulk[q]);
+    if ((parts & kPartEPre) && des_on)
+        k_e_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2]);
     for (int a = 0; a < 3; ++a)
         if ((parts & kPartEPre) && sh[a].np > 0) {
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
@@ -1246,6 +1253,9 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
             k_adj_e_cpml<IdxT><<<cdiv(tot, 256), 256, 0, s>>>(g, cpE, lpE, a, lEx, lEy, lEz, lHx, lHy, lHz, idx,
                                                               d_tup, dd[a]);
     }
+    if (des_on)
+        k_adj_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, lEx, lEy, lEz, lHx, lHy, lHz, d_den[0], d_den[1],
+                                                           d_den[2]);
     k_adj_h_gather<IdxT><<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8), v.NZ - 1), bb, 0, s>>>(
         g, lHx, lHy, lHz, lEx, lEy, lEz, idx, d_tup, d_den[0], d_den[1], d_den[2]);
     k_adj_e_gather<IdxT><<<dim3(cdiv(v.NX, 32), cdiv(v.NY, 8), v.NZ), bb, 0, s>>>(
@@ -1269,19 +1279,27 @@ const char* adjoint_unsupported(const aem_plan_view& v) {
     return nullptr;
 }
 
-// dJ/d eps_r of every material from the per-edge derivatives dJ/dga, dJ/dgb:
-// each edge's eps and sigma are the four-cell averages of the setup
-// (fdtd_solver.f90:2516-2527), its Ga/Gb the formulas of f90:2837-2853.
-// Only cells inside the design box [lo, hi] count: the result is the
-// derivative for a change of the material's permittivity in those cells.
-void eps_chain_rule(const aem::Plan& P, const AdjRec& r, const int lo[3], const int hi[3],
-                    const std::vector<double>& dga, const std::vector<double>& dgb, int n_mat, double* grad) {
+// dJ/d eps_r of every cell of the design box [lo, hi] (x fastest, the three
+// axes together) from the per-edge derivatives dJ/dga, dJ/dgb of the record
+// box [lo, hi+1].  Each edge's eps and sigma are the four-cell averages of the
+// setup (fdtd_solver.f90:2516-2527) and its Ga/Gb the formulas of
+// f90:2837-2853, so d/d eps_cell = (dGa/d eps dJ/dGa + dGb/d eps dJ/dGb) / 4
+// for each of the edge's cells in the box.  eps3 is the box's per-cell
+// permittivity ([3][cells]) or null for the deck's materials.
+void cell_gradient(const aem::Plan& P, const AdjRec& r, const int lo[3], const int hi[3], const float* eps3,
+                   const std::vector<double>& dga, const std::vector<double>& dgb, std::vector<double>& out) {
     const aem_plan_view& v = P.v;
     const double dt = v.dt, e0 = aem::kEp0;
+    const int nb[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    const size_t ncb = (size_t)nb[0] * nb[1] * nb[2];
     auto cidx = [&](int i, int j, int k) {
         return (size_t)(i - 1) + (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (size_t)(k - 1));
     };
-    for (int m = 0; m < n_mat; ++m) grad[m] = 0.0;
+    auto bidx = [&](int i, int j, int k) -> long long {
+        if (i < lo[0] || i > hi[0] || j < lo[1] || j > hi[1] || k < lo[2] || k > hi[2]) return -1;
+        return (long long)(i - lo[0]) + (long long)nb[0] * ((long long)(j - lo[1]) + (long long)nb[1] * (k - lo[2]));
+    };
+    out.assign(ncb, 0.0);
     for (long long e = 0; e < r.nedge; ++e) {
         const int i = r.lo[0] + (int)(e % r.n[0]);
         const long long rr = e / r.n[0];
@@ -1291,12 +1309,13 @@ void eps_chain_rule(const aem::Plan& P, const AdjRec& r, const int lo[3], const 
             if (c == 0) { int t[4][3] = {{i, j, k}, {i, j - 1, k}, {i, j, k - 1}, {i, j - 1, k - 1}}; std::memcpy(cell, t, sizeof t); }
             else if (c == 1) { int t[4][3] = {{i, j, k}, {i - 1, j, k}, {i, j, k - 1}, {i - 1, j, k - 1}}; std::memcpy(cell, t, sizeof t); }
             else { int t[4][3] = {{i, j, k}, {i - 1, j, k}, {i, j - 1, k}, {i - 1, j - 1, k}}; std::memcpy(cell, t, sizeof t); }
-            int m[4];
+            long long b[4];
             float ev[4], sv[4];
             for (int q = 0; q < 4; ++q) {
-                m[q] = P.cell_mat[cidx(cell[q][0], cell[q][1], cell[q][2])];
-                ev[q] = m[q] ? P.deck.mats[m[q] - 1].eps[c] : 1.0f;
-                sv[q] = m[q] ? P.deck.mats[m[q] - 1].sig[c] : 0.0f;
+                const int m = P.cell_mat[cidx(cell[q][0], cell[q][1], cell[q][2])];
+                b[q] = bidx(cell[q][0], cell[q][1], cell[q][2]);
+                sv[q] = m ? P.deck.mats[m - 1].sig[c] : 0.0f;
+                ev[q] = (eps3 && b[q] >= 0) ? eps3[(size_t)c * ncb + (size_t)b[q]] : (m ? P.deck.mats[m - 1].eps[c] : 1.0f);
             }
             const double eps = (((ev[0] + ev[1]) + ev[2]) + ev[3]) / 4.0f;
             const double sig = (((sv[0] + sv[1]) + sv[2]) + sv[3]) / 4.0f;
@@ -1304,11 +1323,8 @@ void eps_chain_rule(const aem::Plan& P, const AdjRec& r, const int lo[3], const 
             const double dgb_de = -A / (eps * (1.0 + al) * (1.0 + al));
             const double dga_de = 2.0 * al / (eps * (1.0 + al) * (1.0 + al));
             const double dj = dga[c * r.nedge + e] * dga_de + dgb[c * r.nedge + e] * dgb_de;
-            for (int q = 0; q < 4; ++q) {
-                bool in = m[q] >= 1 && m[q] <= n_mat;
-                for (int a = 0; a < 3; ++a) in = in && cell[q][a] >= lo[a] && cell[q][a] <= hi[a];
-                if (in) grad[m[q] - 1] += dj / 4.0;
-            }
+            for (int q = 0; q < 4; ++q)
+                if (b[q] >= 0) out[(size_t)b[q]] += dj / 4.0;
         }
     }
 }
@@ -1403,14 +1419,161 @@ aem_status aem_adjoint_record(aem_sim* s, const int32_t* lo, const int32_t* hi) 
     });
 }
 
+void aem_sim::upload_design() {
+    const long long ne = des.nedge;
+    std::vector<float> coef(6 * (size_t)ne);
+    aem::design_coefficients(*plan, rec_lo, rec_hi, des_eps.data(), coef.data());
+    HIPCHK(hipMemcpyAsync(d_des_coef, coef.data(), coef.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+aem_status aem_design_box(aem_sim* s, const int32_t* lo, const int32_t* hi) {
+    if (s && (s->rec_on || s->des_on)) {
+        aem::set_error("aem_design_box: the simulation already records a box");
+        return AEM_ERR_STATE;
+    }
+    const aem_status st = aem_adjoint_record(s, lo, hi);
+    if (st != AEM_OK) return st;
+    const aem_plan_view& v = s->plan->v;
+    return aem::guard([&]() {
+        const int max_tuples = v.idx_bytes == 1 ? 256 : 65536;
+        if (v.n_tuple + 1 > max_tuples)
+            throw aem::Error(AEM_ERR_UNSUPPORTED, "aem_design_box: the coefficient table has no room for the design marker");
+        DesignDev& d = s->des;
+        for (int a = 0; a < 3; ++a) {
+            d.lo[a] = s->rec.lo[a];
+            d.n[a] = s->rec.n[a];
+        }
+        d.nedge = s->rec.nedge;
+        // initial design: the deck's materials
+        const int nb[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+        const size_t ncb = (size_t)nb[0] * nb[1] * nb[2];
+        s->des_eps.assign(3 * ncb, 1.0f);
+        size_t b = 0;
+        for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i, ++b) {
+                    const int m = s->plan->cell_mat[(size_t)(i - 1) +
+                                                    (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (k - 1))];
+                    for (int c = 0; c < 3; ++c) s->des_eps[c * ncb + b] = m ? s->plan->deck.mats[m - 1].eps[c] : 1.0f;
+                }
+        s->d_des_coef = s->zeros<float>(6 * (size_t)d.nedge);
+        d.coef = s->d_des_coef;
+        // the coefficient table gains the neutral tuple (Ga = 1, Gb = 0), which
+        // every edge of the box now selects
+        const int nt = v.n_tuple;
+        float* tup = s->zeros<float>((size_t)(nt + 1) * 6);
+        HIPCHK(hipMemcpyAsync(tup, s->d_tup, (size_t)nt * 6 * sizeof(float), hipMemcpyDeviceToDevice, s->st));
+        const float neutral[6] = {1.0f, 0.0f, 1.0f, 0.0f, 1.0f, 0.0f};
+        HIPCHK(hipMemcpyAsync(tup + (size_t)nt * 6, neutral, sizeof neutral, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        s->d_tup = tup;
+        s->ea.tup = tup;
+        if (v.idx_bytes == 1)
+            k_set_idx<uint8_t><<<cdiv(d.nedge, 256), 256, 0, s->st>>>(s->g, d, static_cast<uint8_t*>(s->d_idx), (uint8_t)nt);
+        else
+            k_set_idx<uint16_t><<<cdiv(d.nedge, 256), 256, 0, s->st>>>(s->g, d, static_cast<uint16_t*>(s->d_idx),
+                                                                       (uint16_t)nt);
+        HIPCHK(hipGetLastError());
+        s->upload_design();
+        s->des_on = true;
+    });
+}
+
+aem_status aem_set_design_eps(aem_sim* s, const float* eps, int64_t n) {
+    if (!s || !eps || !s->des_on) {
+        aem::set_error("aem_set_design_eps: needs a design box (aem_design_box)");
+        return AEM_ERR_STATE;
+    }
+    if (!fresh(s)) {
+        aem::set_error("aem_set_design_eps: call before the first step");
+        return AEM_ERR_STATE;
+    }
+    const int64_t ncb = (int64_t)s->des_eps.size() / 3;
+    if (n != ncb) {
+        aem::set_error("aem_set_design_eps: n must equal the design box's cell count " + std::to_string(ncb));
+        return AEM_ERR_STATE;
+    }
+    for (int64_t q = 0; q < n; ++q)
+        if (!(eps[q] >= 1.0f) || !std::isfinite(eps[q])) {
+            aem::set_error("aem_set_design_eps: permittivities must be finite and >= 1");
+            return AEM_ERR_INPUT;
+        }
+    return aem::guard([&]() {
+        for (int c = 0; c < 3; ++c) std::memcpy(s->des_eps.data() + c * ncb, eps, (size_t)n * sizeof(float));
+        s->upload_design();
+    });
+}
+
+namespace {
+
+// The full backward sweep of J = |F|^2: dJ/dGa and dJ/dGb of every record
+// edge ([3][edges] each).  Clears the adjoint state, captures one backward
+// step as a graph and replays it time_steps times.
+void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb) {
+    const aem_plan_view& v = s->plan->v;
+    s->ensure_adjoint();
+    double F[2];
+    HIPCHK(hipMemcpy(F, s->obj.F, sizeof F, hipMemcpyDeviceToHost));
+    const long long ne = s->rec.nedge;
+    std::vector<void*> tmp;
+    double* dga = dalloc<double>(3 * ne, tmp);
+    double* dgb = dalloc<double>(3 * ne, tmp);
+    HIPCHK(hipMemsetAsync(dga, 0, 3 * ne * sizeof(double), s->st));
+    HIPCHK(hipMemsetAsync(dgb, 0, 3 * ne * sizeof(double), s->st));
+    for (int q = 0; q < 6; ++q) HIPCHK(hipMemsetAsync(s->lam[q], 0, ((size_t)v.ncell + v.plane) * 4, s->st));
+    for (int f = 0; f < 6; ++f) {
+        const int a = f / 2;
+        const size_t nt = (size_t)std::max(v.npml[f] - 1, 0);
+        const size_t trans = a == 0 ? (size_t)v.NY * v.NZ : (a == 1 ? (size_t)v.PX * v.NZ : (size_t)v.PX * v.NY);
+        for (int q = 0; q < 2; ++q) {
+            if (nt * trans) HIPCHK(hipMemsetAsync(s->lpE.psi[f][q], 0, nt * trans * 4, s->st));
+            if (nt * trans) HIPCHK(hipMemsetAsync(s->lpH.psi[f][q], 0, nt * trans * 4, s->st));
+        }
+    }
+    const int T = v.time_steps;
+    HIPCHK(hipMemcpyAsync(s->d_bstep, &T, sizeof(int), hipMemcpyHostToDevice, s->st));
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t gx = nullptr;
+    HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    if (v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, true, F[0], F[1], dga, dgb);
+    else s->enqueue_backward_t<uint16_t>(s->st, true, F[0], F[1], dga, dgb);
+    HIPCHK(hipStreamEndCapture(s->st, &gr));
+    HIPCHK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    for (int n = 0; n < T; ++n) HIPCHK(hipGraphLaunch(gx, s->st));
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_ms = ms / (float)T;
+    (void)hipGraphExecDestroy(gx);
+    (void)hipGraphDestroy(gr);
+    ha.resize(3 * ne);
+    hb.resize(3 * ne);
+    HIPCHK(hipMemcpy(ha.data(), dga, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hb.data(), dgb, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (void* p : tmp) (void)hipFree(p);
+}
+
+bool gradient_ready(const aem_sim* s, const char* who) {
+    if (!s->obj_on || !s->rec_on || s->steps_done != s->plan->v.time_steps) {
+        aem::set_error(std::string(who) + ": needs an objective, a design record and a completed run");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
 aem_status aem_adjoint_eps_gradient(aem_sim* s, int32_t n_mat, double* grad) {
     if (!s || !grad || n_mat < 0) {
         aem::set_error("aem_adjoint_eps_gradient: bad argument");
         return AEM_ERR_STATE;
     }
-    const aem_plan_view& v = s->plan->v;
-    if (!s->obj_on || !s->rec_on || s->steps_done != v.time_steps) {
-        aem::set_error("aem_adjoint_eps_gradient: needs an objective, a design record and a completed run");
+    if (!gradient_ready(s, "aem_adjoint_eps_gradient")) return AEM_ERR_STATE;
+    if (s->des_on) {
+        aem::set_error("aem_adjoint_eps_gradient: the design box sets its own permittivity; use aem_adjoint_design_gradient");
         return AEM_ERR_STATE;
     }
     if (n_mat > (int)s->plan->deck.mats.size()) {
@@ -1418,49 +1581,42 @@ aem_status aem_adjoint_eps_gradient(aem_sim* s, int32_t n_mat, double* grad) {
         return AEM_ERR_STATE;
     }
     return aem::guard([&]() {
-        s->ensure_adjoint();
-        double F[2];
-        HIPCHK(hipMemcpy(F, s->obj.F, sizeof F, hipMemcpyDeviceToHost));
-        const long long ne = s->rec.nedge;
-        std::vector<void*> tmp;
-        double* dga = dalloc<double>(3 * ne, tmp);
-        double* dgb = dalloc<double>(3 * ne, tmp);
-        HIPCHK(hipMemsetAsync(dga, 0, 3 * ne * sizeof(double), s->st));
-        HIPCHK(hipMemsetAsync(dgb, 0, 3 * ne * sizeof(double), s->st));
-        for (int q = 0; q < 6; ++q) HIPCHK(hipMemsetAsync(s->lam[q], 0, ((size_t)v.ncell + v.plane) * 4, s->st));
-        for (int f = 0; f < 6; ++f) {
-            const int a = f / 2;
-            const size_t nt = (size_t)std::max(v.npml[f] - 1, 0);
-            const size_t trans = a == 0 ? (size_t)v.NY * v.NZ : (a == 1 ? (size_t)v.PX * v.NZ : (size_t)v.PX * v.NY);
-            for (int q = 0; q < 2; ++q) {
-                if (nt * trans) HIPCHK(hipMemsetAsync(s->lpE.psi[f][q], 0, nt * trans * 4, s->st));
-                if (nt * trans) HIPCHK(hipMemsetAsync(s->lpH.psi[f][q], 0, nt * trans * 4, s->st));
-            }
-        }
-        const int T = v.time_steps;
-        HIPCHK(hipMemcpyAsync(s->d_bstep, &T, sizeof(int), hipMemcpyHostToDevice, s->st));
-        // one captured backward step, replayed T times
-        hipGraph_t gr = nullptr;
-        hipGraphExec_t gx = nullptr;
-        HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-        if (v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, true, F[0], F[1], dga, dgb);
-        else s->enqueue_backward_t<uint16_t>(s->st, true, F[0], F[1], dga, dgb);
-        HIPCHK(hipStreamEndCapture(s->st, &gr));
-        HIPCHK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
-        HIPCHK(hipEventRecord(s->ev0, s->st));
-        for (int n = 0; n < T; ++n) HIPCHK(hipGraphLaunch(gx, s->st));
-        HIPCHK(hipEventRecord(s->ev1, s->st));
-        HIPCHK(hipEventSynchronize(s->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-        s->last_ms = ms / (float)T;
-        (void)hipGraphExecDestroy(gx);
-        (void)hipGraphDestroy(gr);
-        std::vector<double> ha(3 * ne), hb(3 * ne);
-        HIPCHK(hipMemcpy(ha.data(), dga, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(hb.data(), dgb, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (void* p : tmp) (void)hipFree(p);
-        eps_chain_rule(*s->plan, s->rec, s->rec_lo, s->rec_hi, ha, hb, n_mat, grad);
+        std::vector<double> ha, hb, cells;
+        backward_sweep(s, ha, hb);
+        cell_gradient(*s->plan, s->rec, s->rec_lo, s->rec_hi, nullptr, ha, hb, cells);
+        const aem_plan_view& v = s->plan->v;
+        for (int m = 0; m < n_mat; ++m) grad[m] = 0.0;
+        size_t b = 0;
+        for (int k = s->rec_lo[2]; k <= s->rec_hi[2]; ++k)
+            for (int j = s->rec_lo[1]; j <= s->rec_hi[1]; ++j)
+                for (int i = s->rec_lo[0]; i <= s->rec_hi[0]; ++i, ++b) {
+                    const int m = s->plan->cell_mat[(size_t)(i - 1) +
+                                                    (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (k - 1))];
+                    if (m >= 1 && m <= n_mat) grad[m - 1] += cells[b];
+                }
+    });
+}
+
+aem_status aem_adjoint_design_gradient(aem_sim* s, double* grad, int64_t n) {
+    if (!s || !grad) {
+        aem::set_error("aem_adjoint_design_gradient: bad argument");
+        return AEM_ERR_STATE;
+    }
+    if (!s->des_on) {
+        aem::set_error("aem_adjoint_design_gradient: no design box (aem_design_box)");
+        return AEM_ERR_STATE;
+    }
+    if (!gradient_ready(s, "aem_adjoint_design_gradient")) return AEM_ERR_STATE;
+    const int64_t ncb = (int64_t)s->des_eps.size() / 3;
+    if (n != ncb) {
+        aem::set_error("aem_adjoint_design_gradient: n must equal the design box's cell count " + std::to_string(ncb));
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        std::vector<double> ha, hb, cells;
+        backward_sweep(s, ha, hb);
+        cell_gradient(*s->plan, s->rec, s->rec_lo, s->rec_hi, s->des_eps.data(), ha, hb, cells);
+        std::memcpy(grad, cells.data(), cells.size() * sizeof(double));
     });
 }
 

@@ -163,6 +163,33 @@ class Simulation:
         check(lib().aem_adjoint_eps_gradient(self._h, int(n_materials), out.ctypes.data))
         return out
 
+    def design_box(self, lo, hi) -> None:
+        """Give the cells [lo, hi] (1-based, inclusive) per-cell permittivity
+        (aem_design_box); call before stepping.  Also records the box for the
+        adjoint."""
+        a = np.ascontiguousarray(lo, np.int32)
+        b = np.ascontiguousarray(hi, np.int32)
+        check(lib().aem_design_box(self._h, a.ctypes.data, b.ctypes.data))
+        self._design_shape = tuple(int(b[q] - a[q] + 1) for q in (2, 1, 0))
+
+    @property
+    def design_shape(self):
+        """(nz, ny, nx) of the design box."""
+        return self._design_shape
+
+    def set_design_eps(self, eps) -> None:
+        """Relative permittivity of every design cell, shape ``design_shape``
+        (or flat, x fastest)."""
+        a = np.ascontiguousarray(eps, np.float32).reshape(-1)
+        check(lib().aem_set_design_eps(self._h, a.ctypes.data, a.size))
+
+    def design_gradient(self) -> np.ndarray:
+        """d|F|^2 / d eps_r of every design cell, shape ``design_shape``
+        (aem_adjoint_design_gradient); after the full run."""
+        out = np.zeros(int(np.prod(self._design_shape)), np.float64)
+        check(lib().aem_adjoint_design_gradient(self._h, out.ctypes.data, out.size))
+        return out.reshape(self._design_shape)
+
     def adjoint_step(self, steps: int = 1) -> None:
         check(lib().aem_adjoint_step(self._h, int(steps)))
 

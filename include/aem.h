@@ -210,6 +210,21 @@ aem_status aem_adjoint_record(aem_sim* sim, const int32_t* lo, const int32_t* hi
  * the setup's four-cell edge averaging (fdtd_solver.f90:2516-2527). */
 aem_status aem_adjoint_eps_gradient(aem_sim* sim, int32_t n_mat, double* grad_eps);
 
+/* Design region: the cells [lo, hi] (as for aem_adjoint_record, which this
+ * implies) take their relative permittivity per cell from
+ * aem_set_design_eps instead of the deck's materials (initially the deck's
+ * values).  Conductivity stays the deck's.  The reference reaches per-voxel
+ * permittivity only through optional_geom_bulk.bin with one registered
+ * material per voxel (fdtd_solver.f90:2051-2095); here the box's edges keep
+ * their own Ga/Gb.  Call before the first step. */
+aem_status aem_design_box(aem_sim* sim, const int32_t* lo, const int32_t* hi);
+/* eps: one isotropic value (>= 1) per cell of the design box, x fastest;
+ * n = the box's cell count.  Call before the first step. */
+aem_status aem_set_design_eps(aem_sim* sim, const float* eps, int64_t n);
+/* After the full run: dJ/d eps_r of every cell of the design box (x fastest,
+ * n = the box's cell count), J = |F|^2 of the objective. */
+aem_status aem_adjoint_design_gradient(aem_sim* sim, double* grad, int64_t n);
+
 /* Low-level adjoint access: apply the transposed step (no source) n_steps
  * times to the adjoint state, and read/write its six field components
  * (layout of aem_get_field).  The adjoint CPML state starts at zero. */

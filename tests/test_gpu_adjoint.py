@@ -269,3 +269,135 @@ def test_eps_gradient_matches_finite_difference(scatter_run):
     print("adjoint", g_adj, "\nfinite difference", g_fd)
     assert np.linalg.norm(g_adj - g_fd) <= 1e-3 * np.linalg.norm(g_fd)
     assert np.all(np.abs(g_adj - g_fd) <= 2e-3 * np.abs(g_fd) + 1e-4 * np.abs(g_fd).max())
+
+
+# ---------------------------------------------------------------------------
+# per-cell design region
+# ---------------------------------------------------------------------------
+BOX_LO, BOX_HI = (18, 18, 19), (20, 20, 20)      # 3 x 3 x 2 design cells
+
+
+def design_eps(seed=5):
+    """Per-cell permittivity of the design box, U[1, 4], shape (nz, ny, nx)."""
+    rng = np.random.default_rng(seed)
+    shape = tuple(BOX_HI[a] - BOX_LO[a] + 1 for a in (2, 1, 0))
+    return rng.uniform(1.0, 4.0, shape).astype(np.float32)
+
+
+def design_deck(eps_cells=None):
+    """The plane-wave scatterer with a sigma = 0.5 block (eps 2) filling the
+    design box; with ``eps_cells`` every box cell is its own material
+    (100, 101, ...) of that permittivity -- the per-voxel geometry the
+    reference reaches through one material per voxel."""
+    d = scatter_deck()
+    d.materials.append(decks.Material(30, (2.0, 2.0, 2.0), (0.5, 0.5, 0.5)))
+    size = tuple(BOX_HI[a] - BOX_LO[a] + 1 for a in range(3))
+    d.objects.append(decks.Block(30, BOX_LO, size))
+    if eps_cells is not None:
+        q = 0
+        for k in range(size[2]):
+            for j in range(size[1]):
+                for i in range(size[0]):
+                    e = float(eps_cells[k, j, i])
+                    d.materials.append(decks.Material(100 + q, (e, e, e), (0.5, 0.5, 0.5)))
+                    d.objects.append(decks.Block(100 + q, (BOX_LO[0] + i, BOX_LO[1] + j, BOX_LO[2] + k), (1, 1, 1)))
+                    q += 1
+    return d
+
+
+def test_design_defaults_reproduce_the_deck():
+    """A design box left at the deck's permittivity changes no bit."""
+    plan = Plan.from_text(design_deck().text())
+    a = Simulation(plan)
+    b = Simulation(plan)
+    b.design_box(BOX_LO, BOX_HI)
+    for s in (a, b):
+        s.run()
+        s.finalize()
+    for n in NAMES:
+        assert np.array_equal(a.field(n), b.field(n)), n
+    assert np.array_equal(a.series("e_theta", 0), b.series("e_theta", 0))
+
+
+def test_design_eps_matches_per_voxel_materials():
+    """Per-cell permittivity through the design box is bit-identical to the
+    same permittivity declared as one material per voxel."""
+    eps = design_eps()
+    a = Simulation(Plan.from_text(design_deck(eps).text()))
+    b = Simulation(Plan.from_text(design_deck().text()))
+    b.design_box(BOX_LO, BOX_HI)
+    b.set_design_eps(eps)
+    for s in (a, b):
+        s.run()
+        s.finalize()
+    for n in NAMES:
+        assert np.array_equal(a.field(n), b.field(n)), n
+    assert np.array_equal(a.series("e_theta", 0), b.series("e_theta", 0))
+
+
+def test_design_transposed_step_dot_product():
+    """Dot-product test of the transposed step with a design box (per-edge
+    coefficients applied after the neutral bulk update)."""
+    plan = Plan.from_text(dot_deck().text())
+    m = masks(plan)
+    rng = np.random.default_rng(29)
+    x = {n: np.where(m[n], rng.standard_normal(plan.shape), 0.0).astype(np.float32) for n in NAMES}
+    lam = {n: np.where(m[n], rng.standard_normal(plan.shape), 0.0).astype(np.float32) for n in NAMES}
+    sim = Simulation(plan)
+    sim.design_box((12, 13, 12), (16, 18, 15))
+    sim.set_design_eps(rng.uniform(1.0, 6.0, sim.design_shape))
+    for n in NAMES:
+        sim.set_field(n, x[n])
+    sim.run(3, graph=False)
+    y = {n: sim.field(n).astype(np.float64) for n in NAMES}
+    for n in NAMES:
+        sim.set_adjoint_field(n, lam[n])
+    sim.adjoint_step(3)
+    mu = {n: sim.adjoint_field(n).astype(np.float64) for n in NAMES}
+    terms_l = np.concatenate([(lam[n] * y[n]).ravel() for n in NAMES])
+    terms_r = np.concatenate([(np.where(m[n], mu[n], 0.0) * x[n]).ravel() for n in NAMES])
+    lhs, rhs = terms_l.sum(), terms_r.sum()
+    scale = math.sqrt((terms_l ** 2).sum() + (terms_r ** 2).sum())
+    print(f"design: <lam,Ax>={lhs:.9e} <A^T lam,x>={rhs:.9e} rel={(lhs - rhs) / scale:.3e}")
+    assert abs(lhs - rhs) <= 1e-5 * scale
+
+
+def test_design_gradient_matches_material_gradient(scatter_run):
+    """On the four single-cell materials of the scatterer, the per-cell
+    gradient of a design box over them equals the per-material gradient."""
+    plan, sim = scatter_run
+    g_mat = sim.eps_gradient(4)
+    d = Simulation(plan)
+    d.set_objective(THETA, PHI, FREQ, "theta")
+    d.design_box(DESIGN_LO, DESIGN_HI)
+    d.run()
+    assert abs(d.objective() - sim.objective()) <= 1e-12 * abs(sim.objective())
+    g = d.design_gradient()           # (1, 2, 2): cells (19..20, 19..20, 20)
+    g_cells = g.reshape(-1)           # material q sits at cell q (x fastest)
+    print("material", g_mat, "\ncells", g_cells)
+    assert np.allclose(g_cells, g_mat, rtol=1e-5, atol=0)
+
+
+def test_design_gradient_matches_finite_difference():
+    eps = design_eps()
+    plan = Plan.from_text(design_deck().text())
+    sim = Simulation(plan)
+    sim.set_objective(THETA, PHI, FREQ, "theta")
+    sim.design_box(BOX_LO, BOX_HI)
+    sim.set_design_eps(eps)
+    sim.run()
+    g_adj = sim.design_gradient()
+    rng = np.random.default_rng(16)
+    cells = [tuple(int(v) for v in np.unravel_index(q, eps.shape)) for q in rng.choice(eps.size, 4, replace=False)]
+    h = 0.05
+    for c in cells:
+        J = []
+        e32 = []
+        for sgn in (1, -1):
+            e = eps.copy()
+            e[c] = np.float32(e[c] + sgn * h)
+            e32.append(float(e[c]))
+            J.append(abs(oracle_objective(Plan.from_text(design_deck(e).text()), True, THETA, PHI, FREQ)) ** 2)
+        g_fd = (J[0] - J[1]) / (e32[0] - e32[1])
+        print(f"cell {c}: adjoint {g_adj[c]:.9e} finite difference {g_fd:.9e}")
+        assert abs(g_adj[c] - g_fd) <= 1e-3 * abs(g_fd)
