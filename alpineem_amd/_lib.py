@@ -96,7 +96,7 @@ def check(status: int) -> None:
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
 
-NTFF_BINNING = {"ordered": 0, "atomic": 1, "histogram": 2}
+NTFF_BINNING = {"ordered": 0, "atomic": 1, "histogram": 2, "off": 3}
 
 
 class SimOptions(C.Structure):

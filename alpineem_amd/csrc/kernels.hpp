@@ -1480,14 +1480,17 @@ __global__ void __launch_bounds__(256) k_dft(const float* __restrict__ x, int n,
 //   L = sum M e^{jk r^.r'} dS,  N = sum J e^{jk r^.r'} dS,
 //   E_theta r = -jk/(4 pi) (L_phi + eta N_theta),  E_phi r = jk/(4 pi) (L_theta - eta N_phi),
 // M sampled at t = n dt (E after step n), J at t = (n - 1/2) dt, r' the
-// face cell's offset from the phase centre.  Accumulated in FP64.
+// face cell's offset from the phase centre.  Accumulated in FP64; the
+// spatial factor pre dS e^{jk r^.r'} of every face cell and the time factors
+// of every step are tables built on the host.
 struct Objective {
     int lo[3], hi[3], pbc[3];
-    int ic, jc, kc;
-    double d[3], dt, omega, k;
-    double u[3];              // direction r^
+    int mu, mv;               // face table extents
     double gM[3], gJ[3];      // real projections of M and J components
-    double pre_re, pre_im;    // complex prefactor
+    const double2* g;         // [6][mv][mu]: pre dS e^{jk r^.r'} of each face cell
+    const double2* tE;        // [T]: e^{-jw n dt}
+    const double2* tH;        // [T]: e^{-jw (n - 1/2) dt}
+    double* part;             // [blocks][2]: per-work-group partial sums of a step
     double* F;                // (re, im), device
 };
 
@@ -1522,21 +1525,14 @@ __device__ __forceinline__ void face_terms(const Grid& g, int a, bool low, FaceT
 }
 
 // Complex weight of each face term at step n:  dF/d(term value).
-__device__ __forceinline__ void face_weights(const Objective& ob, int a, int i, int j, int k, int n, double wr[4],
+__device__ __forceinline__ void face_weights(const Objective& ob, int face, int uu, int vv, int n, double wr[4],
                                              double wi[4]) {
+    const int a = face / 2;
     const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-    const double dS = ob.d[b] * ob.d[c];
-    const double ph = ob.k * (ob.u[0] * (i - ob.ic) * ob.d[0] + ob.u[1] * (j - ob.jc) * ob.d[1] +
-                              ob.u[2] * (k - ob.kc) * ob.d[2]);
-    const double tE = n * ob.dt, tH = (n - 0.5) * ob.dt;
-    double sp, cp, sE, cE, sH, cH;
-    sincos(ph, &sp, &cp);
-    sincos(-ob.omega * tE, &sE, &cE);
-    sincos(-ob.omega * tH, &sH, &cH);
-    // g = pre * dS * e^{j ph}
-    const double gr = (ob.pre_re * cp - ob.pre_im * sp) * dS, gi = (ob.pre_re * sp + ob.pre_im * cp) * dS;
-    const double eMr = gr * cE - gi * sE, eMi = gr * sE + gi * cE;
-    const double eJr = gr * cH - gi * sH, eJi = gr * sH + gi * cH;
+    const double2 g = ob.g[((long long)face * ob.mv + vv) * ob.mu + uu];
+    const double2 e = ob.tE[n - 1], h = ob.tH[n - 1];
+    const double eMr = g.x * e.x - g.y * e.y, eMi = g.x * e.y + g.y * e.x;
+    const double eJr = g.x * h.x - g.y * h.y, eJi = g.x * h.y + g.y * h.x;
     wr[0] = eMr * ob.gM[b]; wi[0] = eMi * ob.gM[b];
     wr[1] = eMr * ob.gM[c]; wi[1] = eMi * ob.gM[c];
     wr[2] = eJr * ob.gJ[b]; wi[2] = eJi * ob.gJ[b];
@@ -1557,7 +1553,7 @@ __device__ __forceinline__ bool face_cell(const Objective& ob, int face, int uu,
     return true;
 }
 
-// F += sum over face cells of this step's terms.  grid (x, y, 6 faces), block 256.
+// Per-work-group partial sums of this step's terms.  grid (x, y, 6 faces), block 256.
 __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const float* Ex, const float* Ey,
                                                    const float* Ez, const float* Hx, const float* Hy,
                                                    const float* Hz, const int* step) {
@@ -1566,13 +1562,13 @@ __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const f
     const float* Fl[6] = {Ex, Ey, Ez, Hx, Hy, Hz};
     double re = 0.0, im = 0.0;
     int i, j, k;
-    if (face_cell(ob, face, blockIdx.x * 32 + (tid & 31), blockIdx.y * 8 + (tid >> 5), i, j, k)) {
+    const int uu = blockIdx.x * 32 + (tid & 31), vv = blockIdx.y * 8 + (tid >> 5);
+    if (face_cell(ob, face, uu, vv, i, j, k)) {
         const int n = *step;
-        const int a = face / 2;
         FaceTerm t[4];
-        face_terms(g, a, (face & 1) == 0, t);
+        face_terms(g, face / 2, (face & 1) == 0, t);
         double wr[4], wi[4];
-        face_weights(ob, a, i, j, k, n, wr, wi);
+        face_weights(ob, face, uu, vv, n, wr, wi);
         const long long o = off(g, i, j, k);
         for (int q = 0; q < 4; ++q) {
             double s = 0.0;
@@ -1594,8 +1590,34 @@ __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const f
         __syncthreads();
     }
     if (tid == 0) {
-        atomicAdd(ob.F, sr[0]);
-        atomicAdd(ob.F + 1, si[0]);
+        const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+        ob.part[2 * blk] = sr[0];
+        ob.part[2 * blk + 1] = si[0];
+    }
+}
+
+// F += the step's partial sums, in a fixed order.  One work-group of 256.
+__global__ void __launch_bounds__(256) k_objective_sum(Objective ob, int nblk) {
+    const int tid = threadIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int q = tid; q < nblk; q += 256) {
+        re += ob.part[2 * q];
+        im += ob.part[2 * q + 1];
+    }
+    __shared__ double sr[256], si[256];
+    sr[tid] = re;
+    si[tid] = im;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sr[tid] += sr[tid + w];
+            si[tid] += si[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        ob.F[0] = ob.F[0] + sr[0];
+        ob.F[1] = ob.F[1] + si[0];
     }
 }
 
@@ -1606,15 +1628,14 @@ __global__ void __launch_bounds__(256) k_adj_source(Grid g, Objective ob, double
                                                     const int* step) {
     const int face = blockIdx.z;
     int i, j, k;
-    if (!face_cell(ob, face, blockIdx.x * 32 + (threadIdx.x & 31), blockIdx.y * 8 + (threadIdx.x >> 5), i, j, k))
-        return;
+    const int uu = blockIdx.x * 32 + (threadIdx.x & 31), vv = blockIdx.y * 8 + (threadIdx.x >> 5);
+    if (!face_cell(ob, face, uu, vv, i, j, k)) return;
     float* L[6] = {lEx, lEy, lEz, lHx, lHy, lHz};
     const int n = *step;
-    const int a = face / 2;
     FaceTerm t[4];
-    face_terms(g, a, (face & 1) == 0, t);
+    face_terms(g, face / 2, (face & 1) == 0, t);
     double wr[4], wi[4];
-    face_weights(ob, a, i, j, k, n, wr, wi);
+    face_weights(ob, face, uu, vv, n, wr, wi);
     const long long o = off(g, i, j, k);
     for (int q = 0; q < 4; ++q) {
         const double dj = 2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w;
@@ -1693,11 +1714,86 @@ __device__ __forceinline__ float adj_mu(const Grid& g, const float* lE, const Id
     return tup[(int)idx[o] * 6 + 2 * c + 1] * lE[o];
 }
 
-// lambda_H += (transposed E-update curl) applied to mu.  One thread per H cell.
+// CPML psi cell of the E (slab_e) or H (slab_h) update normal to axis A at
+// (i,j,k): face f, local layer q and psi offset po (as in k_e_cpml_* /
+// k_h_cpml_*); false outside the slab or its transverse range.
+__device__ __forceinline__ bool e_psi(const Grid& g, const Cpml& p, int A, int i, int j, int k, int& f, int& q,
+                                      long long& po) {
+    const int N[3] = {g.NX, g.NY, g.NZ};
+    const int x[3] = {i, j, k};
+    const int o1 = A == 0 ? 1 : 0, o2 = A == 2 ? 1 : 2;
+    if (x[o1] < 1 || x[o1] > N[o1] - 1 || x[o2] < 1 || x[o2] > N[o2] - 1) return false;
+    int t;
+    const int side = slab_e(x[A], N[A], p.n[2 * A], p.n[2 * A + 1], q, t);
+    if (side < 0) return false;
+    f = 2 * A + side;
+    const long long ntf = p.n[f] - 1;
+    if (A == 0) po = t + ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+    else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * (t + ntf * (k - 1));
+    else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * t);
+    return true;
+}
+__device__ __forceinline__ bool h_psi(const Grid& g, const Cpml& p, int A, int i, int j, int k, int& f, int& q,
+                                      long long& po) {
+    const int N[3] = {g.NX, g.NY, g.NZ};
+    const int x[3] = {i, j, k};
+    const int o1 = A == 0 ? 1 : 0, o2 = A == 2 ? 1 : 2;
+    if (x[o1] < 1 || x[o1] > N[o1] - 1 || x[o2] < 1 || x[o2] > N[o2] - 1 || x[A] < 1) return false;
+    const int side = slab_h(x[A], N[A], p.n[2 * A], p.n[2 * A + 1], q);
+    if (side < 0) return false;
+    f = 2 * A + side;
+    const long long ntf = p.n[f] - 1;
+    if (A == 0) po = (q - 1) + ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+    else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * ((q - 1) + ntf * (k - 1));
+    else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (q - 1));
+    return true;
+}
+
+// Slot conditions of the E psi (slot 0/1 of axis A): x: (Ey, k>=2), (Ez, j>=2);
+// y: (Ex, k>=2), (Ez, i>=2); z: (Ex, j>=2), (Ey, i>=2).
+__device__ __forceinline__ bool e_slot_ok(int A, int slot, int i, int j, int k) {
+    if (A == 0) return slot == 0 ? k >= 2 : j >= 2;
+    if (A == 1) return slot == 0 ? k >= 2 : i >= 2;
+    return slot == 0 ? j >= 2 : i >= 2;
+}
+__device__ __forceinline__ int e_slot_comp(int A, int slot) { return A == 0 ? 1 + slot : (A == 1 ? 2 * slot : slot); }
+__device__ __forceinline__ int h_slot_comp(int A, int slot) { return A == 0 ? 1 + slot : (A == 1 ? 2 * slot : slot); }
+
+// c * nu of an E / H psi cell, 0 where there is none.  The adjoint CPML
+// state holds nu = lambda_psi' + (gb or DB) lambda_field' of the step being
+// transposed; lambda_psi of the step before is b nu, folded into the next
+// update of nu.
+__device__ __forceinline__ float cnu_e(const Grid& g, const Cpml& p, const Cpml& lp, int A, int slot, int i, int j,
+                                       int k) {
+    int f, q;
+    long long po;
+    if (!e_psi(g, p, A, i, j, k, f, q, po) || !e_slot_ok(A, slot, i, j, k)) return 0.0f;
+    return p.c[f][q - 1] * lp.psi[f][slot][po];
+}
+__device__ __forceinline__ float cnu_h(const Grid& g, const Cpml& p, const Cpml& lp, int A, int slot, int i, int j,
+                                       int k) {
+    int f, q;
+    long long po;
+    if (!h_psi(g, p, A, i, j, k, f, q, po)) return 0.0f;
+    return p.c[f][q - 1] * lp.psi[f][slot][po];
+}
+
+// Within one cell of a CPML slab (either field's layers)?
+__device__ __forceinline__ bool near_slab(const Grid& g, const Cpml& p, int i, int j, int k) {
+    return i <= p.n[0] + 1 || i >= g.NX - p.n[1] - 1 || j <= p.n[2] + 1 || j >= g.NY - p.n[3] - 1 ||
+           k <= p.n[4] + 1 || k >= g.NZ - p.n[5] - 1;
+}
+
+// Transposed E phase, gathered per H cell (one thread per cell of
+// [1, N-1]^3, block 64 x 4):
+//   lambda_H += S^T (gb lambda_E')                     (bulk E update)
+//             + D^T (c nu_E) / d                        (E-phase CPML)
+// then nu_H <- b nu_H + DB lambda_H on the H-phase psi cells.
 template <typename IdxT>
-__global__ void k_adj_h_gather(Grid g, float* lHx, float* lHy, float* lHz, const float* lEx, const float* lEy,
-                               const float* lEz, const IdxT* idx, const float* tup, const float* dex,
-                               const float* dey, const float* dez) {
+__global__ void __launch_bounds__(256) k_adj_h(Grid g, float* lHx, float* lHy, float* lHz, const float* lEx,
+                                               const float* lEy, const float* lEz, const IdxT* idx, const float* tup,
+                                               const float* dex, const float* dey, const float* dez, Cpml pe, Cpml lpe,
+                                               Cpml ph, Cpml lph, float DB, float dx, float dy, float dz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -1712,17 +1808,43 @@ __global__ void k_adj_h_gather(Grid g, float* lHx, float* lHy, float* lHz, const
                 mz_j = adj_mu(g, lEz, idx, tup, 2, i, j + 1, k);
     const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
     const float dx_i1 = de(dex, g.NX, i + 1), dy_j1 = de(dey, g.NY, j + 1), dz_k1 = de(dez, g.NZ, k + 1);
-    lHx[o] += my * dz_k - my_k * dz_k1 + mz_j * dy_j1 - mz * dy_j;
-    lHy[o] += mx_k * dz_k1 - mx * dz_k + mz * dx_i - mz_i * dx_i1;
-    lHz[o] += mx * dy_j - mx_j * dy_j1 + my_i * dx_i1 - my * dx_i;
+    float hx = lHx[o] + (my * dz_k - my_k * dz_k1 + mz_j * dy_j1 - mz * dy_j);
+    float hy = lHy[o] + (mx_k * dz_k1 - mx * dz_k + mz * dx_i - mz_i * dx_i1);
+    float hz = lHz[o] + (mx * dy_j - mx_j * dy_j1 + my_i * dx_i1 - my * dx_i);
+    if (near_slab(g, pe, i, j, k)) {
+        hx += (cnu_e(g, pe, lpe, 1, 1, i, j + 1, k) - cnu_e(g, pe, lpe, 1, 1, i, j, k)) / dy +
+              (cnu_e(g, pe, lpe, 2, 1, i, j, k) - cnu_e(g, pe, lpe, 2, 1, i, j, k + 1)) / dz;
+        hy += (cnu_e(g, pe, lpe, 0, 1, i, j, k) - cnu_e(g, pe, lpe, 0, 1, i + 1, j, k)) / dx +
+              (cnu_e(g, pe, lpe, 2, 0, i, j, k + 1) - cnu_e(g, pe, lpe, 2, 0, i, j, k)) / dz;
+        hz += (cnu_e(g, pe, lpe, 0, 0, i + 1, j, k) - cnu_e(g, pe, lpe, 0, 0, i, j, k)) / dx +
+              (cnu_e(g, pe, lpe, 1, 0, i, j, k) - cnu_e(g, pe, lpe, 1, 0, i, j + 1, k)) / dy;
+        const float lh[3] = {hx, hy, hz};
+        for (int A = 0; A < 3; ++A) {
+            int f, q;
+            long long po;
+            if (!h_psi(g, ph, A, i, j, k, f, q, po)) continue;
+            const float b = ph.b[f][q - 1];
+            for (int sl = 0; sl < 2; ++sl) {
+                float* nu = lph.psi[f][sl];
+                nu[po] = b * nu[po] + DB * lh[h_slot_comp(A, sl)];
+            }
+        }
+    }
+    lHx[o] = hx;
+    lHy[o] = hy;
+    lHz[o] = hz;
 }
 
-// lambda_E <- ga lambda_E + DB (transposed H-update curl) lambda_H on updated
-// components; 0 on the fixed outer tangential components.
+// Transposed H phase, gathered per E cell (one thread per cell of
+// [1, NX] x [1, NY] x [1, NZ], block 64 x 4) on the updated components:
+//   lambda_E <- ga lambda_E + DB C^T lambda_H + D^T (c nu_H) / d,
+// 0 on the fixed outer tangential components; then
+// nu_E <- b nu_E + gb lambda_E on the E-phase psi cells.
 template <typename IdxT>
-__global__ void k_adj_e_gather(Grid g, float* lEx, float* lEy, float* lEz, const float* lHx, const float* lHy,
-                               const float* lHz, const IdxT* idx, const float* tup, const float* dhx,
-                               const float* dhy, const float* dhz, float DB) {
+__global__ void __launch_bounds__(256) k_adj_e(Grid g, float* lEx, float* lEy, float* lEz, const float* lHx,
+                                               const float* lHy, const float* lHz, const IdxT* idx, const float* tup,
+                                               const float* dhx, const float* dhy, const float* dhz, Cpml pe, Cpml lpe,
+                                               Cpml ph, Cpml lph, float DB, float dx, float dy, float dz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -1730,161 +1852,71 @@ __global__ void k_adj_e_gather(Grid g, float* lEx, float* lEy, float* lEz, const
     const long long o = off(g, i, j, k);
     auto lh = [&](const float* L, int a, int b, int c) { return h_in(g, a, b, c) ? L[off(g, a, b, c)] : 0.0f; };
     auto dh = [](const float* d, int n, int q) { return (q >= 1 && q <= n - 1) ? d[q - 1] : 0.0f; };
-    const int t = e_upd(g, 0, i, j, k) || e_upd(g, 1, i, j, k) || e_upd(g, 2, i, j, k) ? (int)idx[o] * 6 : 0;
-    if (e_upd(g, 0, i, j, k))
-        lEx[o] = tup[t] * lEx[o] + DB * (lh(lHy, i, j, k) * dh(dhz, g.NZ, k) - lh(lHy, i, j, k - 1) * dh(dhz, g.NZ, k - 1) +
-                                         lh(lHz, i, j - 1, k) * dh(dhy, g.NY, j - 1) - lh(lHz, i, j, k) * dh(dhy, g.NY, j));
-    else if (i <= g.NX && j <= g.NY && k <= g.NZ)
-        lEx[o] = 0.0f;
-    if (e_upd(g, 1, i, j, k))
-        lEy[o] = tup[t + 2] * lEy[o] + DB * (lh(lHx, i, j, k - 1) * dh(dhz, g.NZ, k - 1) - lh(lHx, i, j, k) * dh(dhz, g.NZ, k) +
-                                             lh(lHz, i, j, k) * dh(dhx, g.NX, i) - lh(lHz, i - 1, j, k) * dh(dhx, g.NX, i - 1));
-    else
-        lEy[o] = 0.0f;
-    if (e_upd(g, 2, i, j, k))
-        lEz[o] = tup[t + 4] * lEz[o] + DB * (lh(lHx, i, j, k) * dh(dhy, g.NY, j) - lh(lHx, i, j - 1, k) * dh(dhy, g.NY, j - 1) +
-                                             lh(lHy, i - 1, j, k) * dh(dhx, g.NX, i - 1) - lh(lHy, i, j, k) * dh(dhx, g.NX, i));
-    else
-        lEz[o] = 0.0f;
-}
-
-// Transposed CPML of the E phase, slab normal to axis `ax` (threads as in
-// k_e_cpml_*).  nu = lambda_psi' + gb lambda_E on the psi-updated cells;
-// lambda_psi <- b nu; lambda_H gets the transposed difference.
-template <typename IdxT>
-__global__ void k_adj_e_cpml(Grid g, Cpml p, Cpml lp, int ax, const float* lEx, const float* lEy,
-                             const float* lEz, float* lHx, float* lHy, float* lHz, const IdxT* idx,
-                             const float* tup, float dd) {
-    const int A = ax;
-    const int n1 = p.n[2 * A], n2 = p.n[2 * A + 1];
-    const int nl = max(n1 - 1, 0), nh = max(n2 - 1, 0), nt = nl + nh;
-    const int N[3] = {g.NX, g.NY, g.NZ};
-    const int o1 = A == 0 ? 1 : 0, o2 = A == 2 ? 1 : 2;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long tot = (long long)nt * (N[o1] - 1) * (N[o2] - 1);
-    if (tid >= tot) return;
-    const int t = (int)(tid % nt);
-    const long long r = tid / nt;
-    const int u = (int)(r % (N[o1] - 1)) + 1, w = (int)(r / (N[o1] - 1)) + 1;
-    int pos, f, q, ql;
-    if (t < nl) { pos = t + 2; f = 2 * A; q = pos; ql = t; }
-    else { pos = N[A] + 1 - n2 + (t - nl); f = 2 * A + 1; q = N[A] + 1 - pos; ql = q - 2; }
-    int ijk[3];
-    ijk[A] = pos; ijk[o1] = u; ijk[o2] = w;
-    const int i = ijk[0], j = ijk[1], k = ijk[2];
-    const int ntf = f % 2 == 0 ? nl : nh;
-    long long po;
-    if (A == 0) po = (long long)ql + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
-    else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * ((long long)ql + (long long)ntf * (k - 1));
-    else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * ql);
-    const long long o = off(g, i, j, k);
-    const long long step = A == 0 ? 1 : (A == 1 ? g.PX : g.plane);
-    const float cb = p.b[f][q - 1], cc = p.c[f][q - 1];
-    const int t6 = (int)idx[o] * 6;
-    if (A == 0) {
-        if (k >= 2) {   // psi_Eyx:  Ey += gby psi,  psi' = b psi + c (Hz(i-1) - Hz) / dx
-            const float nu = lp.psi[f][0][po] + tup[t6 + 3] * lEy[o];
-            lp.psi[f][0][po] = cb * nu;
-            atomicAdd(&lHz[o - step], cc * nu / dd);
-            atomicAdd(&lHz[o], -(cc * nu / dd));
-        }
-        if (j >= 2) {   // psi_Ezx:  Ez += gbz psi,  psi' = b psi + c (Hy - Hy(i-1)) / dx
-            const float nu = lp.psi[f][1][po] + tup[t6 + 5] * lEz[o];
-            lp.psi[f][1][po] = cb * nu;
-            atomicAdd(&lHy[o], cc * nu / dd);
-            atomicAdd(&lHy[o - step], -(cc * nu / dd));
-        }
-    } else if (A == 1) {
-        if (k >= 2) {   // psi_Exy:  Ex += gbx psi,  psi' = b psi + c (Hz - Hz(j-1)) / dy
-            const float nu = lp.psi[f][0][po] + tup[t6 + 1] * lEx[o];
-            lp.psi[f][0][po] = cb * nu;
-            atomicAdd(&lHz[o], cc * nu / dd);
-            atomicAdd(&lHz[o - step], -(cc * nu / dd));
-        }
-        if (i >= 2) {   // psi_Ezy:  Ez += gbz psi,  psi' = b psi + c (Hx(j-1) - Hx) / dy
-            const float nu = lp.psi[f][1][po] + tup[t6 + 5] * lEz[o];
-            lp.psi[f][1][po] = cb * nu;
-            atomicAdd(&lHx[o - step], cc * nu / dd);
-            atomicAdd(&lHx[o], -(cc * nu / dd));
-        }
-    } else {
-        if (j >= 2) {   // psi_Exz:  Ex += gbx psi,  psi' = b psi + c (Hy(k-1) - Hy) / dz
-            const float nu = lp.psi[f][0][po] + tup[t6 + 1] * lEx[o];
-            lp.psi[f][0][po] = cb * nu;
-            atomicAdd(&lHy[o - step], cc * nu / dd);
-            atomicAdd(&lHy[o], -(cc * nu / dd));
-        }
-        if (i >= 2) {   // psi_Eyz:  Ey += gby psi,  psi' = b psi + c (Hx - Hx(k-1)) / dz
-            const float nu = lp.psi[f][1][po] + tup[t6 + 3] * lEy[o];
-            lp.psi[f][1][po] = cb * nu;
-            atomicAdd(&lHx[o], cc * nu / dd);
-            atomicAdd(&lHx[o - step], -(cc * nu / dd));
+    const bool ux = e_upd(g, 0, i, j, k), uy = e_upd(g, 1, i, j, k), uz = e_upd(g, 2, i, j, k);
+    const int t = (ux || uy || uz) ? (int)idx[o] * 6 : 0;
+    const bool ns = near_slab(g, pe, i, j, k);
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    if (ux) {
+        ex = tup[t] * lEx[o] + DB * (lh(lHy, i, j, k) * dh(dhz, g.NZ, k) - lh(lHy, i, j, k - 1) * dh(dhz, g.NZ, k - 1) +
+                                     lh(lHz, i, j - 1, k) * dh(dhy, g.NY, j - 1) - lh(lHz, i, j, k) * dh(dhy, g.NY, j));
+        if (ns)
+            ex += (cnu_h(g, ph, lph, 2, 1, i, j, k) - cnu_h(g, ph, lph, 2, 1, i, j, k - 1)) / dz +
+                  (cnu_h(g, ph, lph, 1, 1, i, j - 1, k) - cnu_h(g, ph, lph, 1, 1, i, j, k)) / dy;
+    }
+    if (uy) {
+        ey = tup[t + 2] * lEy[o] + DB * (lh(lHx, i, j, k - 1) * dh(dhz, g.NZ, k - 1) - lh(lHx, i, j, k) * dh(dhz, g.NZ, k) +
+                                         lh(lHz, i, j, k) * dh(dhx, g.NX, i) - lh(lHz, i - 1, j, k) * dh(dhx, g.NX, i - 1));
+        if (ns)
+            ey += (cnu_h(g, ph, lph, 2, 0, i, j, k - 1) - cnu_h(g, ph, lph, 2, 0, i, j, k)) / dz +
+                  (cnu_h(g, ph, lph, 0, 1, i, j, k) - cnu_h(g, ph, lph, 0, 1, i - 1, j, k)) / dx;
+    }
+    if (uz) {
+        ez = tup[t + 4] * lEz[o] + DB * (lh(lHx, i, j, k) * dh(dhy, g.NY, j) - lh(lHx, i, j - 1, k) * dh(dhy, g.NY, j - 1) +
+                                         lh(lHy, i - 1, j, k) * dh(dhx, g.NX, i - 1) - lh(lHy, i, j, k) * dh(dhx, g.NX, i));
+        if (ns)
+            ez += (cnu_h(g, ph, lph, 0, 0, i - 1, j, k) - cnu_h(g, ph, lph, 0, 0, i, j, k)) / dx +
+                  (cnu_h(g, ph, lph, 1, 0, i, j, k) - cnu_h(g, ph, lph, 1, 0, i, j - 1, k)) / dy;
+    }
+    lEx[o] = ex;
+    lEy[o] = ey;
+    lEz[o] = ez;
+    if (ns) {
+        const float le[3] = {ex, ey, ez};
+        for (int A = 0; A < 3; ++A) {
+            int f, q;
+            long long po;
+            if (!e_psi(g, pe, A, i, j, k, f, q, po)) continue;
+            const float b = pe.b[f][q - 1];
+            for (int sl = 0; sl < 2; ++sl) {
+                if (!e_slot_ok(A, sl, i, j, k)) continue;
+                const int c = e_slot_comp(A, sl);
+                float* nu = lpe.psi[f][sl];
+                nu[po] = b * nu[po] + tup[t + 2 * c + 1] * le[c];
+            }
         }
     }
 }
 
-// Transposed CPML of the H phase, slab normal to axis `ax` (threads as in
-// k_h_cpml_*).  nu = lambda_psi' + DB lambda_H; lambda_psi <- b nu.
-__global__ void k_adj_h_cpml(Grid g, Cpml p, Cpml lp, int ax, float* lEx, float* lEy, float* lEz,
-                             const float* lHx, const float* lHy, const float* lHz, float DB, float dd) {
-    const int A = ax;
-    const int n1 = p.n[2 * A], n2 = p.n[2 * A + 1];
-    const int nl = max(n1 - 1, 0), nh = max(n2 - 1, 0), nt = nl + nh;
-    const int N[3] = {g.NX, g.NY, g.NZ};
-    const int o1 = A == 0 ? 1 : 0, o2 = A == 2 ? 1 : 2;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long tot = (long long)nt * (N[o1] - 1) * (N[o2] - 1);
-    if (tid >= tot) return;
-    const int t = (int)(tid % nt);
-    const long long r = tid / nt;
-    const int u = (int)(r % (N[o1] - 1)) + 1, w = (int)(r / (N[o1] - 1)) + 1;
-    int pos, f, q;
-    if (t < nl) { pos = t + 1; f = 2 * A; q = pos; }
-    else { pos = N[A] + 1 - n2 + (t - nl); f = 2 * A + 1; q = N[A] - pos; }
-    int ijk[3];
-    ijk[A] = pos; ijk[o1] = u; ijk[o2] = w;
-    const int i = ijk[0], j = ijk[1], k = ijk[2];
-    const int ntf = f % 2 == 0 ? nl : nh;
-    long long po;
-    if (A == 0) po = (long long)(q - 1) + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
-    else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * ((long long)(q - 1) + (long long)ntf * (k - 1));
-    else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (q - 1));
+// nu_E += sign * gb lambda_E of component `comp` on the E psi cells: keeps
+// nu consistent when the adjoint field is overwritten from the host.
+template <typename IdxT>
+__global__ void k_adj_nu_adjust(Grid g, const float* lE, int comp, float sign, const IdxT* idx, const float* tup,
+                                Cpml pe, Cpml lpe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.NX || j > g.NY) return;
+    if (!near_slab(g, pe, i, j, k)) return;
     const long long o = off(g, i, j, k);
-    const long long step = A == 0 ? 1 : (A == 1 ? g.PX : g.plane);
-    const float cb = p.b[f][q - 1], cc = p.c[f][q - 1];
-    if (A == 0) {
-        // psi_Hyx' = b psi + c (Ez(i+1) - Ez) / dx;  Hy += DB psi'
-        float nu = lp.psi[f][0][po] + DB * lHy[o];
-        lp.psi[f][0][po] = cb * nu;
-        atomicAdd(&lEz[o + step], cc * nu / dd);
-        atomicAdd(&lEz[o], -(cc * nu / dd));
-        // psi_Hzx' = b psi + c (Ey - Ey(i+1)) / dx;  Hz += DB psi'
-        nu = lp.psi[f][1][po] + DB * lHz[o];
-        lp.psi[f][1][po] = cb * nu;
-        atomicAdd(&lEy[o], cc * nu / dd);
-        atomicAdd(&lEy[o + step], -(cc * nu / dd));
-    } else if (A == 1) {
-        // psi_Hxy' = b psi + c (Ez - Ez(j+1)) / dy;  Hx += DB psi'
-        float nu = lp.psi[f][0][po] + DB * lHx[o];
-        lp.psi[f][0][po] = cb * nu;
-        atomicAdd(&lEz[o], cc * nu / dd);
-        atomicAdd(&lEz[o + step], -(cc * nu / dd));
-        // psi_Hzy' = b psi + c (Ex(j+1) - Ex) / dy;  Hz += DB psi'
-        nu = lp.psi[f][1][po] + DB * lHz[o];
-        lp.psi[f][1][po] = cb * nu;
-        atomicAdd(&lEx[o + step], cc * nu / dd);
-        atomicAdd(&lEx[o], -(cc * nu / dd));
-    } else {
-        // psi_Hxz' = b psi + c (Ey(k+1) - Ey) / dz;  Hx += DB psi'
-        float nu = lp.psi[f][0][po] + DB * lHx[o];
-        lp.psi[f][0][po] = cb * nu;
-        atomicAdd(&lEy[o + step], cc * nu / dd);
-        atomicAdd(&lEy[o], -(cc * nu / dd));
-        // psi_Hyz' = b psi + c (Ex - Ex(k+1)) / dz;  Hy += DB psi'
-        nu = lp.psi[f][1][po] + DB * lHy[o];
-        lp.psi[f][1][po] = cb * nu;
-        atomicAdd(&lEx[o], cc * nu / dd);
-        atomicAdd(&lEx[o + step], -(cc * nu / dd));
+    for (int A = 0; A < 3; ++A) {
+        int f, q;
+        long long po;
+        if (!e_psi(g, pe, A, i, j, k, f, q, po)) continue;
+        for (int sl = 0; sl < 2; ++sl) {
+            if (e_slot_comp(A, sl) != comp || !e_slot_ok(A, sl, i, j, k)) continue;
+            float* nu = lpe.psi[f][sl];
+            nu[po] = nu[po] + sign * (tup[(int)idx[o] * 6 + 2 * comp + 1] * lE[o]);
+        }
     }
 }
 

@@ -607,7 +607,7 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             k_plane_sums<<<1, 256, 0, s>>>(g, 2, lo, hi, v.ff_lo[0], v.ff_hi[0], v.ff_lo[1], v.ff_hi[1], Ey, Ex, v.ps_c1,
                                             v.ps_c2, 1.0f, 0, d_refl, d_trans, d_step);
     }
-    if ((parts & kPartNtff) && v.n_angles > 0) {
+    if ((parts & kPartNtff) && v.n_angles > 0 && ntff_mode != AEM_NTFF_OFF) {
         const dim3 gf(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
         k_ntff_currents<<<gf, bb, 0, s>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
         if (ntff_mode == AEM_NTFF_ORDERED) {
@@ -619,8 +619,11 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             k_ntff_bins<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6 * v.n_angles), bb, 0, s>>>(nf, nb, d_step);
         }
     }
-    if ((parts & kPartObjective) && obj_on)
-        k_objective<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6), 256, 0, s>>>(g, obj, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
+    if ((parts & kPartObjective) && obj_on) {
+        const dim3 go(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
+        k_objective<<<go, 256, 0, s>>>(g, obj, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
+        k_objective_sum<<<1, 256, 0, s>>>(obj, (int)(go.x * go.y * go.z));
+    }
     if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
         k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
     if (parts & kPartStep) k_step_inc<<<1, 1, 0, s>>>(d_step);
@@ -652,7 +655,7 @@ aem_status aem_sim_create_ex(const aem_plan* plan, const aem_sim_options* opt, a
         return AEM_ERR_STATE;
     }
     *out = nullptr;
-    if (opt && (opt->ntff_binning < AEM_NTFF_ORDERED || opt->ntff_binning > AEM_NTFF_HISTOGRAM ||
+    if (opt && (opt->ntff_binning < AEM_NTFF_ORDERED || opt->ntff_binning > AEM_NTFF_OFF ||
                 opt->planes < 0 || opt->planes > 64)) {
         aem::set_error("aem_sim_create_ex: ntff_binning must be an AEM_NTFF_* value and planes 0..64");
         return AEM_ERR_STATE;
@@ -789,6 +792,7 @@ const float* series_src(const aem_sim* s, int32_t kind, int32_t idx) {
         case AEM_SERIES_E_THETA:
         case AEM_SERIES_E_PHI:
             if (idx < 0 || idx >= v.n_angles) { aem::set_error("angle index out of range"); return nullptr; }
+            if (s->ntff_mode == AEM_NTFF_OFF) { aem::set_error("the time-domain far field is off (AEM_NTFF_OFF)"); return nullptr; }
             if (!s->finalized) { aem::set_error("call aem_finalize first"); return nullptr; }
             return (kind == AEM_SERIES_E_THETA ? s->d_Eth : s->d_Eph) + (size_t)idx * T;
         case AEM_SERIES_INCIDENT:
@@ -885,8 +889,8 @@ aem_status aem_get_ntff_raw(aem_sim* s, int32_t which, double* out, int64_t n) {
         return AEM_ERR_STATE;
     }
     const aem_plan_view& v = s->plan->v;
-    if (v.n_angles <= 0 || n > (int64_t)v.n_angles * v.len_ff) {
-        aem::set_error("aem_get_ntff_raw: no far field or length too large");
+    if (v.n_angles <= 0 || s->ntff_mode == AEM_NTFF_OFF || n > (int64_t)v.n_angles * v.len_ff) {
+        aem::set_error("aem_get_ntff_raw: no time-domain far field or length too large");
         return AEM_ERR_STATE;
     }
     return aem::guard([&]() {
@@ -910,8 +914,8 @@ aem_status aem_get_ntff_raw(aem_sim* s, int32_t which, double* out, int64_t n) {
 }
 
 aem_status aem_get_ff_time(aem_sim* s, float* out, int64_t n) {
-    if (!s || !out || n > s->plan->v.n_angles || !s->finalized) {
-        aem::set_error("aem_get_ff_time: bad argument or not finalized");
+    if (!s || !out || n > s->plan->v.n_angles || !s->finalized || s->ntff_mode == AEM_NTFF_OFF) {
+        aem::set_error("aem_get_ff_time: bad argument, not finalized or no time-domain far field");
         return AEM_ERR_STATE;
     }
     return aem::guard([&]() {
@@ -1052,7 +1056,7 @@ struct aem_batch {
             aem_sim* s = m[q];
             const aem_plan_view& v = s->plan->v;
             if (v.n_ports > 0) pa.push_back(s->pa);
-            if (v.n_angles > 0) {
+            if (v.n_angles > 0 && s->ntff_mode != AEM_NTFF_OFF) {
                 if (s->ntff_mode == AEM_NTFF_HISTOGRAM) {
                     na.push_back(s->na);
                     max_angles = std::max(max_angles, v.n_angles);
@@ -1234,37 +1238,21 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
     const aem_plan_view& v = plan->v;
     const IdxT* idx = static_cast<const IdxT*>(d_idx);
     float *lEx = lam[0], *lEy = lam[1], *lEz = lam[2], *lHx = lam[3], *lHy = lam[4], *lHz = lam[5];
-    const dim3 bb(32, 8, 1);
-    const float dd[3] = {v.dx, v.dy, v.dz};
-    const int N[3] = {v.NX, v.NY, v.NZ};
     if (objective) {
         k_adj_source<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6), 256, 0, s>>>(g, obj, Fr, Fi, lEx, lEy, lEz, lHx,
                                                                                   lHy, lHz, d_bstep);
         if (rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
     }
-    auto slab_threads = [&](int a) {
-        const int nt = std::max(v.npml[2 * a] - 1, 0) + std::max(v.npml[2 * a + 1] - 1, 0);
-        const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
-        return (long long)nt * (N[o1] - 1) * (N[o2] - 1);
-    };
-    for (int a = 0; a < 3; ++a) {
-        const long long tot = slab_threads(a);
-        if (tot > 0)
-            k_adj_e_cpml<IdxT><<<cdiv(tot, 256), 256, 0, s>>>(g, cpE, lpE, a, lEx, lEy, lEz, lHx, lHy, lHz, idx,
-                                                              d_tup, dd[a]);
-    }
     if (des_on)
         k_adj_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, lEx, lEy, lEz, lHx, lHy, lHz, d_den[0], d_den[1],
                                                            d_den[2]);
-    k_adj_h_gather<IdxT><<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8), v.NZ - 1), bb, 0, s>>>(
-        g, lHx, lHy, lHz, lEx, lEy, lEz, idx, d_tup, d_den[0], d_den[1], d_den[2]);
-    k_adj_e_gather<IdxT><<<dim3(cdiv(v.NX, 32), cdiv(v.NY, 8), v.NZ), bb, 0, s>>>(
-        g, lEx, lEy, lEz, lHx, lHy, lHz, idx, d_tup, d_den[3], d_den[4], d_den[5], v.DB);
-    for (int a = 0; a < 3; ++a) {
-        const long long tot = slab_threads(a);
-        if (tot > 0)
-            k_adj_h_cpml<<<cdiv(tot, 256), 256, 0, s>>>(g, cpH, lpH, a, lEx, lEy, lEz, lHx, lHy, lHz, v.DB, dd[a]);
-    }
+    const dim3 bt(64, 4, 1);
+    k_adj_h<IdxT><<<dim3(cdiv(v.NX - 1, 64), cdiv(v.NY - 1, 4), v.NZ - 1), bt, 0, s>>>(
+        g, lHx, lHy, lHz, lEx, lEy, lEz, idx, d_tup, d_den[0], d_den[1], d_den[2], cpE, lpE, cpH, lpH, v.DB, v.dx,
+        v.dy, v.dz);
+    k_adj_e<IdxT><<<dim3(cdiv(v.NX, 64), cdiv(v.NY, 4), v.NZ), bt, 0, s>>>(
+        g, lEx, lEy, lEz, lHx, lHy, lHz, idx, d_tup, d_den[3], d_den[4], d_den[5], cpE, lpE, cpH, lpH, v.DB, v.dx,
+        v.dy, v.dz);
     k_step_dec<<<1, 1, 0, s>>>(d_bstep);
 }
 
@@ -1348,22 +1336,50 @@ aem_status aem_objective_set(aem_sim* s, float theta_deg, float phi_deg, double 
     return aem::guard([&]() {
         Objective& o = s->obj;
         for (int a = 0; a < 3; ++a) { o.lo[a] = v.ff_lo[a]; o.hi[a] = v.ff_hi[a]; o.pbc[a] = v.pbc[a]; }
-        o.ic = v.ic; o.jc = v.jc; o.kc = v.kc;
-        o.d[0] = v.dx; o.d[1] = v.dy; o.d[2] = v.dz;
-        o.dt = v.dt;
-        o.omega = 2.0 * M_PI * freq_hz;
-        o.k = o.omega / 299792458.0;
+        const double d[3] = {v.dx, v.dy, v.dz};
+        const double omega = 2.0 * M_PI * freq_hz, k = omega / 299792458.0, dt = v.dt;
         const double th = theta_deg * M_PI / 180.0, ph = phi_deg * M_PI / 180.0;
         const double st = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
-        o.u[0] = st * cp; o.u[1] = st * sp; o.u[2] = ct;
+        const double u[3] = {st * cp, st * sp, ct};
         const double eta = std::sqrt((double)aem::kMu0 / (double)aem::kEp0);
         const double th_v[3] = {ct * cp, ct * sp, -st}, ph_v[3] = {-sp, cp, 0.0};
         for (int a = 0; a < 3; ++a) {
             if (pol == 0) { o.gM[a] = ph_v[a]; o.gJ[a] = eta * th_v[a]; }
             else { o.gM[a] = th_v[a]; o.gJ[a] = -eta * ph_v[a]; }
         }
-        o.pre_re = 0.0;
-        o.pre_im = (pol == 0 ? -1.0 : 1.0) * o.k / (4.0 * M_PI);
+        const double pre_im = (pol == 0 ? -1.0 : 1.0) * k / (4.0 * M_PI);   // pre = j pre_im
+        // face-cell table: pre dS e^{jk r^.r'}, r' = ((i - ic) dx, (j - jc) dy, (k - kc) dz)
+        const int mu = s->ntff_mu, mv = s->ntff_mv;
+        std::vector<double2> gt((size_t)6 * mv * mu, make_double2(0.0, 0.0));
+        for (int face = 0; face < 6; ++face) {
+            const int a = face / 2, b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+            if (v.pbc[a]) continue;
+            const double dS = d[b] * d[c];
+            const int pos = (face & 1) ? o.hi[a] : o.lo[a];
+            for (int vv = 0; vv < mv; ++vv)
+                for (int uu = 0; uu < mu; ++uu) {
+                    const int uc = o.lo[b] + uu, vc = o.lo[c] + vv;
+                    if (uc > o.hi[b] || vc > o.hi[c]) continue;
+                    int ijk[3];
+                    ijk[a] = pos; ijk[b] = uc; ijk[c] = vc;
+                    const double phs = k * (u[0] * (ijk[0] - v.ic) * d[0] + u[1] * (ijk[1] - v.jc) * d[1] +
+                                            u[2] * (ijk[2] - v.kc) * d[2]);
+                    gt[((size_t)face * mv + vv) * mu + uu] =
+                        make_double2(-pre_im * dS * std::sin(phs), pre_im * dS * std::cos(phs));
+                }
+        }
+        const int T = v.time_steps;
+        std::vector<double2> te(T), thh(T);
+        for (int n = 1; n <= T; ++n) {
+            te[n - 1] = make_double2(std::cos(omega * n * dt), -std::sin(omega * n * dt));
+            thh[n - 1] = make_double2(std::cos(omega * (n - 0.5) * dt), -std::sin(omega * (n - 0.5) * dt));
+        }
+        o.mu = mu;
+        o.mv = mv;
+        o.g = s->up(gt.data(), gt.size());
+        o.tE = s->up(te.data(), te.size());
+        o.tH = s->up(thh.data(), thh.size());
+        o.part = s->zeros<double>((size_t)2 * cdiv(mu, 32) * cdiv(mv, 8) * 6);
         if (!o.F) o.F = s->zeros<double>(2);
         s->obj_on = true;
         HIPCHK(hipStreamSynchronize(s->st));
@@ -1659,7 +1675,25 @@ aem_status aem_adjoint_set_field(aem_sim* s, int32_t comp, const float* in, int6
     }
     return aem::guard([&]() {
         s->ensure_adjoint();
+        const aem_plan_view& v = s->plan->v;
+        // the adjoint CPML state carries gb lambda_E of the E psi cells: swap
+        // the old field's share for the new one's
+        auto adjust = [&](float sign) {
+            if (comp >= 3) return;
+            const dim3 gr(cdiv(v.NX, 64), cdiv(v.NY, 4), v.NZ), bt(64, 4, 1);
+            if (v.idx_bytes == 1)
+                k_adj_nu_adjust<uint8_t><<<gr, bt, 0, s->st>>>(s->g, s->lam[comp], comp, sign,
+                                                               static_cast<const uint8_t*>(s->d_idx), s->d_tup, s->cpE,
+                                                               s->lpE);
+            else
+                k_adj_nu_adjust<uint16_t><<<gr, bt, 0, s->st>>>(s->g, s->lam[comp], comp, sign,
+                                                                static_cast<const uint16_t*>(s->d_idx), s->d_tup,
+                                                                s->cpE, s->lpE);
+            HIPCHK(hipGetLastError());
+        };
+        adjust(-1.0f);
         HIPCHK(hipMemcpyAsync(s->lam[comp], in, (size_t)n * 4, hipMemcpyHostToDevice, s->st));
+        adjust(1.0f);
         HIPCHK(hipStreamSynchronize(s->st));
     });
 }

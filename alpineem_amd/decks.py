@@ -256,6 +256,26 @@ def layered_slab(nz: int = 1200, nxy: int = 32, steps: int = 12000, d: float = 0
                 sheets=[Sheet("z", 30, min(z, nz - 40), (1, 1), (nxy, nxy))])
 
 
+def adjoint_deck(n: int = 160, steps: int = 4000, box: Tuple[int, int, int] = (40, 40, 20),
+                 angle: Tuple[float, float] = (30.0, 0.0)) -> Tuple[Deck, Tuple[int, int, int], Tuple[int, int, int]]:
+    """The differentiable workload: vacuum, 10-cell CPML, +z x-polarised TF/SF
+    differentiated Gaussian centred on 10 GHz, one far-field angle, and a
+    design box of ``box`` cells at the centre.  Returns (deck, lo, hi) of the
+    box (1-based, inclusive)."""
+    c = n // 2
+    lo = tuple(c - b // 2 + 1 for b in box)
+    hi = tuple(lo[a] + box[a] - 1 for a in range(3))
+    d = Deck(cells=(n, n, n), cell_size=(1e-3, 1e-3, 1e-3), time_steps=steps, factor=1.0, f_center=10e9,
+             pulse_type=2, excitation="plane wave", amplitude=1.0, angles=(0.0, 0.0, 270.0), phase_centre=(c, c, c),
+             far_field=[angle])
+    return d, lo, hi
+
+
+def design_permittivity(shape: Tuple[int, int, int], seed: int = 5, lo: float = 1.0, hi: float = 4.0) -> np.ndarray:
+    """Per-voxel eps_r uniform in [lo, hi] (PCG64 ``seed``), shape (nz, ny, nx)."""
+    return np.random.default_rng(seed).uniform(lo, hi, shape).astype(np.float32)
+
+
 def population(count: int = 64, seed: int = 3, steps: int = 3000, edges=range(80, 161, 16)) -> List[Deck]:
     """Independent dipole antennas of random size, arm and dielectric blocks."""
     rng = np.random.default_rng(seed)

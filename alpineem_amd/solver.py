@@ -77,7 +77,8 @@ class Simulation:
     """One simulation resident on the current HIP device.
 
     ``ntff`` selects the far-field binning ("ordered": the reference's FP32
-    sums bit for bit; "atomic" or "histogram": FP64 accumulation), ``planes``
+    sums bit for bit; "atomic" or "histogram": FP64 accumulation; "off": no
+    time-domain far field, e.g. for objective-only runs), ``planes``
     the z planes each bulk thread marches, ``fuse_cpml=False`` forces the
     separate CPML kernels (aem_sim_options in include/aem.h)."""
 

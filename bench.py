@@ -28,6 +28,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -123,6 +125,46 @@ def population_bench(steps: int, count: int = 64) -> dict:
     return res
 
 
+def adjoint_bench(steps: int) -> dict:
+    """The differentiable workload (decks.adjoint_deck): 160^3, 10-cell CPML,
+    x-polarised TF/SF differentiated Gaussian, a 40 x 40 x 20 design box of
+    eps_r U[1, 4] (seed 5), objective |r E_theta(30 deg, 0, 10 GHz)|^2 by
+    the frequency-domain transform (the time-domain far-field series are off
+    in both runs).  Times one plain forward run (no objective, no design box), the recorded
+    forward run and the adjoint sweep of the gradient (wall clock, device
+    synchronised; each run includes its one graph capture)."""
+    from alpineem_amd import decks
+    from alpineem_amd.solver import Plan, Simulation
+
+    d, lo, hi = decks.adjoint_deck(steps=steps)
+    plan = Plan.from_text(d.text())
+    shape = tuple(hi[a] - lo[a] + 1 for a in (2, 1, 0))
+    eps = decks.design_permittivity(shape, seed=5)
+    plain = Simulation(plan, ntff="off")
+    t0 = time.perf_counter()
+    plain.run()
+    t_plain = time.perf_counter() - t0
+    plain.close()
+    sim = Simulation(plan, ntff="off")
+    sim.set_objective(30.0, 0.0, 10e9, "theta")
+    sim.design_box(lo, hi)
+    sim.set_design_eps(eps)
+    t0 = time.perf_counter()
+    sim.run()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = sim.design_gradient()
+    t_bwd = time.perf_counter() - t0
+    J = abs(sim.objective()) ** 2
+    sim.close()
+    n = plan.view.nx * plan.view.ny * plan.view.nz
+    return {"workload": f"{plan.view.nx}^3, {steps} steps, design box {shape[2]}x{shape[1]}x{shape[0]}",
+            "forward_s": round(t_plain, 4), "recorded_forward_s": round(t_fwd, 4), "adjoint_s": round(t_bwd, 4),
+            "gradient_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
+            "forward_gcells": round(n * steps / t_plain / 1e9, 2), "J": J,
+            "grad_norm": float(np.linalg.norm(g))}
+
+
 def _short_name(kernel: str) -> str:
     """'void aemk::k_e_bulk<unsigned char, 1>(aemk::Grid, ...)' -> 'k_e_bulk<unsigned char, 1>'."""
     s = kernel.split("(")[0].replace("void ", "").strip()
@@ -179,6 +221,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", action="store_true", help="also run a known-size device copy (counter scaling)")
     ap.add_argument("--population-steps", type=int, default=1000,
                     help="steps of the 64-member batched-population comparison (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="steps of the 160^3 design-gradient timing (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -236,6 +280,12 @@ def main() -> int:
         print(f"[bench] population: 64 members, {pop['cells_per_step'] / 1e6:.1f} Mcells per step, "
               f"sequential {pop['sequential_gcells']} Gcell/s, batched {pop['batched_gcells']} Gcell/s, "
               f"speedup {pop['speedup']}x", file=sys.stderr, flush=True)
+    adj = None
+    if args.adjoint_steps > 0:
+        adj = adjoint_bench(args.adjoint_steps)
+        print(f"[bench] adjoint: {adj['workload']}: forward {adj['forward_s']} s, recorded forward "
+              f"{adj['recorded_forward_s']} s, adjoint {adj['adjoint_s']} s, gradient/forward "
+              f"{adj['gradient_over_forward']}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(n, args.cpu_steps)
     line = {
         "metric": "Gcell-updates/s at 400^3 FP32 (CPML, lumped port)",
@@ -256,6 +306,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "population": pop,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -107,12 +107,15 @@ aem_status aem_create_from_inputs(const char* inputs_path, aem_sim** out);
  *   HISTOGRAM  FP64 accumulation through a per-step LDS histogram of the
  *              bin window (fastest with many angles; required for batched
  *              far-field work).
+ *   OFF        no time-domain far field (its series are unavailable); for
+ *              runs that only need the frequency-domain objective.
  * ATOMIC and HISTOGRAM keep the reference's FP32 index and weight arithmetic
  * and agree with it to FP32 rounding. */
 enum {
     AEM_NTFF_ORDERED = 0,
     AEM_NTFF_ATOMIC = 1,
-    AEM_NTFF_HISTOGRAM = 2
+    AEM_NTFF_HISTOGRAM = 2,
+    AEM_NTFF_OFF = 3
 };
 
 /* Execution options (no counterpart in the reference, whose build flags

@@ -401,3 +401,28 @@ def test_design_gradient_matches_finite_difference():
         g_fd = (J[0] - J[1]) / (e32[0] - e32[1])
         print(f"cell {c}: adjoint {g_adj[c]:.9e} finite difference {g_fd:.9e}")
         assert abs(g_adj[c] - g_fd) <= 1e-3 * abs(g_fd)
+
+
+def test_autograd_wrapper():
+    """torch.autograd: J and dJ/d eps through far_field_power equal the direct
+    API, and a small step along the gradient changes J as the gradient
+    predicts to first order."""
+    torch = pytest.importorskip("torch")
+    from alpineem_amd.autograd import DesignProblem, far_field_power
+
+    plan = Plan.from_text(design_deck().text())
+    prob = DesignProblem(plan, BOX_LO, BOX_HI, THETA, PHI, FREQ)
+    base = np.maximum(design_eps(), 1.1).astype(np.float32)
+    eps = torch.tensor(base, dtype=torch.float64, requires_grad=True)
+    J = far_field_power(eps, prob)
+    J.backward()
+    sim = prob.simulate(base)
+    g = sim.design_gradient()
+    assert J.item() == pytest.approx(abs(sim.objective()) ** 2, rel=1e-12)
+    assert np.allclose(eps.grad.numpy(), g, rtol=1e-6, atol=0)
+    step = (0.01 * g / np.abs(g).max()).astype(np.float32)
+    with torch.no_grad():
+        J2 = far_field_power(torch.tensor(base + step, dtype=torch.float64), prob).item()
+    first = float((g * step.astype(np.float64)).sum())
+    print(f"J={J.item():.9e} J(eps+s)={J2:.9e} predicted change {first:.3e} actual {J2 - J.item():.3e}")
+    assert abs((J2 - J.item()) - first) <= 0.02 * abs(first)
