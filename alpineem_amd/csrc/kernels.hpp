@@ -1491,6 +1491,7 @@ struct Objective {
     const double2* tE;        // [T]: e^{-jw n dt}
     const double2* tH;        // [T]: e^{-jw (n - 1/2) dt}
     double* part;             // [blocks][2]: per-work-group partial sums of a step
+    unsigned* done;           // work-groups finished this step (reset by the last)
     double* F;                // (re, im), device
 };
 
@@ -1502,16 +1503,17 @@ struct FaceTerm {
 };
 
 // The four current terms of a face cell (M slot 0/1, J slot 0/1) exactly as
-// k_ntff_currents forms them; slot s belongs to the component (b, c)[s].
-__device__ __forceinline__ void face_terms(const Grid& g, int a, bool low, FaceTerm t[4]) {
-    const double s = low ? 1.0 : -1.0;
+// k_ntff_currents forms them, for faces normal to axis A; s = +1 on the low
+// face, -1 on the high face.  Slot q belongs to component (b, c)[q % 2].
+template <int A>
+__device__ __forceinline__ void face_terms(const Grid& g, double s, FaceTerm t[4]) {
     const long long X = g.PX, P = g.plane;
-    if (a == 0) {
+    if (A == 0) {
         t[0] = FaceTerm{2, 4, {0, X, 1, 1 + X}, -s / 4};
         t[1] = FaceTerm{1, 4, {0, 1, P, 1 + P}, s / 4};
         t[2] = FaceTerm{5, 2, {0, P, 0, 0}, s / 2};
         t[3] = FaceTerm{4, 2, {0, X, 0, 0}, -s / 2};
-    } else if (a == 1) {
+    } else if (A == 1) {
         t[0] = FaceTerm{2, 4, {0, 1, X, 1 + X}, s / 4};
         t[1] = FaceTerm{0, 4, {0, X, P, X + P}, -s / 4};
         t[2] = FaceTerm{5, 2, {0, P, 0, 0}, -s / 2};
@@ -1524,11 +1526,13 @@ __device__ __forceinline__ void face_terms(const Grid& g, int a, bool low, FaceT
     }
 }
 
-// Complex weight of each face term at step n:  dF/d(term value).
+// Complex weight of each face term at step n:  dF/d(term value), for a
+// face normal to axis A (axes as template parameters keep the kernel
+// argument struct out of scratch).
+template <int A>
 __device__ __forceinline__ void face_weights(const Objective& ob, int face, int uu, int vv, int n, double wr[4],
                                              double wi[4]) {
-    const int a = face / 2;
-    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    constexpr int b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
     const double2 g = ob.g[((long long)face * ob.mv + vv) * ob.mu + uu];
     const double2 e = ob.tE[n - 1], h = ob.tH[n - 1];
     const double eMr = g.x * e.x - g.y * e.y, eMi = g.x * e.y + g.y * e.x;
@@ -1539,46 +1543,69 @@ __device__ __forceinline__ void face_weights(const Objective& ob, int face, int 
     wr[3] = eJr * ob.gJ[c]; wi[3] = eJi * ob.gJ[c];
 }
 
-// Face cell (face, u, v) of the NTFF box -> (i, j, k); false outside.
-__device__ __forceinline__ bool face_cell(const Objective& ob, int face, int uu, int vv, int& i, int& j, int& k) {
-    const int a = face / 2;
-    if (ob.pbc[a]) return false;
-    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+// Face cell (u, v) of the NTFF box face normal to A (high face if `high`)
+// -> cell offset; false outside.
+template <int A>
+__device__ __forceinline__ bool face_cell(const Grid& g, const Objective& ob, bool high, int uu, int vv, long long& o) {
+    constexpr int b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
+    if (ob.pbc[A]) return false;
     const int u = ob.lo[b] + uu, v = ob.lo[c] + vv;
     if (u > ob.hi[b] || v > ob.hi[c]) return false;
-    const int pos = (face & 1) ? ob.hi[a] : ob.lo[a];
-    if (a == 0) { i = pos; j = u; k = v; }
-    else if (a == 1) { i = u; j = pos; k = v; }
-    else { i = u; j = v; k = pos; }
+    const int pos = high ? ob.hi[A] : ob.lo[A];
+    if (A == 0) o = off(g, pos, u, v);
+    else if (A == 1) o = off(g, u, pos, v);
+    else o = off(g, u, v, pos);
     return true;
 }
 
-// Per-work-group partial sums of this step's terms.  grid (x, y, 6 faces), block 256.
+// Field component c (0..5 = Ex..Hz) by selects rather than an indexed array
+// of pointers (which would live in scratch).
+__device__ __forceinline__ const float* pick6(int c, const float* Ex, const float* Ey, const float* Ez,
+                                              const float* Hx, const float* Hy, const float* Hz) {
+    return c == 0 ? Ex : c == 1 ? Ey : c == 2 ? Ez : c == 3 ? Hx : c == 4 ? Hy : Hz;
+}
+__device__ __forceinline__ float* pick6(int c, float* Ex, float* Ey, float* Ez, float* Hx, float* Hy, float* Hz) {
+    return c == 0 ? Ex : c == 1 ? Ey : c == 2 ? Ez : c == 3 ? Hx : c == 4 ? Hy : Hz;
+}
+
+template <int A>
+__device__ __forceinline__ void objective_terms(const Grid& g, const Objective& ob, int face, int uu, int vv,
+                                                const int* step, const float* Ex, const float* Ey, const float* Ez,
+                                                const float* Hx, const float* Hy, const float* Hz, double& re,
+                                                double& im) {
+    long long o;
+    if (!face_cell<A>(g, ob, face & 1, uu, vv, o)) return;
+    double wr[4], wi[4];
+    face_weights<A>(ob, face, uu, vv, *step, wr, wi);
+    FaceTerm t[4];
+    face_terms<A>(g, (face & 1) ? -1.0 : 1.0, t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float* Fq = pick6(t[q].comp, Ex, Ey, Ez, Hx, Hy, Hz) + o;
+        double v = (double)Fq[t[q].offs[0]] + (double)Fq[t[q].offs[1]];
+        if (t[q].n == 4) v = (v + (double)Fq[t[q].offs[2]]) + (double)Fq[t[q].offs[3]];
+        const double val = t[q].w * v;
+        re += wr[q] * val;
+        im += wi[q] * val;
+    }
+}
+
+// Per-work-group partial sums of this step's terms; the last work-group to
+// finish adds them to F in a fixed order.  grid (x, y, 6 faces), block 256.
 __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const float* Ex, const float* Ey,
                                                    const float* Ez, const float* Hx, const float* Hy,
                                                    const float* Hz, const int* step) {
     const int face = blockIdx.z;
     const int tid = threadIdx.x;
-    const float* Fl[6] = {Ex, Ey, Ez, Hx, Hy, Hz};
     double re = 0.0, im = 0.0;
-    int i, j, k;
     const int uu = blockIdx.x * 32 + (tid & 31), vv = blockIdx.y * 8 + (tid >> 5);
-    if (face_cell(ob, face, uu, vv, i, j, k)) {
-        const int n = *step;
-        FaceTerm t[4];
-        face_terms(g, face / 2, (face & 1) == 0, t);
-        double wr[4], wi[4];
-        face_weights(ob, face, uu, vv, n, wr, wi);
-        const long long o = off(g, i, j, k);
-        for (int q = 0; q < 4; ++q) {
-            double s = 0.0;
-            for (int r = 0; r < t[q].n; ++r) s += (double)Fl[t[q].comp][o + t[q].offs[r]];
-            const double val = t[q].w * s;
-            re += wr[q] * val;
-            im += wi[q] * val;
-        }
+    switch (face / 2) {
+        case 0: objective_terms<0>(g, ob, face, uu, vv, step, Ex, Ey, Ez, Hx, Hy, Hz, re, im); break;
+        case 1: objective_terms<1>(g, ob, face, uu, vv, step, Ex, Ey, Ez, Hx, Hy, Hz, re, im); break;
+        default: objective_terms<2>(g, ob, face, uu, vv, step, Ex, Ey, Ez, Hx, Hy, Hz, re, im); break;
     }
     __shared__ double sr[256], si[256];
+    __shared__ bool last;
     sr[tid] = re;
     si[tid] = im;
     __syncthreads();
@@ -1589,22 +1616,23 @@ __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const f
         }
         __syncthreads();
     }
+    const int nblk = gridDim.x * gridDim.y * gridDim.z;
     if (tid == 0) {
         const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
         ob.part[2 * blk] = sr[0];
         ob.part[2 * blk + 1] = si[0];
+        __threadfence();
+        last = atomicAdd(ob.done, 1u) == (unsigned)nblk - 1;
     }
-}
-
-// F += the step's partial sums, in a fixed order.  One work-group of 256.
-__global__ void __launch_bounds__(256) k_objective_sum(Objective ob, int nblk) {
-    const int tid = threadIdx.x;
-    double re = 0.0, im = 0.0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    re = 0.0;
+    im = 0.0;
     for (int q = tid; q < nblk; q += 256) {
         re += ob.part[2 * q];
         im += ob.part[2 * q + 1];
     }
-    __shared__ double sr[256], si[256];
     sr[tid] = re;
     si[tid] = im;
     __syncthreads();
@@ -1618,6 +1646,30 @@ __global__ void __launch_bounds__(256) k_objective_sum(Objective ob, int nblk) {
     if (tid == 0) {
         ob.F[0] = ob.F[0] + sr[0];
         ob.F[1] = ob.F[1] + si[0];
+        *ob.done = 0u;
+    }
+}
+
+template <int A>
+__device__ __forceinline__ void source_terms(const Grid& g, const Objective& ob, int face, int uu, int vv,
+                                             const int* step, double Fr, double Fi, float* lEx, float* lEy,
+                                             float* lEz, float* lHx, float* lHy, float* lHz) {
+    long long o;
+    if (!face_cell<A>(g, ob, face & 1, uu, vv, o)) return;
+    double wr[4], wi[4];
+    face_weights<A>(ob, face, uu, vv, *step, wr, wi);
+    FaceTerm t[4];
+    face_terms<A>(g, (face & 1) ? -1.0 : 1.0, t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float dj = (float)(2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w);
+        float* Lq = pick6(t[q].comp, lEx, lEy, lEz, lHx, lHy, lHz) + o;
+        atomicAdd(&Lq[t[q].offs[0]], dj);
+        atomicAdd(&Lq[t[q].offs[1]], dj);
+        if (t[q].n == 4) {
+            atomicAdd(&Lq[t[q].offs[2]], dj);
+            atomicAdd(&Lq[t[q].offs[3]], dj);
+        }
     }
 }
 
@@ -1627,19 +1679,11 @@ __global__ void __launch_bounds__(256) k_adj_source(Grid g, Objective ob, double
                                                     float* lEy, float* lEz, float* lHx, float* lHy, float* lHz,
                                                     const int* step) {
     const int face = blockIdx.z;
-    int i, j, k;
     const int uu = blockIdx.x * 32 + (threadIdx.x & 31), vv = blockIdx.y * 8 + (threadIdx.x >> 5);
-    if (!face_cell(ob, face, uu, vv, i, j, k)) return;
-    float* L[6] = {lEx, lEy, lEz, lHx, lHy, lHz};
-    const int n = *step;
-    FaceTerm t[4];
-    face_terms(g, face / 2, (face & 1) == 0, t);
-    double wr[4], wi[4];
-    face_weights(ob, face, uu, vv, n, wr, wi);
-    const long long o = off(g, i, j, k);
-    for (int q = 0; q < 4; ++q) {
-        const double dj = 2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w;
-        for (int r = 0; r < t[q].n; ++r) atomicAdd(&L[t[q].comp][o + t[q].offs[r]], (float)dj);
+    switch (face / 2) {
+        case 0: source_terms<0>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz); break;
+        case 1: source_terms<1>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz); break;
+        default: source_terms<2>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz); break;
     }
 }
 
@@ -1940,8 +1984,11 @@ __global__ void k_set_idx(Grid g, DesignDev d, IdxT* idx, IdxT value) {
     idx[off(g, i, j, k)] = value;
 }
 
+// With `record`, also stores the adjoint record of the step (as k_adj_record:
+// E before the update and the curl terms), the box being the record box.
 __global__ void k_e_design(Grid g, DesignDev d, float* Ex, float* Ey, float* Ez, const float* Hx, const float* Hy,
-                           const float* Hz, const float* dex, const float* dey, const float* dez) {
+                           const float* Hz, const float* dex, const float* dey, const float* dez, AdjRec r,
+                           int record, const int* step) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= d.nedge) return;
     const int i = d.lo[0] + (int)(e % d.n[0]);
@@ -1953,9 +2000,22 @@ __global__ void k_e_design(Grid g, DesignDev d, float* Ex, float* Ey, float* Ez,
     const float hz_ym = Hz[o - g.PX], hz_xm = Hz[o - 1], hy_xm = Hy[o - 1], hx_ym = Hx[o - g.PX];
     const float hx_km = Hx[o - g.plane], hy_km = Hy[o - g.plane];
     const float* c = d.coef;
-    Ex[o] = c[e] * Ex[o] + c[n + e] * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
-    Ey[o] = c[2 * n + e] * Ey[o] + c[3 * n + e] * ((hz_xm - hz) * dx_i + (hx - hx_km) * dz_k);
-    Ez[o] = c[4 * n + e] * Ez[o] + c[5 * n + e] * ((hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j);
+    const float ex0 = Ex[o], ey0 = Ey[o], ez0 = Ez[o];
+    const float s0 = (hz - hz_ym) * dy_j + (hy_km - hy) * dz_k;
+    const float s1 = (hz_xm - hz) * dx_i + (hx - hx_km) * dz_k;
+    const float s2 = (hy - hy_xm) * dx_i + (hx_ym - hx) * dy_j;
+    Ex[o] = c[e] * ex0 + c[n + e] * s0;
+    Ey[o] = c[2 * n + e] * ey0 + c[3 * n + e] * s1;
+    Ez[o] = c[4 * n + e] * ez0 + c[5 * n + e] * s2;
+    if (record) {
+        float* h = r.hist + (long long)(*step - 1) * 6 * n;
+        h[e] = ex0;
+        h[n + e] = ey0;
+        h[2 * n + e] = ez0;
+        h[3 * n + e] = s0;
+        h[4 * n + e] = s1;
+        h[5 * n + e] = s2;
+    }
 }
 
 // Transposed k_e_design: lambda_H += S^T (gb lambda_E'), lambda_E <- ga lambda_E'.

@@ -538,7 +538,7 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Hy, Hx);
     }
 
-    if ((parts & kPartRecord) && rec_on)
+    if ((parts & kPartRecord) && rec_on && !des_on)   // a design box records in k_e_design
         k_adj_record<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2],
                                                           d_step);
 
@@ -549,7 +549,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     if (parts & kPartEBulk)
         for (size_t q = 1; q < bulk.size(); ++q) e_bulk(bulk[q]);
     if ((parts & kPartEPre) && des_on)
-        k_e_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2]);
+        k_e_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2],
+                                                         rec, rec_on ? 1 : 0, d_step);
     for (int a = 0; a < 3; ++a)
         if ((parts & kPartEPre) && sh[a].np > 0) {
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
@@ -622,7 +623,6 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     if ((parts & kPartObjective) && obj_on) {
         const dim3 go(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
         k_objective<<<go, 256, 0, s>>>(g, obj, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
-        k_objective_sum<<<1, 256, 0, s>>>(obj, (int)(go.x * go.y * go.z));
     }
     if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
         k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
@@ -1380,6 +1380,7 @@ aem_status aem_objective_set(aem_sim* s, float theta_deg, float phi_deg, double 
         o.tE = s->up(te.data(), te.size());
         o.tH = s->up(thh.data(), thh.size());
         o.part = s->zeros<double>((size_t)2 * cdiv(mu, 32) * cdiv(mv, 8) * 6);
+        o.done = s->zeros<unsigned>(1);
         if (!o.F) o.F = s->zeros<double>(2);
         s->obj_on = true;
         HIPCHK(hipStreamSynchronize(s->st));
