@@ -1165,62 +1165,171 @@ struct NtffExact {
     const int* win;         // per angle: {lowest bin offset relative to n, window length} (host bound)
 };
 
+// Ordered binning, one thread per (bin of the step's window, accumulator,
+// angle), 256 bins per work-group.  Each thread walks the face cells in the
+// reference's loop order -- face pairs x, y, z; rows v (outer) and u (inner);
+// low then high face of each (u, v) -- and adds the contributions that land
+// in its bin, so each FP32 sum is the serial reference's bit for bit.
+// The work-group stages chunks of rows in LDS: every cell's bin and weight
+// (the reference's exact FP32 expressions) and its current difference.  A
+// thread finds a row's candidate cells from an FP64 linear model of the
+// retarded time, widened by a margin that covers the FP32 evaluation error
+// (at most a tenth of it is used on the tested geometries), and checks each
+// candidate's staged bin.
+constexpr int kExactCells = 4096;   // staged cells per chunk (48 KB of LDS); rows of <= 2048 cells
+
 __global__ void __launch_bounds__(256) k_ntff_exact(NtffFaces f, NtffBins bn, NtffExact ex, const int* step) {
+    __shared__ int sii[kExactCells];
+    __shared__ float sw[kExactCells], sd[kExactCells];
+    __shared__ int s_vr[2];
     const int n = *step;
-    const int kk = blockIdx.x;
+    const int kk = blockIdx.z;
     const int arr = blockIdx.y;               // 0..2 U, 3..5 W
     const int comp = arr % 3;                 // x, y, z component of U or W
     const bool isW = arr >= 3;
-    const int slot_in_win = blockIdx.z * blockDim.x + threadIdx.x;
+    const int slot_in_win = blockIdx.x * blockDim.x + threadIdx.x;
     const int bin = n + ex.win[2 * kk] + slot_in_win;
     const bool own = slot_in_win < ex.win[2 * kk + 1] && bin >= 0 && bin < bn.LL;
     float* A = ex.acc[arr] + kk * bn.LL;
     float acc = own ? A[bin] : 0.0f;
     const float* an = bn.ang + 6 * kk;
     const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
-    __shared__ int r_ii[256];
-    __shared__ float r_lo[256], r_hi[256];
+    const double cdt = (double)bn.c_dt;
+    const double tn = (double)n + (isW ? 0.0 : 0.5) - 1.0;
+    const double gg[3] = {(double)bn.dx * s1 * c2, (double)bn.dy * s1 * s2, (double)bn.dz * c1};
     for (int a = 0; a < 3; ++a) {             // face pairs in the reference's order
         if (a == comp || f.pbc[a]) continue;  // faces normal to a feed the other two components
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-        // slot of this component on faces normal to a: (b, c) -> (0, 1)
-        const int slot = comp == b ? 0 : 1;
-        const int nu = f.hi[b] - f.lo[b] + 1, nv = f.hi[c] - f.lo[c] + 1;
-        const long long ncell = (long long)nu * nv * 2;
+        const int slot = comp == b ? 0 : 1;   // slot of this component on faces normal to a
         const float K = bn.K[a];
-        for (long long base = 0; base < ncell; base += blockDim.x) {
-            const long long cid = base + threadIdx.x;
-            if (cid < ncell) {
-                const int side = (int)(cid & 1);
-                const long long uv = cid >> 1;
-                const int u = f.lo[b] + (int)(uv % nu), v = f.lo[c] + (int)(uv / nu);
+        const int nu = f.hi[b] - f.lo[b] + 1, nv = f.hi[c] - f.lo[c] + 1;
+        const int rows = max(1, kExactCells / (2 * nu));
+        // FP64 model of the retarded time on side s: t = base[s] + mu * u + mv * v
+        const double mu = -(b == 0 ? gg[0] : gg[1]) / cdt, mv = -(c == 1 ? gg[1] : gg[2]) / cdt;
+        const double ga = a == 0 ? gg[0] : (a == 1 ? gg[1] : gg[2]);
+        const int ica = a == 0 ? bn.ic : (a == 1 ? bn.jc : bn.kc);
+        const int icb = b == 0 ? bn.ic : bn.jc, icc = c == 1 ? bn.jc : bn.kc;
+        double base[2];
+        for (int side = 0; side < 2; ++side) {
+            const int pos = side ? f.hi[a] : f.lo[a];
+            const double S0 = (double)(pos - ica) * ga - (double)icb * (b == 0 ? gg[0] : gg[1]) -
+                              (double)icc * (c == 1 ? gg[1] : gg[2]);
+            base[side] = tn + ((double)bn.r_relay - S0) / cdt;
+        }
+        const double eps = 4e-3 + 4e-7 * fabs(base[0] + mu * f.lo[b] + mv * f.lo[c]);
+        const double lo_t = (double)bin - 1.0 - eps, hi_t = (double)bin + 1.0 + eps;
+        const double umin = fmin(mu * f.lo[b], mu * f.hi[b]), umax = fmax(mu * f.lo[b], mu * f.hi[b]);
+        // rows (per side) whose t range over u meets [lo_t, hi_t]
+        int sa[2] = {f.hi[c] + 1, f.hi[c] + 1}, sb[2] = {f.lo[c] - 1, f.lo[c] - 1};
+        if (own)
+            for (int side = 0; side < 2; ++side) {
+                if (fabs(mv) < 1e-12) {
+                    const bool in = base[side] + umax >= lo_t && base[side] + umin <= hi_t;
+                    sa[side] = in ? f.lo[c] : f.hi[c] + 1;
+                    sb[side] = in ? f.hi[c] : f.lo[c] - 1;
+                } else {
+                    const double v1 = (lo_t - base[side] - umax) / mv, v2 = (hi_t - base[side] - umin) / mv;
+                    sa[side] = max(f.lo[c], (int)ceil(fmax(fmin(v1, v2), -1e9)));
+                    sb[side] = min(f.hi[c], (int)floor(fmin(fmax(v1, v2), 1e9)));
+                }
+            }
+        const int va = min(sa[0], sa[1]), vb = max(sb[0], sb[1]);
+        // rows the work-group needs
+        if (threadIdx.x == 0) { s_vr[0] = f.hi[c] + 1; s_vr[1] = f.lo[c] - 1; }
+        __syncthreads();
+        if (va <= vb) {
+            atomicMin(&s_vr[0], va);
+            atomicMax(&s_vr[1], vb);
+        }
+        __syncthreads();
+        const int gva = s_vr[0], gvb = s_vr[1];
+        const double inv_mu = fabs(mu) < 1e-12 ? 0.0 : 1.0 / mu;
+        for (int v0 = gva; v0 <= gvb; v0 += rows) {
+            const int nr = min(rows, gvb - v0 + 1);
+            __syncthreads();
+            // stage rows v0 .. v0+nr-1 of both faces: index (r * 2 + side) * nu + (u - lo)
+            for (int q = threadIdx.x; q < nr * 2 * nu; q += blockDim.x) {
+                const int uu = q % nu, rs = q / nu, side = rs & 1, r = rs >> 1;
+                const int u = f.lo[b] + uu, v = v0 + r;
                 const int face = 2 * a + side;
                 const int pos = side ? f.hi[a] : f.lo[a];
                 int i, j, k;
                 if (a == 0) { i = pos; j = u; k = v; }
                 else if (a == 1) { i = u; j = pos; k = v; }
                 else { i = u; j = v; k = pos; }
-                const float tfv = (bn.r_relay - ((float)(i - bn.ic) * bn.dx * s1 * c2 + (float)(j - bn.jc) * bn.dy * s1 * s2 +
+                const float tfv = (bn.r_relay - ((float)(i - bn.ic) * bn.dx * s1 * c2 +
+                                                 (float)(j - bn.jc) * bn.dy * s1 * s2 +
                                                  (float)(k - bn.kc) * bn.dz * c1)) / bn.c_dt;
-                const long long fo = (long long)u + (long long)f.fd0[face] * v;
-                const float d = isW ? (f.J[face][slot][fo] - f.Jo[face][slot][fo])
-                                    : (f.M[face][slot][fo] - f.Mo[face][slot][fo]);
                 const float t = isW ? ((float)n + tfv - 1.0f) : ((float)n + 0.5f + tfv - 1.0f);
                 const int ii = (int)t;
-                const float w = t - (float)ii;
-                r_ii[threadIdx.x] = ii;
-                r_lo[threadIdx.x] = (1.0f - w) * K * d;
-                r_hi[threadIdx.x] = w * K * d;
+                const long long fo = (long long)u + (long long)f.fd0[face] * v;
+                sii[q] = ii;
+                sw[q] = t - (float)ii;
+                sd[q] = isW ? (f.J[face][slot][fo] - f.Jo[face][slot][fo]) : (f.M[face][slot][fo] - f.Mo[face][slot][fo]);
             }
             __syncthreads();
-            const int m = (int)min((long long)blockDim.x, ncell - base);
-            if (own)
-                for (int q = 0; q < m; ++q) {
-                    const int ii = r_ii[q];
-                    if (ii == bin) acc = acc + r_lo[q];
-                    else if (ii + 1 == bin) acc = acc + r_hi[q];
+            // this thread's rows in the chunk: the union of the two sides'
+            // ranges, walked in order (the wave then runs as many row
+            // iterations as its busiest thread, not the union of all rows)
+            const int ra0 = max(sa[0], v0), rb0 = min(sb[0], v0 + nr - 1);
+            const int ra1 = max(sa[1], v0), rb1 = min(sb[1], v0 + nr - 1);
+            const int vend = max(ra0 <= rb0 ? rb0 : v0 - 1, ra1 <= rb1 ? rb1 : v0 - 1);
+            int v = min(ra0 <= rb0 ? ra0 : v0 + nr, ra1 <= rb1 ? ra1 : v0 + nr);
+            for (; v <= vend; ++v) {
+                if (ra0 <= rb0 && ra1 <= rb1) {          // jump the gap between the ranges
+                    if (v > rb0 && v < ra1) v = ra1;
+                    if (v > rb1 && v < ra0) v = ra0;
                 }
-            __syncthreads();
+                const int r = v - v0;
+                int ulo[2], uhi[2];
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const double bv = base[side] + mv * v;
+                    const bool row_in = side ? (v >= ra1 && v <= rb1) : (v >= ra0 && v <= rb0);
+                    if (!row_in) {
+                        ulo[side] = f.hi[b] + 1;
+                        uhi[side] = f.lo[b] - 1;
+                    } else if (inv_mu == 0.0) {
+                        const bool in = bv >= lo_t && bv <= hi_t;
+                        ulo[side] = in ? f.lo[b] : f.hi[b] + 1;
+                        uhi[side] = in ? f.hi[b] : f.lo[b] - 1;
+                    } else {
+                        const double u1 = (lo_t - bv) * inv_mu, u2 = (hi_t - bv) * inv_mu;
+                        ulo[side] = max(f.lo[b], (int)ceil(fmax(fmin(u1, u2), -1e9)));
+                        uhi[side] = min(f.hi[b], (int)floor(fmin(fmax(u1, u2), 1e9)));
+                    }
+                }
+                // Branch-free over the merged candidate range (the LDS loads of
+                // successive cells can then issue together); a cell outside
+                // its side's range or bin leaves acc unchanged through a select.
+                const int row0 = r * 2 * nu - f.lo[b];
+                const bool e0 = ulo[0] <= uhi[0], e1 = ulo[1] <= uhi[1];
+                if (e0 && e1) {
+                    const int ua = min(ulo[0], ulo[1]), ub = max(uhi[0], uhi[1]);
+#pragma unroll 4
+                    for (int u = ua; u <= ub; ++u) {
+#pragma unroll
+                        for (int side = 0; side < 2; ++side) {
+                            const int q = row0 + side * nu + u;
+                            const int ii = sii[q];
+                            const float w = sw[q], d = sd[q];
+                            const bool in = u >= ulo[side] && u <= uhi[side];
+                            const float a_lo = acc + (1.0f - w) * K * d, a_hi = acc + w * K * d;
+                            acc = (in && ii == bin) ? a_lo : ((in && ii + 1 == bin) ? a_hi : acc);
+                        }
+                    }
+                } else if (e0 || e1) {                   // one face only: its own run
+                    const int side = e0 ? 0 : 1;
+                    const int q0 = row0 + side * nu;
+#pragma unroll 4
+                    for (int u = ulo[side]; u <= uhi[side]; ++u) {
+                        const int ii = sii[q0 + u];
+                        const float w = sw[q0 + u], d = sd[q0 + u];
+                        const float a_lo = acc + (1.0f - w) * K * d, a_hi = acc + w * K * d;
+                        acc = ii == bin ? a_lo : (ii + 1 == bin ? a_hi : acc);
+                    }
+                }
+            }
         }
     }
     if (own) A[bin] = acc;

@@ -450,6 +450,13 @@ void aem_sim::create(const aem_sim_options* opt) {
         // the histogram keeps one window of bins in LDS; larger windows bin
         // through global atomics
         if (ntff_mode == AEM_NTFF_HISTOGRAM && win_max > kHistMax) ntff_mode = AEM_NTFF_ATOMIC;
+        if (ntff_mode == AEM_NTFF_ORDERED)
+            for (int a = 0; a < 2; ++a)
+                if (!v.pbc[a] && 2 * (v.ff_hi[a] - v.ff_lo[a] + 1) > kExactCells)
+                    throw aem::Error(AEM_ERR_UNSUPPORTED,
+                                     "ordered far-field binning stages rows of at most " +
+                                         std::to_string(kExactCells / 2) +
+                                         " cells; use AEM_NTFF_ATOMIC or AEM_NTFF_HISTOGRAM for this grid");
         d_Eth = zeros<float>((size_t)v.n_angles * T);
         d_Eph = zeros<float>((size_t)v.n_angles * T);
         d_fftime = zeros<float>(v.n_angles);
@@ -613,7 +620,7 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         k_ntff_currents<<<gf, bb, 0, s>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
         if (ntff_mode == AEM_NTFF_ORDERED) {
             if (win_max > 0)
-                k_ntff_exact<<<dim3(v.n_angles, 6, cdiv(win_max, 256)), 256, 0, s>>>(nf, nb, nx_acc, d_step);
+                k_ntff_exact<<<dim3(cdiv(win_max, 256), 6, v.n_angles), 256, 0, s>>>(nf, nb, nx_acc, d_step);
         } else if (ntff_mode == AEM_NTFF_HISTOGRAM) {
             k_ntff_hist<<<dim3(v.n_angles, 6), 256, 0, s>>>(nf, nb, nx_acc.win, d_step);
         } else {
