@@ -1759,10 +1759,11 @@ __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const f
     }
 }
 
-template <int A>
+template <int A, typename IdxT>
 __device__ __forceinline__ void source_terms(const Grid& g, const Objective& ob, int face, int uu, int vv,
                                              const int* step, double Fr, double Fi, float* lEx, float* lEy,
-                                             float* lEz, float* lHx, float* lHy, float* lHz) {
+                                             float* lEz, float* lHx, float* lHy, float* lHz, float* mEx, float* mEy,
+                                             float* mEz, const IdxT* idx, const float* tup) {
     long long o;
     if (!face_cell<A>(g, ob, face & 1, uu, vv, o)) return;
     double wr[4], wi[4];
@@ -1773,26 +1774,32 @@ __device__ __forceinline__ void source_terms(const Grid& g, const Objective& ob,
     for (int q = 0; q < 4; ++q) {
         const float dj = (float)(2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w);
         float* Lq = pick6(t[q].comp, lEx, lEy, lEz, lHx, lHy, lHz) + o;
-        atomicAdd(&Lq[t[q].offs[0]], dj);
-        atomicAdd(&Lq[t[q].offs[1]], dj);
-        if (t[q].n == 4) {
-            atomicAdd(&Lq[t[q].offs[2]], dj);
-            atomicAdd(&Lq[t[q].offs[3]], dj);
+        const int nq = t[q].n;
+        for (int r = 0; r < 4; ++r) {
+            if (r >= nq) break;
+            atomicAdd(&Lq[t[q].offs[r]], dj);
+            if (t[q].comp < 3) {   // keep mu = gb lambda_E of the E components
+                const long long oc = o + t[q].offs[r];
+                float* Mq = pick6(t[q].comp, mEx, mEy, mEz, mEx, mEy, mEz);
+                atomicAdd(&Mq[oc], tup[(int)idx[oc] * 6 + 2 * t[q].comp + 1] * dj);
+            }
         }
     }
 }
 
 // Adjoint source of J = |F|^2 at step n: dJ/d(term) = 2 Re(conj(F) w),
 // scattered onto the adjoint fields through each term's average.
+template <typename IdxT>
 __global__ void __launch_bounds__(256) k_adj_source(Grid g, Objective ob, double Fr, double Fi, float* lEx,
                                                     float* lEy, float* lEz, float* lHx, float* lHy, float* lHz,
-                                                    const int* step) {
+                                                    float* mEx, float* mEy, float* mEz, const IdxT* idx,
+                                                    const float* tup, const int* step) {
     const int face = blockIdx.z;
     const int uu = blockIdx.x * 32 + (threadIdx.x & 31), vv = blockIdx.y * 8 + (threadIdx.x >> 5);
     switch (face / 2) {
-        case 0: source_terms<0>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz); break;
-        case 1: source_terms<1>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz); break;
-        default: source_terms<2>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz); break;
+        case 0: source_terms<0>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, tup); break;
+        case 1: source_terms<1>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, tup); break;
+        default: source_terms<2>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, tup); break;
     }
 }
 
@@ -1937,30 +1944,31 @@ __device__ __forceinline__ bool near_slab(const Grid& g, const Cpml& p, int i, i
            k <= p.n[4] + 1 || k >= g.NZ - p.n[5] - 1;
 }
 
+// The adjoint state keeps mu_c = gb_c lambda_E_c beside lambda_E (0 on the
+// fixed outer tangential components): k_adj_e writes it with the new
+// lambda_E, k_adj_source adds gb times its own increments, and k_adj_mu
+// rebuilds it after a host write.  The H gather then needs no tuple lookups.
+
 // Transposed E phase, gathered per H cell (one thread per cell of
 // [1, N-1]^3, block 64 x 4):
-//   lambda_H += S^T (gb lambda_E')                     (bulk E update)
+//   lambda_H += S^T mu                                 (bulk E update)
 //             + D^T (c nu_E) / d                        (E-phase CPML)
 // then nu_H <- b nu_H + DB lambda_H on the H-phase psi cells.
-template <typename IdxT>
-__global__ void __launch_bounds__(256) k_adj_h(Grid g, float* lHx, float* lHy, float* lHz, const float* lEx,
-                                               const float* lEy, const float* lEz, const IdxT* idx, const float* tup,
-                                               const float* dex, const float* dey, const float* dez, Cpml pe, Cpml lpe,
-                                               Cpml ph, Cpml lph, float DB, float dx, float dy, float dz) {
+__global__ void __launch_bounds__(256) k_adj_h(Grid g, float* lHx, float* lHy, float* lHz, const float* mEx,
+                                               const float* mEy, const float* mEz, const float* dex,
+                                               const float* dey, const float* dez, Cpml pe, Cpml lpe, Cpml ph,
+                                               Cpml lph, float DB, float dx, float dy, float dz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.NX - 1 || j > g.NY - 1) return;
-    const long long o = off(g, i, j, k);
-    auto de = [](const float* d, int n, int q) { return q <= n - 1 ? d[q - 1] : 0.0f; };
-    const float mx = adj_mu(g, lEx, idx, tup, 0, i, j, k), mx_j = adj_mu(g, lEx, idx, tup, 0, i, j + 1, k),
-                mx_k = adj_mu(g, lEx, idx, tup, 0, i, j, k + 1);
-    const float my = adj_mu(g, lEy, idx, tup, 1, i, j, k), my_i = adj_mu(g, lEy, idx, tup, 1, i + 1, j, k),
-                my_k = adj_mu(g, lEy, idx, tup, 1, i, j, k + 1);
-    const float mz = adj_mu(g, lEz, idx, tup, 2, i, j, k), mz_i = adj_mu(g, lEz, idx, tup, 2, i + 1, j, k),
-                mz_j = adj_mu(g, lEz, idx, tup, 2, i, j + 1, k);
+    const long long o = off(g, i, j, k), X = g.PX, P = g.plane;
+    const float mx = mEx[o], mx_j = mEx[o + X], mx_k = mEx[o + P];
+    const float my = mEy[o], my_i = mEy[o + 1], my_k = mEy[o + P];
+    const float mz = mEz[o], mz_i = mEz[o + 1], mz_j = mEz[o + X];
     const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
-    const float dx_i1 = de(dex, g.NX, i + 1), dy_j1 = de(dey, g.NY, j + 1), dz_k1 = de(dez, g.NZ, k + 1);
+    const float dx_i1 = i + 1 <= g.NX - 1 ? dex[i] : 0.0f, dy_j1 = j + 1 <= g.NY - 1 ? dey[j] : 0.0f,
+                dz_k1 = k + 1 <= g.NZ - 1 ? dez[k] : 0.0f;
     float hx = lHx[o] + (my * dz_k - my_k * dz_k1 + mz_j * dy_j1 - mz * dy_j);
     float hy = lHy[o] + (mx_k * dz_k1 - mx * dz_k + mz * dx_i - mz_i * dx_i1);
     float hz = lHz[o] + (mx * dy_j - mx_j * dy_j1 + my_i * dx_i1 - my * dx_i);
@@ -1971,15 +1979,15 @@ __global__ void __launch_bounds__(256) k_adj_h(Grid g, float* lHx, float* lHy, f
               (cnu_e(g, pe, lpe, 2, 0, i, j, k + 1) - cnu_e(g, pe, lpe, 2, 0, i, j, k)) / dz;
         hz += (cnu_e(g, pe, lpe, 0, 0, i + 1, j, k) - cnu_e(g, pe, lpe, 0, 0, i, j, k)) / dx +
               (cnu_e(g, pe, lpe, 1, 0, i, j, k) - cnu_e(g, pe, lpe, 1, 0, i, j + 1, k)) / dy;
-        const float lh[3] = {hx, hy, hz};
         for (int A = 0; A < 3; ++A) {
             int f, q;
             long long po;
             if (!h_psi(g, ph, A, i, j, k, f, q, po)) continue;
             const float b = ph.b[f][q - 1];
             for (int sl = 0; sl < 2; ++sl) {
+                const int c = h_slot_comp(A, sl);
                 float* nu = lph.psi[f][sl];
-                nu[po] = b * nu[po] + DB * lh[h_slot_comp(A, sl)];
+                nu[po] = b * nu[po] + DB * (c == 0 ? hx : (c == 1 ? hy : hz));
             }
         }
     }
@@ -1991,50 +1999,60 @@ __global__ void __launch_bounds__(256) k_adj_h(Grid g, float* lHx, float* lHy, f
 // Transposed H phase, gathered per E cell (one thread per cell of
 // [1, NX] x [1, NY] x [1, NZ], block 64 x 4) on the updated components:
 //   lambda_E <- ga lambda_E + DB C^T lambda_H + D^T (c nu_H) / d,
-// 0 on the fixed outer tangential components; then
-// nu_E <- b nu_E + gb lambda_E on the E-phase psi cells.
+// 0 on the fixed outer tangential components, and mu = gb lambda_E; then
+// nu_E <- b nu_E + mu on the E-phase psi cells.  Every lambda_H an updated
+// component reads lies in [1, N-1]^3; the i-1 / j-1 / k-1 offsets are
+// clamped so the other cells' loads stay in bounds.
 template <typename IdxT>
-__global__ void __launch_bounds__(256) k_adj_e(Grid g, float* lEx, float* lEy, float* lEz, const float* lHx,
-                                               const float* lHy, const float* lHz, const IdxT* idx, const float* tup,
-                                               const float* dhx, const float* dhy, const float* dhz, Cpml pe, Cpml lpe,
-                                               Cpml ph, Cpml lph, float DB, float dx, float dy, float dz) {
+__global__ void __launch_bounds__(256) k_adj_e(Grid g, float* lEx, float* lEy, float* lEz, float* mEx, float* mEy,
+                                               float* mEz, const float* lHx, const float* lHy, const float* lHz,
+                                               const IdxT* idx, const float* tup, const float* dhx, const float* dhy,
+                                               const float* dhz, Cpml pe, Cpml lpe, Cpml ph, Cpml lph, float DB,
+                                               float dx, float dy, float dz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.NX || j > g.NY) return;
     const long long o = off(g, i, j, k);
-    auto lh = [&](const float* L, int a, int b, int c) { return h_in(g, a, b, c) ? L[off(g, a, b, c)] : 0.0f; };
-    auto dh = [](const float* d, int n, int q) { return (q >= 1 && q <= n - 1) ? d[q - 1] : 0.0f; };
-    const bool ux = e_upd(g, 0, i, j, k), uy = e_upd(g, 1, i, j, k), uz = e_upd(g, 2, i, j, k);
-    const int t = (ux || uy || uz) ? (int)idx[o] * 6 : 0;
+    const bool in = i <= g.NX - 1 && j <= g.NY - 1 && k <= g.NZ - 1;
+    const bool ux = in && j >= 2 && k >= 2, uy = in && i >= 2 && k >= 2, uz = in && i >= 2 && j >= 2;
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+    int t = 0;
+    if (ux || uy || uz) {
+        const long long xm = i >= 2 ? 1 : 0, ym = j >= 2 ? g.PX : 0, zm = k >= 2 ? g.plane : 0;
+        const float hx = lHx[o], hy = lHy[o], hz = lHz[o];
+        const float hx_ym = lHx[o - ym], hx_km = lHx[o - zm], hy_xm = lHy[o - xm], hy_km = lHy[o - zm];
+        const float hz_xm = lHz[o - xm], hz_ym = lHz[o - ym];
+        const float dh_x = in ? dhx[i - 1] : 0.0f, dh_y = in ? dhy[j - 1] : 0.0f, dh_z = in ? dhz[k - 1] : 0.0f;
+        const float dh_xm = i >= 2 ? dhx[i - 2] : 0.0f, dh_ym = j >= 2 ? dhy[j - 2] : 0.0f,
+                    dh_zm = k >= 2 ? dhz[k - 2] : 0.0f;
+        t = (int)idx[o] * 6;
+        if (ux) ex = tup[t] * lEx[o] + DB * (hy * dh_z - hy_km * dh_zm + hz_ym * dh_ym - hz * dh_y);
+        if (uy) ey = tup[t + 2] * lEy[o] + DB * (hx_km * dh_zm - hx * dh_z + hz * dh_x - hz_xm * dh_xm);
+        if (uz) ez = tup[t + 4] * lEz[o] + DB * (hx * dh_y - hx_ym * dh_ym + hy_xm * dh_xm - hy * dh_x);
+    }
     const bool ns = near_slab(g, pe, i, j, k);
-    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    if (ux) {
-        ex = tup[t] * lEx[o] + DB * (lh(lHy, i, j, k) * dh(dhz, g.NZ, k) - lh(lHy, i, j, k - 1) * dh(dhz, g.NZ, k - 1) +
-                                     lh(lHz, i, j - 1, k) * dh(dhy, g.NY, j - 1) - lh(lHz, i, j, k) * dh(dhy, g.NY, j));
-        if (ns)
+    if (ns) {
+        if (ux)
             ex += (cnu_h(g, ph, lph, 2, 1, i, j, k) - cnu_h(g, ph, lph, 2, 1, i, j, k - 1)) / dz +
                   (cnu_h(g, ph, lph, 1, 1, i, j - 1, k) - cnu_h(g, ph, lph, 1, 1, i, j, k)) / dy;
-    }
-    if (uy) {
-        ey = tup[t + 2] * lEy[o] + DB * (lh(lHx, i, j, k - 1) * dh(dhz, g.NZ, k - 1) - lh(lHx, i, j, k) * dh(dhz, g.NZ, k) +
-                                         lh(lHz, i, j, k) * dh(dhx, g.NX, i) - lh(lHz, i - 1, j, k) * dh(dhx, g.NX, i - 1));
-        if (ns)
+        if (uy)
             ey += (cnu_h(g, ph, lph, 2, 0, i, j, k - 1) - cnu_h(g, ph, lph, 2, 0, i, j, k)) / dz +
                   (cnu_h(g, ph, lph, 0, 1, i, j, k) - cnu_h(g, ph, lph, 0, 1, i - 1, j, k)) / dx;
-    }
-    if (uz) {
-        ez = tup[t + 4] * lEz[o] + DB * (lh(lHx, i, j, k) * dh(dhy, g.NY, j) - lh(lHx, i, j - 1, k) * dh(dhy, g.NY, j - 1) +
-                                         lh(lHy, i - 1, j, k) * dh(dhx, g.NX, i - 1) - lh(lHy, i, j, k) * dh(dhx, g.NX, i));
-        if (ns)
+        if (uz)
             ez += (cnu_h(g, ph, lph, 0, 0, i - 1, j, k) - cnu_h(g, ph, lph, 0, 0, i, j, k)) / dx +
                   (cnu_h(g, ph, lph, 1, 0, i, j, k) - cnu_h(g, ph, lph, 1, 0, i, j - 1, k)) / dy;
     }
+    if (ux) mx = tup[t + 1] * ex;
+    if (uy) my = tup[t + 3] * ey;
+    if (uz) mz = tup[t + 5] * ez;
     lEx[o] = ex;
     lEy[o] = ey;
     lEz[o] = ez;
+    mEx[o] = mx;
+    mEy[o] = my;
+    mEz[o] = mz;
     if (ns) {
-        const float le[3] = {ex, ey, ez};
         for (int A = 0; A < 3; ++A) {
             int f, q;
             long long po;
@@ -2044,10 +2062,22 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid g, float* lEx, float* lEy, f
                 if (!e_slot_ok(A, sl, i, j, k)) continue;
                 const int c = e_slot_comp(A, sl);
                 float* nu = lpe.psi[f][sl];
-                nu[po] = b * nu[po] + tup[t + 2 * c + 1] * le[c];
+                nu[po] = b * nu[po] + (c == 0 ? mx : (c == 1 ? my : mz));
             }
         }
     }
+}
+
+// mu_c = gb_c lambda_E_c of one component over the grid (0 where the
+// component is fixed); after a host write of lambda_E.
+template <typename IdxT>
+__global__ void k_adj_mu(Grid g, const float* lE, float* m, int c, const IdxT* idx, const float* tup) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.NX || j > g.NY) return;
+    const long long o = off(g, i, j, k);
+    m[o] = e_upd(g, c, i, j, k) ? tup[(int)idx[o] * 6 + 2 * c + 1] * lE[o] : 0.0f;
 }
 
 // nu_E += sign * gb lambda_E of component `comp` on the E psi cells: keeps

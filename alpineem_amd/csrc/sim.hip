@@ -229,7 +229,7 @@ struct aem_sim {
     std::vector<float> des_eps;    // [3][cells of the box], x fastest
     float* d_des_coef = nullptr;
     void upload_design();
-    float* lam[6] = {};            // adjoint fields (allocated at the first backward step)
+    float* lam[9] = {};            // adjoint fields and mu = gb lambda_E (allocated at the first backward step)
     Cpml lpE{}, lpH{};             // adjoint CPML states
     int* d_bstep = nullptr;
     void ensure_adjoint();
@@ -1218,7 +1218,7 @@ void aem_batch_destroy(aem_batch* b) { delete b; }
 void aem_sim::ensure_adjoint() {
     if (lam[0]) return;
     const aem_plan_view& v = plan->v;
-    for (int q = 0; q < 6; ++q) lam[q] = zeros<float>((size_t)v.ncell + v.plane);
+    for (int q = 0; q < 9; ++q) lam[q] = zeros<float>((size_t)v.ncell + v.plane);
     lpE = cpE;
     lpH = cpH;
     for (int f = 0; f < 6; ++f) {
@@ -1245,21 +1245,21 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
     const aem_plan_view& v = plan->v;
     const IdxT* idx = static_cast<const IdxT*>(d_idx);
     float *lEx = lam[0], *lEy = lam[1], *lEz = lam[2], *lHx = lam[3], *lHy = lam[4], *lHz = lam[5];
+    float *mEx = lam[6], *mEy = lam[7], *mEz = lam[8];
     if (objective) {
-        k_adj_source<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6), 256, 0, s>>>(g, obj, Fr, Fi, lEx, lEy, lEz, lHx,
-                                                                                  lHy, lHz, d_bstep);
+        k_adj_source<IdxT><<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6), 256, 0, s>>>(
+            g, obj, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, d_tup, d_bstep);
         if (rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
     }
     if (des_on)
         k_adj_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, lEx, lEy, lEz, lHx, lHy, lHz, d_den[0], d_den[1],
                                                            d_den[2]);
     const dim3 bt(64, 4, 1);
-    k_adj_h<IdxT><<<dim3(cdiv(v.NX - 1, 64), cdiv(v.NY - 1, 4), v.NZ - 1), bt, 0, s>>>(
-        g, lHx, lHy, lHz, lEx, lEy, lEz, idx, d_tup, d_den[0], d_den[1], d_den[2], cpE, lpE, cpH, lpH, v.DB, v.dx,
-        v.dy, v.dz);
+    k_adj_h<<<dim3(cdiv(v.NX - 1, 64), cdiv(v.NY - 1, 4), v.NZ - 1), bt, 0, s>>>(
+        g, lHx, lHy, lHz, mEx, mEy, mEz, d_den[0], d_den[1], d_den[2], cpE, lpE, cpH, lpH, v.DB, v.dx, v.dy, v.dz);
     k_adj_e<IdxT><<<dim3(cdiv(v.NX, 64), cdiv(v.NY, 4), v.NZ), bt, 0, s>>>(
-        g, lEx, lEy, lEz, lHx, lHy, lHz, idx, d_tup, d_den[3], d_den[4], d_den[5], cpE, lpE, cpH, lpH, v.DB, v.dx,
-        v.dy, v.dz);
+        g, lEx, lEy, lEz, mEx, mEy, mEz, lHx, lHy, lHz, idx, d_tup, d_den[3], d_den[4], d_den[5], cpE, lpE, cpH, lpH,
+        v.DB, v.dx, v.dy, v.dz);
     k_step_dec<<<1, 1, 0, s>>>(d_bstep);
 }
 
@@ -1545,7 +1545,7 @@ void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb
     double* dgb = dalloc<double>(3 * ne, tmp);
     HIPCHK(hipMemsetAsync(dga, 0, 3 * ne * sizeof(double), s->st));
     HIPCHK(hipMemsetAsync(dgb, 0, 3 * ne * sizeof(double), s->st));
-    for (int q = 0; q < 6; ++q) HIPCHK(hipMemsetAsync(s->lam[q], 0, ((size_t)v.ncell + v.plane) * 4, s->st));
+    for (int q = 0; q < 9; ++q) HIPCHK(hipMemsetAsync(s->lam[q], 0, ((size_t)v.ncell + v.plane) * 4, s->st));
     for (int f = 0; f < 6; ++f) {
         const int a = f / 2;
         const size_t nt = (size_t)std::max(v.npml[f] - 1, 0);
@@ -1702,6 +1702,16 @@ aem_status aem_adjoint_set_field(aem_sim* s, int32_t comp, const float* in, int6
         adjust(-1.0f);
         HIPCHK(hipMemcpyAsync(s->lam[comp], in, (size_t)n * 4, hipMemcpyHostToDevice, s->st));
         adjust(1.0f);
+        if (comp < 3) {   // rebuild mu of this component
+            const dim3 gr(cdiv(v.NX, 64), cdiv(v.NY, 4), v.NZ), bt(64, 4, 1);
+            if (v.idx_bytes == 1)
+                k_adj_mu<uint8_t><<<gr, bt, 0, s->st>>>(s->g, s->lam[comp], s->lam[6 + comp], comp,
+                                                        static_cast<const uint8_t*>(s->d_idx), s->d_tup);
+            else
+                k_adj_mu<uint16_t><<<gr, bt, 0, s->st>>>(s->g, s->lam[comp], s->lam[6 + comp], comp,
+                                                         static_cast<const uint16_t*>(s->d_idx), s->d_tup);
+            HIPCHK(hipGetLastError());
+        }
         HIPCHK(hipStreamSynchronize(s->st));
     });
 }
