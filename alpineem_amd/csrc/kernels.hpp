@@ -1865,15 +1865,6 @@ __device__ __forceinline__ bool h_in(const Grid& g, int i, int j, int k) {
     return i >= 1 && j >= 1 && k >= 1 && i <= g.NX - 1 && j <= g.NY - 1 && k <= g.NZ - 1;
 }
 
-// mu_c = gb_c * lambda_E_c on updated components, 0 elsewhere.
-template <typename IdxT>
-__device__ __forceinline__ float adj_mu(const Grid& g, const float* lE, const IdxT* idx, const float* tup, int c,
-                                        int i, int j, int k) {
-    if (!e_upd(g, c, i, j, k)) return 0.0f;
-    const long long o = off(g, i, j, k);
-    return tup[(int)idx[o] * 6 + 2 * c + 1] * lE[o];
-}
-
 // CPML psi cell of the E (slab_e) or H (slab_h) update normal to axis A at
 // (i,j,k): face f, local layer q and psi offset po (as in k_e_cpml_* /
 // k_h_cpml_*); false outside the slab or its transverse range.
@@ -1938,6 +1929,63 @@ __device__ __forceinline__ float cnu_h(const Grid& g, const Cpml& p, const Cpml&
     return p.c[f][q - 1] * lp.psi[f][slot][po];
 }
 
+// c * nu of both slots of the E psi normal to axis A at the cell and at its
+// +1 neighbour along A (cn[slot][0: cell, 1: neighbour]); 0 where absent.
+// The transverse coordinates are those of an H cell, inside the psi range.
+__device__ __forceinline__ void cnu_e_pair(const Grid& g, const Cpml& p, const Cpml& lp, int A, int i, int j, int k,
+                                           float cn[2][2]) {
+    const int N[3] = {g.NX, g.NY, g.NZ};
+    const int x[3] = {i, j, k};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        int q, t;
+        const int side = slab_e(x[A] + h, N[A], p.n[2 * A], p.n[2 * A + 1], q, t);
+        if (side < 0) {
+            cn[0][h] = cn[1][h] = 0.0f;
+            continue;
+        }
+        const int f = 2 * A + side;
+        const long long ntf = p.n[f] - 1;
+        long long po;
+        if (A == 0) po = t + ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+        else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * (t + ntf * (k - 1));
+        else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * t);
+        const float cc = p.c[f][q - 1];
+        int xx[3] = {i, j, k};
+        xx[A] += h;
+        cn[0][h] = e_slot_ok(A, 0, xx[0], xx[1], xx[2]) ? cc * lp.psi[f][0][po] : 0.0f;
+        cn[1][h] = e_slot_ok(A, 1, xx[0], xx[1], xx[2]) ? cc * lp.psi[f][1][po] : 0.0f;
+    }
+}
+
+// c * nu of both slots of the H psi normal to axis A at the cell and at its
+// -1 neighbour along A (cn[slot][0: cell, 1: neighbour]); 0 where absent.
+// The transverse coordinates are those of an updated E cell.
+__device__ __forceinline__ void cnu_h_pair(const Grid& g, const Cpml& p, const Cpml& lp, int A, int i, int j, int k,
+                                           float cn[2][2]) {
+    const int N[3] = {g.NX, g.NY, g.NZ};
+    const int x[3] = {i, j, k};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int xa = x[A] - h;
+        int q;
+        const int side = xa >= 1 ? slab_h(xa, N[A], p.n[2 * A], p.n[2 * A + 1], q) : -1;
+        if (side < 0) {
+            cn[0][h] = cn[1][h] = 0.0f;
+            continue;
+        }
+        const int f = 2 * A + side;
+        const long long ntf = p.n[f] - 1;
+        long long po;
+        if (A == 0) po = (q - 1) + ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
+        else if (A == 1) po = (long long)(i - 1) + (long long)g.PX * ((q - 1) + ntf * (k - 1));
+        else po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (q - 1));
+        const float cc = p.c[f][q - 1];
+        cn[0][h] = cc * lp.psi[f][0][po];
+        cn[1][h] = cc * lp.psi[f][1][po];
+    }
+}
+
 // Within one cell of a CPML slab (either field's layers)?
 __device__ __forceinline__ bool near_slab(const Grid& g, const Cpml& p, int i, int j, int k) {
     return i <= p.n[0] + 1 || i >= g.NX - p.n[1] - 1 || j <= p.n[2] + 1 || j >= g.NY - p.n[3] - 1 ||
@@ -1973,17 +2021,20 @@ __global__ void __launch_bounds__(256) k_adj_h(Grid g, float* lHx, float* lHy, f
     float hy = lHy[o] + (mx_k * dz_k1 - mx * dz_k + mz * dx_i - mz_i * dx_i1);
     float hz = lHz[o] + (mx * dy_j - mx_j * dy_j1 + my_i * dx_i1 - my * dx_i);
     if (near_slab(g, pe, i, j, k)) {
-        hx += (cnu_e(g, pe, lpe, 1, 1, i, j + 1, k) - cnu_e(g, pe, lpe, 1, 1, i, j, k)) / dy +
-              (cnu_e(g, pe, lpe, 2, 1, i, j, k) - cnu_e(g, pe, lpe, 2, 1, i, j, k + 1)) / dz;
-        hy += (cnu_e(g, pe, lpe, 0, 1, i, j, k) - cnu_e(g, pe, lpe, 0, 1, i + 1, j, k)) / dx +
-              (cnu_e(g, pe, lpe, 2, 0, i, j, k + 1) - cnu_e(g, pe, lpe, 2, 0, i, j, k)) / dz;
-        hz += (cnu_e(g, pe, lpe, 0, 0, i + 1, j, k) - cnu_e(g, pe, lpe, 0, 0, i, j, k)) / dx +
-              (cnu_e(g, pe, lpe, 1, 0, i, j, k) - cnu_e(g, pe, lpe, 1, 0, i, j + 1, k)) / dy;
+        float cx[2][2], cy[2][2], cz[2][2];   // [slot][cell, +1]: x (Eyx, Ezx), y (Exy, Ezy), z (Exz, Eyz)
+        cnu_e_pair(g, pe, lpe, 0, i, j, k, cx);
+        cnu_e_pair(g, pe, lpe, 1, i, j, k, cy);
+        cnu_e_pair(g, pe, lpe, 2, i, j, k, cz);
+        hx += (cy[1][1] - cy[1][0]) / dy + (cz[1][0] - cz[1][1]) / dz;
+        hy += (cx[1][0] - cx[1][1]) / dx + (cz[0][1] - cz[0][0]) / dz;
+        hz += (cx[0][1] - cx[0][0]) / dx + (cy[0][0] - cy[0][1]) / dy;
+#pragma unroll
         for (int A = 0; A < 3; ++A) {
             int f, q;
             long long po;
             if (!h_psi(g, ph, A, i, j, k, f, q, po)) continue;
             const float b = ph.b[f][q - 1];
+#pragma unroll
             for (int sl = 0; sl < 2; ++sl) {
                 const int c = h_slot_comp(A, sl);
                 float* nu = lph.psi[f][sl];
@@ -2032,16 +2083,14 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid g, float* lEx, float* lEy, f
         if (uz) ez = tup[t + 4] * lEz[o] + DB * (hx * dh_y - hx_ym * dh_ym + hy_xm * dh_xm - hy * dh_x);
     }
     const bool ns = near_slab(g, pe, i, j, k);
-    if (ns) {
-        if (ux)
-            ex += (cnu_h(g, ph, lph, 2, 1, i, j, k) - cnu_h(g, ph, lph, 2, 1, i, j, k - 1)) / dz +
-                  (cnu_h(g, ph, lph, 1, 1, i, j - 1, k) - cnu_h(g, ph, lph, 1, 1, i, j, k)) / dy;
-        if (uy)
-            ey += (cnu_h(g, ph, lph, 2, 0, i, j, k - 1) - cnu_h(g, ph, lph, 2, 0, i, j, k)) / dz +
-                  (cnu_h(g, ph, lph, 0, 1, i, j, k) - cnu_h(g, ph, lph, 0, 1, i - 1, j, k)) / dx;
-        if (uz)
-            ez += (cnu_h(g, ph, lph, 0, 0, i - 1, j, k) - cnu_h(g, ph, lph, 0, 0, i, j, k)) / dx +
-                  (cnu_h(g, ph, lph, 1, 0, i, j, k) - cnu_h(g, ph, lph, 1, 0, i, j - 1, k)) / dy;
+    if (ns && (ux || uy || uz)) {
+        float cx[2][2], cy[2][2], cz[2][2];   // [slot][cell, -1]: x (Hyx, Hzx), y (Hxy, Hzy), z (Hxz, Hyz)
+        cnu_h_pair(g, ph, lph, 0, i, j, k, cx);
+        cnu_h_pair(g, ph, lph, 1, i, j, k, cy);
+        cnu_h_pair(g, ph, lph, 2, i, j, k, cz);
+        if (ux) ex += (cz[1][0] - cz[1][1]) / dz + (cy[1][1] - cy[1][0]) / dy;
+        if (uy) ey += (cz[0][1] - cz[0][0]) / dz + (cx[1][0] - cx[1][1]) / dx;
+        if (uz) ez += (cx[0][1] - cx[0][0]) / dx + (cy[0][0] - cy[0][1]) / dy;
     }
     if (ux) mx = tup[t + 1] * ex;
     if (uy) my = tup[t + 3] * ey;
@@ -2053,11 +2102,13 @@ __global__ void __launch_bounds__(256) k_adj_e(Grid g, float* lEx, float* lEy, f
     mEy[o] = my;
     mEz[o] = mz;
     if (ns) {
+#pragma unroll
         for (int A = 0; A < 3; ++A) {
             int f, q;
             long long po;
             if (!e_psi(g, pe, A, i, j, k, f, q, po)) continue;
             const float b = pe.b[f][q - 1];
+#pragma unroll
             for (int sl = 0; sl < 2; ++sl) {
                 if (!e_slot_ok(A, sl, i, j, k)) continue;
                 const int c = e_slot_comp(A, sl);
