@@ -426,3 +426,103 @@ def test_autograd_wrapper():
     first = float((g * step.astype(np.float64)).sum())
     print(f"J={J.item():.9e} J(eps+s)={J2:.9e} predicted change {first:.3e} actual {J2 - J.item():.3e}")
     assert abs((J2 - J.item()) - first) <= 0.02 * abs(first)
+
+
+# ---------------------------------------------------------------------------
+# the reference's own gradient formula, on this solver's fields
+# ---------------------------------------------------------------------------
+def _edge_dft(plan, freq, lo, hi, with_objective=False):
+    """FP32 oracle run of the whole deck: DFT at ``freq`` of Ex, Ey, Ez on
+    the edges of the cell box [lo, hi] (array (3, nz+1, ny+1, nx+1)), and the
+    objective when asked."""
+    v = plan.view
+    orc = Oracle(plan)
+    w = 2 * math.pi * freq
+    dt = float(v.dt)
+    acc = 0.0
+    ref = FarFieldRef(plan, THETA, PHI, freq, "theta") if with_objective else None
+    for n in range(1, plan.time_steps + 1):
+        orc.step(1)
+        comps = [orc.field(c).astype(np.float64)[lo[2] - 1:hi[2] + 1, lo[1] - 1:hi[1] + 1, lo[0] - 1:hi[0] + 1]
+                 for c in ("Ex", "Ey", "Ez")]
+        acc = acc + np.stack(comps) * np.exp(-1j * w * n * dt)
+        if ref is not None:
+            ref.add({c: orc.field(c).astype(np.float64) for c in NAMES}, n)
+    return acc, (ref.value() if ref is not None else None)
+
+
+def test_design_gradient_matches_reference_workflow_formula():
+    """The reference's adjoint workflow (optimization_scripts/
+    adjoint_optimization/adjoint_optimization.py:76-134) computes
+    dJ/d eps = Re(sum E_fwd E_adj) dV eps0 w^2 x (adjoint source scaling) from
+    a second simulation.  Restated for J = |F|^2 with F = r E_theta: by
+    reciprocity the adjoint simulation is a plane wave arriving from the
+    observation direction with theta polarisation, and
+        dF/d eps_v = k^2 dV / (4 pi E0) * sum over the 12 edges of voxel v of
+                     (1/4) E_fwd . E_adj        (each edge averages 4 cells)
+        dJ/d eps_v = 2 Re(conj(F) dF/d eps_v),
+    E0 being the adjoint wave's amplitude at the phase reference (measured on
+    an empty grid).  This continuous-adjoint formula approximates the exact
+    discrete gradient to the grid's discretisation error, O((k dx)^2) = 4.4% at
+    30 cells per wavelength; the test bounds the gap."""
+    T = 800
+    eps = design_eps()
+    fwd = design_deck(eps)
+    fwd.time_steps = T
+    # exact discrete gradient on the device
+    base = design_deck()
+    base.time_steps = T
+    plan = Plan.from_text(base.text())
+    sim = Simulation(plan)
+    sim.set_objective(THETA, PHI, FREQ, "theta")
+    sim.design_box(BOX_LO, BOX_HI)
+    sim.set_design_eps(eps)
+    sim.run()
+    g_adj = sim.design_gradient()
+    # forward, adjoint and empty-grid fields from the oracle
+    p_fwd = Plan.from_text(fwd.text())
+    E1, F = _edge_dft(p_fwd, FREQ, BOX_LO, BOX_HI, with_objective=True)
+    adj = design_deck(eps)
+    adj.time_steps = T
+    adj.angles = (180.0 - THETA, (PHI + 180.0) % 360.0, 270.0)   # arriving from (THETA, PHI), E along theta^
+    E2, _ = _edge_dft(Plan.from_text(adj.text()), FREQ, BOX_LO, BOX_HI)
+    adj.objects = []
+    p_emp = Plan.from_text(adj.text())
+    E2e, _ = _edge_dft(p_emp, FREQ, BOX_LO, BOX_HI)
+    v = p_fwd.view
+    k = 2 * math.pi * FREQ / C0
+    th, ph = math.radians(THETA), math.radians(PHI)
+    rh = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    thv = np.array([math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th)])
+    d3 = np.array([float(v.dx), float(v.dy), float(v.dz)])
+    p0 = (np.array([v.ic, v.jc, v.kc]) + 0.5) * d3          # phase reference of F (centre of the phase-centre cell)
+    K, J, I = np.meshgrid(*(np.arange(BOX_LO[a], BOX_HI[a] + 2) for a in (2, 1, 0)), indexing="ij")
+    est = []
+    for c in range(3):
+        if abs(thv[c]) < 1e-9:
+            continue
+        half = np.zeros(3)
+        half[c] = 0.5                                          # Yee position of component c
+        pos = np.stack([(I + half[0]) * d3[0], (J + half[1]) * d3[1], (K + half[2]) * d3[2]])
+        phase = np.exp(1j * k * np.tensordot(rh, pos - p0[:, None, None, None], axes=1))
+        est.append((E2e[c] / (thv[c] * phase)).ravel())
+    est = np.concatenate(est)
+    E0 = est.mean()
+    assert np.abs(est - E0).max() < 1e-3 * abs(E0)            # the empty-grid wave is the assumed plane wave
+    P = E1 * E2
+    g_rec = np.zeros_like(g_adj)
+    dV = float(d3.prod())
+    for kk in range(g_adj.shape[0]):
+        for jj in range(g_adj.shape[1]):
+            for ii in range(g_adj.shape[2]):
+                s = (P[0, kk:kk + 2, jj:jj + 2, ii].sum() + P[1, kk:kk + 2, jj, ii:ii + 2].sum() +
+                     P[2, kk, jj:jj + 2, ii:ii + 2].sum())
+                dF = k * k * dV / (4 * math.pi) * 0.25 * s / E0
+                g_rec[kk, jj, ii] = 2 * np.real(np.conj(F) * dF)
+    rel = np.abs(g_rec - g_adj) / np.abs(g_adj)
+    cos = float((g_rec * g_adj).sum() / (np.linalg.norm(g_rec) * np.linalg.norm(g_adj)))
+    print("discrete adjoint", g_adj.ravel(), "\nreference formula", g_rec.ravel(),
+          f"\nmax rel gap {rel.max():.3f}, cosine {cos:.5f}")
+    # measured: cosine 1.00000, a uniform 5.4% scale -- the O((k dx)^2) = 4.4% order of the gap
+    assert cos > 0.999
+    assert rel.max() < 0.08
