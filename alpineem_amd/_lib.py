@@ -50,6 +50,8 @@ SIGNATURES = [
     ("aem_get_dft", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     ("aem_objective_set", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_double, C.c_int32]),
     ("aem_objective_get", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("aem_ffdft_set", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int32]),
+    ("aem_ffdft_get", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_adjoint_record", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("aem_adjoint_eps_gradient", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("aem_design_box", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -1787,6 +1787,158 @@ __device__ __forceinline__ void source_terms(const Grid& g, const Objective& ob,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Running-DFT far field: the frequency-domain near-to-far-field transform of
+// the same face currents, at n_ang directions and n_freq frequencies
+// f0 + q df, accumulated every step.  Per (direction a, frequency q):
+//   S_theta = sum (M.phi^ e_M + eta J.theta^ e_J) dS e^{jk r^.r'}
+//   S_phi   = sum (M.theta^ e_M - eta J.phi^ e_J) dS e^{jk r^.r'}
+// with e_M = e^{-jw n dt}, e_J = e^{-jw (n-1/2) dt}; r E_theta = -jk/(4 pi)
+// S_theta and r E_phi = jk/(4 pi) S_phi are formed when read.  The spatial
+// factor steps through the frequencies by one complex product per q.  Each
+// work-group reduces its face cells in a fixed order (wave shuffles, then the
+// four waves) and writes one partial per value; k_ffdft_sum adds the
+// partials in order and folds them into FP64 totals with Kahan compensation.
+// ---------------------------------------------------------------------------
+struct FfDft {
+    int lo[3], hi[3], pbc[3];
+    int ic, jc, kc;
+    int mu, mv;              // face extents of the launch grid
+    int n_ang, n_freq;
+    double d[3], dt, w0, dw, k0, dk, eta;
+    const double* dir;       // [n_ang][9]: r^, theta^, phi^
+    double* part;            // [blocks][n_ang][n_freq][4]
+    double* tot;             // [n_ang][n_freq][4]: S_theta (re, im), S_phi (re, im)
+    double* comp;            // Kahan compensation of tot
+};
+
+constexpr int kFfChunk = 8;   // frequencies reduced together
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int A>
+__device__ __forceinline__ bool ffdft_cell(const Grid& g, const FfDft& p, int face, int uu, int vv, const float* Ex,
+                                           const float* Ey, const float* Ez, const float* Hx, const float* Hy,
+                                           const float* Hz, double cur[4], double r[3]) {
+    constexpr int b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
+    if (p.pbc[A]) return false;
+    const int u = p.lo[b] + uu, v = p.lo[c] + vv;
+    if (u > p.hi[b] || v > p.hi[c]) return false;
+    const int pos = (face & 1) ? p.hi[A] : p.lo[A];
+    int ijk[3];
+    ijk[A] = pos;
+    ijk[b] = u;
+    ijk[c] = v;
+    const long long o = off(g, ijk[0], ijk[1], ijk[2]);
+    FaceTerm t[4];
+    face_terms<A>(g, (face & 1) ? -1.0 : 1.0, t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float* Fq = pick6(t[q].comp, Ex, Ey, Ez, Hx, Hy, Hz) + o;
+        double s = (double)Fq[t[q].offs[0]] + (double)Fq[t[q].offs[1]];
+        if (t[q].n == 4) s = (s + (double)Fq[t[q].offs[2]]) + (double)Fq[t[q].offs[3]];
+        cur[q] = t[q].w * s;
+    }
+    r[0] = (ijk[0] - p.ic) * p.d[0];
+    r[1] = (ijk[1] - p.jc) * p.d[1];
+    r[2] = (ijk[2] - p.kc) * p.d[2];
+    return true;
+}
+
+// grid (x, y, 6 faces) over the face cells, block 256.
+__global__ void __launch_bounds__(256) k_ffdft(Grid g, FfDft p, const float* Ex, const float* Ey, const float* Ez,
+                                               const float* Hx, const float* Hy, const float* Hz, const int* step) {
+    const int face = blockIdx.z, a = face / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int uu = blockIdx.x * 32 + (tid & 31), vv = blockIdx.y * 8 + (tid >> 5);
+    double cur[4] = {0, 0, 0, 0}, r[3] = {0, 0, 0};
+    bool live;
+    switch (a) {
+        case 0: live = ffdft_cell<0>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur, r); break;
+        case 1: live = ffdft_cell<1>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur, r); break;
+        default: live = ffdft_cell<2>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur, r); break;
+    }
+    // M = (cur0 along b, cur1 along c), J = (cur2, cur3)
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const double dS = p.d[b] * p.d[c];
+    const int n = *step;
+    const double tE = n * p.dt, tH = (n - 0.5) * p.dt;
+    double sE0, cE0, sEd, cEd, sH0, cH0, sHd, cHd;
+    sincos(-p.w0 * tE, &sE0, &cE0);
+    sincos(-p.dw * tE, &sEd, &cEd);
+    sincos(-p.w0 * tH, &sH0, &cH0);
+    sincos(-p.dw * tH, &sHd, &cHd);
+    __shared__ double red[4][kFfChunk * 4];
+    const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    for (int ka = 0; ka < p.n_ang; ++ka) {
+        const double* dv = p.dir + 9 * ka;
+        const double th[3] = {dv[3], dv[4], dv[5]}, ph[3] = {dv[6], dv[7], dv[8]};
+        const double mth = cur[0] * th[b] + cur[1] * th[c], mph = cur[0] * ph[b] + cur[1] * ph[c];
+        const double jth = p.eta * (cur[2] * th[b] + cur[3] * th[c]), jph = p.eta * (cur[2] * ph[b] + cur[3] * ph[c]);
+        const double x = dv[0] * r[0] + dv[1] * r[1] + dv[2] * r[2];
+        double zs, zc, ds, dc;
+        sincos(p.k0 * x, &zs, &zc);
+        sincos(p.dk * x, &ds, &dc);
+        double zr = live ? dS * zc : 0.0, zi = live ? dS * zs : 0.0;   // dS e^{jk_q x}
+        double eMr = cE0, eMi = sE0, eJr = cH0, eJi = sH0;            // e^{-jw_q t}
+        for (int q0 = 0; q0 < p.n_freq; q0 += kFfChunk) {
+            double acc[kFfChunk][4];
+#pragma unroll
+            for (int j = 0; j < kFfChunk; ++j) {
+                const double cMr = zr * eMr - zi * eMi, cMi = zr * eMi + zi * eMr;
+                const double cJr = zr * eJr - zi * eJi, cJi = zr * eJi + zi * eJr;
+                acc[j][0] = mph * cMr + jth * cJr;
+                acc[j][1] = mph * cMi + jth * cJi;
+                acc[j][2] = mth * cMr - jph * cJr;
+                acc[j][3] = mth * cMi - jph * cJi;
+                double t0 = zr * dc - zi * ds;
+                zi = zr * ds + zi * dc;
+                zr = t0;
+                t0 = eMr * cEd - eMi * sEd;
+                eMi = eMr * sEd + eMi * cEd;
+                eMr = t0;
+                t0 = eJr * cHd - eJi * sHd;
+                eJi = eJr * sHd + eJi * cHd;
+                eJr = t0;
+            }
+#pragma unroll
+            for (int j = 0; j < kFfChunk; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double s = wave_sum(acc[j][e]);
+                    if (lane == 0) red[wave][j * 4 + e] = s;
+                }
+            __syncthreads();
+            if (tid < kFfChunk * 4) {
+                const int j = tid / 4, q = q0 + j;
+                if (q < p.n_freq) {
+                    const double s = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+                    p.part[((blk * p.n_ang + ka) * p.n_freq + q) * 4 + (tid & 3)] = s;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// tot += sum over work-groups of the step's partials (fixed order), with
+// Kahan compensation.  One thread per value.
+__global__ void k_ffdft_sum(FfDft p, int nblk) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nv = (long long)p.n_ang * p.n_freq * 4;
+    if (v >= nv) return;
+    double s = 0.0;
+    for (int q = 0; q < nblk; ++q) s += p.part[(long long)q * nv + v];
+    const double y = s - p.comp[v];
+    const double t = p.tot[v] + y;
+    p.comp[v] = (t - p.tot[v]) - y;
+    p.tot[v] = t;
+}
+
 // Adjoint source of J = |F|^2 at step n: dJ/d(term) = 2 Re(conj(F) w),
 // scattered onto the adjoint fields through each term's average.
 template <typename IdxT>

@@ -221,6 +221,8 @@ struct aem_sim {
     // differentiation
     Objective obj{};
     bool obj_on = false;
+    FfDft ff{};                    // running-DFT far field
+    bool ff_on = false;
     AdjRec rec{};
     bool rec_on = false;
     int rec_lo[3] = {0, 0, 0}, rec_hi[3] = {0, 0, 0};   // design cell box
@@ -630,6 +632,12 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     if ((parts & kPartObjective) && obj_on) {
         const dim3 go(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
         k_objective<<<go, 256, 0, s>>>(g, obj, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
+    }
+    if ((parts & kPartObjective) && ff_on) {
+        const dim3 gf(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
+        k_ffdft<<<gf, 256, 0, s>>>(g, ff, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
+        const long long nv = (long long)ff.n_ang * ff.n_freq * 4;
+        k_ffdft_sum<<<cdiv(nv, 256), 256, 0, s>>>(ff, (int)(gf.x * gf.y * gf.z));
     }
     if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
         k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
@@ -1391,6 +1399,69 @@ aem_status aem_objective_set(aem_sim* s, float theta_deg, float phi_deg, double 
         if (!o.F) o.F = s->zeros<double>(2);
         s->obj_on = true;
         HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_ffdft_set(aem_sim* s, int32_t n_ang, const float* theta_deg, const float* phi_deg, double f0_hz,
+                         double df_hz, int32_t n_freq) {
+    if (!s || n_ang <= 0 || n_freq <= 0 || !theta_deg || !phi_deg || f0_hz < 0 || df_hz < 0) {
+        aem::set_error("aem_ffdft_set: bad argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (!fresh(s) || v.n_angles <= 0 || s->ff_on) {
+        aem::set_error("aem_ffdft_set: needs a deck with a far-field box and a sim that has not stepped (once)");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        FfDft& p = s->ff;
+        for (int a = 0; a < 3; ++a) { p.lo[a] = v.ff_lo[a]; p.hi[a] = v.ff_hi[a]; p.pbc[a] = v.pbc[a]; }
+        p.ic = v.ic; p.jc = v.jc; p.kc = v.kc;
+        p.mu = s->ntff_mu; p.mv = s->ntff_mv;
+        p.n_ang = n_ang; p.n_freq = n_freq;
+        p.d[0] = v.dx; p.d[1] = v.dy; p.d[2] = v.dz;
+        p.dt = v.dt;
+        p.w0 = 2.0 * M_PI * f0_hz; p.dw = 2.0 * M_PI * df_hz;
+        p.k0 = p.w0 / 299792458.0; p.dk = p.dw / 299792458.0;
+        p.eta = std::sqrt((double)aem::kMu0 / (double)aem::kEp0);
+        std::vector<double> dir(9 * (size_t)n_ang);
+        for (int a = 0; a < n_ang; ++a) {
+            const double th = theta_deg[a] * M_PI / 180.0, ph = phi_deg[a] * M_PI / 180.0;
+            const double st = std::sin(th), ct = std::cos(th), sp = std::sin(ph), cp = std::cos(ph);
+            const double d9[9] = {st * cp, st * sp, ct, ct * cp, ct * sp, -st, -sp, cp, 0.0};
+            std::copy(d9, d9 + 9, dir.begin() + 9 * a);
+        }
+        p.dir = s->up(dir.data(), dir.size());
+        const size_t nv = (size_t)n_ang * n_freq * 4;
+        const size_t nblk = (size_t)cdiv(p.mu, 32) * cdiv(p.mv, 8) * 6;
+        p.part = s->zeros<double>(nblk * nv);
+        p.tot = s->zeros<double>(nv);
+        p.comp = s->zeros<double>(nv);
+        s->ff_on = true;
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
+aem_status aem_ffdft_get(aem_sim* s, double* out, int64_t n) {
+    if (!s || !out || !s->ff_on || n != (int64_t)s->ff.n_ang * s->ff.n_freq * 4) {
+        aem::set_error("aem_ffdft_get: no running-DFT far field, or n is not n_ang * n_freq * 4");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        std::vector<double> S((size_t)n);
+        HIPCHK(hipMemcpyAsync(S.data(), s->ff.tot, S.size() * sizeof(double), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        const FfDft& p = s->ff;
+        for (int a = 0; a < p.n_ang; ++a)
+            for (int q = 0; q < p.n_freq; ++q) {
+                const size_t o = ((size_t)a * p.n_freq + q) * 4;
+                const double kq = (p.k0 + q * p.dk) / (4.0 * M_PI);
+                // r E_theta = -j k/(4 pi) S_theta,  r E_phi = j k/(4 pi) S_phi
+                out[o] = kq * S[o + 1];
+                out[o + 1] = -kq * S[o];
+                out[o + 2] = -kq * S[o + 3];
+                out[o + 3] = kq * S[o + 2];
+            }
     });
 }
 

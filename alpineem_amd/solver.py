@@ -145,6 +145,25 @@ class Simulation:
         check(lib().aem_objective_set(self._h, float(theta_deg), float(phi_deg), float(freq_hz),
                                       0 if pol == "theta" else 1))
 
+    def set_farfield_dft(self, angles, f0_hz: float, df_hz: float, n_freq: int) -> None:
+        """Accumulate r E_theta and r E_phi at the directions ``angles``
+        [(theta, phi) in degrees] and frequencies f0 + q df by a running DFT
+        of the far-field box currents (aem_ffdft_set); call before stepping."""
+        ang = np.asarray(angles, np.float32).reshape(-1, 2)
+        th = np.ascontiguousarray(ang[:, 0])
+        ph = np.ascontiguousarray(ang[:, 1])
+        check(lib().aem_ffdft_set(self._h, len(ang), th.ctypes.data, ph.ctypes.data, float(f0_hz), float(df_hz),
+                                  int(n_freq)))
+        self._ff_shape = (len(ang), int(n_freq))
+
+    def farfield_dft(self) -> np.ndarray:
+        """Complex (n_angles, n_freq, 2): r E_theta and r E_phi."""
+        na, nf = self._ff_shape
+        out = np.zeros(na * nf * 4, np.float64)
+        check(lib().aem_ffdft_get(self._h, out.ctypes.data, out.size))
+        o = out.reshape(na, nf, 2, 2)
+        return o[..., 0] + 1j * o[..., 1]
+
     def objective(self) -> complex:
         out = np.zeros(2, np.float64)
         check(lib().aem_objective_get(self._h, out.ctypes.data))

@@ -200,6 +200,20 @@ aem_status aem_get_dft(aem_sim* sim, int32_t kind, int32_t idx, const double* fr
 aem_status aem_objective_set(aem_sim* sim, float theta_deg, float phi_deg, double freq_hz, int32_t pol);
 aem_status aem_objective_get(aem_sim* sim, double* re_im);
 
+/* Running-DFT far field (no counterpart in the reference, whose far field is
+ * the time-domain transform of f90:7643-7927 followed by the post-processor's
+ * FFT, utility_scripts/post_processor.py:245-266): the frequency-domain
+ * near-to-far-field transform of the same face currents as the objective,
+ * accumulated every step in FP64 with Kahan compensation, at n_ang
+ * directions (degrees) and the n_freq frequencies f0 + q df.  Call before the
+ * first step; the deck needs a far-field box. */
+aem_status aem_ffdft_set(aem_sim* sim, int32_t n_ang, const float* theta_deg, const float* phi_deg, double f0_hz,
+                         double df_hz, int32_t n_freq);
+/* out[((a * n_freq + q) * 4) + 0..3] = re, im of r E_theta and of r E_phi
+ * (e^{jkr} dropped, phase reference at the centre of the phase-centre cell);
+ * n = n_ang * n_freq * 4. */
+aem_status aem_ffdft_get(aem_sim* sim, double* out, int64_t n);
+
 /* Record the design box of cells [lo, hi] (1-based, inclusive, in the
  * interior outside the CPML slabs) during the run: every step stores E and
  * the update's curl term on the box's edges (time_steps * 6 * edges floats).

@@ -526,3 +526,37 @@ def test_design_gradient_matches_reference_workflow_formula():
     # measured: cosine 1.00000, a uniform 5.4% scale -- the O((k dx)^2) = 4.4% order of the gap
     assert cos > 0.999
     assert rel.max() < 0.08
+
+
+# ---------------------------------------------------------------------------
+# running-DFT far field at many directions and frequencies
+# ---------------------------------------------------------------------------
+def test_farfield_dft_matches_objective_and_restatement():
+    """aem_ffdft_*: at the objective's (theta, phi, f) it equals the
+    objective F; at every (direction, frequency) it equals the numpy
+    restatement on the oracle's fields, for both polarisations."""
+    plan = Plan.from_text(scatter_deck().text())
+    angles = [(THETA, PHI), (60.0, 30.0), (150.0, 300.0)]
+    f0, df, nf = 6e9, 2e9, 5                       # 6, 8, 10, 12, 14 GHz
+    sim = Simulation(plan)
+    sim.set_objective(THETA, PHI, FREQ, "theta")
+    sim.set_farfield_dft(angles, f0, df, nf)
+    sim.run()
+    ff = sim.farfield_dft()
+    F = sim.objective()
+    assert abs(ff[0, 2, 0] - F) <= 1e-9 * abs(F)
+    orc = Oracle(plan)
+    refs = {(a, q, pol): FarFieldRef(plan, th, ph, f0 + q * df, pol)
+            for a, (th, ph) in enumerate(angles) for q in range(nf) for pol in ("theta", "phi")}
+    for n in range(1, plan.time_steps + 1):
+        orc.step(1)
+        fields = {c: orc.field(c).astype(np.float64) for c in NAMES}
+        for r in refs.values():
+            r.add(fields, n)
+    worst = 0.0
+    for (a, q, pol), r in refs.items():
+        want = r.value()
+        got = ff[a, q, 0 if pol == "theta" else 1]
+        worst = max(worst, abs(got - want) / np.abs(ff[a, q]).max())
+    print(f"running-DFT far field vs restatement: worst {worst:.2e}")
+    assert worst <= 1e-5
