@@ -1790,40 +1790,37 @@ __device__ __forceinline__ void source_terms(const Grid& g, const Objective& ob,
 // ---------------------------------------------------------------------------
 // Running-DFT far field: the frequency-domain near-to-far-field transform of
 // the same face currents, at n_ang directions and n_freq frequencies
-// f0 + q df, accumulated every step.  Per (direction a, frequency q):
-//   S_theta = sum (M.phi^ e_M + eta J.theta^ e_J) dS e^{jk r^.r'}
-//   S_phi   = sum (M.theta^ e_M - eta J.phi^ e_J) dS e^{jk r^.r'}
+// f0 + q df, accumulated every step.  Per (direction, frequency) pair:
+//   S_theta = e_M U_theta + e_J V_theta,   U_theta = sum dS e^{jk r^.r'} M.phi^,
+//                                          V_theta = sum dS e^{jk r^.r'} eta J.theta^
+//   S_phi   = e_M U_phi   - e_J V_phi,     U_phi = ... M.theta^, V_phi = ... eta J.phi^
 // with e_M = e^{-jw n dt}, e_J = e^{-jw (n-1/2) dt}; r E_theta = -jk/(4 pi)
-// S_theta and r E_phi = jk/(4 pi) S_phi are formed when read.  The spatial
-// factor steps through the frequencies by one complex product per q.  Each
-// work-group reduces its face cells in a fixed order (wave shuffles, then the
-// four waves) and writes one partial per value; k_ffdft_sum adds the
-// partials in order and folds them into FP64 totals with Kahan compensation.
+// S_theta and r E_phi = jk/(4 pi) S_phi are formed when read.  r^.r' is
+// linear along a face, so within a work-group's 32 x 8 tile the spatial
+// factor advances by one complex product per cell from a host-built table
+// (the tile's first cell and the per-row and per-column steps).  A
+// work-group stages its tile's currents in LDS, each thread then owns whole
+// (direction, frequency) pairs and writes the tile's sums; k_ffdft_sum adds
+// the tiles in order, applies the step's time factors and folds the result
+// into FP64 totals with Kahan compensation.
 // ---------------------------------------------------------------------------
 struct FfDft {
     int lo[3], hi[3], pbc[3];
-    int ic, jc, kc;
     int mu, mv;              // face extents of the launch grid
     int n_ang, n_freq;
-    double d[3], dt, w0, dw, k0, dk, eta;
+    double dt, w0, dw, eta;
     const double* dir;       // [n_ang][9]: r^, theta^, phi^
-    double* part;            // [blocks][n_ang][n_freq][4]
-    double* tot;             // [n_ang][n_freq][4]: S_theta (re, im), S_phi (re, im)
+    const double* step;      // [6 faces][n_pairs][4]: e^{jk r^.d_u} (re, im), e^{jk r^.d_v} (re, im)
+    const double* z0;        // [tiles][n_pairs][2]: dS e^{jk r^.r'} at each tile's first cell
+    double* part;            // [n_pairs][tiles][8]: U_theta, V_theta, U_phi, V_phi (re, im)
+    double* tot;             // [n_pairs][4]: S_theta (re, im), S_phi (re, im)
     double* comp;            // Kahan compensation of tot
 };
-
-constexpr int kFfChunk = 8;   // frequencies reduced together
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
 
 template <int A>
 __device__ __forceinline__ bool ffdft_cell(const Grid& g, const FfDft& p, int face, int uu, int vv, const float* Ex,
                                            const float* Ey, const float* Ez, const float* Hx, const float* Hy,
-                                           const float* Hz, double cur[4], double r[3]) {
+                                           const float* Hz, double cur[4]) {
     constexpr int b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
     if (p.pbc[A]) return false;
     const int u = p.lo[b] + uu, v = p.lo[c] + vv;
@@ -1843,100 +1840,106 @@ __device__ __forceinline__ bool ffdft_cell(const Grid& g, const FfDft& p, int fa
         if (t[q].n == 4) s = (s + (double)Fq[t[q].offs[2]]) + (double)Fq[t[q].offs[3]];
         cur[q] = t[q].w * s;
     }
-    r[0] = (ijk[0] - p.ic) * p.d[0];
-    r[1] = (ijk[1] - p.jc) * p.d[1];
-    r[2] = (ijk[2] - p.kc) * p.d[2];
     return true;
 }
 
-// grid (x, y, 6 faces) over the face cells, block 256.
+// grid (x, y, 6 faces) of 32 x 8 tiles over the face cells, block 256.
 __global__ void __launch_bounds__(256) k_ffdft(Grid g, FfDft p, const float* Ex, const float* Ey, const float* Ez,
-                                               const float* Hx, const float* Hy, const float* Hz, const int* step) {
+                                               const float* Hx, const float* Hy, const float* Hz) {
+    __shared__ double sc[256][4];
     const int face = blockIdx.z, a = face / 2;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int uu = blockIdx.x * 32 + (tid & 31), vv = blockIdx.y * 8 + (tid >> 5);
-    double cur[4] = {0, 0, 0, 0}, r[3] = {0, 0, 0};
-    bool live;
-    switch (a) {
-        case 0: live = ffdft_cell<0>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur, r); break;
-        case 1: live = ffdft_cell<1>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur, r); break;
-        default: live = ffdft_cell<2>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur, r); break;
-    }
-    // M = (cur0 along b, cur1 along c), J = (cur2, cur3)
-    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-    const double dS = p.d[b] * p.d[c];
-    const int n = *step;
-    const double tE = n * p.dt, tH = (n - 0.5) * p.dt;
-    double sE0, cE0, sEd, cEd, sH0, cH0, sHd, cHd;
-    sincos(-p.w0 * tE, &sE0, &cE0);
-    sincos(-p.dw * tE, &sEd, &cEd);
-    sincos(-p.w0 * tH, &sH0, &cH0);
-    sincos(-p.dw * tH, &sHd, &cHd);
-    __shared__ double red[4][kFfChunk * 4];
-    const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-    for (int ka = 0; ka < p.n_ang; ++ka) {
-        const double* dv = p.dir + 9 * ka;
-        const double th[3] = {dv[3], dv[4], dv[5]}, ph[3] = {dv[6], dv[7], dv[8]};
-        const double mth = cur[0] * th[b] + cur[1] * th[c], mph = cur[0] * ph[b] + cur[1] * ph[c];
-        const double jth = p.eta * (cur[2] * th[b] + cur[3] * th[c]), jph = p.eta * (cur[2] * ph[b] + cur[3] * ph[c]);
-        const double x = dv[0] * r[0] + dv[1] * r[1] + dv[2] * r[2];
-        double zs, zc, ds, dc;
-        sincos(p.k0 * x, &zs, &zc);
-        sincos(p.dk * x, &ds, &dc);
-        double zr = live ? dS * zc : 0.0, zi = live ? dS * zs : 0.0;   // dS e^{jk_q x}
-        double eMr = cE0, eMi = sE0, eJr = cH0, eJi = sH0;            // e^{-jw_q t}
-        for (int q0 = 0; q0 < p.n_freq; q0 += kFfChunk) {
-            double acc[kFfChunk][4];
-#pragma unroll
-            for (int j = 0; j < kFfChunk; ++j) {
-                const double cMr = zr * eMr - zi * eMi, cMi = zr * eMi + zi * eMr;
-                const double cJr = zr * eJr - zi * eJi, cJi = zr * eJi + zi * eJr;
-                acc[j][0] = mph * cMr + jth * cJr;
-                acc[j][1] = mph * cMi + jth * cJi;
-                acc[j][2] = mth * cMr - jph * cJr;
-                acc[j][3] = mth * cMi - jph * cJi;
-                double t0 = zr * dc - zi * ds;
-                zi = zr * ds + zi * dc;
-                zr = t0;
-                t0 = eMr * cEd - eMi * sEd;
-                eMi = eMr * sEd + eMi * cEd;
-                eMr = t0;
-                t0 = eJr * cHd - eJi * sHd;
-                eJi = eJr * sHd + eJi * cHd;
-                eJr = t0;
-            }
-#pragma unroll
-            for (int j = 0; j < kFfChunk; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const double s = wave_sum(acc[j][e]);
-                    if (lane == 0) red[wave][j * 4 + e] = s;
-                }
-            __syncthreads();
-            if (tid < kFfChunk * 4) {
-                const int j = tid / 4, q = q0 + j;
-                if (q < p.n_freq) {
-                    const double s = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-                    p.part[((blk * p.n_ang + ka) * p.n_freq + q) * 4 + (tid & 3)] = s;
-                }
-            }
-            __syncthreads();
+    const int tid = threadIdx.x;
+    {
+        const int uu = blockIdx.x * 32 + (tid & 31), vv = blockIdx.y * 8 + (tid >> 5);
+        double cur[4] = {0, 0, 0, 0};
+        switch (a) {
+            case 0: ffdft_cell<0>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur); break;
+            case 1: ffdft_cell<1>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur); break;
+            default: ffdft_cell<2>(g, p, face, uu, vv, Ex, Ey, Ez, Hx, Hy, Hz, cur); break;
         }
+        sc[tid][0] = cur[0];   // outside the face: 0
+        sc[tid][1] = cur[1];
+        sc[tid][2] = cur[2];
+        sc[tid][3] = cur[3];
+    }
+    __syncthreads();
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const int npair = p.n_ang * p.n_freq;
+    const long long tile = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    for (int pr = tid; pr < npair; pr += blockDim.x) {
+        const double* dv = p.dir + 9 * (pr / p.n_freq);
+        const double tb = dv[3 + b], tc = dv[3 + c], pb = dv[6 + b], pc = dv[6 + c];
+        const double* st = p.step + ((long long)face * npair + pr) * 4;
+        const double ar = st[0], ai = st[1], br = st[2], bi = st[3];
+        double zr0 = p.z0[(tile * npair + pr) * 2], zi0 = p.z0[(tile * npair + pr) * 2 + 1];
+        double uth_r = 0, uth_i = 0, vth_r = 0, vth_i = 0, uph_r = 0, uph_i = 0, vph_r = 0, vph_i = 0;
+        for (int vv = 0; vv < 8; ++vv) {
+            double zr = zr0, zi = zi0;
+#pragma unroll 4
+            for (int uu = 0; uu < 32; ++uu) {
+                const double* cl = sc[vv * 32 + uu];
+                const double mth = cl[0] * tb + cl[1] * tc, mph = cl[0] * pb + cl[1] * pc;
+                const double jth = p.eta * (cl[2] * tb + cl[3] * tc), jph = p.eta * (cl[2] * pb + cl[3] * pc);
+                uth_r += zr * mph; uth_i += zi * mph;
+                vth_r += zr * jth; vth_i += zi * jth;
+                uph_r += zr * mth; uph_i += zi * mth;
+                vph_r += zr * jph; vph_i += zi * jph;
+                const double t0 = zr * ar - zi * ai;
+                zi = zr * ai + zi * ar;
+                zr = t0;
+            }
+            const double t0 = zr0 * br - zi0 * bi;
+            zi0 = zr0 * bi + zi0 * br;
+            zr0 = t0;
+        }
+        double* out = p.part + ((long long)pr * gridDim.x * gridDim.y * gridDim.z + tile) * 8;
+        out[0] = uth_r; out[1] = uth_i; out[2] = vth_r; out[3] = vth_i;
+        out[4] = uph_r; out[5] = uph_i; out[6] = vph_r; out[7] = vph_i;
     }
 }
 
-// tot += sum over work-groups of the step's partials (fixed order), with
-// Kahan compensation.  One thread per value.
-__global__ void k_ffdft_sum(FfDft p, int nblk) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nv = (long long)p.n_ang * p.n_freq * 4;
-    if (v >= nv) return;
-    double s = 0.0;
-    for (int q = 0; q < nblk; ++q) s += p.part[(long long)q * nv + v];
-    const double y = s - p.comp[v];
-    const double t = p.tot[v] + y;
-    p.comp[v] = (t - p.tot[v]) - y;
-    p.tot[v] = t;
+// One work-group of 256 per (direction, frequency) pair: the tiles' sums in
+// a fixed order (strided per thread, then a tree), the step's time factors,
+// and the Kahan-compensated totals.
+__global__ void __launch_bounds__(256) k_ffdft_sum(FfDft p, int ntile, const int* step) {
+    __shared__ double red[8][256];
+    const int pr = blockIdx.x, tid = threadIdx.x;
+    double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = tid; t < ntile; t += 256) {
+        const double* in = p.part + ((long long)pr * ntile + t) * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) u[e] += in[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) red[e][tid] = u[e];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) red[e][tid] += red[e][tid + w];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) u[e] = red[e][0];
+    const int n = *step;
+    const double w = p.w0 + (pr % p.n_freq) * p.dw;
+    double sM, cM, sJ, cJ;
+    sincos(-w * (n * p.dt), &sM, &cM);
+    sincos(-w * ((n - 0.5) * p.dt), &sJ, &cJ);
+    // S_theta = e_M U_theta + e_J V_theta,  S_phi = e_M U_phi - e_J V_phi
+    const double add[4] = {cM * u[0] - sM * u[1] + (cJ * u[2] - sJ * u[3]),
+                           cM * u[1] + sM * u[0] + (cJ * u[3] + sJ * u[2]),
+                           cM * u[4] - sM * u[5] - (cJ * u[6] - sJ * u[7]),
+                           cM * u[5] + sM * u[4] - (cJ * u[7] + sJ * u[6])};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const long long v = (long long)pr * 4 + e;
+        const double y = add[e] - p.comp[v];
+        const double t = p.tot[v] + y;
+        p.comp[v] = (t - p.tot[v]) - y;
+        p.tot[v] = t;
+    }
 }
 
 // Adjoint source of J = |F|^2 at step n: dJ/d(term) = 2 Re(conj(F) w),

@@ -635,9 +635,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     }
     if ((parts & kPartObjective) && ff_on) {
         const dim3 gf(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
-        k_ffdft<<<gf, 256, 0, s>>>(g, ff, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
-        const long long nv = (long long)ff.n_ang * ff.n_freq * 4;
-        k_ffdft_sum<<<cdiv(nv, 256), 256, 0, s>>>(ff, (int)(gf.x * gf.y * gf.z));
+        k_ffdft<<<gf, 256, 0, s>>>(g, ff, Ex, Ey, Ez, Hx, Hy, Hz);
+        k_ffdft_sum<<<ff.n_ang * ff.n_freq, 256, 0, s>>>(ff, (int)(gf.x * gf.y * gf.z), d_step);
     }
     if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
         k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
@@ -1416,14 +1415,14 @@ aem_status aem_ffdft_set(aem_sim* s, int32_t n_ang, const float* theta_deg, cons
     return aem::guard([&]() {
         FfDft& p = s->ff;
         for (int a = 0; a < 3; ++a) { p.lo[a] = v.ff_lo[a]; p.hi[a] = v.ff_hi[a]; p.pbc[a] = v.pbc[a]; }
-        p.ic = v.ic; p.jc = v.jc; p.kc = v.kc;
         p.mu = s->ntff_mu; p.mv = s->ntff_mv;
         p.n_ang = n_ang; p.n_freq = n_freq;
-        p.d[0] = v.dx; p.d[1] = v.dy; p.d[2] = v.dz;
         p.dt = v.dt;
         p.w0 = 2.0 * M_PI * f0_hz; p.dw = 2.0 * M_PI * df_hz;
-        p.k0 = p.w0 / 299792458.0; p.dk = p.dw / 299792458.0;
         p.eta = std::sqrt((double)aem::kMu0 / (double)aem::kEp0);
+        const double d[3] = {v.dx, v.dy, v.dz};
+        const int ctr[3] = {v.ic, v.jc, v.kc};
+        const int npair = n_ang * n_freq;
         std::vector<double> dir(9 * (size_t)n_ang);
         for (int a = 0; a < n_ang; ++a) {
             const double th = theta_deg[a] * M_PI / 180.0, ph = phi_deg[a] * M_PI / 180.0;
@@ -1431,12 +1430,48 @@ aem_status aem_ffdft_set(aem_sim* s, int32_t n_ang, const float* theta_deg, cons
             const double d9[9] = {st * cp, st * sp, ct, ct * cp, ct * sp, -st, -sp, cp, 0.0};
             std::copy(d9, d9 + 9, dir.begin() + 9 * a);
         }
+        auto kq = [&](int pr) { return (p.w0 + (pr % n_freq) * p.dw) / 299792458.0; };
+        // per face: the spatial factor's step along u (axis b) and v (axis c)
+        std::vector<double> stp((size_t)6 * npair * 4);
+        for (int face = 0; face < 6; ++face) {
+            const int A = face / 2, b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
+            for (int pr = 0; pr < npair; ++pr) {
+                const double* rh = &dir[9 * (pr / n_freq)];
+                const double k = kq(pr), xu = k * rh[b] * d[b], xv = k * rh[c] * d[c];
+                double* o = &stp[((size_t)face * npair + pr) * 4];
+                o[0] = std::cos(xu); o[1] = std::sin(xu); o[2] = std::cos(xv); o[3] = std::sin(xv);
+            }
+        }
+        // per tile: dS e^{jk r^.r'} at its first cell
+        const int gx = cdiv(p.mu, 32), gy = cdiv(p.mv, 8);
+        const size_t ntile = (size_t)gx * gy * 6;
+        std::vector<double> z0(ntile * npair * 2);
+        for (int face = 0; face < 6; ++face) {
+            const int A = face / 2, b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
+            const double dS = d[b] * d[c];
+            for (int by = 0; by < gy; ++by)
+                for (int bx = 0; bx < gx; ++bx) {
+                    int ijk[3];
+                    ijk[A] = (face & 1) ? p.hi[A] : p.lo[A];
+                    ijk[b] = p.lo[b] + bx * 32;
+                    ijk[c] = p.lo[c] + by * 8;
+                    const size_t tile = (size_t)bx + (size_t)gx * ((size_t)by + (size_t)gy * face);
+                    for (int pr = 0; pr < npair; ++pr) {
+                        const double* rh = &dir[9 * (pr / n_freq)];
+                        double x = 0.0;
+                        for (int q = 0; q < 3; ++q) x += rh[q] * (ijk[q] - ctr[q]) * d[q];
+                        const double phs = kq(pr) * x;
+                        z0[(tile * npair + pr) * 2] = dS * std::cos(phs);
+                        z0[(tile * npair + pr) * 2 + 1] = dS * std::sin(phs);
+                    }
+                }
+        }
         p.dir = s->up(dir.data(), dir.size());
-        const size_t nv = (size_t)n_ang * n_freq * 4;
-        const size_t nblk = (size_t)cdiv(p.mu, 32) * cdiv(p.mv, 8) * 6;
-        p.part = s->zeros<double>(nblk * nv);
-        p.tot = s->zeros<double>(nv);
-        p.comp = s->zeros<double>(nv);
+        p.step = s->up(stp.data(), stp.size());
+        p.z0 = s->up(z0.data(), z0.size());
+        p.part = s->zeros<double>(ntile * npair * 8);
+        p.tot = s->zeros<double>((size_t)npair * 4);
+        p.comp = s->zeros<double>((size_t)npair * 4);
         s->ff_on = true;
         HIPCHK(hipStreamSynchronize(s->st));
     });
@@ -1455,7 +1490,7 @@ aem_status aem_ffdft_get(aem_sim* s, double* out, int64_t n) {
         for (int a = 0; a < p.n_ang; ++a)
             for (int q = 0; q < p.n_freq; ++q) {
                 const size_t o = ((size_t)a * p.n_freq + q) * 4;
-                const double kq = (p.k0 + q * p.dk) / (4.0 * M_PI);
+                const double kq = (p.w0 + q * p.dw) / 299792458.0 / (4.0 * M_PI);
                 // r E_theta = -j k/(4 pi) S_theta,  r E_phi = j k/(4 pi) S_phi
                 out[o] = kq * S[o + 1];
                 out[o + 1] = -kq * S[o];
