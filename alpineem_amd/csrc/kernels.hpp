@@ -605,7 +605,24 @@ __device__ __forceinline__ float inc_fn(const Tfsf& p, float w, float xf, float 
 }
 
 // face: 0 = y faces, 1 = z faces, 2 = x faces.  (u,v) spans the face.
+__device__ __forceinline__ void h_tfsf_body(const Grid& g, const Tfsf& p, int face, float* Hx, float* Hy, float* Hz,
+                                            const int* step);
+
 __global__ void k_h_tfsf(Grid g, Tfsf p, int face, float* Hx, float* Hy, float* Hz, const int* step) {
+    h_tfsf_body(g, p, face, Hx, Hy, Hz, step);
+}
+
+// The three face pairs in one launch (blockIdx.z = face).  Their H updates
+// touch disjoint elements (each face's ranges stop one short of the others'
+// planes), so the order between faces does not matter.
+__global__ void k_h_tfsf3(Grid g, Tfsf p, float* Hx, float* Hy, float* Hz, const int* step) {
+    const int face = blockIdx.z;
+    if (p.pbc[face == 0 ? 1 : (face == 1 ? 2 : 0)]) return;
+    h_tfsf_body(g, p, face, Hx, Hy, Hz, step);
+}
+
+__device__ __forceinline__ void h_tfsf_body(const Grid& g, const Tfsf& p, int face, float* Hx, float* Hy, float* Hz,
+                                            const int* step) {
     const int n = *step;
     const float cf = (float)n - 1.0f;
     const int xl = p.lo[0], xh = p.hi[0], yl = p.lo[1], yh = p.hi[1], zl = p.lo[2], zh = p.hi[2];
@@ -1447,6 +1464,29 @@ __global__ void k_phase_centre(Grid g, int ic, int jc, int kc, int T, const floa
 }
 
 __global__ void k_step_inc(int* step) { *step = *step + 1; }
+
+// Phase-centre samples (when do_pc) and the step counter in one launch.
+__global__ void k_step_end(Grid g, int ic, int jc, int kc, int T, const float* Ex, const float* Ey, const float* Ez,
+                           const float* Hx, const float* Hy, const float* Hz, float* pc, int* step, int do_pc) {
+    const int n = *step;
+    if (do_pc) {
+        const long long o = off(g, ic, jc, kc);
+        float* P0 = pc;
+        float* P1 = pc + (long long)(T + 1);
+        float* P2 = pc + 2 * (long long)(T + 1);
+        float* P3 = pc + 3 * (long long)(T + 1);
+        float* P4 = pc + 4 * (long long)(T + 1);
+        float* P5 = pc + 5 * (long long)(T + 1);
+        P0[n - 1] = (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f;
+        P1[n - 1] = (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f;
+        P2[n - 1] = (Ez[o] + Ez[o + 1] + Ez[o + g.PX] + Ez[o + 1 + g.PX]) / 4.0f;
+        const float hx = (Hx[o] + Hx[o + 1]) / 4.0f, hy = (Hy[o] + Hy[o + g.PX]) / 4.0f,
+                    hz = (Hz[o] + Hz[o + g.plane]) / 4.0f;
+        P3[n - 1] = hx; P4[n - 1] = hy; P5[n - 1] = hz;
+        if (n < T) { P3[n] = hx; P4[n] = hy; P5[n] = hz; }
+    }
+    *step = n + 1;
+}
 
 // ---------------------------------------------------------------------------
 // Post-loop products on the device

@@ -530,9 +530,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         }
     if ((parts & kPartHPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Hx, Hy, Hz, d_step);
-        if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Hx, Hy, Hz, d_step);
-        if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Hx, Hy, Hz, d_step);
+        if (nx > 0 && ny > 0 && nz > 0)
+            k_h_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, tf, Hx, Hy, Hz,
+                                                                                                    d_step);
     }
     if ((parts & kPartHPort) && v.n_ports > 0) k_ports_h<<<1, 256, 0, s>>>(pa);
     if (parts & kPartHPost) {
@@ -638,9 +638,11 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         k_ffdft<<<gf, 256, 0, s>>>(g, ff, Ex, Ey, Ez, Hx, Hy, Hz);
         k_ffdft_sum<<<ff.n_ang * ff.n_freq, 256, 0, s>>>(ff, (int)(gf.x * gf.y * gf.z), d_step);
     }
-    if ((parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0)
+    const bool pc_on = (parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0;
+    if (parts & kPartStep)
+        k_step_end<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step, pc_on ? 1 : 0);
+    else if (pc_on)
         k_phase_centre<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step);
-    if (parts & kPartStep) k_step_inc<<<1, 1, 0, s>>>(d_step);
     if (mark) HIPCHK(hipEventRecord(mark[5], st));
 }
 
