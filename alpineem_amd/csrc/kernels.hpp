@@ -702,6 +702,98 @@ __global__ void k_e_tfsf(Grid g, Tfsf p, int face, float* Ex, float* Ey, float* 
     }
 }
 
+// The three E face pairs in one launch (blockIdx.z = face).  Unlike H, the
+// E corrections of two faces meet on the TF/SF box edges (Ex on the y/z
+// edges, Ez on x/y, Ey on x/z), where the reference adds the earlier face's
+// term first.  Here the thread of the earlier face adds both terms, in that
+// order, and the later face skips those elements, so every element gets the
+// reference's sum bit for bit.
+__global__ void k_e_tfsf3(Grid g, Tfsf p, float* Ex, float* Ey, float* Ez, const int* step) {
+    const int face = blockIdx.z;
+    const int n = *step;
+    const float cf = (float)n - 0.5f;
+    const int xl = p.lo[0], xh = p.hi[0], yl = p.lo[1], yh = p.hi[1], zl = p.lo[2], zh = p.hi[2];
+    const int px = p.pbc[0], py = p.pbc[1], pz = p.pbc[2];
+    const bool f0 = !py, f1 = !pz, f2 = !px;   // face pairs present: y, z, x
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y * blockDim.y + threadIdx.y;
+    // x-face terms (face 2) of Ez and Ey at the low (xl) or high (xh) face
+    auto ez_x = [&](int x, int j, int k) {
+        return x == xl ? -(p.kE[0] * inc_fn(p, p.WH[1], (float)xl - 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f)
+                       : p.kE[0] * inc_fn(p, p.WH[1], (float)xh + 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f;
+    };
+    auto ey_x = [&](int x, int j, int k) {
+        return x == xl ? p.kE[0] * inc_fn(p, p.WH[2], (float)xl - 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f
+                       : -(p.kE[0] * inc_fn(p, p.WH[2], (float)xh + 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f);
+    };
+    // z-face term (face 1) of Ex at the low (zl) or high (zh) face
+    auto ex_z = [&](int i, int j, int z) {
+        return z == zl ? p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zl - 0.5f, cf) * 1.0f
+                       : -(p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zh + 0.5f, cf) * 1.0f);
+    };
+    if (face == 0) {
+        if (!f0) return;
+        const int i = xl + a, k = zl + b;
+        if (k <= zh - 1 + pz && i >= xl + px && i <= xh) {
+            const bool e2 = f2 && (i == xl || i == xh);
+            long long o = off(g, i, yl, k);
+            float e = Ez[o] + p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yl - 0.5f, (float)k + 0.5f, cf) * 1.0f;
+            if (e2) e = e + ez_x(i, yl, k);
+            Ez[o] = e;
+            o = off(g, i, yh, k);
+            e = Ez[o] - p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yh + 0.5f, (float)k + 0.5f, cf) * 1.0f;
+            if (e2) e = e + ez_x(i, yh, k);
+            Ez[o] = e;
+        }
+        if (k >= zl + pz && k <= zh && i <= xh - 1 + px) {
+            const bool e1 = f1 && (k == zl || k == zh);
+            long long o = off(g, i, yl, k);
+            float e = Ex[o] - p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yl - 0.5f, (float)k, cf) * 1.0f;
+            if (e1) e = e + ex_z(i, yl, k);
+            Ex[o] = e;
+            o = off(g, i, yh, k);
+            e = Ex[o] + p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yh + 0.5f, (float)k, cf) * 1.0f;
+            if (e1) e = e + ex_z(i, yh, k);
+            Ex[o] = e;
+        }
+    } else if (face == 1) {
+        if (!f1) return;
+        const int i = xl + a, j = yl + b;
+        if (j <= yh - 1 + py && i >= xl + px && i <= xh) {
+            const bool e2 = f2 && (i == xl || i == xh);
+            long long o = off(g, i, j, zl);
+            float e = Ey[o] - p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zl - 0.5f, cf) * 1.0f;
+            if (e2) e = e + ey_x(i, j, zl);
+            Ey[o] = e;
+            o = off(g, i, j, zh);
+            e = Ey[o] + p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zh + 0.5f, cf) * 1.0f;
+            if (e2) e = e + ey_x(i, j, zh);
+            Ey[o] = e;
+        }
+        if (j >= yl + py && j <= yh && i <= xh - 1 + px && !(f0 && (j == yl || j == yh))) {
+            long long o = off(g, i, j, zl);
+            Ex[o] = Ex[o] + ex_z(i, j, zl);
+            o = off(g, i, j, zh);
+            Ex[o] = Ex[o] + ex_z(i, j, zh);
+        }
+    } else {
+        if (!f2) return;
+        const int j = yl + a, k = zl + b;
+        if (k <= zh - 1 + pz && j >= yl + py && j <= yh && !(f0 && (j == yl || j == yh))) {
+            long long o = off(g, xl, j, k);
+            Ez[o] = Ez[o] + ez_x(xl, j, k);
+            o = off(g, xh, j, k);
+            Ez[o] = Ez[o] + ez_x(xh, j, k);
+        }
+        if (k >= zl + pz && k <= zh && j <= yh - 1 + py && !(f1 && (k == zl || k == zh))) {
+            long long o = off(g, xl, j, k);
+            Ey[o] = Ey[o] + ey_x(xl, j, k);
+            o = off(g, xh, j, k);
+            Ey[o] = Ey[o] + ey_x(xh, j, k);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Lumped ports (f90:4800-5013 H phase, 6120-6283 E phase).  One work-group
 // handles every port in order; the 1-D line updates are spread over threads.

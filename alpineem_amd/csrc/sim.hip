@@ -530,9 +530,15 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         }
     if ((parts & kPartHPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (nx > 0 && ny > 0 && nz > 0)
-            k_h_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, tf, Hx, Hy, Hz,
-                                                                                                    d_step);
+        if (fuse_cpml) {   // fused path: the three face pairs in one launch
+            if (nx > 0 && ny > 0 && nz > 0)
+                k_h_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, tf, Hx, Hy,
+                                                                                                        Hz, d_step);
+        } else {           // one launch per face pair, in the reference's order
+            if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Hx, Hy, Hz, d_step);
+            if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Hx, Hy, Hz, d_step);
+            if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Hx, Hy, Hz, d_step);
+        }
     }
     if ((parts & kPartHPort) && v.n_ports > 0) k_ports_h<<<1, 256, 0, s>>>(pa);
     if (parts & kPartHPost) {
@@ -570,9 +576,15 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             if (db[c].nb > 0) k_drude<IdxT><<<cdiv(db[c].nb, 256), 256, 0, s>>>(g, db[c], c, v.n_poles, F[c], idx, d_tup);
     if ((parts & kPartEPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Ex, Ey, Ez, d_step);
-        if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Ex, Ey, Ez, d_step);
-        if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Ex, Ey, Ez, d_step);
+        if (fuse_cpml) {
+            if (nx > 0 && ny > 0 && nz > 0)
+                k_e_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, tf, Ex, Ey,
+                                                                                                        Ez, d_step);
+        } else {
+            if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Ex, Ey, Ez, d_step);
+            if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Ex, Ey, Ez, d_step);
+            if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Ex, Ey, Ez, d_step);
+        }
     }
     if ((parts & kPartEPort) && v.n_ports > 0) k_ports_e<<<1, 256, 0, s>>>(pa);
     if (parts & kPartEPost) {

@@ -123,7 +123,7 @@ enum {
 typedef struct aem_sim_options {
     int32_t ntff_binning;   /* AEM_NTFF_* */
     int32_t planes;         /* z planes marched per thread in the bulk kernels; 0 = default (4) */
-    int32_t no_cpml_fusion; /* 1: always run the separate CPML kernels after ports/TF/SF */
+    int32_t no_cpml_fusion; /* 1: the separate CPML kernels after ports/TF/SF and one TF/SF launch per face pair */
 } aem_sim_options;
 
 /* aem_sim_create with options (NULL = defaults). */

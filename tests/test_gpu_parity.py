@@ -129,12 +129,13 @@ def test_antenna_decks_bit_exact(runs, name):
         np.testing.assert_array_equal(sim.series("e_phi", a), g["e_phi"][a], err_msg=f"E_phi {a}")
 
 
-@pytest.mark.parametrize("name", ["c1_mini", "drude_slab", "master_ports"])
+@pytest.mark.parametrize("name", ["c1_mini", "drude_slab", "master_ports", "sphere_oblique", "c2_mini"])
 @pytest.mark.parametrize("planes", [0, 1, 7])
 def test_cpml_fusion_matches_separate_kernels(golden_dir, name, planes):
-    """Folding the CPML correction into the bulk kernels keeps the
-    reference's per-cell add order: fields equal the separate-kernel path,
-    for any number of z planes per bulk thread."""
+    """Folding the CPML correction into the bulk kernels, and the TF/SF face
+    pairs into one launch per phase, keeps the reference's per-element add
+    order: fields equal the separate-kernel path (one launch per CPML axis
+    and per TF/SF face pair), for any number of z planes per bulk thread."""
     deck = os.path.join(golden_dir, "decks", name + ".txt")
     fused = Simulation(Plan.from_inputs(deck), planes=planes)
     plain = Simulation(Plan.from_inputs(deck), fuse_cpml=False)
