@@ -64,6 +64,7 @@ SIGNATURES = [
     ("aem_last_run_ms", C.c_float, [C.c_void_p]),
     ("aem_time_phases", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_float), C.c_int32]),
     ("aem_core_cells", C.c_int64, [C.c_void_p]),
+    ("aem_single_pass", C.c_int32, [C.c_void_p]),
 ]
 
 _lib = None
@@ -103,7 +104,8 @@ NTFF_BINNING = {"ordered": 0, "atomic": 1, "histogram": 2, "off": 3}
 
 class SimOptions(C.Structure):
     """aem_sim_options (include/aem.h)."""
-    _fields_ = [("ntff_binning", C.c_int32), ("planes", C.c_int32), ("no_cpml_fusion", C.c_int32)]
+    _fields_ = [("ntff_binning", C.c_int32), ("planes", C.c_int32), ("no_cpml_fusion", C.c_int32),
+                ("no_single_pass", C.c_int32)]
 
 
 class PortView(C.Structure):

@@ -38,6 +38,7 @@ struct Cpml {
     const float* b[6];         // bh (H) or be (E)
     const float* c[6];         // ch or ce
     float* psi[6][2];          // per face, two components
+    float* pso[6][2];          // where a ping-pong bulk launch stores the new psi (x faces)
 };
 
 // Slab membership along one axis for H (layers 1..n1-1 and N+1-n2..N-1,
@@ -121,13 +122,15 @@ __device__ __forceinline__ void st_once(T v, T* p) { __builtin_nontemporal_store
 // of the box then add their CPML corrections in the reference's per-cell
 // order (x, then y, then z slab), reusing the loaded neighbours.
 // ---------------------------------------------------------------------------
-template <int C>
+template <int C, bool PP>
 __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx, int by, int bz,
                                             float* __restrict__ Hx, float* __restrict__ Hy, float* __restrict__ Hz,
-                                            const float* __restrict__ Ex, const float* __restrict__ Ey,
-                                            const float* __restrict__ Ez, const float* __restrict__ dhx,
-                                            const float* __restrict__ dhy, const float* __restrict__ dhz, float DB,
-                                            const Cpml& cp, float dx, float dy, float dz) {
+                                            float* __restrict__ Hxo, float* __restrict__ Hyo,
+                                            float* __restrict__ Hzo, const float* __restrict__ Ex,
+                                            const float* __restrict__ Ey, const float* __restrict__ Ez,
+                                            const float* __restrict__ dhx, const float* __restrict__ dhy,
+                                            const float* __restrict__ dhz, float DB, const Cpml& cp, float dx,
+                                            float dy, float dz) {
     const int i = b.i0 + bx * kTX + (int)threadIdx.x;
     const int j = b.j0 + by * kTY + (int)threadIdx.y;
     if (i >= b.i0 + b.ni || j >= b.j0 + b.nj) return;
@@ -158,10 +161,10 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
                 const long long po = (long long)(qx - 1) + (long long)ntf * ((long long)(j - 1) + (long long)g.NY * (k - 1));
                 const float cb = cp.b[f][qx - 1], cc = cp.c[f][qx - 1];
                 float p0 = cb * cp.psi[f][0][po] + cc * (ez_xp - ez) / dx;
-                cp.psi[f][0][po] = p0;
+                (PP ? cp.pso[f][0] : cp.psi[f][0])[po] = p0;
                 hy = hy + DB * p0;
                 float p1 = cb * cp.psi[f][1][po] + cc * (ey_k - ey_xp) / dx;
-                cp.psi[f][1][po] = p1;
+                (PP ? cp.pso[f][1] : cp.psi[f][1])[po] = p1;
                 hz = hz + DB * p1;
             }
             if ((C & kCy) && fy >= 0) {                      // psi_Hxy, psi_Hzy
@@ -170,10 +173,10 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
                 const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(qy - 1) + (long long)ntf * (k - 1));
                 const float cb = cp.b[f][qy - 1], cc = cp.c[f][qy - 1];
                 float p0 = cb * cp.psi[f][0][po] + cc * (ez - ez_yp) / dy;
-                cp.psi[f][0][po] = p0;
+                (PP ? cp.pso[f][0] : cp.psi[f][0])[po] = p0;
                 hx = hx + DB * p0;
                 float p1 = cb * cp.psi[f][1][po] + cc * (ex_yp - ex_k) / dy;
-                cp.psi[f][1][po] = p1;
+                (PP ? cp.pso[f][1] : cp.psi[f][1])[po] = p1;
                 hz = hz + DB * p1;
             }
             int qz = 0;
@@ -183,16 +186,16 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
                 const long long po = (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (qz - 1));
                 const float cb = cp.b[f][qz - 1], cc = cp.c[f][qz - 1];
                 float p0 = cb * cp.psi[f][0][po] + cc * (ey_k1 - ey_k) / dz;
-                cp.psi[f][0][po] = p0;
+                (PP ? cp.pso[f][0] : cp.psi[f][0])[po] = p0;
                 hx = hx + DB * p0;
                 float p1 = cb * cp.psi[f][1][po] + cc * (ex_k - ex_k1) / dz;
-                cp.psi[f][1][po] = p1;
+                (PP ? cp.pso[f][1] : cp.psi[f][1])[po] = p1;
                 hy = hy + DB * p1;
             }
         }
-        st_once(hx, &Hx[o]);
-        st_once(hy, &Hy[o]);
-        st_once(hz, &Hz[o]);
+        st_once(hx, PP ? &Hxo[o] : &Hx[o]);
+        st_once(hy, PP ? &Hyo[o] : &Hy[o]);
+        st_once(hz, PP ? &Hzo[o] : &Hz[o]);
         ey_k = ey_k1;
         ex_k = ex_k1;
         o += g.plane;
@@ -204,10 +207,11 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
 // Component ranges: Ex i>=1,j>=2,k>=2; Ey i>=2,j>=1,k>=2; Ez i>=2,j>=2,k>=1.
 // Slab cells of the box append their CPML corrections as in k_h_bulk.
 // ---------------------------------------------------------------------------
-template <typename IdxT, int C>
+template <typename IdxT, int C, bool PP>
 __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx, int by, int bz,
                                             float* __restrict__ Ex, float* __restrict__ Ey, float* __restrict__ Ez,
-                                            const float* __restrict__ Hx, const float* __restrict__ Hy,
+                                            float* __restrict__ Exo, float* __restrict__ Eyo,
+                                            float* __restrict__ Ezo, const float* __restrict__ Hx, const float* __restrict__ Hy,
                                             const float* __restrict__ Hz, const IdxT* __restrict__ idx,
                                             const float* __restrict__ tup, const float* __restrict__ dex,
                                             const float* __restrict__ dey, const float* __restrict__ dez,
@@ -258,12 +262,12 @@ __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx,
                 const float cb = cp.b[f][qx - 1], cc = cp.c[f][qx - 1];
                 if (k >= 2) {
                     float p = cb * cp.psi[f][0][po] + cc * (hz_xm - hz) / dx;
-                    cp.psi[f][0][po] = p;
+                    (PP ? cp.pso[f][0] : cp.psi[f][0])[po] = p;
                     ey = ey + gby * p;
                 }
                 if (jy) {
                     float p = cb * cp.psi[f][1][po] + cc * (hy - hy_xm) / dx;
-                    cp.psi[f][1][po] = p;
+                    (PP ? cp.pso[f][1] : cp.psi[f][1])[po] = p;
                     ez = ez + gbz * p;
                 }
             }
@@ -274,12 +278,12 @@ __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx,
                 const float cb = cp.b[f][qy - 1], cc = cp.c[f][qy - 1];
                 if (k >= 2) {
                     float p = cb * cp.psi[f][0][po] + cc * (hz - hz_ym) / dy;
-                    cp.psi[f][0][po] = p;
+                    (PP ? cp.pso[f][0] : cp.psi[f][0])[po] = p;
                     ex = ex + gbx * p;
                 }
                 if (ix) {
                     float p = cb * cp.psi[f][1][po] + cc * (hx_ym - hx) / dy;
-                    cp.psi[f][1][po] = p;
+                    (PP ? cp.pso[f][1] : cp.psi[f][1])[po] = p;
                     ez = ez + gbz * p;
                 }
             }
@@ -291,21 +295,21 @@ __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx,
                 const float cb = cp.b[f][qz - 1], cc = cp.c[f][qz - 1];
                 if (jy) {
                     float p = cb * cp.psi[f][0][po] + cc * (hy_km - hy) / dz;
-                    cp.psi[f][0][po] = p;
+                    (PP ? cp.pso[f][0] : cp.psi[f][0])[po] = p;
                     ex = ex + gbx * p;
                 }
                 if (ix) {
                     float p = cb * cp.psi[f][1][po] + cc * (hx - hx_km) / dz;
-                    cp.psi[f][1][po] = p;
+                    (PP ? cp.pso[f][1] : cp.psi[f][1])[po] = p;
                     ey = ey + gby * p;
                 }
             }
         }
         // components outside the reference's update ranges get their own
         // value back; storing every component keeps the loads unconditional
-        st_once(ex_on ? ex : ex0, &Ex[o]);
-        st_once(ey_on ? ey : ey0, &Ey[o]);
-        st_once(ez_on ? ez : ez0, &Ez[o]);
+        st_once(ex_on ? ex : ex0, PP ? &Exo[o] : &Ex[o]);
+        st_once(ey_on ? ey : ey0, PP ? &Eyo[o] : &Ey[o]);
+        st_once(ez_on ? ez : ez0, PP ? &Ezo[o] : &Ez[o]);
         hx_km = hx;
         hy_km = hy;
         o += g.plane;
@@ -322,6 +326,7 @@ __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx,
 struct HArgs {
     Grid g;
     float *Hx, *Hy, *Hz;
+    float *Hxo, *Hyo, *Hzo;   // new H of a ping-pong launch (unused in place)
     const float *Ex, *Ey, *Ez;
     const float *dhx, *dhy, *dhz;
     float DB;
@@ -332,6 +337,7 @@ struct HArgs {
 struct EArgs {
     Grid g;
     float *Ex, *Ey, *Ez;
+    float *Exo, *Eyo, *Ezo;   // new E of a ping-pong launch (unused in place)
     const float *Hx, *Hy, *Hz;
     const void* idx;   // uint8_t or uint16_t per the kernel's IdxT
     const float* tup;
@@ -365,12 +371,12 @@ __device__ __forceinline__ BatchEntry entry_of(const BatchEntry* __restrict__ e,
     return r;
 }
 
-template <int C>
+template <int C, bool PP = false>
 __global__ void __launch_bounds__(kTX * kTY) k_h_bulk(HArgs a, BoxSet bs) {
     int bx, by, bz;
     const Box b = box_of(bs, bx, by, bz);
-    h_bulk_body<C>(a.g, b, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy, a.dhz, a.DB, a.cp, a.dx,
-                   a.dy, a.dz);
+    h_bulk_body<C, PP>(a.g, b, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Hxo, a.Hyo, a.Hzo, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy,
+                       a.dhz, a.DB, a.cp, a.dx, a.dy, a.dz);
 }
 
 template <int C>
@@ -379,16 +385,16 @@ __global__ void __launch_bounds__(kTX * kTY) k_h_bulk_batch(const HArgs* __restr
     int bx, by, bz;
     const BatchEntry e = entry_of(ents, n_ents, bx, by, bz);
     const HArgs& a = args[e.member];
-    h_bulk_body<C>(a.g, e.box, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy, a.dhz, a.DB, a.cp,
-                   a.dx, a.dy, a.dz);
+    h_bulk_body<C, false>(a.g, e.box, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy,
+                          a.dhz, a.DB, a.cp, a.dx, a.dy, a.dz);
 }
 
-template <typename IdxT, int C>
+template <typename IdxT, int C, bool PP = false>
 __global__ void __launch_bounds__(kTX * kTY) k_e_bulk(EArgs a, BoxSet bs) {
     int bx, by, bz;
     const Box b = box_of(bs, bx, by, bz);
-    e_bulk_body<IdxT, C>(a.g, b, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz, static_cast<const IdxT*>(a.idx),
-                         a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
+    e_bulk_body<IdxT, C, PP>(a.g, b, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Exo, a.Eyo, a.Ezo, a.Hx, a.Hy, a.Hz,
+                             static_cast<const IdxT*>(a.idx), a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
 }
 
 template <typename IdxT, int C>
@@ -397,8 +403,262 @@ __global__ void __launch_bounds__(kTX * kTY) k_e_bulk_batch(const EArgs* __restr
     int bx, by, bz;
     const BatchEntry e = entry_of(ents, n_ents, bx, by, bz);
     const EArgs& a = args[e.member];
-    e_bulk_body<IdxT, C>(a.g, e.box, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz,
-                         static_cast<const IdxT*>(a.idx), a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
+    e_bulk_body<IdxT, C, false>(a.g, e.box, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz,
+                                static_cast<const IdxT*>(a.idx), a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass step of the core box: the H and the E update of one leapfrog
+// step in one launch, from the fields at the start of the step (E, H and the
+// x-face psi) into a second set of buffers.  In the core box nothing touches
+// a cell between its bulk H and bulk E updates when the deck has no sheets,
+// TF/SF, PBC or Drude media (ports only read H there), so E(k) can follow
+// H(k) at once: each field is read and written once per step instead of
+// twice read and once written per phase (49 instead of 73 B per cell).
+//
+// Every wave is independent (no LDS, no barrier): it owns a 64-cell x row at
+// one y and marches b.kc planes.  The E update needs new H from outside the
+// wave: the x neighbour comes from the lane below (a lane shuffle), lane 0
+// forms Hy, Hz of the cell i0-1 itself; the row j-1 (Hx, Hz) is formed by
+// every lane for its column; Hx, Hy of the plane k0-1 before the first
+// plane.  A value formed again this way comes from the step-start state by
+// the same expression as its owner's, so it is bit-identical; only the owner
+// stores H and the new psi.  The y and z CPML slabs (outside the core box)
+// run after this launch as ping-pong bulk launches (k_h_bulk / k_e_bulk
+// with PP).
+// ---------------------------------------------------------------------------
+constexpr int kSY = 4;   // rows (waves) per work-group
+
+struct SpArgs {
+    Grid g;
+    const float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;   // fields at the start of the step
+    float *Exo, *Eyo, *Ezo, *Hxo, *Hyo, *Hzo;   // fields at its end
+    const void* idx;                            // uint8_t or uint16_t per IdxT
+    const float* tup;
+    const float *dex, *dey, *dez, *dhx, *dhy, *dhz;
+    float DB;
+    Cpml cpH, cpE;                              // x faces: psi read, pso written
+    float dx, dy, dz;
+};
+
+// x-face CPML term of an H cell: slot 0 (psi_Hyx) or 1 (psi_Hzx), read only.
+__device__ __forceinline__ float psi_hx_read(const Cpml& cp, const Grid& g, int f, int q, int slot, int j, int k,
+                                             float diff, float dx) {
+    const uint32_t po = (uint32_t)(q - 1) + (uint32_t)(cp.n[f] - 1) * ((uint32_t)(j - 1) + (uint32_t)g.NY * (k - 1));
+    return cp.b[f][q - 1] * cp.psi[f][slot][po] + cp.c[f][q - 1] * diff / dx;
+}
+
+// Field access by 32-bit byte offset from a uniform base: one VGPR per
+// distinct offset, shared by all arrays (global loads with a scalar base).
+__device__ __forceinline__ float ldb(const float* p, uint32_t ob) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + ob);
+}
+__device__ __forceinline__ void stb(float v, float* p, uint32_t ob) {
+    __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(p) + ob));
+}
+
+template <typename IdxT, int C>
+__global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
+    const Grid& g = a.g;
+    const float* __restrict__ Ex = a.Ex;
+    const float* __restrict__ Ey = a.Ey;
+    const float* __restrict__ Ez = a.Ez;
+    const float* __restrict__ Hx = a.Hx;
+    const float* __restrict__ Hy = a.Hy;
+    const float* __restrict__ Hz = a.Hz;
+    const char* __restrict__ idx = static_cast<const char*>(a.idx);
+    const float* __restrict__ tup = a.tup;
+    const float DB = a.DB;
+    const int nbx = (b.ni + 63) / 64, nby = (b.nj + kSY - 1) / kSY;
+    // Work-groups are dealt to the 8 XCDs in turn; XCD x takes the x-th
+    // contiguous range of tiles (x fastest), so the tiles whose halo lines a
+    // work-group reads run on its own XCD, through its L2.
+    const int nb = (int)gridDim.x, xcd = (int)blockIdx.x % 8, rk = (int)blockIdx.x / 8;
+    const int l = xcd * (nb / 8) + min(xcd, nb % 8) + rk;
+    const int bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
+    const int lane = (int)threadIdx.x;
+    const int i0 = b.i0 + bx * 64;
+    const int imax = b.i0 + b.ni - 1;
+    const int j = b.j0 + by * kSY + (int)threadIdx.y;
+    if (j > b.j0 + b.nj - 1) return;          // a whole wave
+    const int i = i0 + lane;
+    const bool act = i <= imax;               // lanes past the box compute a copy of the last cell
+    const int ic = min(i, imax);
+    const int k0 = b.k0 + bz * b.kc;
+    const int k1 = min(k0 + b.kc, b.k0 + b.nk) - 1;
+    const bool ix = ic >= 2, jy = j >= 2;
+    const int jm = jy ? j - 1 : j;            // row j-1 (its values are used only when jy)
+    const bool x0 = lane == 0 && i0 >= 2;     // lane 0 takes the x neighbour from the cell i0-1
+    const int ih = max(i0 - 1, 1);            // that cell (same for the whole wave)
+    const float dxh = a.dhx[ic - 1], dyh = a.dhy[j - 1], dyhm = a.dhy[jm - 1];
+    const float dxe = a.dex[ic - 1], dye = a.dey[j - 1], dxc = a.dhx[ih - 1];
+    const uint32_t NY = (uint32_t)g.NY;
+    // x-slab CPML of the cell (H and E) and of the cell i0-1: coefficients
+    // and psi row pointers, fixed over the march; psi index per plane
+    // n*((j-1) + NY*(k-1)) (+ the layer)
+    int qxh = 0, qxe = 0, txe = 0, qxc = 0;
+    const int fxh = (C & kCx) ? slab_h(ic, g.NX, a.cpH.n[0], a.cpH.n[1], qxh) : -1;
+    const int fxe = (C & kCx) ? slab_e(ic, g.NX, a.cpE.n[0], a.cpE.n[1], qxe, txe) : -1;
+    const int fxc = (C & kCx) ? slab_h(ih, g.NX, a.cpH.n[0], a.cpH.n[1], qxc) : -1;
+    float cbh = 0.f, cch = 0.f, cbe = 0.f, cce = 0.f, cbc = 0.f, ccc = 0.f;
+    const float *ph0 = nullptr, *ph1 = nullptr, *pe0 = nullptr, *pe1 = nullptr, *pc0 = nullptr, *pc1 = nullptr;
+    float *qh0 = nullptr, *qh1 = nullptr, *qe0 = nullptr, *qe1 = nullptr;
+    uint32_t nth = 0, nte = 0, ntc = 0;
+    if ((C & kCx) && fxh >= 0) {
+        const int f = fxh;
+        nth = (uint32_t)(a.cpH.n[f] - 1);
+        cbh = a.cpH.b[f][qxh - 1];
+        cch = a.cpH.c[f][qxh - 1];
+        ph0 = a.cpH.psi[f][0] + (qxh - 1);
+        ph1 = a.cpH.psi[f][1] + (qxh - 1);
+        qh0 = a.cpH.pso[f][0] + (qxh - 1);
+        qh1 = a.cpH.pso[f][1] + (qxh - 1);
+    }
+    if ((C & kCx) && fxe >= 0) {
+        const int f = fxe;
+        nte = (uint32_t)(a.cpE.n[f] - 1);
+        cbe = a.cpE.b[f][qxe - 1];
+        cce = a.cpE.c[f][qxe - 1];
+        pe0 = a.cpE.psi[f][0] + txe;
+        pe1 = a.cpE.psi[f][1] + txe;
+        qe0 = a.cpE.pso[f][0] + txe;
+        qe1 = a.cpE.pso[f][1] + txe;
+    }
+    if ((C & kCx) && fxc >= 0) {
+        const int f = fxc;
+        ntc = (uint32_t)(a.cpH.n[f] - 1);
+        cbc = a.cpH.b[f][qxc - 1];
+        ccc = a.cpH.c[f][qxc - 1];
+        pc0 = a.cpH.psi[f][0] + (qxc - 1);
+        pc1 = a.cpH.psi[f][1] + (qxc - 1);
+    }
+    uint32_t prow = (uint32_t)(j - 1) + NY * (uint32_t)(k0 - 1);   // (j-1) + NY*(k-1)
+    const uint32_t drow = jy ? 1u : 0u;                             // row j-1
+    // byte offsets (the host admits grids whose arrays stay under 4 GiB)
+    const uint32_t PB = (uint32_t)g.PX * 4u, ZB = (uint32_t)g.plane * 4u;
+    const uint32_t cell = (uint32_t)off(g, ic, j, k0);
+    uint32_t ob = cell * 4u;
+    const uint32_t dm = jy ? PB : 0u;         // ob - dm: the row j-1
+    uint32_t oc = (uint32_t)off(g, ih, j, k0) * 4u;
+    float exk = ldb(Ex, ob), eyk = ldb(Ey, ob);   // E(k) of the cell, carried
+    float eym = ldb(Ey, ob - dm);                 // Ey(k) of the row j-1, carried
+    float exc = ldb(Ex, oc);                      // Ex(k) of the cell i0-1, carried
+    float hxm = 0.f, hym = 0.f;                   // new Hx, Hy of the cell at k-1
+    if (k0 >= 2) {                                // Hx, Hy of the plane below the chunk
+        const uint32_t om = ob - ZB;
+        const float eyb = ldb(Ey, om), exb = ldb(Ex, om);
+        const float ez = ldb(Ez, om), ez_xp = ldb(Ez, om + 4u), ez_yp = ldb(Ez, om + PB);
+        const float dzm = a.dhz[k0 - 2];
+        hxm = ldb(Hx, om) + DB * ((ez - ez_yp) * dyh + (eyk - eyb) * dzm);
+        hym = ldb(Hy, om) + DB * ((ez_xp - ez) * dxh + (exb - exk) * dzm);
+        if ((C & kCx) && fxh >= 0) hym = hym + DB * (cbh * ph0[nth * (prow - NY)] + cch * (ez_xp - ez) / a.dx);
+    }
+    // coefficient tuples one plane ahead, their indices two planes ahead
+    const uint32_t IB = (uint32_t)sizeof(IdxT), ZI = (uint32_t)g.plane * IB;
+    uint32_t oi = cell * IB;
+    int t = (int)*reinterpret_cast<const IdxT*>(idx + oi) * 6;
+    if (k0 < k1) oi += ZI;
+    int tn = (int)*reinterpret_cast<const IdxT*>(idx + oi) * 6;
+    float gax = tup[t + 0], gbx = tup[t + 1], gay = tup[t + 2], gby = tup[t + 3], gaz = tup[t + 4], gbz = tup[t + 5];
+    for (int k = k0; k <= k1; ++k) {
+        const float dzh = a.dhz[k - 1], dze = a.dez[k - 1];
+        // every load of the plane first: the cell, its +x/+y/+z neighbours,
+        // the row j-1, the cell i0-1 (one address per wave), psi, the next
+        // plane's tuple and index
+        const float eyk1 = ldb(Ey, ob + ZB), exk1 = ldb(Ex, ob + ZB);
+        const float ez = ldb(Ez, ob), ez_xp = ldb(Ez, ob + 4u), ey_xp = ldb(Ey, ob + 4u);
+        const float ez_yp = ldb(Ez, ob + PB), ex_yp = ldb(Ex, ob + PB);
+        const float hx0 = ldb(Hx, ob), hy0 = ldb(Hy, ob), hz0 = ldb(Hz, ob);
+        const uint32_t om = ob - dm;
+        const float ezm = ldb(Ez, om), eym1 = ldb(Ey, om + ZB), eym_xp = ldb(Ey, om + 4u), exm = ldb(Ex, om);
+        const float hxm0 = ldb(Hx, om), hzm0 = ldb(Hz, om);
+        const float eyc = ldb(Ey, oc), ezc = ldb(Ez, oc), exc_yp = ldb(Ex, oc + PB), exc1 = ldb(Ex, oc + ZB);
+        const float hyc0 = ldb(Hy, oc), hzc0 = ldb(Hz, oc);
+        float sh0 = 0.f, sh1 = 0.f, sr1 = 0.f, se0 = 0.f, se1 = 0.f, sc0 = 0.f, sc1 = 0.f;
+        if ((C & kCx) && fxh >= 0) {
+            sh0 = ph0[nth * prow];
+            sh1 = ph1[nth * prow];
+            sr1 = ph1[nth * (prow - drow)];
+        }
+        if ((C & kCx) && fxe >= 0) {
+            se0 = pe0[nte * prow];
+            se1 = pe1[nte * prow];
+        }
+        if ((C & kCx) && fxc >= 0) {
+            sc0 = pc0[ntc * prow];
+            sc1 = pc1[ntc * prow];
+        }
+        const float nax = tup[tn + 0], nbx_ = tup[tn + 1], nay = tup[tn + 2], nby_ = tup[tn + 3], naz = tup[tn + 4],
+                    nbz = tup[tn + 5];
+        if (k + 1 < k1) oi += ZI;
+        const int tnn = (int)*reinterpret_cast<const IdxT*>(idx + oi) * 6;
+        // new H of the cell
+        float hx = hx0 + DB * ((ez - ez_yp) * dyh + (eyk1 - eyk) * dzh);
+        float hy = hy0 + DB * ((ez_xp - ez) * dxh + (exk - exk1) * dzh);
+        float hz = hz0 + DB * ((eyk - ey_xp) * dxh + (ex_yp - exk) * dyh);
+        if ((C & kCx) && fxh >= 0) {                          // psi_Hyx, psi_Hzx
+            const float p0 = cbh * sh0 + cch * (ez_xp - ez) / a.dx;
+            const float p1 = cbh * sh1 + cch * (eyk - ey_xp) / a.dx;
+            if (act) {
+                qh0[nth * prow] = p0;
+                qh1[nth * prow] = p1;
+            }
+            hy = hy + DB * p0;
+            hz = hz + DB * p1;
+        }
+        if (act) {
+            stb(hx, a.Hxo, ob);
+            stb(hy, a.Hyo, ob);
+            stb(hz, a.Hzo, ob);
+        }
+        // new Hx, Hz of the row j-1 (its +y neighbours are this row's E)
+        const float hxr = hxm0 + DB * ((ezm - ez) * dyhm + (eym1 - eym) * dzh);
+        float hzr = hzm0 + DB * ((eym - eym_xp) * dxh + (exk - exm) * dyhm);
+        if ((C & kCx) && fxh >= 0) hzr = hzr + DB * (cbh * sr1 + cch * (eym - eym_xp) / a.dx);
+        // new Hy, Hz of the cell i0-1 (its +x neighbour is lane 0's cell)
+        float hyc = hyc0 + DB * ((ez - ezc) * dxc + (exc - exc1) * dzh);
+        float hzc = hzc0 + DB * ((eyc - eyk) * dxc + (exc_yp - exc) * dyh);
+        if ((C & kCx) && fxc >= 0) {
+            hyc = hyc + DB * (cbc * sc0 + ccc * (ez - ezc) / a.dx);
+            hzc = hzc + DB * (cbc * sc1 + ccc * (eyc - eyk) / a.dx);
+        }
+        // x neighbour: the lane below, lane 0 the cell i0-1
+        const float sy_ = __shfl_up(hy, 1), sz_ = __shfl_up(hz, 1);
+        const float hy_xm = x0 ? hyc : sy_, hz_xm = x0 ? hzc : sz_;
+        // new E of the cell
+        float ex = gax * exk + gbx * ((hz - hzr) * dye + (hym - hy) * dze);
+        float ey = gay * eyk + gby * ((hz_xm - hz) * dxe + (hx - hxm) * dze);
+        float ezn = gaz * ez + gbz * ((hy - hy_xm) * dxe + (hxr - hx) * dye);
+        if ((C & kCx) && fxe >= 0) {                          // psi_Eyx, psi_Ezx
+            if (k >= 2) {
+                const float q = cbe * se0 + cce * (hz_xm - hz) / a.dx;
+                if (act) qe0[nte * prow] = q;
+                ey = ey + gby * q;
+            }
+            if (jy) {
+                const float q = cbe * se1 + cce * (hy - hy_xm) / a.dx;
+                if (act) qe1[nte * prow] = q;
+                ezn = ezn + gbz * q;
+            }
+        }
+        if (act) {
+            const bool ex_on = jy && k >= 2, ey_on = ix && k >= 2, ez_on = ix && jy;
+            stb(ex_on ? ex : exk, a.Exo, ob);
+            stb(ey_on ? ey : eyk, a.Eyo, ob);
+            stb(ez_on ? ezn : ez, a.Ezo, ob);
+        }
+        hxm = hx;
+        hym = hy;
+        exk = exk1;
+        eyk = eyk1;
+        eym = eym1;
+        exc = exc1;
+        gax = nax; gbx = nbx_; gay = nay; gby = nby_; gaz = naz; gbz = nbz;
+        tn = tnn;
+        ob += ZB;
+        oc += ZB;
+        prow += NY;
+    }
 }
 
 // ---------------------------------------------------------------------------

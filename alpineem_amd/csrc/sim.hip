@@ -93,6 +93,17 @@ bool cpml_fusable(const aem_plan_view& v) {
     return true;
 }
 
+// The single-pass step (k_step_sp) needs the CPML folded into the bulk and
+// nothing applied between the bulk H and bulk E updates of a core cell: no
+// sheets, TF/SF, PBC or Drude media.  Ports qualify: their H phase only reads
+// H (fdtd_solver.f90:4917-4957) and their E phase follows the bulk E.
+bool single_pass_ok(const aem_plan_view& v) {
+    if (v.has_tfsf || v.has_drude) return false;
+    for (int a = 0; a < 3; ++a)
+        if (v.pbc[a] || v.n_sheet_planes[a] > 0) return false;
+    return true;
+}
+
 // One bulk launch: a set of boxes and the slab tests compiled into its kernel.
 struct Launch {
     BoxSet bs;
@@ -105,7 +116,8 @@ struct Launch {
     }
 };
 
-constexpr int kPlanes = 4;   // default z planes marched by one thread of a bulk kernel
+constexpr int kPlanes = 4;     // default z planes marched by one thread of a bulk kernel
+constexpr int kSpPlanes = 16;  // default z planes marched by one column of the single-pass kernel
 
 // Parts of a step, in the order the reference applies them.
 enum : unsigned {
@@ -189,7 +201,8 @@ struct aem_sim {
     Grid g{};
     int* d_step = nullptr;
     int64_t steps_done = 0;
-    float* F[6] = {};
+    float* F[6] = {};              // fields now (Ex, Ey, Ez, Hx, Hy, Hz)
+    float* F2[6] = {};             // second field set of the single-pass step
     void* d_idx = nullptr;
     float* d_tup = nullptr;
     float* d_den[6] = {};
@@ -217,7 +230,22 @@ struct aem_sim {
     int ntff_mode = AEM_NTFF_ORDERED;
     int ntff_mu = 0, ntff_mv = 0;  // largest face extents of the far-field box
     int planes = kPlanes;
+    bool planes_set = false;       // planes given by the caller (also used by the single-pass kernel)
     bool allow_fuse = true;
+    // single-pass step: fields and x-face psi alternate between two buffer
+    // sets, so a step reads one set and writes the other
+    bool allow_sp = true;
+    bool sp_ok = false;            // the deck admits it (set at creation)
+    bool batched = false;          // a batch steps this simulation: in-place kernels only
+    Box sp_box{};                  // core box of the single-pass kernel
+    std::vector<std::pair<Box, int>> sp_parts;   // its launches: x tiles free of slab cells (no slab
+                                                 // test) and those holding x-slab cells (kCx)
+    int par = 0;                   // which buffer set holds the fields now
+    // k_adj_record reads E before its update, which the single pass has
+    // already written; a design box records inside k_e_design, after it
+    bool sp_active() const { return sp_ok && !batched && !(rec_on && !des_on); }
+    void flip();
+    void bind();
     // differentiation
     Objective obj{};
     bool obj_on = false;
@@ -239,6 +267,11 @@ struct aem_sim {
     void enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, double* dga, double* dgb);
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph2 = nullptr;   // single-pass steps: the step from the other buffer set
+    hipGraphExec_t gexec2 = nullptr;
+    int gpar = 0;                  // buffer set the first graph starts from
+    bool gsp = false;              // the graphs are single-pass steps
+    void drop_graphs();
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     bool finalized = false;
@@ -247,8 +280,7 @@ struct aem_sim {
 
     ~aem_sim() {
         if (st) (void)hipStreamSynchronize(st);
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
+        drop_graphs();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (d_dft) (void)hipFree(d_dft);
@@ -272,6 +304,8 @@ struct aem_sim {
     }
 
     void create(const aem_sim_options* opt);
+    // one full step; a single-pass step leaves the fields in the other buffer
+    // set and flips the host's pointers to it
     void enqueue_step(hipEvent_t* mark = nullptr) { enqueue_parts(mark, st, kPartAll); }
     void enqueue_parts(hipEvent_t* mark, hipStream_t s, unsigned parts);
     template <class IdxT>
@@ -282,8 +316,12 @@ void aem_sim::create(const aem_sim_options* opt) {
     const aem_plan_view& v = plan->v;
     if (opt) {
         ntff_mode = opt->ntff_binning;
-        if (opt->planes > 0) planes = opt->planes;
+        if (opt->planes > 0) {
+            planes = opt->planes;
+            planes_set = true;
+        }
         allow_fuse = opt->no_cpml_fusion == 0;
+        allow_sp = opt->no_single_pass == 0;
     }
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
@@ -474,18 +512,76 @@ void aem_sim::create(const aem_sim_options* opt) {
 
     fuse_cpml = allow_fuse && cpml_fusable(v);
     bulk = bulk_launches(v, fuse_cpml, planes);
-    ha = HArgs{g, F[3], F[4], F[5], F[0], F[1], F[2], d_den[3], d_den[4], d_den[5], v.DB, cpH, v.dx, v.dy, v.dz};
-    ea = EArgs{g, F[0], F[1], F[2], F[3], F[4], F[5], d_idx, d_tup, d_den[0], d_den[1], d_den[2], cpE,
-               v.dx, v.dy, v.dz};
-    pa = PortArgs{g, d_ports, v.n_ports, v.M, T, d_V, d_I, d_Vout, d_Iout, F[0], F[1], F[2], F[3], F[4], F[5],
-                  v.dx, v.dy, v.dz, pu, d_step};
+    for (int f = 0; f < 6; ++f)
+        for (int q = 0; q < 2; ++q) {
+            cpH.pso[f][q] = cpH.psi[f][q];
+            cpE.pso[f][q] = cpE.psi[f][q];
+        }
+    // the single-pass kernel takes the core launch's box (every x, y and z
+    // outside the y/z slabs); the slab launch stays as it is
+    // k_step_sp addresses the fields by 32-bit byte offsets
+    const bool small = ((unsigned long long)v.ncell + (unsigned long long)v.plane) * 4ull < (1ull << 32);
+    sp_ok = allow_sp && fuse_cpml && small && single_pass_ok(v) && !bulk.empty() && bulk[0].c == kCx &&
+            bulk[0].bs.n == 1;
+    if (sp_ok) {
+        sp_box = bulk[0].bs.b[0];
+        sp_box.kc = planes_set ? planes : kSpPlanes;
+        // one launch with the x-slab test compiled in: measured at 400^3,
+        // separate launches for the slab-free tiles (no slab test, fewer
+        // registers) and the two x-slab tiles of each row took 5% longer
+        sp_parts.push_back({sp_box, (int)kCx});
+        for (int q = 0; q < 6; ++q) F2[q] = zeros<float>(nc + v.plane);
+        for (int f = 0; f < 2; ++f) {
+            const size_t nt = (size_t)std::max(v.npml[f] - 1, 0), trans = (size_t)v.NY * v.NZ;
+            for (int q = 0; q < 2; ++q) {
+                cpH.pso[f][q] = zeros<float>(nt * trans);
+                cpE.pso[f][q] = zeros<float>(nt * trans);
+            }
+        }
+    }
     for (int a = 0; a < 3; ++a) {
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
         ntff_mu = std::max(ntff_mu, v.ff_hi[b] - v.ff_lo[b] + 1);
         ntff_mv = std::max(ntff_mv, v.ff_hi[c] - v.ff_lo[c] + 1);
     }
-    na = NtffArgs{g, nf, nb, nx_acc.win, F[0], F[1], F[2], F[3], F[4], F[5], d_step};
+    bind();
     HIPCHK(hipStreamSynchronize(st));
+}
+
+// Kernel argument blocks from the current field and psi pointers.  In place,
+// the bulk kernels read and write F and psi; ping-pong launches write F2 and
+// the x-face pso.
+void aem_sim::bind() {
+    const aem_plan_view& v = plan->v;
+    float* const* O = sp_ok ? F2 : F;
+    ha = HArgs{g, F[3], F[4], F[5], O[3], O[4], O[5], F[0], F[1], F[2], d_den[3], d_den[4], d_den[5], v.DB, cpH,
+               v.dx, v.dy, v.dz};
+    ea = EArgs{g, F[0], F[1], F[2], O[0], O[1], O[2], F[3], F[4], F[5], d_idx, d_tup, d_den[0], d_den[1], d_den[2],
+               cpE, v.dx, v.dy, v.dz};
+    pa = PortArgs{g, d_ports, v.n_ports, v.M, v.time_steps, d_V, d_I, d_Vout, d_Iout, F[0], F[1], F[2], F[3], F[4],
+                  F[5], v.dx, v.dy, v.dz, pu, d_step};
+    na = NtffArgs{g, nf, nb, nx_acc.win, F[0], F[1], F[2], F[3], F[4], F[5], d_step};
+}
+
+// After a single-pass step: the other buffer set now holds the fields.
+void aem_sim::flip() {
+    for (int q = 0; q < 6; ++q) std::swap(F[q], F2[q]);
+    for (int f = 0; f < 2; ++f)
+        for (int q = 0; q < 2; ++q) {
+            std::swap(cpH.psi[f][q], cpH.pso[f][q]);
+            std::swap(cpE.psi[f][q], cpE.pso[f][q]);
+        }
+    par ^= 1;
+    bind();
+}
+
+void aem_sim::drop_graphs() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (gexec2) (void)hipGraphExecDestroy(gexec2);
+    if (graph2) (void)hipGraphDestroy(graph2);
+    gexec = gexec2 = nullptr;
+    graph = graph2 = nullptr;
 }
 
 // Enqueue the selected parts of one step on stream s, in the reference's
@@ -515,12 +611,40 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         }
     };
 
-    // ---- H phase ----  (marks: [0,1] the core bulk launch, [1,2] the rest of H)
+    // single pass: the core box's H and E in one launch, then the y/z slab
+    // boxes' H and E as ping-pong bulk launches; the rest of the step then
+    // runs on the new buffer set
+    const bool sp = (parts & kPartHBulk) && (parts & kPartEBulk) && sp_active();
+
+    // ---- H phase ----  (marks: [0,1] the core bulk launch -- the single-pass
+    // launch when sp --, [1,2] the rest of H)
     if (mark) HIPCHK(hipEventRecord(mark[0], st));
-    if (parts & kPartHBulk) h_bulk(bulk[0]);
+    if (sp) {
+        const SpArgs sa{g, Ex, Ey, Ez, Hx, Hy, Hz, F2[0], F2[1], F2[2], F2[3], F2[4], F2[5], d_idx, d_tup,
+                        d_den[0], d_den[1], d_den[2], d_den[3], d_den[4], d_den[5], v.DB, cpH, cpE, v.dx, v.dy, v.dz};
+        for (const auto& part : sp_parts) {
+            const Box& b = part.first;
+            const unsigned nblk = cdiv(b.ni, 64) * cdiv(b.nj, kSY) * cdiv(b.nk, b.kc);
+            if (part.second == kCx) k_step_sp<IdxT, kCx><<<nblk, dim3(64, kSY), 0, s>>>(sa, b);
+            else k_step_sp<IdxT, 0><<<nblk, dim3(64, kSY), 0, s>>>(sa, b);
+        }
+    } else if (parts & kPartHBulk) {
+        h_bulk(bulk[0]);
+    }
     if (mark) HIPCHK(hipEventRecord(mark[1], st));
-    if (parts & kPartHBulk)
+    if (sp) {
+        EArgs es = ea;   // E of the slabs reads the new H
+        es.Hx = F2[3];
+        es.Hy = F2[4];
+        es.Hz = F2[5];
+        for (size_t q = 1; q < bulk.size(); ++q) k_h_bulk<kCx | kCy | kCz, true><<<bulk[q].blocks(), bt, 0, s>>>(ha, bulk[q].bs);
+        for (size_t q = 1; q < bulk.size(); ++q)
+            k_e_bulk<IdxT, kCx | kCy | kCz, true><<<bulk[q].blocks(), bt, 0, s>>>(es, bulk[q].bs);
+        flip();
+        Ex = F[0]; Ey = F[1]; Ez = F[2]; Hx = F[3]; Hy = F[4]; Hz = F[5];
+    } else if (parts & kPartHBulk) {
         for (size_t q = 1; q < bulk.size(); ++q) h_bulk(bulk[q]);
+    }
     const float dd[3] = {v.dx, v.dy, v.dz};
     for (int a = 0; a < 3; ++a)
         if ((parts & kPartHPre) && sh[a].np > 0) {
@@ -559,9 +683,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
 
     // ---- E phase ----  (marks: [2,3] the core bulk launch, [3,4] the rest of E)
     if (mark) HIPCHK(hipEventRecord(mark[2], st));
-    if (parts & kPartEBulk) e_bulk(bulk[0]);
+    if ((parts & kPartEBulk) && !sp) e_bulk(bulk[0]);
     if (mark) HIPCHK(hipEventRecord(mark[3], st));
-    if (parts & kPartEBulk)
+    if ((parts & kPartEBulk) && !sp)
         for (size_t q = 1; q < bulk.size(); ++q) e_bulk(bulk[q]);
     if ((parts & kPartEPre) && des_on)
         k_e_design<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2],
@@ -734,15 +858,29 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
     return aem::guard([&]() {
         if (n_steps == 0) return;
         if (use_graph && !s->gexec) {
+            // a single-pass step alternates buffer sets: one graph per set
+            const int p0 = s->par;
             HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
             s->enqueue_step();
             HIPCHK(hipStreamEndCapture(s->st, &s->graph));
             HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+            s->gsp = s->par != p0;
+            s->gpar = p0;
+            if (s->gsp) {
+                HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+                s->enqueue_step();
+                HIPCHK(hipStreamEndCapture(s->st, &s->graph2));
+                HIPCHK(hipGraphInstantiate(&s->gexec2, s->graph2, nullptr, nullptr, 0));
+            }
         }
         HIPCHK(hipEventRecord(s->ev0, s->st));
         for (int64_t q = 0; q < n_steps; ++q) {
-            if (use_graph) HIPCHK(hipGraphLaunch(s->gexec, s->st));
-            else s->enqueue_step();
+            if (use_graph) {
+                HIPCHK(hipGraphLaunch(s->par == s->gpar ? s->gexec : s->gexec2, s->st));
+                if (s->gsp) s->flip();
+            } else {
+                s->enqueue_step();
+            }
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(s->ev1, s->st));
@@ -760,6 +898,8 @@ int64_t aem_core_cells(const aem_sim* s) {
     if (!s || s->bulk.empty()) return -1;
     return (int64_t)s->bulk[0].cells();
 }
+
+int32_t aem_single_pass(const aem_sim* s) { return s ? (s->sp_active() ? 1 : 0) : -1; }
 
 float aem_last_run_ms(const aem_sim* s) { return s ? s->last_ms : 0.f; }
 
@@ -907,6 +1047,8 @@ aem_status aem_set_field(aem_sim* s, int32_t comp, const float* in, int64_t n) {
     }
     return aem::guard([&]() {
         HIPCHK(hipMemcpyAsync(s->F[comp], in, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s->st));
+        // cells no step writes keep their value in both buffer sets
+        if (s->F2[comp]) HIPCHK(hipMemcpyAsync(s->F2[comp], in, (size_t)n * sizeof(float), hipMemcpyHostToDevice, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
     });
 }
@@ -1169,7 +1311,12 @@ aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
     }
     b->m.assign(sims, sims + n);
     aem_status stt = aem::guard([&]() {
-        for (aem_sim* s : b->m) HIPCHK(hipStreamSynchronize(s->st));
+        for (aem_sim* s : b->m) {
+            HIPCHK(hipStreamSynchronize(s->st));
+            // the batch steps its members with in-place kernels from here on
+            s->batched = true;
+            s->drop_graphs();
+        }
         b->create();
     });
     if (stt != AEM_OK) {

@@ -80,11 +80,13 @@ class Simulation:
     sums bit for bit; "atomic" or "histogram": FP64 accumulation; "off": no
     time-domain far field, e.g. for objective-only runs), ``planes``
     the z planes each bulk thread marches, ``fuse_cpml=False`` forces the
-    separate CPML kernels (aem_sim_options in include/aem.h)."""
+    separate CPML kernels, ``single_pass=False`` separate H and E bulk
+    launches (aem_sim_options in include/aem.h)."""
 
-    def __init__(self, plan: Plan, ntff: str = "ordered", planes: int = 0, fuse_cpml: bool = True):
+    def __init__(self, plan: Plan, ntff: str = "ordered", planes: int = 0, fuse_cpml: bool = True,
+                 single_pass: bool = True):
         self.plan = plan
-        opt = _lib.SimOptions(_lib.NTFF_BINNING[ntff], int(planes), 0 if fuse_cpml else 1)
+        opt = _lib.SimOptions(_lib.NTFF_BINNING[ntff], int(planes), 0 if fuse_cpml else 1, 0 if single_pass else 1)
         h = C.c_void_p()
         check(lib().aem_sim_create_ex(plan.handle, C.byref(opt), C.byref(h)))
         self._h = h
@@ -115,6 +117,12 @@ class Simulation:
         """Cells updated by the core bulk launch (phases 0 and 2 of
         :meth:`time_phases`)."""
         return int(lib().aem_core_cells(self._h))
+
+    @property
+    def single_pass(self) -> bool:
+        """True if a step runs the core box as one single-pass launch (H and
+        E together, timed as phase 0 of :meth:`time_phases`)."""
+        return int(lib().aem_single_pass(self._h)) == 1
 
     @property
     def steps_done(self) -> int:

@@ -37,6 +37,7 @@ PEAK_HBM_GBS = 8000.0
 BYTES_H = 36          # read Hx,Hy,Hz,Ex,Ey,Ez + write Hx,Hy,Hz (FP32)
 BYTES_E = 37          # read Ex,Ey,Ez,Hx,Hy,Hz + 1-byte coefficient index, write Ex,Ey,Ez
 BYTES_STEP = BYTES_H + BYTES_E
+BYTES_SP = 49         # single pass: read Ex..Hz + 1-byte index, write Ex..Hz once per step
 
 
 def cpu_baseline(n: int, steps: int) -> dict:
@@ -249,12 +250,16 @@ def main() -> int:
     ms_step = wall / K * 1e3
     print(f"[bench] timed {K} steps: {ms_step:.4f} ms/step, {value:.2f} Gcell/s", file=sys.stderr, flush=True)
 
+    single = sim.single_pass                      # core box as one H+E launch (k_step_sp)
     ph = sim.time_phases(P)                       # ms per step per phase, HIP events on the sim stream
-    names = ["h_core", "h_rest", "e_core", "e_rest", "outputs"]
+    names = (["core_step", "rest_h", "core_e", "rest_e", "outputs"] if single
+             else ["h_core", "h_rest", "e_core", "e_rest", "outputs"])
     print("[bench] phases (ms): " + ", ".join(f"{a}={b:.4f}" for a, b in zip(names, ph)), file=sys.stderr, flush=True)
-    core = sim.core_cells                         # cells of the core bulk launch timed in phases 0 and 2
+    core = sim.core_cells                         # cells of the core launch timed in phase 0 (and 2 if split)
     c = 1 if core < cells else 0                  # fused CPML: the core kernel carries the x-slab test (kCx)
-    if ph[2] >= ph[0]:
+    if single:
+        dom, bpc, t_ms = f"k_step_sp<unsigned char, {c}>", BYTES_SP, ph[0]
+    elif ph[2] >= ph[0]:
         dom, bpc, t_ms = f"k_e_bulk<unsigned char, {c}>", BYTES_E, ph[2]
     else:
         dom, bpc, t_ms = f"k_h_bulk<{c}>", BYTES_H, ph[0]
@@ -262,7 +267,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_cell": bpc, "cells_per_launch": core, "launch_ms": round(float(t_ms), 5),
-                "step_frac": round(value * BYTES_STEP / PEAK_HBM_GBS, 4)}
+                "step_frac": round(value * (BYTES_SP if single else BYTES_STEP) / PEAK_HBM_GBS, 4)}
     traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(traffic_file):
         entry = json.load(open(traffic_file)).get(dom)

@@ -124,6 +124,7 @@ typedef struct aem_sim_options {
     int32_t ntff_binning;   /* AEM_NTFF_* */
     int32_t planes;         /* z planes marched per thread in the bulk kernels; 0 = default (4) */
     int32_t no_cpml_fusion; /* 1: the separate CPML kernels after ports/TF/SF and one TF/SF launch per face pair */
+    int32_t no_single_pass; /* 1: separate H and E bulk launches even where one single-pass launch qualifies */
 } aem_sim_options;
 
 /* aem_sim_create with options (NULL = defaults). */
@@ -290,7 +291,9 @@ float aem_last_run_ms(const aem_sim* sim);
  * events on the sim's stream between the five phases of a step and return
  * the average ms per phase in ms[0..4]: the core H bulk launch, the rest of
  * H (slab boxes, sheets, TF/SF, ports, separate CPML, PBC), the core E bulk
- * launch, the rest of E, per-step outputs. */
+ * launch, the rest of E, per-step outputs.  A single-pass step (see
+ * aem_single_pass) times its core launch as phase 0 and its slab launches
+ * in phase 1. */
 aem_status aem_time_phases(aem_sim* sim, int64_t n_steps, float* ms, int32_t n_ms);
 
 /* Instrumentation, no counterpart in the reference: the number of cells of
@@ -298,6 +301,15 @@ aem_status aem_time_phases(aem_sim* sim, int64_t n_steps, float* ms, int32_t n_m
  * free of CPML slabs when the CPML correction is fused, else the whole
  * updated grid).  -1 for a null handle. */
 int64_t aem_core_cells(const aem_sim* sim);
+
+/* Instrumentation, no counterpart in the reference: 1 if a full step of this
+ * sim runs its core box as one single-pass launch (H and E of the step,
+ * timed as phase 0 by aem_time_phases; phase 2 is then empty), 0 if as
+ * separate H and E bulk launches, -1 for a null handle.  The single pass
+ * needs a deck with the CPML folded into the bulk and no sheets, TF/SF, PBC
+ * or Drude media, and is off for batch members and for material-gradient
+ * records (aem_adjoint_record without a design box). */
+int32_t aem_single_pass(const aem_sim* sim);
 
 #ifdef __cplusplus
 }

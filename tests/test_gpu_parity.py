@@ -145,6 +145,66 @@ def test_cpml_fusion_matches_separate_kernels(golden_dir, name, planes):
         np.testing.assert_array_equal(fused.field(comp), plain.field(comp), err_msg=comp)
 
 
+def single_pass_deck(cells=(70, 45, 39), steps=160):
+    """Dipole with a port, a dielectric sphere and an anisotropic block in a
+    CPML corner on a grid whose x extent spans two 64-cell tiles, the tile
+    seam (x = 64) lying in the upper x slab."""
+    from alpineem_amd import decks
+    nx, ny, nz = cells
+    c = (nx // 2, ny // 2, nz // 2)
+    d = decks.dipole(n=40, steps=steps, arm=6, angles=3)
+    d.cells = cells
+    d.phase_centre = c
+    d.materials = [decks.Material(4, sigma=(1e8, 1e8, 1e8)), decks.Material(5, eps=(2.5, 3.0, 4.0)),
+                   decks.Material(6, eps=(1.5, 1.5, 1.5), sigma=(0.02, 0.01, 0.03))]
+    d.objects = [decks.Block(4, (c[0], c[1], c[2] - 6), (1, 1, 6)), decks.Block(4, (c[0], c[1], c[2] + 1), (1, 1, 6)),
+                 decks.Sphere(5, (c[0] + 14, c[1], c[2]), 5), decks.Block(6, (3, 3, 3), (12, 8, 6)),
+                 decks.Block(5, (nx - 12, ny - 15, nz - 19), (11, 9, 10))]
+    d.ports = [decks.Port("z", 50.0, c, (0, 0, 1))]
+    return d
+
+
+@pytest.mark.parametrize("cells", [(70, 45, 39), (150, 37, 29)])
+@pytest.mark.parametrize("planes", [0, 1, 5])
+def test_single_pass_matches_split_launches(planes, cells):
+    """The single-pass core launch (H and E of a step together, halo H
+    formed again from the step-start fields) leaves fields, port series and
+    far-field state equal, bit for bit, to separate H and E bulk launches --
+    across x/y tile seams (in an x slab on the 70-cell grid; slab-free and
+    slab tiles as separate launches on the 150-cell grid), z chunks, graph
+    and eager steps and odd step counts (the two buffer sets alternate)."""
+    from alpineem_amd.solver import Plan as P
+    text = single_pass_deck(cells).text()
+    sp = Simulation(P.from_text(text), planes=planes)
+    ref = Simulation(P.from_text(text), planes=planes, single_pass=False)
+    assert sp.single_pass and not ref.single_pass
+    for steps, graph in ((37, True), (1, False), (2, False), (60, True), (3, False), (57, True)):
+        sp.run(steps, graph=graph)
+        ref.run(steps, graph=graph)
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        np.testing.assert_array_equal(sp.field(comp), ref.field(comp), err_msg=comp)
+    sp.finalize()
+    ref.finalize()
+    np.testing.assert_array_equal(sp.series("port_v", 0), ref.series("port_v", 0))
+    np.testing.assert_array_equal(sp.series("port_i", 0), ref.series("port_i", 0))
+    for a in range(3):
+        np.testing.assert_array_equal(sp.series("e_theta", a), ref.series("e_theta", a))
+
+
+@pytest.mark.parametrize("name", ["dipole36", "c1_mini", "antenna_x"])
+def test_single_pass_golden_decks(golden_dir, name):
+    """Decks that qualify run the single-pass step and stay equal to the
+    split launches and, through the parity tests above, to the oracle."""
+    deck = os.path.join(golden_dir, "decks", name + ".txt")
+    sp = Simulation(Plan.from_inputs(deck), planes=3)
+    ref = Simulation(Plan.from_inputs(deck), single_pass=False)
+    assert sp.single_pass
+    sp.run(101)
+    ref.run(101)
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        np.testing.assert_array_equal(sp.field(comp), ref.field(comp), err_msg=comp)
+
+
 @pytest.mark.parametrize("mode", ["atomic", "histogram"])
 @pytest.mark.parametrize("name", ["c1_mini", "sphere_oblique"])
 def test_ntff_fp64_modes_close(golden_dir, name, mode):
