@@ -441,13 +441,6 @@ struct SpArgs {
     float dx, dy, dz;
 };
 
-// x-face CPML term of an H cell: slot 0 (psi_Hyx) or 1 (psi_Hzx), read only.
-__device__ __forceinline__ float psi_hx_read(const Cpml& cp, const Grid& g, int f, int q, int slot, int j, int k,
-                                             float diff, float dx) {
-    const uint32_t po = (uint32_t)(q - 1) + (uint32_t)(cp.n[f] - 1) * ((uint32_t)(j - 1) + (uint32_t)g.NY * (k - 1));
-    return cp.b[f][q - 1] * cp.psi[f][slot][po] + cp.c[f][q - 1] * diff / dx;
-}
-
 // Field access by 32-bit byte offset from a uniform base: one VGPR per
 // distinct offset, shared by all arrays (global loads with a scalar base).
 __device__ __forceinline__ float ldb(const float* p, uint32_t ob) {
@@ -457,8 +450,20 @@ __device__ __forceinline__ void stb(float v, float* p, uint32_t ob) {
     __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(p) + ob));
 }
 
+// Values across the lanes of a wave (DPP wave shifts): lane L gets lane
+// L-1's v (lane 0: first), or lane L+1's v (lane 63: last).
+__device__ __forceinline__ float from_lane_below(float v, float first) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_lane_above(float v, float last) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(last), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+constexpr int kSpRows = 2;   // y rows marched together by one wave
+
 template <typename IdxT, int C>
 __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
+    constexpr int R = kSpRows;
     const Grid& g = a.g;
     const float* __restrict__ Ex = a.Ex;
     const float* __restrict__ Ey = a.Ey;
@@ -469,7 +474,7 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
     const char* __restrict__ idx = static_cast<const char*>(a.idx);
     const float* __restrict__ tup = a.tup;
     const float DB = a.DB;
-    const int nbx = (b.ni + 63) / 64, nby = (b.nj + kSY - 1) / kSY;
+    const int nbx = (b.ni + 63) / 64, nby = (b.nj + kSY * R - 1) / (kSY * R);
     // Work-groups are dealt to the 8 XCDs in turn; XCD x takes the x-th
     // contiguous range of tiles (x fastest), so the tiles whose halo lines a
     // work-group reads run on its own XCD, through its L2.
@@ -478,185 +483,247 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
     const int bx = l % nbx, by = (l / nbx) % nby, bz = l / (nbx * nby);
     const int lane = (int)threadIdx.x;
     const int i0 = b.i0 + bx * 64;
-    const int imax = b.i0 + b.ni - 1;
-    const int j = b.j0 + by * kSY + (int)threadIdx.y;
-    if (j > b.j0 + b.nj - 1) return;          // a whole wave
+    const int imax = b.i0 + b.ni - 1, jmax = b.j0 + b.nj - 1;
+    // rows jw .. jw+nr-1 of this wave (threadIdx.y is one value per wave)
+    const int jw = __builtin_amdgcn_readfirstlane(b.j0 + (by * kSY + (int)threadIdx.y) * R);
+    if (jw > jmax) return;
+    const int nr = min(R, jmax - jw + 1);
     const int i = i0 + lane;
-    const bool act = i <= imax;               // lanes past the box compute a copy of the last cell
-    const int ic = min(i, imax);
+    const bool act = i <= imax;
+    const int it = min(i, imax);              // tables and slab tests
+    const int ic = min(i, imax + 1);          // loads: the lane past the last cell holds its +x neighbour
     const int k0 = b.k0 + bz * b.kc;
     const int k1 = min(k0 + b.kc, b.k0 + b.nk) - 1;
-    const bool ix = ic >= 2, jy = j >= 2;
-    const int jm = jy ? j - 1 : j;            // row j-1 (its values are used only when jy)
-    const bool x0 = lane == 0 && i0 >= 2;     // lane 0 takes the x neighbour from the cell i0-1
-    const int ih = max(i0 - 1, 1);            // that cell (same for the whole wave)
-    const float dxh = a.dhx[ic - 1], dyh = a.dhy[j - 1], dyhm = a.dhy[jm - 1];
-    const float dxe = a.dex[ic - 1], dye = a.dey[j - 1], dxc = a.dhx[ih - 1];
+    const bool ix = i >= 2;
+    const bool hrow = jw >= 2;                // the row jw-1 exists
+    const int ih = max(i0 - 1, 1);            // the cell i0-1 (its H is lane 0's x neighbour)
+    const int inx = min(i0 + 64, imax + 1);   // the first cell past this tile
+    const float dxh = a.dhx[it - 1], dxe = a.dex[it - 1], dxc = a.dhx[ih - 1];
+    const float dyhm = a.dhy[max(jw - 1, 1) - 1];
+    float dyh[R], dye[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int jr = min(jw + r, jmax);
+        dyh[r] = a.dhy[jr - 1];
+        dye[r] = a.dey[jr - 1];
+    }
     const uint32_t NY = (uint32_t)g.NY;
-    // x-slab CPML of the cell (H and E) and of the cell i0-1: coefficients
-    // and psi row pointers, fixed over the march; psi index per plane
-    // n*((j-1) + NY*(k-1)) (+ the layer)
+    // x-slab CPML of the lane's cell (H, E) and of the cell i0-1: coefficients
+    // and psi offsets fixed over the march.  The two x faces of a slot share
+    // one allocation (face 1 after face 0, the same layout in both buffer
+    // sets), so the lane keeps an element offset from the face-0 base; psi
+    // index n*((j-1) + NY*(k-1)) on top
     int qxh = 0, qxe = 0, txe = 0, qxc = 0;
-    const int fxh = (C & kCx) ? slab_h(ic, g.NX, a.cpH.n[0], a.cpH.n[1], qxh) : -1;
-    const int fxe = (C & kCx) ? slab_e(ic, g.NX, a.cpE.n[0], a.cpE.n[1], qxe, txe) : -1;
+    const int fxh = (C & kCx) ? slab_h(it, g.NX, a.cpH.n[0], a.cpH.n[1], qxh) : -1;
+    const int fxe = (C & kCx) ? slab_e(it, g.NX, a.cpE.n[0], a.cpE.n[1], qxe, txe) : -1;
     const int fxc = (C & kCx) ? slab_h(ih, g.NX, a.cpH.n[0], a.cpH.n[1], qxc) : -1;
+    const float* __restrict__ ph0 = a.cpH.psi[0][0];
+    const float* __restrict__ ph1 = a.cpH.psi[0][1];
+    const float* __restrict__ pe0 = a.cpE.psi[0][0];
+    const float* __restrict__ pe1 = a.cpE.psi[0][1];
+    float* __restrict__ qh0 = a.cpH.pso[0][0];
+    float* __restrict__ qh1 = a.cpH.pso[0][1];
+    float* __restrict__ qe0 = a.cpE.pso[0][0];
+    float* __restrict__ qe1 = a.cpE.pso[0][1];
+    const uint32_t fH1 = (uint32_t)(a.cpH.psi[1][0] - a.cpH.psi[0][0]);   // face 1 after face 0
+    const uint32_t fE1 = (uint32_t)(a.cpE.psi[1][0] - a.cpE.psi[0][0]);
     float cbh = 0.f, cch = 0.f, cbe = 0.f, cce = 0.f, cbc = 0.f, ccc = 0.f;
-    const float *ph0 = nullptr, *ph1 = nullptr, *pe0 = nullptr, *pe1 = nullptr, *pc0 = nullptr, *pc1 = nullptr;
-    float *qh0 = nullptr, *qh1 = nullptr, *qe0 = nullptr, *qe1 = nullptr;
-    uint32_t nth = 0, nte = 0, ntc = 0;
+    uint32_t nth = 0, nte = 0, ntc = 0, lh = 0, le = 0, lc = 0;
     if ((C & kCx) && fxh >= 0) {
         const int f = fxh;
         nth = (uint32_t)(a.cpH.n[f] - 1);
         cbh = a.cpH.b[f][qxh - 1];
         cch = a.cpH.c[f][qxh - 1];
-        ph0 = a.cpH.psi[f][0] + (qxh - 1);
-        ph1 = a.cpH.psi[f][1] + (qxh - 1);
-        qh0 = a.cpH.pso[f][0] + (qxh - 1);
-        qh1 = a.cpH.pso[f][1] + (qxh - 1);
+        lh = (f ? fH1 : 0u) + (uint32_t)(qxh - 1);
     }
     if ((C & kCx) && fxe >= 0) {
         const int f = fxe;
         nte = (uint32_t)(a.cpE.n[f] - 1);
         cbe = a.cpE.b[f][qxe - 1];
         cce = a.cpE.c[f][qxe - 1];
-        pe0 = a.cpE.psi[f][0] + txe;
-        pe1 = a.cpE.psi[f][1] + txe;
-        qe0 = a.cpE.pso[f][0] + txe;
-        qe1 = a.cpE.pso[f][1] + txe;
+        le = (f ? fE1 : 0u) + (uint32_t)txe;
     }
     if ((C & kCx) && fxc >= 0) {
         const int f = fxc;
         ntc = (uint32_t)(a.cpH.n[f] - 1);
         cbc = a.cpH.b[f][qxc - 1];
         ccc = a.cpH.c[f][qxc - 1];
-        pc0 = a.cpH.psi[f][0] + (qxc - 1);
-        pc1 = a.cpH.psi[f][1] + (qxc - 1);
+        lc = (f ? fH1 : 0u) + (uint32_t)(qxc - 1);
     }
-    uint32_t prow = (uint32_t)(j - 1) + NY * (uint32_t)(k0 - 1);   // (j-1) + NY*(k-1)
-    const uint32_t drow = jy ? 1u : 0u;                             // row j-1
-    // byte offsets (the host admits grids whose arrays stay under 4 GiB)
+    uint32_t prow = (uint32_t)(jw - 1) + NY * (uint32_t)(k0 - 1);   // (j-1) + NY*(k-1) of the row jw
+    // byte offsets (the host admits grids whose arrays stay under 4 GiB); row r at + r*PB
     const uint32_t PB = (uint32_t)g.PX * 4u, ZB = (uint32_t)g.plane * 4u;
-    const uint32_t cell = (uint32_t)off(g, ic, j, k0);
-    uint32_t ob = cell * 4u;
-    const uint32_t dm = jy ? PB : 0u;         // ob - dm: the row j-1
-    uint32_t oc = (uint32_t)off(g, ih, j, k0) * 4u;
-    float exk = ldb(Ex, ob), eyk = ldb(Ey, ob);   // E(k) of the cell, carried
-    float eym = ldb(Ey, ob - dm);                 // Ey(k) of the row j-1, carried
-    float exc = ldb(Ex, oc);                      // Ex(k) of the cell i0-1, carried
-    float hxm = 0.f, hym = 0.f;                   // new Hx, Hy of the cell at k-1
-    if (k0 >= 2) {                                // Hx, Hy of the plane below the chunk
-        const uint32_t om = ob - ZB;
-        const float eyb = ldb(Ey, om), exb = ldb(Ex, om);
-        const float ez = ldb(Ez, om), ez_xp = ldb(Ez, om + 4u), ez_yp = ldb(Ez, om + PB);
-        const float dzm = a.dhz[k0 - 2];
-        hxm = ldb(Hx, om) + DB * ((ez - ez_yp) * dyh + (eyk - eyb) * dzm);
-        hym = ldb(Hy, om) + DB * ((ez_xp - ez) * dxh + (exb - exk) * dzm);
-        if ((C & kCx) && fxh >= 0) hym = hym + DB * (cbh * ph0[nth * (prow - NY)] + cch * (ez_xp - ez) / a.dx);
-    }
-    // coefficient tuples one plane ahead, their indices two planes ahead
+    const uint32_t dH = hrow ? PB : 0u;       // ob - dH: the row jw-1 (the row jw itself when absent)
+    uint32_t ob = (uint32_t)off(g, ic, jw, k0) * 4u;
+    uint32_t oc = (uint32_t)off(g, ih, jw, k0) * 4u;    // the cell i0-1, row jw
+    uint32_t on = (uint32_t)off(g, inx, jw, k0) * 4u;   // the cell past the tile, row jw
+    float exk[R], eyk[R], exc[R], hxm[R], hym[R];
+    int t[R], tn[R];
     const uint32_t IB = (uint32_t)sizeof(IdxT), ZI = (uint32_t)g.plane * IB;
-    uint32_t oi = cell * IB;
-    int t = (int)*reinterpret_cast<const IdxT*>(idx + oi) * 6;
-    if (k0 < k1) oi += ZI;
-    int tn = (int)*reinterpret_cast<const IdxT*>(idx + oi) * 6;
-    float gax = tup[t + 0], gbx = tup[t + 1], gay = tup[t + 2], gby = tup[t + 3], gaz = tup[t + 4], gbz = tup[t + 5];
+    const uint32_t PI = (uint32_t)g.PX * IB;
+    uint32_t oi = (uint32_t)off(g, ic, jw, k0) * IB;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        exk[r] = eyk[r] = exc[r] = hxm[r] = hym[r] = 0.f;
+        t[r] = tn[r] = 0;
+        if (r < nr) {
+            const uint32_t o = ob + r * PB;
+            exk[r] = ldb(Ex, o);
+            eyk[r] = ldb(Ey, o);
+            exc[r] = ldb(Ex, oc + r * PB);
+            t[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
+            if (k0 >= 2) {                    // Hx, Hy of the plane below the chunk
+                const int j = jw + r;
+                const uint32_t om = o - ZB;
+                const float eyb = ldb(Ey, om), exb = ldb(Ex, om);
+                const float ez = ldb(Ez, om), ez_xp = ldb(Ez, om + 4u), ez_yp = ldb(Ez, om + PB);
+                const float dzm = a.dhz[k0 - 2];
+                hxm[r] = ldb(Hx, om) + DB * ((ez - ez_yp) * dyh[r] + (eyk[r] - eyb) * dzm);
+                hym[r] = ldb(Hy, om) + DB * ((ez_xp - ez) * dxh + (exb - exk[r]) * dzm);
+                if ((C & kCx) && fxh >= 0)
+                    hym[r] = hym[r] + DB * (cbh * ph0[lh + nth * ((uint32_t)(j - 1) + NY * (uint32_t)(k0 - 2))] +
+                                            cch * (ez_xp - ez) / a.dx);
+            }
+        }
+    }
+    float eyh = ldb(Ey, ob - dH);             // Ey(k) of the row jw-1, carried
     for (int k = k0; k <= k1; ++k) {
         const float dzh = a.dhz[k - 1], dze = a.dez[k - 1];
-        // every load of the plane first: the cell, its +x/+y/+z neighbours,
-        // the row j-1, the cell i0-1 (one address per wave), psi, the next
-        // plane's tuple and index
-        const float eyk1 = ldb(Ey, ob + ZB), exk1 = ldb(Ex, ob + ZB);
-        const float ez = ldb(Ez, ob), ez_xp = ldb(Ez, ob + 4u), ey_xp = ldb(Ey, ob + 4u);
-        const float ez_yp = ldb(Ez, ob + PB), ex_yp = ldb(Ex, ob + PB);
-        const float hx0 = ldb(Hx, ob), hy0 = ldb(Hy, ob), hz0 = ldb(Hz, ob);
-        const uint32_t om = ob - dm;
-        const float ezm = ldb(Ez, om), eym1 = ldb(Ey, om + ZB), eym_xp = ldb(Ey, om + 4u), exm = ldb(Ex, om);
-        const float hxm0 = ldb(Hx, om), hzm0 = ldb(Hz, om);
-        const float eyc = ldb(Ey, oc), ezc = ldb(Ez, oc), exc_yp = ldb(Ex, oc + PB), exc1 = ldb(Ex, oc + ZB);
-        const float hyc0 = ldb(Hy, oc), hzc0 = ldb(Hz, oc);
-        float sh0 = 0.f, sh1 = 0.f, sr1 = 0.f, se0 = 0.f, se1 = 0.f, sc0 = 0.f, sc1 = 0.f;
-        if ((C & kCx) && fxh >= 0) {
-            sh0 = ph0[nth * prow];
-            sh1 = ph1[nth * prow];
-            sr1 = ph1[nth * (prow - drow)];
+        // every load of the plane first
+        float eyk1[R], exk1[R], ez[R], hx0[R], hy0[R], hz0[R], ezn[R], eyn[R];
+        float eyc[R], ezc[R], exc1[R], excy[R], hyc0[R], hzc0[R], ezi[R], eyi[R];
+        float ga[R][6];
+        float sh0[R], sh1[R], se0[R], se1[R], sc0[R], sc1[R];
+        if (k < k1) oi += ZI;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r >= nr) continue;
+            const uint32_t o = ob + r * PB, c = oc + r * PB, u = on + r * PB;
+            eyk1[r] = ldb(Ey, o + ZB);
+            exk1[r] = ldb(Ex, o + ZB);
+            ez[r] = ldb(Ez, o);
+            hx0[r] = ldb(Hx, o);
+            hy0[r] = ldb(Hy, o);
+            hz0[r] = ldb(Hz, o);
+            ezn[r] = ldb(Ez, u);              // one address per wave: the next tile's first cell
+            eyn[r] = ldb(Ey, u);
+            eyc[r] = ldb(Ey, c);              // the cell i0-1 and its +x, +y, +z neighbours
+            ezc[r] = ldb(Ez, c);
+            exc1[r] = ldb(Ex, c + ZB);
+            excy[r] = ldb(Ex, c + PB);
+            hyc0[r] = ldb(Hy, c);
+            hzc0[r] = ldb(Hz, c);
+            ezi[r] = ldb(Ez, c + 4u);
+            eyi[r] = ldb(Ey, c + 4u);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ga[r][q] = tup[t[r] + q];
+            tn[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
+            const uint32_t pr = prow + (uint32_t)r;
+            sh0[r] = sh1[r] = se0[r] = se1[r] = sc0[r] = sc1[r] = 0.f;
+            if ((C & kCx) && fxh >= 0) {
+                sh0[r] = ph0[lh + nth * pr];
+                sh1[r] = ph1[lh + nth * pr];
+            }
+            if ((C & kCx) && fxe >= 0) {
+                se0[r] = pe0[le + nte * pr];
+                se1[r] = pe1[le + nte * pr];
+            }
+            if ((C & kCx) && fxc >= 0) {
+                sc0[r] = ph0[lc + ntc * pr];
+                sc1[r] = ph1[lc + ntc * pr];
+            }
         }
-        if ((C & kCx) && fxe >= 0) {
-            se0 = pe0[nte * prow];
-            se1 = pe1[nte * prow];
+        const uint32_t oT = ob + (uint32_t)nr * PB;   // the row past the last: +y neighbours
+        const float ezT = ldb(Ez, oT), exT = ldb(Ex, oT);
+        const uint32_t oH = ob - dH;                  // the row jw-1
+        const float ezh = ldb(Ez, oH), eyh1 = ldb(Ey, oH + ZB), exh = ldb(Ex, oH);
+        const float hxh0 = ldb(Hx, oH), hzh0 = ldb(Hz, oH), eyhn = ldb(Ey, on - dH);
+        const float srh = ((C & kCx) && fxh >= 0) ? ph1[lh + nth * (prow - (hrow ? 1u : 0u))] : 0.f;
+        // new Hx, Hz of the row jw-1 (from the step-start state, as its owner forms them)
+        float hxp, hzp;
+        {
+            const float eyh_xp = from_lane_above(eyh, eyhn);
+            hxp = hxh0 + DB * ((ezh - ez[0]) * dyhm + (eyh1 - eyh) * dzh);
+            hzp = hzh0 + DB * ((eyh - eyh_xp) * dxh + (exk[0] - exh) * dyhm);
+            if ((C & kCx) && fxh >= 0) hzp = hzp + DB * (cbh * srh + cch * (eyh - eyh_xp) / a.dx);
         }
-        if ((C & kCx) && fxc >= 0) {
-            sc0 = pc0[ntc * prow];
-            sc1 = pc1[ntc * prow];
-        }
-        const float nax = tup[tn + 0], nbx_ = tup[tn + 1], nay = tup[tn + 2], nby_ = tup[tn + 3], naz = tup[tn + 4],
-                    nbz = tup[tn + 5];
-        if (k + 1 < k1) oi += ZI;
-        const int tnn = (int)*reinterpret_cast<const IdxT*>(idx + oi) * 6;
-        // new H of the cell
-        float hx = hx0 + DB * ((ez - ez_yp) * dyh + (eyk1 - eyk) * dzh);
-        float hy = hy0 + DB * ((ez_xp - ez) * dxh + (exk - exk1) * dzh);
-        float hz = hz0 + DB * ((eyk - ey_xp) * dxh + (ex_yp - exk) * dyh);
-        if ((C & kCx) && fxh >= 0) {                          // psi_Hyx, psi_Hzx
-            const float p0 = cbh * sh0 + cch * (ez_xp - ez) / a.dx;
-            const float p1 = cbh * sh1 + cch * (eyk - ey_xp) / a.dx;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r >= nr) continue;
+            const int j = jw + r;
+            const bool jy = j >= 2;
+            const uint32_t o = ob + r * PB;
+            const uint32_t pr = prow + (uint32_t)r;
+            const float ez_xp = from_lane_above(ez[r], ezn[r]), ey_xp = from_lane_above(eyk[r], eyn[r]);
+            const float ez_yp = (r + 1 < nr) ? ez[(r + 1) % R] : ezT;
+            const float ex_yp = (r + 1 < nr) ? exk[(r + 1) % R] : exT;
+            // new H of the cell
+            float hx = hx0[r] + DB * ((ez[r] - ez_yp) * dyh[r] + (eyk1[r] - eyk[r]) * dzh);
+            float hy = hy0[r] + DB * ((ez_xp - ez[r]) * dxh + (exk[r] - exk1[r]) * dzh);
+            float hz = hz0[r] + DB * ((eyk[r] - ey_xp) * dxh + (ex_yp - exk[r]) * dyh[r]);
+            if ((C & kCx) && fxh >= 0) {                      // psi_Hyx, psi_Hzx
+                const float p0 = cbh * sh0[r] + cch * (ez_xp - ez[r]) / a.dx;
+                const float p1 = cbh * sh1[r] + cch * (eyk[r] - ey_xp) / a.dx;
+                if (act) {
+                    qh0[lh + nth * pr] = p0;
+                    qh1[lh + nth * pr] = p1;
+                }
+                hy = hy + DB * p0;
+                hz = hz + DB * p1;
+            }
             if (act) {
-                qh0[nth * prow] = p0;
-                qh1[nth * prow] = p1;
+                stb(hx, a.Hxo, o);
+                stb(hy, a.Hyo, o);
+                stb(hz, a.Hzo, o);
             }
-            hy = hy + DB * p0;
-            hz = hz + DB * p1;
-        }
-        if (act) {
-            stb(hx, a.Hxo, ob);
-            stb(hy, a.Hyo, ob);
-            stb(hz, a.Hzo, ob);
-        }
-        // new Hx, Hz of the row j-1 (its +y neighbours are this row's E)
-        const float hxr = hxm0 + DB * ((ezm - ez) * dyhm + (eym1 - eym) * dzh);
-        float hzr = hzm0 + DB * ((eym - eym_xp) * dxh + (exk - exm) * dyhm);
-        if ((C & kCx) && fxh >= 0) hzr = hzr + DB * (cbh * sr1 + cch * (eym - eym_xp) / a.dx);
-        // new Hy, Hz of the cell i0-1 (its +x neighbour is lane 0's cell)
-        float hyc = hyc0 + DB * ((ez - ezc) * dxc + (exc - exc1) * dzh);
-        float hzc = hzc0 + DB * ((eyc - eyk) * dxc + (exc_yp - exc) * dyh);
-        if ((C & kCx) && fxc >= 0) {
-            hyc = hyc + DB * (cbc * sc0 + ccc * (ez - ezc) / a.dx);
-            hzc = hzc + DB * (cbc * sc1 + ccc * (eyc - eyk) / a.dx);
-        }
-        // x neighbour: the lane below, lane 0 the cell i0-1
-        const float sy_ = __shfl_up(hy, 1), sz_ = __shfl_up(hz, 1);
-        const float hy_xm = x0 ? hyc : sy_, hz_xm = x0 ? hzc : sz_;
-        // new E of the cell
-        float ex = gax * exk + gbx * ((hz - hzr) * dye + (hym - hy) * dze);
-        float ey = gay * eyk + gby * ((hz_xm - hz) * dxe + (hx - hxm) * dze);
-        float ezn = gaz * ez + gbz * ((hy - hy_xm) * dxe + (hxr - hx) * dye);
-        if ((C & kCx) && fxe >= 0) {                          // psi_Eyx, psi_Ezx
-            if (k >= 2) {
-                const float q = cbe * se0 + cce * (hz_xm - hz) / a.dx;
-                if (act) qe0[nte * prow] = q;
-                ey = ey + gby * q;
+            // new Hy, Hz of the cell i0-1 (lane 0's x neighbour)
+            float hyc = hyc0[r] + DB * ((ezi[r] - ezc[r]) * dxc + (exc[r] - exc1[r]) * dzh);
+            float hzc = hzc0[r] + DB * ((eyc[r] - eyi[r]) * dxc + (excy[r] - exc[r]) * dyh[r]);
+            if ((C & kCx) && fxc >= 0) {
+                hyc = hyc + DB * (cbc * sc0[r] + ccc * (ezi[r] - ezc[r]) / a.dx);
+                hzc = hzc + DB * (cbc * sc1[r] + ccc * (eyc[r] - eyi[r]) / a.dx);
             }
-            if (jy) {
-                const float q = cbe * se1 + cce * (hy - hy_xm) / a.dx;
-                if (act) qe1[nte * prow] = q;
-                ezn = ezn + gbz * q;
+            const float hy_xm = from_lane_below(hy, hyc), hz_xm = from_lane_below(hz, hzc);
+            // new E of the cell
+            const float* gq = ga[r];
+            float ex = gq[0] * exk[r] + gq[1] * ((hz - hzp) * dye[r] + (hym[r] - hy) * dze);
+            float ey = gq[2] * eyk[r] + gq[3] * ((hz_xm - hz) * dxe + (hx - hxm[r]) * dze);
+            float ezn_ = gq[4] * ez[r] + gq[5] * ((hy - hy_xm) * dxe + (hxp - hx) * dye[r]);
+            if ((C & kCx) && fxe >= 0) {                      // psi_Eyx, psi_Ezx
+                if (k >= 2) {
+                    const float q = cbe * se0[r] + cce * (hz_xm - hz) / a.dx;
+                    if (act) qe0[le + nte * pr] = q;
+                    ey = ey + gq[3] * q;
+                }
+                if (jy) {
+                    const float q = cbe * se1[r] + cce * (hy - hy_xm) / a.dx;
+                    if (act) qe1[le + nte * pr] = q;
+                    ezn_ = ezn_ + gq[5] * q;
+                }
             }
+            if (act) {
+                const bool ex_on = jy && k >= 2, ey_on = ix && k >= 2, ez_on = ix && jy;
+                stb(ex_on ? ex : exk[r], a.Exo, o);
+                stb(ey_on ? ey : eyk[r], a.Eyo, o);
+                stb(ez_on ? ezn_ : ez[r], a.Ezo, o);
+            }
+            hxp = hx;
+            hzp = hz;
+            hxm[r] = hx;
+            hym[r] = hy;
         }
-        if (act) {
-            const bool ex_on = jy && k >= 2, ey_on = ix && k >= 2, ez_on = ix && jy;
-            stb(ex_on ? ex : exk, a.Exo, ob);
-            stb(ey_on ? ey : eyk, a.Eyo, ob);
-            stb(ez_on ? ezn : ez, a.Ezo, ob);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r >= nr) continue;
+            exk[r] = exk1[r];
+            eyk[r] = eyk1[r];
+            exc[r] = exc1[r];
+            t[r] = tn[r];
         }
-        hxm = hx;
-        hym = hy;
-        exk = exk1;
-        eyk = eyk1;
-        eym = eym1;
-        exc = exc1;
-        gax = nax; gbx = nbx_; gay = nay; gby = nby_; gaz = naz; gbz = nbz;
-        tn = tnn;
+        eyh = eyh1;
         ob += ZB;
         oc += ZB;
+        on += ZB;
         prow += NY;
     }
 }

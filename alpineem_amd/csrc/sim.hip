@@ -238,8 +238,7 @@ struct aem_sim {
     bool sp_ok = false;            // the deck admits it (set at creation)
     bool batched = false;          // a batch steps this simulation: in-place kernels only
     Box sp_box{};                  // core box of the single-pass kernel
-    std::vector<std::pair<Box, int>> sp_parts;   // its launches: x tiles free of slab cells (no slab
-                                                 // test) and those holding x-slab cells (kCx)
+    std::vector<std::pair<Box, int>> sp_parts;   // its launches (box, slab tests compiled in)
     int par = 0;                   // which buffer set holds the fields now
     // k_adj_record reads E before its update, which the single pass has
     // already written; a design box records inside k_e_design, after it
@@ -341,7 +340,9 @@ void aem_sim::create(const aem_sim_options* opt) {
     const float* dsrc[6] = {v.den_ex, v.den_ey, v.den_ez, v.den_hx, v.den_hy, v.den_hz};
     for (int q = 0; q < 6; ++q) d_den[q] = up<float>(dsrc[q], std::max(dl[q], 1));
 
-    // CPML: coefficients and psi slabs
+    // CPML: coefficients and psi slabs (the two x faces of a slot in one
+    // allocation, face 0 first: the single-pass kernel reaches both from
+    // one base)
     for (int f = 0; f < 6; ++f) {
         const int n = v.npml[f];
         const int a = f / 2;
@@ -356,6 +357,15 @@ void aem_sim::create(const aem_sim_options* opt) {
         else if (a == 1) trans = (size_t)v.PX * v.NZ;
         else trans = (size_t)v.PX * v.NY;
         for (int s = 0; s < 2; ++s) {
+            if (f == 1) continue;
+            if (f == 0) {
+                const size_t nt1 = (size_t)std::max(v.npml[1] - 1, 0);
+                cpH.psi[0][s] = zeros<float>((nt + nt1) * trans);
+                cpE.psi[0][s] = zeros<float>((nt + nt1) * trans);
+                cpH.psi[1][s] = cpH.psi[0][s] + nt * trans;
+                cpE.psi[1][s] = cpE.psi[0][s] + nt * trans;
+                continue;
+            }
             cpH.psi[f][s] = zeros<float>(nt * trans);
             cpE.psi[f][s] = zeros<float>(nt * trans);
         }
@@ -527,15 +537,19 @@ void aem_sim::create(const aem_sim_options* opt) {
         sp_box = bulk[0].bs.b[0];
         sp_box.kc = planes_set ? planes : kSpPlanes;
         // one launch with the x-slab test compiled in: measured at 400^3,
-        // separate launches for the slab-free tiles (no slab test, fewer
-        // registers) and the two x-slab tiles of each row took 5% longer
+        // separate launches for the slab-free x tiles (no slab test, fewer
+        // registers) and the two x-slab tiles of each row took 5-12% longer,
+        // in sequence or on a second stream beside them
         sp_parts.push_back({sp_box, (int)kCx});
         for (int q = 0; q < 6; ++q) F2[q] = zeros<float>(nc + v.plane);
-        for (int f = 0; f < 2; ++f) {
-            const size_t nt = (size_t)std::max(v.npml[f] - 1, 0), trans = (size_t)v.NY * v.NZ;
-            for (int q = 0; q < 2; ++q) {
-                cpH.pso[f][q] = zeros<float>(nt * trans);
-                cpE.pso[f][q] = zeros<float>(nt * trans);
+        {
+            const size_t nt0 = (size_t)std::max(v.npml[0] - 1, 0), nt1 = (size_t)std::max(v.npml[1] - 1, 0);
+            const size_t trans = (size_t)v.NY * v.NZ;
+            for (int q = 0; q < 2; ++q) {   // same layout as psi: face 0, then face 1
+                cpH.pso[0][q] = zeros<float>((nt0 + nt1) * trans);
+                cpE.pso[0][q] = zeros<float>((nt0 + nt1) * trans);
+                cpH.pso[1][q] = cpH.pso[0][q] + nt0 * trans;
+                cpE.pso[1][q] = cpE.pso[0][q] + nt0 * trans;
             }
         }
     }
@@ -624,7 +638,7 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
                         d_den[0], d_den[1], d_den[2], d_den[3], d_den[4], d_den[5], v.DB, cpH, cpE, v.dx, v.dy, v.dz};
         for (const auto& part : sp_parts) {
             const Box& b = part.first;
-            const unsigned nblk = cdiv(b.ni, 64) * cdiv(b.nj, kSY) * cdiv(b.nk, b.kc);
+            const unsigned nblk = cdiv(b.ni, 64) * cdiv(b.nj, kSY * kSpRows) * cdiv(b.nk, b.kc);
             if (part.second == kCx) k_step_sp<IdxT, kCx><<<nblk, dim3(64, kSY), 0, s>>>(sa, b);
             else k_step_sp<IdxT, 0><<<nblk, dim3(64, kSY), 0, s>>>(sa, b);
         }
