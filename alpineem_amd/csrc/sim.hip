@@ -536,6 +536,12 @@ void aem_sim::create(const aem_sim_options* opt) {
     if (sp_ok) {
         sp_box = bulk[0].bs.b[0];
         sp_box.kc = planes_set ? planes : kSpPlanes;
+        // small grids: fewer planes per column until the launch has enough
+        // work-groups to fill the chip (16 per CU)
+        auto groups = [&](int kc) {
+            return (long long)cdiv(sp_box.ni, 64) * cdiv(sp_box.nj, kSY * kSpRows) * cdiv(sp_box.nk, kc);
+        };
+        while (!planes_set && sp_box.kc > 2 && groups(sp_box.kc) < 4096) sp_box.kc /= 2;
         // one launch with the x-slab test compiled in: measured at 400^3,
         // separate launches for the slab-free x tiles (no slab test, fewer
         // registers) and the two x-slab tiles of each row took 5-12% longer,
