@@ -220,6 +220,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="also run a known-size device copy (counter scaling)")
+    ap.add_argument("--no-single-pass", action="store_true",
+                    help="separate H and E bulk launches instead of the single-pass core launch")
     ap.add_argument("--population-steps", type=int, default=1000,
                     help="steps of the 64-member batched-population comparison (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
@@ -237,7 +239,7 @@ def main() -> int:
     deck = decks.roofline_deck(n, W + K + P)
     t0 = time.perf_counter()
     plan = Plan.from_text(deck.text())
-    sim = Simulation(plan)
+    sim = Simulation(plan, single_pass=not args.no_single_pass)
     setup_s = time.perf_counter() - t0
     print(f"[bench] setup {setup_s:.1f}s, grid {n}^3, tuples {plan.view.n_tuple}", file=sys.stderr, flush=True)
 
@@ -260,9 +262,9 @@ def main() -> int:
     if single:
         dom, bpc, t_ms = f"k_step_sp<unsigned char, {c}>", BYTES_SP, ph[0]
     elif ph[2] >= ph[0]:
-        dom, bpc, t_ms = f"k_e_bulk<unsigned char, {c}>", BYTES_E, ph[2]
+        dom, bpc, t_ms = f"k_e_bulk<unsigned char, {c}, false>", BYTES_E, ph[2]
     else:
-        dom, bpc, t_ms = f"k_h_bulk<{c}>", BYTES_H, ph[0]
+        dom, bpc, t_ms = f"k_h_bulk<{c}, false>", BYTES_H, ph[0]
     achieved = bpc * core / (t_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
