@@ -2417,6 +2417,136 @@ __global__ void __launch_bounds__(256) k_adj_source(Grid g, Objective ob, double
     }
 }
 
+// The same source in a fixed summation order (run to run bit-identical):
+// k_adj_source_terms stores each face cell's four increments dj, and
+// k_adj_source_gather gives every adjoint element near the far-field box to
+// one thread, which adds the increments that reach it in face, then term
+// offset order.  D is [6 faces][4 terms][mv][mu].
+template <int A>
+__device__ __forceinline__ void source_dj(const Grid& g, const Objective& ob, int face, int uu, int vv,
+                                          const int* step, double Fr, double Fi, float* D) {
+    long long o;
+    if (!face_cell<A>(g, ob, face & 1, uu, vv, o)) return;
+    double wr[4], wi[4];
+    face_weights<A>(ob, face, uu, vv, *step, wr, wi);
+    FaceTerm t[4];
+    face_terms<A>(g, (face & 1) ? -1.0 : 1.0, t);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        D[(((long long)face * 4 + q) * ob.mv + vv) * ob.mu + uu] = (float)(2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w);
+}
+
+__global__ void __launch_bounds__(256) k_adj_source_terms(Grid g, Objective ob, double Fr, double Fi, float* D,
+                                                          const int* step) {
+    const int face = blockIdx.z;
+    const int uu = blockIdx.x * 32 + (threadIdx.x & 31), vv = blockIdx.y * 8 + (threadIdx.x >> 5);
+    switch (face / 2) {
+        case 0: source_dj<0>(g, ob, face, uu, vv, step, Fr, Fi, D); break;
+        case 1: source_dj<1>(g, ob, face, uu, vv, step, Fr, Fi, D); break;
+        default: source_dj<2>(g, ob, face, uu, vv, step, Fr, Fi, D); break;
+    }
+}
+
+// Term q of a face normal to A: its field component and its n cell offsets
+// (di, dj, dk) -- the same stencils as face_terms.
+__device__ __forceinline__ int term_stencil(int A, int q, int d[4][3]) {
+    for (int r = 0; r < 4; ++r) d[r][0] = d[r][1] = d[r][2] = 0;
+    // the two tangential axes (b, c) and the normal A
+    const int b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
+    if (q == 0 || q == 1) {          // averages of an E component over 4 edges
+        // q0: component c-axis E with offsets {0, +b, +A, +A+b}; q1: component b-axis E, {0, +A, +c, +A+c}
+        // (the order of face_terms for each A)
+        if (A == 0) {
+            if (q == 0) { d[1][1] = 1; d[2][0] = 1; d[3][0] = 1; d[3][1] = 1; return 2 * 10 + 4; }   // Ez {0,X,1,1+X}
+            d[1][0] = 1; d[2][2] = 1; d[3][0] = 1; d[3][2] = 1; return 1 * 10 + 4;                    // Ey {0,1,P,1+P}
+        } else if (A == 1) {
+            if (q == 0) { d[1][0] = 1; d[2][1] = 1; d[3][0] = 1; d[3][1] = 1; return 2 * 10 + 4; }   // Ez {0,1,X,1+X}
+            d[1][1] = 1; d[2][2] = 1; d[3][1] = 1; d[3][2] = 1; return 0 * 10 + 4;                    // Ex {0,X,P,X+P}
+        } else {
+            if (q == 0) { d[1][0] = 1; d[2][2] = 1; d[3][0] = 1; d[3][2] = 1; return 1 * 10 + 4; }   // Ey {0,1,P,1+P}
+            d[1][1] = 1; d[2][2] = 1; d[3][1] = 1; d[3][2] = 1; return 0 * 10 + 4;                    // Ex {0,X,P,X+P}
+        }
+    }
+    (void)b; (void)c;
+    if (A == 0) {
+        if (q == 2) { d[1][2] = 1; return 5 * 10 + 2; }   // Hz {0,P}
+        d[1][1] = 1; return 4 * 10 + 2;                   // Hy {0,X}
+    } else if (A == 1) {
+        if (q == 2) { d[1][2] = 1; return 5 * 10 + 2; }   // Hz {0,P}
+        d[1][0] = 1; return 3 * 10 + 2;                   // Hx {0,1}
+    } else {
+        if (q == 2) { d[1][1] = 1; return 4 * 10 + 2; }   // Hy {0,X}
+        d[1][0] = 1; return 3 * 10 + 2;                   // Hx {0,1}
+    }
+}
+
+// One thread per cell of the shells {lo, lo+1} and {hi, hi+1} of each axis
+// of the far-field box (tangentially lo..hi+1), each cell taken by the first
+// shell holding it; grid z = shell * 2 + layer.
+template <typename IdxT>
+__global__ void __launch_bounds__(256) k_adj_source_gather(Grid g, Objective ob, const float* __restrict__ D,
+                                                           float* lEx, float* lEy, float* lEz, float* lHx,
+                                                           float* lHy, float* lHz, float* mEx, float* mEy,
+                                                           float* mEz, const IdxT* idx, const float* tup) {
+    const int sh = (int)blockIdx.z >> 1, layer = (int)blockIdx.z & 1;
+    const int A = sh / 2;
+    if (ob.pbc[A]) return;
+    const int b = A == 0 ? 1 : 0, c = A == 2 ? 1 : 2;
+    const int uu = blockIdx.x * 32 + (threadIdx.x & 31), vv = blockIdx.y * 8 + (threadIdx.x >> 5);
+    if (uu > ob.hi[b] - ob.lo[b] + 1 || vv > ob.hi[c] - ob.lo[c] + 1) return;
+    int x[3];
+    x[A] = ((sh & 1) ? ob.hi[A] : ob.lo[A]) + layer;
+    x[b] = ob.lo[b] + uu;
+    x[c] = ob.lo[c] + vv;
+    // taken by an earlier shell?
+    for (int s2 = 0; s2 < sh; ++s2) {
+        const int A2 = s2 / 2;
+        if (ob.pbc[A2]) continue;
+        const int p2 = (s2 & 1) ? ob.hi[A2] : ob.lo[A2];
+        bool in = x[A2] == p2 || x[A2] == p2 + 1;
+        for (int a = 0; a < 3; ++a)
+            if (a != A2 && (x[a] < ob.lo[a] || x[a] > ob.hi[a] + 1)) in = false;
+        if (in) return;
+    }
+    const long long o = off(g, x[0], x[1], x[2]);
+    float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    bool hit[6] = {false, false, false, false, false, false};
+    for (int f = 0; f < 6; ++f) {
+        const int Af = f / 2;
+        if (ob.pbc[Af]) continue;
+        const int bf = Af == 0 ? 1 : 0, cf = Af == 2 ? 1 : 2;
+        const int pf = (f & 1) ? ob.hi[Af] : ob.lo[Af];
+        // a face reaches only the two planes pf, pf+1 and lo..hi+1 across
+        if ((x[Af] != pf && x[Af] != pf + 1) || x[bf] < ob.lo[bf] || x[bf] > ob.hi[bf] + 1 || x[cf] < ob.lo[cf] ||
+            x[cf] > ob.hi[cf] + 1)
+            continue;
+        for (int q = 0; q < 4; ++q) {
+            int d[4][3];
+            const int code = term_stencil(Af, q, d);
+            const int comp = code / 10, n = code % 10;
+            for (int r = 0; r < n; ++r) {
+                const int y0 = x[0] - d[r][0], y1 = x[1] - d[r][1], y2 = x[2] - d[r][2];
+                const int y[3] = {y0, y1, y2};
+                if (y[Af] != pf || y[bf] < ob.lo[bf] || y[bf] > ob.hi[bf] || y[cf] < ob.lo[cf] || y[cf] > ob.hi[cf])
+                    continue;
+                const float dj = D[(((long long)f * 4 + q) * ob.mv + (y[cf] - ob.lo[cf])) * ob.mu + (y[bf] - ob.lo[bf])];
+                acc[comp] = acc[comp] + dj;
+                hit[comp] = true;
+            }
+        }
+    }
+#pragma unroll
+    for (int comp = 0; comp < 6; ++comp) {
+        if (!hit[comp]) continue;
+        float* L = pick6(comp, lEx, lEy, lEz, lHx, lHy, lHz);
+        L[o] = L[o] + acc[comp];
+        if (comp < 3) {   // keep mu = gb lambda_E of the E components
+            float* M = pick6(comp, mEx, mEy, mEz, mEx, mEy, mEz);
+            M[o] = M[o] + tup[(int)idx[o] * 6 + 2 * comp + 1] * acc[comp];
+        }
+    }
+}
+
 // Design-box record of one step (between the H and E phases): the E value
 // each design edge starts the step with and the curl term S of its update
 // E' = ga E + gb S, both in the bulk kernel's FP32 arithmetic.
@@ -2853,6 +2983,66 @@ __global__ void k_adj_design(Grid g, DesignDev d, float* lEx, float* lEy, float*
     atomicAdd(&lHy[o - 1], -(mz * dx_i));
     atomicAdd(&lHx[o - g.PX], mz * dy_j);
     atomicAdd(&lHx[o], -(mz * dy_j));
+}
+
+// The same transpose in a fixed order: k_adj_design_m stores m = gb lambda_E
+// of every design edge ([3][nedge]) and scales lambda_E by ga;
+// k_adj_design_gather gives every H cell of the box extended by one below
+// to one thread, which adds the terms of the edges that reach it.
+__global__ void k_adj_design_m(Grid g, DesignDev d, float* lEx, float* lEy, float* lEz, float* m) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= d.nedge) return;
+    const int i = d.lo[0] + (int)(e % d.n[0]);
+    const long long rr = e / d.n[0];
+    const int j = d.lo[1] + (int)(rr % d.n[1]), k = d.lo[2] + (int)(rr / d.n[1]);
+    const long long o = off(g, i, j, k), n = d.nedge;
+    const float* c = d.coef;
+    const float lx = lEx[o], ly = lEy[o], lz = lEz[o];
+    m[e] = c[n + e] * lx;
+    m[n + e] = c[3 * n + e] * ly;
+    m[2 * n + e] = c[5 * n + e] * lz;
+    lEx[o] = c[e] * lx;
+    lEy[o] = c[2 * n + e] * ly;
+    lEz[o] = c[4 * n + e] * lz;
+}
+
+__global__ void k_adj_design_gather(Grid g, DesignDev d, const float* __restrict__ m, float* lHx, float* lHy,
+                                    float* lHz, const float* dex, const float* dey, const float* dez) {
+    const int ex = d.n[0] + 1, ey = d.n[1] + 1, ez = d.n[2] + 1;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ex * ey * ez) return;
+    const int i = d.lo[0] - 1 + (int)(t % ex);
+    const long long rr = t / ex;
+    const int j = d.lo[1] - 1 + (int)(rr % ey), k = d.lo[2] - 1 + (int)(rr / ey);
+    const long long n = d.nedge;
+    // edge index of cell (a, b, c), -1 outside the design box
+    auto edge = [&](int a, int b2, int c2) -> long long {
+        const int u = a - d.lo[0], v = b2 - d.lo[1], w = c2 - d.lo[2];
+        if (u < 0 || v < 0 || w < 0 || u >= d.n[0] || v >= d.n[1] || w >= d.n[2]) return -1;
+        return (long long)u + (long long)d.n[0] * ((long long)v + (long long)d.n[1] * w);
+    };
+    const long long e0 = edge(i, j, k), ei = edge(i + 1, j, k), ej = edge(i, j + 1, k), ek = edge(i, j, k + 1);
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    bool hx = false, hy = false, hz = false;
+    // Hx: + my dz (edge t), - my dz (edge k+1), + mz dy (edge j+1), - mz dy (edge t)
+    if (e0 >= 0) { sx = sx + m[n + e0] * dez[k - 1]; hx = true; }
+    if (ek >= 0) { sx = sx + -(m[n + ek] * dez[k]); hx = true; }
+    if (ej >= 0) { sx = sx + m[2 * n + ej] * dey[j]; hx = true; }
+    if (e0 >= 0) { sx = sx + -(m[2 * n + e0] * dey[j - 1]); }
+    // Hy: + mx dz (edge k+1), - mx dz (edge t), + mz dx (edge t), - mz dx (edge i+1)
+    if (ek >= 0) { sy = sy + m[ek] * dez[k]; hy = true; }
+    if (e0 >= 0) { sy = sy + -(m[e0] * dez[k - 1]); hy = true; }
+    if (e0 >= 0) { sy = sy + m[2 * n + e0] * dex[i - 1]; }
+    if (ei >= 0) { sy = sy + -(m[2 * n + ei] * dex[i]); hy = true; }
+    // Hz: + mx dy (edge t), - mx dy (edge j+1), + my dx (edge i+1), - my dx (edge t)
+    if (e0 >= 0) { sz = sz + m[e0] * dey[j - 1]; hz = true; }
+    if (ej >= 0) { sz = sz + -(m[ej] * dey[j]); hz = true; }
+    if (ei >= 0) { sz = sz + m[n + ei] * dex[i]; hz = true; }
+    if (e0 >= 0) { sz = sz + -(m[n + e0] * dex[i - 1]); }
+    const long long o = off(g, i, j, k);
+    if (hx) lHx[o] = lHx[o] + sx;
+    if (hy) lHy[o] = lHy[o] + sy;
+    if (hz) lHz[o] = lHz[o] + sz;
 }
 
 __global__ void k_step_dec(int* step) { *step = *step - 1; }

@@ -560,3 +560,56 @@ def test_farfield_dft_matches_objective_and_restatement():
         worst = max(worst, abs(got - want) / np.abs(ff[a, q]).max())
     print(f"running-DFT far field vs restatement: worst {worst:.2e}")
     assert worst <= 1e-5
+
+
+def test_design_gradient_single_pass():
+    """A deck that qualifies for the single-pass step (CPML, no TF/SF; here
+    source-free with an initial field, as the adjoint takes no ports) runs
+    its recorded forward as the single pass, the design box's update
+    following it in k_e_design: fields and objective equal the split
+    launches bit for bit, and so does the design gradient (the backward
+    sweep is deterministic: test_gradient_is_deterministic)."""
+    from tests.test_gpu_parity import single_pass_deck
+    deck = single_pass_deck(steps=240)
+    deck.ports = []
+    deck.excitation = "none"
+    text = deck.text()
+    lo, hi = (25, 15, 22), (30, 19, 25)   # clear of the wire and the sphere
+    shape = (hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+    eps = np.random.default_rng(7).uniform(1.0, 3.0, shape).astype(np.float32)
+    out = []
+    for sp in (True, False):
+        sim = Simulation(Plan.from_text(text), ntff="off", single_pass=sp)
+        ez = np.zeros(sim.plan.shape, np.float32)
+        ez[17:21, 20:25, 33:38] = np.random.default_rng(3).standard_normal((4, 5, 5)).astype(np.float32)
+        sim.set_field("Ez", ez)
+        sim.set_objective(60.0, 0.0, 4e9, "theta")
+        sim.design_box(lo, hi)
+        sim.set_design_eps(eps)
+        assert sim.single_pass == sp
+        sim.run()
+        fields = [sim.field(n) for n in NAMES]
+        out.append((fields, sim.objective(), sim.design_gradient()))
+        sim.close()
+    (fa, Fa, ga), (fb, Fb, gb) = out
+    for n, x, y in zip(NAMES, fa, fb):
+        assert np.array_equal(x, y), n
+    assert Fa == Fb and abs(Fa) > 0
+    assert np.array_equal(ga, gb) and np.abs(ga).max() > 0
+
+
+def test_gradient_is_deterministic():
+    """The backward sweep sums in a fixed order (no atomics): two runs of the
+    same design problem give bit-identical gradients, per voxel and per
+    material."""
+    eps = design_eps()
+    g = []
+    for _ in range(2):
+        sim = Simulation(Plan.from_text(design_deck().text()), ntff="off")
+        sim.set_objective(THETA, PHI, FREQ, "theta")
+        sim.design_box(BOX_LO, BOX_HI)
+        sim.set_design_eps(eps)
+        sim.run()
+        g.append(sim.design_gradient())
+        sim.close()
+    assert np.array_equal(g[0], g[1]) and np.abs(g[0]).max() > 0
