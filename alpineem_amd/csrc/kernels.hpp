@@ -2218,34 +2218,6 @@ __global__ void __launch_bounds__(256) k_objective(Grid g, Objective ob, const f
     }
 }
 
-template <int A, typename IdxT>
-__device__ __forceinline__ void source_terms(const Grid& g, const Objective& ob, int face, int uu, int vv,
-                                             const int* step, double Fr, double Fi, float* lEx, float* lEy,
-                                             float* lEz, float* lHx, float* lHy, float* lHz, float* mEx, float* mEy,
-                                             float* mEz, const IdxT* idx, const float* tup) {
-    long long o;
-    if (!face_cell<A>(g, ob, face & 1, uu, vv, o)) return;
-    double wr[4], wi[4];
-    face_weights<A>(ob, face, uu, vv, *step, wr, wi);
-    FaceTerm t[4];
-    face_terms<A>(g, (face & 1) ? -1.0 : 1.0, t);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const float dj = (float)(2.0 * (Fr * wr[q] + Fi * wi[q]) * t[q].w);
-        float* Lq = pick6(t[q].comp, lEx, lEy, lEz, lHx, lHy, lHz) + o;
-        const int nq = t[q].n;
-        for (int r = 0; r < 4; ++r) {
-            if (r >= nq) break;
-            atomicAdd(&Lq[t[q].offs[r]], dj);
-            if (t[q].comp < 3) {   // keep mu = gb lambda_E of the E components
-                const long long oc = o + t[q].offs[r];
-                float* Mq = pick6(t[q].comp, mEx, mEy, mEz, mEx, mEy, mEz);
-                atomicAdd(&Mq[oc], tup[(int)idx[oc] * 6 + 2 * t[q].comp + 1] * dj);
-            }
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Running-DFT far field: the frequency-domain near-to-far-field transform of
 // the same face currents, at n_ang directions and n_freq frequencies
@@ -2402,22 +2374,8 @@ __global__ void __launch_bounds__(256) k_ffdft_sum(FfDft p, int ntile, const int
 }
 
 // Adjoint source of J = |F|^2 at step n: dJ/d(term) = 2 Re(conj(F) w),
-// scattered onto the adjoint fields through each term's average.
-template <typename IdxT>
-__global__ void __launch_bounds__(256) k_adj_source(Grid g, Objective ob, double Fr, double Fi, float* lEx,
-                                                    float* lEy, float* lEz, float* lHx, float* lHy, float* lHz,
-                                                    float* mEx, float* mEy, float* mEz, const IdxT* idx,
-                                                    const float* tup, const int* step) {
-    const int face = blockIdx.z;
-    const int uu = blockIdx.x * 32 + (threadIdx.x & 31), vv = blockIdx.y * 8 + (threadIdx.x >> 5);
-    switch (face / 2) {
-        case 0: source_terms<0>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, tup); break;
-        case 1: source_terms<1>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, tup); break;
-        default: source_terms<2>(g, ob, face, uu, vv, step, Fr, Fi, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, tup); break;
-    }
-}
-
-// The same source in a fixed summation order (run to run bit-identical):
+// spread onto the adjoint fields through each term's average, in a fixed
+// summation order (run to run bit-identical):
 // k_adj_source_terms stores each face cell's four increments dj, and
 // k_adj_source_gather gives every adjoint element near the far-field box to
 // one thread, which adds the increments that reach it in face, then term
@@ -2738,7 +2696,7 @@ __device__ __forceinline__ bool near_slab(const Grid& g, const Cpml& p, int i, i
 
 // The adjoint state keeps mu_c = gb_c lambda_E_c beside lambda_E (0 on the
 // fixed outer tangential components): k_adj_e writes it with the new
-// lambda_E, k_adj_source adds gb times its own increments, and k_adj_mu
+// lambda_E, k_adj_source_gather adds gb times its own increments, and k_adj_mu
 // rebuilds it after a host write.  The H gather then needs no tuple lookups.
 
 // Transposed E phase, gathered per H cell (one thread per cell of
@@ -2952,40 +2910,8 @@ __global__ void k_e_design(Grid g, DesignDev d, float* Ex, float* Ey, float* Ez,
     }
 }
 
-// Transposed k_e_design: lambda_H += S^T (gb lambda_E'), lambda_E <- ga lambda_E'.
-__global__ void k_adj_design(Grid g, DesignDev d, float* lEx, float* lEy, float* lEz, float* lHx, float* lHy,
-                             float* lHz, const float* dex, const float* dey, const float* dez) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= d.nedge) return;
-    const int i = d.lo[0] + (int)(e % d.n[0]);
-    const long long rr = e / d.n[0];
-    const int j = d.lo[1] + (int)(rr % d.n[1]), k = d.lo[2] + (int)(rr / d.n[1]);
-    const long long o = off(g, i, j, k), n = d.nedge;
-    const float dx_i = dex[i - 1], dy_j = dey[j - 1], dz_k = dez[k - 1];
-    const float* c = d.coef;
-    const float lx = lEx[o], ly = lEy[o], lz = lEz[o];
-    const float mx = c[n + e] * lx, my = c[3 * n + e] * ly, mz = c[5 * n + e] * lz;
-    lEx[o] = c[e] * lx;
-    lEy[o] = c[2 * n + e] * ly;
-    lEz[o] = c[4 * n + e] * lz;
-    // Ex: (Hz - Hz(j-1)) dy + (Hy(k-1) - Hy) dz
-    atomicAdd(&lHz[o], mx * dy_j);
-    atomicAdd(&lHz[o - g.PX], -(mx * dy_j));
-    atomicAdd(&lHy[o - g.plane], mx * dz_k);
-    atomicAdd(&lHy[o], -(mx * dz_k));
-    // Ey: (Hz(i-1) - Hz) dx + (Hx - Hx(k-1)) dz
-    atomicAdd(&lHz[o - 1], my * dx_i);
-    atomicAdd(&lHz[o], -(my * dx_i));
-    atomicAdd(&lHx[o], my * dz_k);
-    atomicAdd(&lHx[o - g.plane], -(my * dz_k));
-    // Ez: (Hy - Hy(i-1)) dx + (Hx(j-1) - Hx) dy
-    atomicAdd(&lHy[o], mz * dx_i);
-    atomicAdd(&lHy[o - 1], -(mz * dx_i));
-    atomicAdd(&lHx[o - g.PX], mz * dy_j);
-    atomicAdd(&lHx[o], -(mz * dy_j));
-}
-
-// The same transpose in a fixed order: k_adj_design_m stores m = gb lambda_E
+// Transposed k_e_design, lambda_H += S^T (gb lambda_E'), lambda_E <- ga lambda_E',
+// in a fixed order: k_adj_design_m stores m = gb lambda_E
 // of every design edge ([3][nedge]) and scales lambda_E by ga;
 // k_adj_design_gather gives every H cell of the box extended by one below
 // to one thread, which adds the terms of the edges that reach it.
