@@ -116,13 +116,27 @@ __device__ __forceinline__ T ld_once(const T* p) { return __builtin_nontemporal_
 template <class T>
 __device__ __forceinline__ void st_once(T v, T* p) { __builtin_nontemporal_store(v, p); }
 
+// The same by byte offset from an array base.
+template <class T, class OffT>
+__device__ __forceinline__ T ldo(const T* p, OffT ob) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + ob);
+}
+template <class OffT>
+__device__ __forceinline__ float ldo_once(const float* p, OffT ob) {
+    return __builtin_nontemporal_load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + ob));
+}
+template <class OffT>
+__device__ __forceinline__ void sto_once(float v, float* p, OffT ob) {
+    __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(p) + ob));
+}
+
 // ---------------------------------------------------------------------------
 // Bulk H (f90:4442-4484).  One thread per (i,j) column marching b.kc planes;
 // Ex(k+1), Ey(k+1) are carried in registers to the next plane.  Slab cells
 // of the box then add their CPML corrections in the reference's per-cell
 // order (x, then y, then z slab), reusing the loaded neighbours.
 // ---------------------------------------------------------------------------
-template <int C, bool PP>
+template <int C, bool PP, typename OffT>
 __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx, int by, int bz,
                                             float* __restrict__ Hx, float* __restrict__ Hy, float* __restrict__ Hz,
                                             float* __restrict__ Hxo, float* __restrict__ Hyo,
@@ -140,20 +154,23 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
     int qx = 0, qy = 0;
     const int fx = (C & kCx) ? slab_h(i, g.NX, cp.n[0], cp.n[1], qx) : -1;
     const int fy = (C & kCy) ? slab_h(j, g.NY, cp.n[2], cp.n[3], qy) : -1;
-    long long o = off(g, i, j, k0);
-    float ey_k = Ey[o], ex_k = Ex[o];
+    // byte offsets from the array bases (OffT: 32 bits where the arrays stay
+    // under 4 GiB, so one VGPR addresses every array)
+    const OffT PB = (OffT)g.PX * 4u, ZB = (OffT)g.plane * 4u;
+    OffT o = (OffT)off(g, i, j, k0) * 4u;
+    float ey_k = ldo(Ey, o), ex_k = ldo(Ex, o);
     for (int k = k0; k <= k1; ++k) {
-        const float ey_k1 = Ey[o + g.plane];
-        const float ex_k1 = Ex[o + g.plane];
-        const float ez = Ez[o];
-        const float ez_xp = Ez[o + 1];
-        const float ez_yp = Ez[o + g.PX];
-        const float ey_xp = Ey[o + 1];
-        const float ex_yp = Ex[o + g.PX];
+        const float ey_k1 = ldo(Ey, o + ZB);
+        const float ex_k1 = ldo(Ex, o + ZB);
+        const float ez = ldo(Ez, o);
+        const float ez_xp = ldo(Ez, o + 4u);
+        const float ez_yp = ldo(Ez, o + PB);
+        const float ey_xp = ldo(Ey, o + 4u);
+        const float ex_yp = ldo(Ex, o + PB);
         const float dz_k = dhz[k - 1];
-        float hx = ld_once(&Hx[o]) + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
-        float hy = ld_once(&Hy[o]) + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
-        float hz = ld_once(&Hz[o]) + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
+        float hx = ldo_once(Hx, o) + DB * ((ez - ez_yp) * dy_j + (ey_k1 - ey_k) * dz_k);
+        float hy = ldo_once(Hy, o) + DB * ((ez_xp - ez) * dx_i + (ex_k - ex_k1) * dz_k);
+        float hz = ldo_once(Hz, o) + DB * ((ey_k - ey_xp) * dx_i + (ex_yp - ex_k) * dy_j);
         if (C) {
             if ((C & kCx) && fx >= 0) {                      // psi_Hyx, psi_Hzx
                 const int f = fx;
@@ -193,12 +210,12 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
                 hy = hy + DB * p1;
             }
         }
-        st_once(hx, PP ? &Hxo[o] : &Hx[o]);
-        st_once(hy, PP ? &Hyo[o] : &Hy[o]);
-        st_once(hz, PP ? &Hzo[o] : &Hz[o]);
+        sto_once(hx, PP ? Hxo : Hx, o);
+        sto_once(hy, PP ? Hyo : Hy, o);
+        sto_once(hz, PP ? Hzo : Hz, o);
         ey_k = ey_k1;
         ex_k = ex_k1;
-        o += g.plane;
+        o += ZB;
     }
 }
 
@@ -207,7 +224,7 @@ __device__ __forceinline__ void h_bulk_body(const Grid& g, const Box& b, int bx,
 // Component ranges: Ex i>=1,j>=2,k>=2; Ey i>=2,j>=1,k>=2; Ez i>=2,j>=2,k>=1.
 // Slab cells of the box append their CPML corrections as in k_h_bulk.
 // ---------------------------------------------------------------------------
-template <typename IdxT, int C, bool PP>
+template <typename IdxT, int C, bool PP, typename OffT>
 __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx, int by, int bz,
                                             float* __restrict__ Ex, float* __restrict__ Ey, float* __restrict__ Ez,
                                             float* __restrict__ Exo, float* __restrict__ Eyo,
@@ -226,29 +243,32 @@ __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx,
     int qx = 0, tx = 0, qy = 0, ty = 0;
     const int fx = (C & kCx) ? slab_e(i, g.NX, cp.n[0], cp.n[1], qx, tx) : -1;
     const int fy = (C & kCy) ? slab_e(j, g.NY, cp.n[2], cp.n[3], qy, ty) : -1;
-    long long o = off(g, i, j, k0);
+    const OffT PB = (OffT)g.PX * 4u, ZB = (OffT)g.plane * 4u;
+    OffT o = (OffT)off(g, i, j, k0) * 4u;   // byte offset, as in h_bulk_body
+    OffT oi = (OffT)off(g, i, j, k0) * (OffT)sizeof(IdxT);
+    const OffT ZI = (OffT)g.plane * (OffT)sizeof(IdxT);
     float hx_km = 0.f, hy_km = 0.f;
     if (k0 >= 2) {
-        hx_km = Hx[o - g.plane];
-        hy_km = Hy[o - g.plane];
+        hx_km = ldo(Hx, o - ZB);
+        hy_km = ldo(Hy, o - ZB);
     }
     // i-1 / j-1 neighbours through clamped offsets: at i = 1 (j = 1) the load
     // reads the cell itself, and every component that would use it (the
     // outer PEC tangential E) is left unwritten, so all loads of an
     // iteration issue together without branches.
-    const long long dxm = ix ? 1 : 0, dym = jy ? g.PX : 0;
-    int t_next = (int)idx[o];   // coefficient index, loaded one plane ahead
+    const OffT dxm = ix ? 4u : 0u, dym = jy ? PB : (OffT)0;
+    int t_next = (int)ldo(idx, oi);   // coefficient index, loaded one plane ahead
     for (int k = k0; k <= k1; ++k) {
-        const float hx = Hx[o], hy = Hy[o], hz = Hz[o];
+        const float hx = ldo(Hx, o), hy = ldo(Hy, o), hz = ldo(Hz, o);
         const int t = t_next * 6;
-        t_next = (int)idx[o + (k < k1 ? g.plane : 0)];
+        t_next = (int)ldo(idx, oi + (k < k1 ? ZI : (OffT)0));
         const float dz_k = dez[k - 1];
         const bool ex_on = jy && k >= 2, ey_on = ix && k >= 2, ez_on = ix && jy;
-        const float hz_ym = Hz[o - dym];
-        const float hz_xm = Hz[o - dxm];
-        const float hy_xm = Hy[o - dxm];
-        const float hx_ym = Hx[o - dym];
-        const float ex0 = ld_once(&Ex[o]), ey0 = ld_once(&Ey[o]), ez0 = ld_once(&Ez[o]);
+        const float hz_ym = ldo(Hz, o - dym);
+        const float hz_xm = ldo(Hz, o - dxm);
+        const float hy_xm = ldo(Hy, o - dxm);
+        const float hx_ym = ldo(Hx, o - dym);
+        const float ex0 = ldo_once(Ex, o), ey0 = ldo_once(Ey, o), ez0 = ldo_once(Ez, o);
         const float gax = tup[t + 0], gbx = tup[t + 1], gay = tup[t + 2], gby = tup[t + 3], gaz = tup[t + 4],
                     gbz = tup[t + 5];
         float ex = gax * ex0 + gbx * ((hz - hz_ym) * dy_j + (hy_km - hy) * dz_k);
@@ -307,12 +327,13 @@ __device__ __forceinline__ void e_bulk_body(const Grid& g, const Box& b, int bx,
         }
         // components outside the reference's update ranges get their own
         // value back; storing every component keeps the loads unconditional
-        st_once(ex_on ? ex : ex0, PP ? &Exo[o] : &Ex[o]);
-        st_once(ey_on ? ey : ey0, PP ? &Eyo[o] : &Ey[o]);
-        st_once(ez_on ? ez : ez0, PP ? &Ezo[o] : &Ez[o]);
+        sto_once(ex_on ? ex : ex0, PP ? Exo : Ex, o);
+        sto_once(ey_on ? ey : ey0, PP ? Eyo : Ey, o);
+        sto_once(ez_on ? ez : ez0, PP ? Ezo : Ez, o);
         hx_km = hx;
         hy_km = hy;
-        o += g.plane;
+        o += ZB;
+        oi += ZI;
     }
 }
 
@@ -371,39 +392,39 @@ __device__ __forceinline__ BatchEntry entry_of(const BatchEntry* __restrict__ e,
     return r;
 }
 
-template <int C, bool PP = false>
+template <int C, bool PP = false, typename OffT = uint64_t>
 __global__ void __launch_bounds__(kTX * kTY) k_h_bulk(HArgs a, BoxSet bs) {
     int bx, by, bz;
     const Box b = box_of(bs, bx, by, bz);
-    h_bulk_body<C, PP>(a.g, b, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Hxo, a.Hyo, a.Hzo, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy,
+    h_bulk_body<C, PP, OffT>(a.g, b, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Hxo, a.Hyo, a.Hzo, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy,
                        a.dhz, a.DB, a.cp, a.dx, a.dy, a.dz);
 }
 
-template <int C>
+template <int C, typename OffT = uint64_t>
 __global__ void __launch_bounds__(kTX * kTY) k_h_bulk_batch(const HArgs* __restrict__ args,
                                                             const BatchEntry* __restrict__ ents, int n_ents) {
     int bx, by, bz;
     const BatchEntry e = entry_of(ents, n_ents, bx, by, bz);
     const HArgs& a = args[e.member];
-    h_bulk_body<C, false>(a.g, e.box, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy,
+    h_bulk_body<C, false, OffT>(a.g, e.box, bx, by, bz, a.Hx, a.Hy, a.Hz, a.Hx, a.Hy, a.Hz, a.Ex, a.Ey, a.Ez, a.dhx, a.dhy,
                           a.dhz, a.DB, a.cp, a.dx, a.dy, a.dz);
 }
 
-template <typename IdxT, int C, bool PP = false>
+template <typename IdxT, int C, bool PP = false, typename OffT = uint64_t>
 __global__ void __launch_bounds__(kTX * kTY) k_e_bulk(EArgs a, BoxSet bs) {
     int bx, by, bz;
     const Box b = box_of(bs, bx, by, bz);
-    e_bulk_body<IdxT, C, PP>(a.g, b, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Exo, a.Eyo, a.Ezo, a.Hx, a.Hy, a.Hz,
+    e_bulk_body<IdxT, C, PP, OffT>(a.g, b, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Exo, a.Eyo, a.Ezo, a.Hx, a.Hy, a.Hz,
                              static_cast<const IdxT*>(a.idx), a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
 }
 
-template <typename IdxT, int C>
+template <typename IdxT, int C, typename OffT = uint64_t>
 __global__ void __launch_bounds__(kTX * kTY) k_e_bulk_batch(const EArgs* __restrict__ args,
                                                             const BatchEntry* __restrict__ ents, int n_ents) {
     int bx, by, bz;
     const BatchEntry e = entry_of(ents, n_ents, bx, by, bz);
     const EArgs& a = args[e.member];
-    e_bulk_body<IdxT, C, false>(a.g, e.box, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz,
+    e_bulk_body<IdxT, C, false, OffT>(a.g, e.box, bx, by, bz, a.Ex, a.Ey, a.Ez, a.Ex, a.Ey, a.Ez, a.Hx, a.Hy, a.Hz,
                                 static_cast<const IdxT*>(a.idx), a.tup, a.dex, a.dey, a.dez, a.cp, a.dx, a.dy, a.dz);
 }
 

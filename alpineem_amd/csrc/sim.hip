@@ -234,6 +234,7 @@ struct aem_sim {
     int ntff_mu = 0, ntff_mv = 0;  // largest face extents of the far-field box
     int planes = kPlanes;
     bool planes_set = false;       // planes given by the caller (also used by the single-pass kernel)
+    bool off32 = false;            // every field array under 4 GiB: 32-bit byte offsets in the kernels
     bool allow_fuse = true;
     // single-pass step: fields and x-face psi alternate between two buffer
     // sets, so a step reads one set and writes the other
@@ -534,8 +535,10 @@ void aem_sim::create(const aem_sim_options* opt) {
         }
     // the single-pass kernel takes the core launch's box (every x, y and z
     // outside the y/z slabs); the slab launch stays as it is
-    // k_step_sp addresses the fields by 32-bit byte offsets
+    // k_step_sp addresses the fields by 32-bit byte offsets, as the bulk
+    // kernels do where they can
     const bool small = ((unsigned long long)v.ncell + (unsigned long long)v.plane) * 4ull < (1ull << 32);
+    off32 = small;
     sp_ok = allow_sp && fuse_cpml && small && single_pass_ok(v) && !bulk.empty() && bulk[0].c == kCx &&
             bulk[0].bs.n == 1;
     if (sp_ok) {
@@ -625,15 +628,26 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     const dim3 bb(32, 8, 1);
     const dim3 bt(kTX, kTY, 1);   // bulk tile
     const int T = v.time_steps;
+    // 32-bit byte offsets where the arrays stay under 4 GiB
     auto h_bulk = [&](const Launch& L) {
-        switch (L.c) {
+        if (off32) switch (L.c) {
+            case 0: k_h_bulk<0, false, uint32_t><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
+            case kCx: k_h_bulk<kCx, false, uint32_t><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
+            default: k_h_bulk<kCx | kCy | kCz, false, uint32_t><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
+        }
+        else switch (L.c) {
             case 0: k_h_bulk<0><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
             case kCx: k_h_bulk<kCx><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
             default: k_h_bulk<kCx | kCy | kCz><<<L.blocks(), bt, 0, s>>>(ha, L.bs); break;
         }
     };
     auto e_bulk = [&](const Launch& L) {
-        switch (L.c) {
+        if (off32) switch (L.c) {
+            case 0: k_e_bulk<IdxT, 0, false, uint32_t><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
+            case kCx: k_e_bulk<IdxT, kCx, false, uint32_t><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
+            default: k_e_bulk<IdxT, kCx | kCy | kCz, false, uint32_t><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
+        }
+        else switch (L.c) {
             case 0: k_e_bulk<IdxT, 0><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
             case kCx: k_e_bulk<IdxT, kCx><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
             default: k_e_bulk<IdxT, kCx | kCy | kCz><<<L.blocks(), bt, 0, s>>>(ea, L.bs); break;
@@ -666,9 +680,11 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         es.Hx = F2[3];
         es.Hy = F2[4];
         es.Hz = F2[5];
-        for (size_t q = 1; q < bulk.size(); ++q) k_h_bulk<kCx | kCy | kCz, true><<<bulk[q].blocks(), bt, 0, s>>>(ha, bulk[q].bs);
+        // (single-pass decks always have 32-bit offsets)
         for (size_t q = 1; q < bulk.size(); ++q)
-            k_e_bulk<IdxT, kCx | kCy | kCz, true><<<bulk[q].blocks(), bt, 0, s>>>(es, bulk[q].bs);
+            k_h_bulk<kCx | kCy | kCz, true, uint32_t><<<bulk[q].blocks(), bt, 0, s>>>(ha, bulk[q].bs);
+        for (size_t q = 1; q < bulk.size(); ++q)
+            k_e_bulk<IdxT, kCx | kCy | kCz, true, uint32_t><<<bulk[q].blocks(), bt, 0, s>>>(es, bulk[q].bs);
         flip();
         Ex = F[0]; Ey = F[1]; Ez = F[2]; Hx = F[3]; Hy = F[4]; Hz = F[5];
     } else if (parts & kPartHBulk) {
@@ -1173,6 +1189,7 @@ struct aem_batch {
         unsigned blocks = 0;
     };
     std::vector<Group> hg, eg;
+    bool off32 = true;             // every member's arrays under 4 GiB
     HArgs* d_hargs = nullptr;
     EArgs* d_eargs = nullptr;
     PortArgs* d_pargs = nullptr;
@@ -1216,6 +1233,7 @@ struct aem_batch {
         }
         d_hargs = upload(ha);
         d_eargs = upload(ea);
+        for (int q = 0; q < n; ++q) off32 = off32 && m[q]->off32;
         // bulk groups: one launch per kernel variant (H: by slab tests; E: by
         // slab tests and index width), entries in member order
         auto build = [&](bool e_phase, std::vector<Group>& out) {
@@ -1275,30 +1293,46 @@ struct aem_batch {
         HIPCHK(hipStreamSynchronize(st));
     }
 
+    // batched bulk launches; 32-bit byte offsets when every member's arrays
+    // stay under 4 GiB
+    template <class OffT>
+    void launch_h_t(const Group& gr, hipStream_t s) {
+        const dim3 bt(kTX, kTY, 1);
+        if (gr.c == 0) k_h_bulk_batch<0, OffT><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
+        else if (gr.c == kCx) k_h_bulk_batch<kCx, OffT><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
+        else k_h_bulk_batch<kCx | kCy | kCz, OffT><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
+    }
+    void launch_h(const Group& gr, hipStream_t s) {
+        if (off32) launch_h_t<uint32_t>(gr, s);
+        else launch_h_t<uint64_t>(gr, s);
+    }
+    template <class IdxT, class OffT>
+    void launch_e_t(const Group& gr, hipStream_t s) {
+        const dim3 bt(kTX, kTY, 1);
+        if (gr.c == 0) k_e_bulk_batch<IdxT, 0, OffT><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+        else if (gr.c == kCx) k_e_bulk_batch<IdxT, kCx, OffT><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+        else k_e_bulk_batch<IdxT, kCx | kCy | kCz, OffT><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
+    }
+    template <class IdxT>
+    void launch_e(const Group& gr, hipStream_t s) {
+        if (off32) launch_e_t<IdxT, uint32_t>(gr, s);
+        else launch_e_t<IdxT, uint64_t>(gr, s);
+    }
+
     void enqueue(hipStream_t s) {
-        const dim3 bt(kTX, kTY, 1), bb(32, 8, 1);
+        const dim3 bb(32, 8, 1);
         const int n = (int)m.size();
         auto each = [&](unsigned part) {
             for (aem_sim* x : m) x->enqueue_parts(nullptr, s, part);
         };
-        for (const Group& gr : hg) {
-            if (gr.c == 0) k_h_bulk_batch<0><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
-            else if (gr.c == kCx) k_h_bulk_batch<kCx><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
-            else k_h_bulk_batch<kCx | kCy | kCz><<<gr.blocks, bt, 0, s>>>(d_hargs, gr.ents, gr.n_ents);
-        }
+        for (const Group& gr : hg)
+            launch_h(gr, s);
         each(kPartHPre);
         if (n_port) k_ports_h_batch<<<n_port, 256, 0, s>>>(d_pargs);
         each(kPartHPost | kPartRecord);
         for (const Group& gr : eg) {
-            if (gr.idx_bytes == 1) {
-                if (gr.c == 0) k_e_bulk_batch<uint8_t, 0><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
-                else if (gr.c == kCx) k_e_bulk_batch<uint8_t, kCx><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
-                else k_e_bulk_batch<uint8_t, kCx | kCy | kCz><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
-            } else {
-                if (gr.c == 0) k_e_bulk_batch<uint16_t, 0><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
-                else if (gr.c == kCx) k_e_bulk_batch<uint16_t, kCx><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
-                else k_e_bulk_batch<uint16_t, kCx | kCy | kCz><<<gr.blocks, bt, 0, s>>>(d_eargs, gr.ents, gr.n_ents);
-            }
+            if (gr.idx_bytes == 1) launch_e<uint8_t>(gr, s);
+            else launch_e<uint16_t>(gr, s);
         }
         each(kPartEPre);
         if (n_port) k_ports_e_batch<<<n_port, 256, 0, s>>>(d_pargs);
