@@ -262,9 +262,9 @@ def main() -> int:
     if single:
         dom, bpc, t_ms = f"k_step_sp<unsigned char, {c}>", BYTES_SP, ph[0]
     elif ph[2] >= ph[0]:
-        dom, bpc, t_ms = f"k_e_bulk<unsigned char, {c}, false>", BYTES_E, ph[2]
+        dom, bpc, t_ms = f"k_e_bulk<unsigned char, {c}, false, unsigned int>", BYTES_E, ph[2]
     else:
-        dom, bpc, t_ms = f"k_h_bulk<{c}, false>", BYTES_H, ph[0]
+        dom, bpc, t_ms = f"k_h_bulk<{c}, false, unsigned int>", BYTES_H, ph[0]
     achieved = bpc * core / (t_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
