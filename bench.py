@@ -228,8 +228,8 @@ def main() -> int:
                     help="steps of the 160^3 design-gradient timing (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(z-slab sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this solver runs on one GPU; runs over several GPUs (z-slab sharding with one "
+              "halo plane per half-step) are not built", file=sys.stderr)
         return 2
 
     from alpineem_amd import decks

@@ -1,8 +1,8 @@
 """HIP path (libaem on gfx950) against the oracle and the reference fixtures.
 
 For every golden deck the device run must reproduce the reference outputs
-(tests/golden/*.npz) to a relative L2 of 1e-5 or better -- the accuracy bar
-of the north star -- and the fields left in device memory must match the
+(tests/golden/*.npz) to a relative L2 of 1e-5 or better -- the FP32 parity
+bar of this path -- and the fields left in device memory must match the
 oracle's.  Bulk, CPML, sheet, Drude and port arithmetic is order-identical
 FP32, so port voltages and fields of decks without plane-wave sources are
 expected bit-exact; the residual differences elsewhere come from the device
