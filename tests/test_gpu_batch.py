@@ -85,3 +85,24 @@ def test_batch_rejects_overrun_and_duplicates(golden_dir):
     b = Batch([a])
     with pytest.raises(RuntimeError):
         b.run(a.plan.time_steps + 1)
+
+
+def test_batch_after_single_pass_steps():
+    """A simulation that took single-pass steps first (its fields in the
+    second buffer set, after an odd number of steps) joins a batch and
+    continues with the batch's in-place launches, then alone again: its
+    state equals a run of the separate H/E launches throughout."""
+    from tests.test_gpu_parity import single_pass_deck
+    text = single_pass_deck(steps=140).text()
+    a = Simulation(Plan.from_text(text))
+    other = Simulation(Plan.from_text(single_pass_deck((40, 38, 36), steps=140).text()))
+    ref = Simulation(Plan.from_text(text), single_pass=False)
+    assert a.single_pass
+    a.run(31)
+    b = Batch([a, other])
+    assert not a.single_pass
+    b.run(40)
+    a.run(29)
+    ref.run(100)
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        np.testing.assert_array_equal(a.field(comp), ref.field(comp), err_msg=comp)
