@@ -634,8 +634,6 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
             excy[r] = ldb(Ex, c + PB);
             hyc0[r] = ldb(Hy, c);
             hzc0[r] = ldb(Hz, c);
-            ezi[r] = ldb(Ez, c + 4u);
-            eyi[r] = ldb(Ey, c + 4u);
 #pragma unroll
             for (int q = 0; q < 6; ++q) ga[r][q] = tup[t[r] + q];
             tn[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
@@ -697,7 +695,9 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
                 stb(hy, a.Hyo, o);
                 stb(hz, a.Hzo, o);
             }
-            // new Hy, Hz of the cell i0-1 (lane 0's x neighbour)
+            // new Hy, Hz of the cell i0-1 (lane 0's x neighbour); E of the cell i0 is lane 0's
+            ezi[r] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ez[r])));
+            eyi[r] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(eyk[r])));
             float hyc = hyc0[r] + DB * ((ezi[r] - ezc[r]) * dxc + (exc[r] - exc1[r]) * dzh);
             float hzc = hzc0[r] + DB * ((eyc[r] - eyi[r]) * dxc + (excy[r] - exc[r]) * dyh[r]);
             if ((C & kCx) && fxc >= 0) {
