@@ -162,9 +162,12 @@ Launch make_launch(const std::vector<Box>& boxes, int c) {
 std::vector<Launch> bulk_launches(const aem_plan_view& v, bool fuse, int planes) {
     const int N[3] = {v.NX, v.NY, v.NZ};
     auto box = [&](int a0, int a1, int b0, int b1, int c0, int c1) {
-        // z chunks of equal length, at most `planes` each
+        // z chunks of equal length, at most `planes` each; a box of at most
+        // 16 planes (a z slab) in chunks of 2, for more work-groups (at 400^3
+        // 2.5% faster slab launches than 4 planes; 1 plane the same, the whole
+        // box 23% slower)
         const int nk = c1 - c0 + 1;
-        const int kc = nk > 0 ? (int)cdiv(nk, cdiv(nk, planes)) : planes;
+        const int kc = nk > 0 ? (nk <= 16 ? 2 : (int)cdiv(nk, cdiv(nk, planes))) : planes;
         return Box{a0, b0, c0, a1 - a0 + 1, b1 - b0 + 1, nk, kc};
     };
     std::vector<Launch> out;
