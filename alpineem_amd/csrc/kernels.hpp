@@ -6,6 +6,10 @@
 //   E: bulk -> sheets -> Drude -> TF/SF (y,z,x) -> ports -> CPML (x,y,z) -> PBC -> E_old
 //   outputs: plane sums, NTFF currents + retarded-time binning, phase centre,
 //            step counter
+// Decks with nothing between the bulk H and bulk E updates of a core cell
+// (CPML, ports, materials; no sheets, TF/SF, PBC, Drude) run the bulk H and
+// E of the core box as one single-pass launch (k_step_sp) between two buffer
+// sets, the y/z CPML slabs as ping-pong bulk launches after it.
 // Every FP32 expression keeps the reference's left-to-right order and the
 // file is compiled with -ffp-contract=off, so the bulk, CPML, sheet, Drude
 // and port arithmetic is bit-identical to the serial reference.

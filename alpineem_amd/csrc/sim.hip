@@ -2,8 +2,9 @@
 //
 // aem_sim_create uploads a plan (the reference's "!$acc data" region,
 // fdtd_solver.f90:4380-4430); aem_run enqueues the leapfrog step
-// (fdtd_solver.f90:4440-8299) on the sim's stream, optionally replaying one
-// captured hipGraph per step; aem_finalize runs the post-loop products
+// (fdtd_solver.f90:4440-8299) on the sim's stream, optionally replaying a
+// captured hipGraph per step (one per buffer set when the single-pass step
+// alternates two sets of fields); aem_finalize runs the post-loop products
 // (fdtd_solver.f90:8340-8534) on the device.
 #include <hip/hip_runtime.h>
 
