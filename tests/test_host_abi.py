@@ -1,7 +1,8 @@
 """Host side of the C ABI, without a GPU.
 
 * libaem.so loads and exports every entry point that include/aem.h declares,
-  and the ctypes table in alpineem_amd/_lib.py binds exactly that set;
+  and the ctypes table in alpineem_amd/_lib.py binds exactly that set, each
+  with the declared argument count, widths and pointer positions;
 * the ctypes mirrors of aem_plan_view / aem_port_view have the C layout
   (offsets printed by a program compiled against include/aem_plan.h);
 * deck errors come back as status codes with the reference's STOP text
@@ -40,6 +41,62 @@ def test_every_declared_entry_point_is_exported():
 def test_ctypes_table_matches_header():
     bound = sorted(n for n, _, _ in _lib.SIGNATURES)
     assert bound == _declared("aem.h")
+
+
+_SCALARS = {"int32_t": C.c_int32, "int64_t": C.c_int64, "float": C.c_float, "double": C.c_double,
+            "aem_status": C.c_int}
+_OPAQUE = ("aem_plan", "aem_sim", "aem_batch", "aem_sim_options", "aem_plan_view")
+
+
+def _prototypes(header):
+    """name -> (return type, [parameter declarations]) of every aem_ function."""
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"\s+", " ", text)
+    out = {}
+    for ret, name, params in re.findall(r"([A-Za-z_][\w ]*?\s*\**)\s*\b(aem_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        ps = [p.strip() for p in params.split(",")]
+        out[name] = (ret.strip(), [] if ps == ["void"] else ps)
+    return out
+
+
+def _ctype_ok(decl, ct):
+    """Whether ctypes type ct passes a C value of declaration decl."""
+    stars = decl.count("*")
+    base = re.sub(r"\bconst\b", "", decl.split("*")[0]).split()
+    base = base[0] if base else ""
+    if stars == 0:
+        if decl in ("void",):
+            return ct is None
+        return ct is _SCALARS[base]
+    if stars == 1 and base == "char":
+        return ct is C.c_char_p
+    if stars >= 2:
+        return ct in (C.c_void_p, C.POINTER(C.c_void_p))
+    if base in _OPAQUE:
+        return ct is C.c_void_p
+    return ct is C.c_void_p or ct is C.POINTER(_SCALARS[base])
+
+
+def test_ctypes_arguments_match_prototypes():
+    """Every bound entry point passes as many arguments as include/aem.h
+    declares, each of the declared width (int32/int64/float/double), pointer
+    or not; the return types agree too."""
+    protos = _prototypes("aem.h")
+    bad = []
+    for name, res, args in _lib.SIGNATURES:
+        ret, params = protos[name]
+        if len(params) != len(args):
+            bad.append(f"{name}: {len(params)} parameters declared, {len(args)} bound")
+            continue
+        for p, a in zip(params, args):
+            decl = re.sub(r"\s*\b\w+$", "", p) if not p.endswith("*") else p   # drop the parameter name
+            if not _ctype_ok(decl, a):
+                bad.append(f"{name}: '{p}' bound as {a}")
+        if not _ctype_ok(ret, res):
+            bad.append(f"{name}: returns '{ret}', bound as {res}")
+    assert not bad, bad
 
 
 def test_version_and_error_strings():
