@@ -77,12 +77,15 @@ def test_series_match_reference(runs, name):
     o64 = Oracle(plan, fp64=True)
     o64.step(plan.time_steps)
     o64.finalize()
-    floors = {label: rel_l2(got, ref) for label, got, ref in _series_sets(o64, g, v)}
+    sets64 = _series_sets(o64, g, v)
+    floors = {label: rel_l2(got, ref) for label, got, ref in sets64}
+    fp64 = {label: got for label, got, _ in sets64}
     bad = []
     for label, got, ref in _series_sets(sim, g, v):
         r = rel_l2(got, ref)
         exact = int(np.sum(got == ref))
-        print(f"  {label:14s} relL2={r:.2e}  (reference vs FP64: {floors[label]:.2e})  exact {exact}/{ref.size}")
+        print(f"  {label:14s} relL2={r:.2e}  (reference vs FP64: {floors[label]:.2e}, device vs FP64: "
+              f"{rel_l2(got, fp64[label]):.2e})  exact {exact}/{ref.size}")
         if not (r <= TOL or r <= floors[label]):
             bad.append((label, r, floors[label]))
     if "ff_time" in g:
