@@ -1822,11 +1822,18 @@ __device__ __forceinline__ void ntff_hist_body(const NtffFaces& f, const NtffBin
         const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
         const int slot = comp == b ? 0 : 1;
         const int nu = f.hi[b] - f.lo[b] + 1, nv = f.hi[c] - f.lo[c] + 1;
-        const long long ncell = (long long)nu * nv * 2;
+        // each wave takes 64 consecutive cells of one face side, so at
+        // angles where a row of cells shares one time bin the whole wave
+        // does (below); the loop bound is a multiple of 128 and whole waves
+        // stay converged through it
+        const long long nuv = (long long)nu * nv;
+        const long long ncell = ((nuv + 63) >> 6) << 7;
         const float K = bn.K[a];
-        for (long long cid = threadIdx.x; cid < ncell; cid += blockDim.x) {
-            const int side = (int)(cid & 1);
-            const long long uv = cid >> 1;
+        for (long long cid = threadIdx.x; cid - (long long)(threadIdx.x & 63) < ncell; cid += blockDim.x) {
+            const int side = (int)((cid >> 6) & 1);
+            const long long uvr = ((cid >> 7) << 6) | (cid & 63);
+            const bool live = uvr < nuv;
+            const long long uv = live ? uvr : 0;
             const int u = f.lo[b] + (int)(uv % nu), v = f.lo[c] + (int)(uv / nu);
             const int face = 2 * a + side;
             const int pos = side ? f.hi[a] : f.lo[a];
@@ -1842,10 +1849,28 @@ __device__ __forceinline__ void ntff_hist_body(const NtffFaces& f, const NtffBin
             const float t = isW ? ((float)n + tfv - 1.0f) : ((float)n + 0.5f + tfv - 1.0f);
             const int ii = (int)t;
             const float w = t - (float)ii;
-            const int q = ii - lo;
-            if (q >= 0 && q + 1 < len) {
-                atomicAdd(&h[q], (double)((1.0f - w) * K * d));
-                atomicAdd(&h[q + 1], (double)(w * K * d));
+            int q = ii - lo;
+            if (!live || q < 0 || q + 1 >= len) q = -1;
+            const float clo = (1.0f - w) * K * d, chi = w * K * d;
+            // a wave whose cells all fall in one bin adds its FP64 wave sum
+            // once instead of 64 contending LDS atomics on one address
+            const int q0 = __builtin_amdgcn_readfirstlane(q);
+            if (__all(q == q0)) {
+                if (q0 >= 0) {
+                    double slo = (double)clo, shi = (double)chi;
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) {
+                        slo += __shfl_xor(slo, o);
+                        shi += __shfl_xor(shi, o);
+                    }
+                    if ((threadIdx.x & 63) == 0) {
+                        atomicAdd(&h[q0], slo);
+                        atomicAdd(&h[q0 + 1], shi);
+                    }
+                }
+            } else if (q >= 0) {
+                atomicAdd(&h[q], (double)clo);
+                atomicAdd(&h[q + 1], (double)chi);
             }
         }
     }
