@@ -45,6 +45,7 @@ SIGNATURES = [
     ("aem_get_series", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_get_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_set_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    ("aem_get_video", C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_int64]),
     ("aem_get_ntff_raw", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_get_ff_time", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_get_dft", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),

@@ -1,10 +1,12 @@
 """Command-line drop-in for the reference executable: ``python -m alpineem_amd.cli inputs.txt``.
 
 Mirrors ``./FDTD inputs.txt`` (main_fdtd/master.py:438-441): parses the deck,
-runs every time step on the GPU and writes ``data.dat`` and
-``<name>_geometry.bin`` next to the deck, printing the reference's progress
-and timing lines (fdtd_solver.f90:724, 4366-4368, 8313-8316).  Deck errors
-print the reference's ``ERROR: ...`` message and exit with status 1.
+runs every time step on the GPU in the deck's directory and writes
+``data.dat``, ``<name>_geometry.bin`` and, when the deck asks for them, the
+video and full-field files there, printing the reference's progress and
+timing lines (fdtd_solver.f90:724, 4366-4368, 8313-8316).  Deck errors print
+the reference's ``ERROR: ...`` message and exit with status 1, as does a run
+stopped by the non-finite field check.
 """
 from __future__ import annotations
 
@@ -22,14 +24,23 @@ def main(argv=None) -> int:
         print("usage: python -m alpineem_amd.cli <inputs.txt>", file=sys.stderr)
         return 2
     deck = os.path.abspath(argv[0])
-    t0 = time.time()
-    print("Reading in the inputs file and setting up the simulation...")
+    cwd = os.getcwd()
+    # the reference runs in the deck's directory and writes its files there
+    os.chdir(os.path.dirname(deck))
     try:
-        plan = Plan.from_inputs(deck)
-        sim = Simulation(plan)
+        return _run(deck)
     except AemError as e:
         print(str(e).split(": ", 1)[-1])
         return 1
+    finally:
+        os.chdir(cwd)
+
+
+def _run(deck: str) -> int:
+    t0 = time.time()
+    print("Reading in the inputs file and setting up the simulation...")
+    plan = Plan.from_inputs(deck)
+    sim = Simulation(plan)
     print(f"Setup is finished. Time to complete was {time.time() - t0:.3f} seconds.")
     print("Starting main FDTD algorithm...")
     T = plan.time_steps
@@ -45,17 +56,13 @@ def main(argv=None) -> int:
     v = plan.view
     cells = (v.NX - 1) * (v.NY - 1) * (v.NZ - 1)
     print(f"Finished main FDTD algorithm. Time to complete was {el:.3f} seconds.")
-    print(f"Average of {1e-6 * cells * T / el:.2f} Mcells per second.")
+    print(f"Average of {1e-6 * cells * T / max(el, 1e-9):.2f} Mcells per second.")
     print("Post processing some data...")
     t2 = time.time()
-    cwd = os.getcwd()
-    try:
-        os.chdir(os.path.dirname(deck))
-        sim.finalize()
-        sim.write_outputs()
-    finally:
-        os.chdir(cwd)
+    sim.finalize()
+    sim.write_outputs()
     print(f"Finished post processing. Time to complete was {time.time() - t2:.3f} seconds.")
+    sim.close()
     return 0
 
 

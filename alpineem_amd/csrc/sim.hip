@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -18,6 +19,7 @@
 #include "../../include/aem.h"
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "kernels_out.hpp"
 #include "plan.hpp"
 
 using namespace aemk;
@@ -41,6 +43,38 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// Replay loops keep at most two chunks of steps queued ahead of the device:
+// after every kRunAheadChunk launches the host records an event and waits
+// for the one it recorded a chunk earlier.  The device always has a chunk
+// of work queued, while the runtime is never handed a whole run at once (a
+// 3000-step, 64-member population is about 10^6 kernel dispatches).
+constexpr int kRunAheadChunk = 32;
+
+// A run checks every field for NaN or infinity after every kWatchSteps
+// steps and after its last step (one read of the six fields: about 0.3 ms
+// at 400^3, 0.1% of 256 steps).
+constexpr int kWatchSteps = 256;
+struct RunAhead {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool live[2] = {false, false};
+    void create() {
+        for (auto& e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    void destroy() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        ev[0] = ev[1] = nullptr;
+    }
+    // after launch q (0-based) of a replay loop on stream s
+    void pace(hipStream_t s, int64_t q) {
+        if ((q + 1) % kRunAheadChunk) return;
+        const int slot = (int)(((q + 1) / kRunAheadChunk) & 1);
+        HIPCHK(hipEventRecord(ev[slot], s));
+        live[slot] = true;
+        if (live[slot ^ 1]) HIPCHK(hipEventSynchronize(ev[slot ^ 1]));
+    }
+};
 
 // The CPML correction can be folded into the bulk update when no cell of a
 // slab also receives a sheet, TF/SF, port or Drude update: the reference
@@ -282,16 +316,36 @@ struct aem_sim {
     bool gsp = false;              // the graphs are single-pass steps
     void drop_graphs();
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    RunAhead ra;                   // bounds the steps a replay loop queues
     float last_ms = 0.f;
     bool finalized = false;
     double* d_dft = nullptr;   // aem_get_dft staging: f*dt, then (re,im) pairs
     int dft_cap = 0;
+    // exports (f90:6833-7003): the video slice stays on the device for the
+    // run; the full-field records go to their files after every step
+    Video vid{};
+    bool vid_on = false;
+    bool full_on = false;
+    int full_n[3] = {0, 0, 0};     // cells of a full-field record per axis
+    float* d_full[6] = {};
+    float** d_fullp = nullptr;     // device copy of d_full for k_full
+    float* h_full = nullptr;       // pinned staging of one record set
+    std::FILE* full_f[6] = {};
+    void export_step();
+    // numeric watchdog: set by k_watch, read after every run
+    int* d_watch = nullptr;
+    void enqueue_watch(hipStream_t s);
+    void check_watch();
 
     ~aem_sim() {
         if (st) (void)hipStreamSynchronize(st);
+        for (auto& f : full_f)
+            if (f) std::fclose(f);
+        if (h_full) (void)hipHostFree(h_full);
         drop_graphs();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        ra.destroy();
         if (d_dft) (void)hipFree(d_dft);
         for (void* p : owned) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
@@ -335,6 +389,7 @@ void aem_sim::create(const aem_sim_options* opt) {
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
+    ra.create();
     g.NX = v.NX; g.NY = v.NY; g.NZ = v.NZ; g.PX = v.PX; g.plane = v.plane;
     const size_t nc = (size_t)v.ncell;
     for (int q = 0; q < 6; ++q) F[q] = zeros<float>(nc + v.plane);  // one spare plane keeps halo reads in bounds
@@ -529,6 +584,37 @@ void aem_sim::create(const aem_sim_options* opt) {
     d_pc = zeros<float>((size_t)6 * (T + 1));
     d_ffpc = zeros<float>(T);
     d_incident = zeros<float>(T);
+    d_watch = zeros<int>(1);
+
+    // exports: a component's video (one Fortran record) and a full-field
+    // record must each fit the 4-byte record length of the files
+    const unsigned long long rec_max = 0x7fffffffull;
+    if (v.video_on) {
+        vid.slice = v.slice;
+        vid.loc = v.slice_location;
+        vid.v1 = v.vid_size1;
+        vid.v2 = v.vid_size2;
+        vid.T = T;
+        const unsigned long long n = (unsigned long long)vid.v1 * vid.v2 * (unsigned long long)T;
+        if (n * 4 > rec_max)
+            throw aem::Error(AEM_ERR_UNSUPPORTED, "video slice of " + std::to_string(n * 4) +
+                                                      " bytes per component exceeds one 2 GiB file record");
+        for (int q = 0; q < 6; ++q) vid.V[q] = zeros<float>((size_t)n);
+        vid_on = true;
+    }
+    if (plan->deck.export_all == "yes") {
+        full_n[0] = v.NX - 1 - v.pbc[0];
+        full_n[1] = v.NY - 1 - v.pbc[1];
+        full_n[2] = v.NZ - 1 - v.pbc[2];
+        const unsigned long long n = (unsigned long long)full_n[0] * full_n[1] * full_n[2];
+        if (n * 4 > rec_max)
+            throw aem::Error(AEM_ERR_UNSUPPORTED, "full-field record of " + std::to_string(n * 4) +
+                                                      " bytes exceeds one 2 GiB file record");
+        for (int q = 0; q < 6; ++q) d_full[q] = zeros<float>((size_t)n);
+        d_fullp = up<float*>(d_full, 6);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_full), (size_t)n * 6 * sizeof(float)));
+        full_on = true;
+    }
 
     fuse_cpml = allow_fuse && cpml_fusable(v);
     bulk = bulk_launches(v, fuse_cpml, planes);
@@ -823,6 +909,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         k_ffdft<<<gf, 256, 0, s>>>(g, ff, Ex, Ey, Ez, Hx, Hy, Hz);
         k_ffdft_sum<<<ff.n_ang * ff.n_freq, 256, 0, s>>>(ff, (int)(gf.x * gf.y * gf.z), d_step);
     }
+    if ((parts & kPartOutPost) && vid_on)
+        k_video<<<dim3(cdiv(vid.v1, 32), cdiv(vid.v2, 8)), bb, 0, s>>>(g, vid, Ex, Ey, Ez, Hx, Hy, Hz, d_step);
     const bool pc_on = (parts & kPartOutPost) && v.n_angles > 0 && v.plane_wave_amp > 0;
     if (parts & kPartStep)
         k_step_end<<<1, 1, 0, s>>>(g, v.ic, v.jc, v.kc, T, Ex, Ey, Ez, Hx, Hy, Hz, d_pc, d_step, pc_on ? 1 : 0);
@@ -834,6 +922,49 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
 void aem_sim::enqueue_parts(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     if (plan->v.idx_bytes == 1) enqueue_step_t<uint8_t>(mark, s, parts);
     else enqueue_step_t<uint16_t>(mark, s, parts);
+}
+
+// One record of each <name>_full_*.bin file after the step just enqueued
+// (f90:6992-7003; files opened at the first step as the reference opens
+// them at setup, f90:4348-4355).  Each record is a Fortran unformatted
+// sequential record: a 4-byte length, the cell-centred floats, the length.
+void aem_sim::export_step() {
+    if (!full_f[0]) {
+        const std::string& fn = plan->deck.filename;
+        const std::string stem = fn.size() > 4 ? fn.substr(0, fn.size() - 4) : fn;
+        static const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+        for (int q = 0; q < 6; ++q) {
+            const std::string p = stem + "_full_" + names[q] + ".bin";
+            full_f[q] = std::fopen(p.c_str(), "wb");
+            if (!full_f[q]) throw aem::Error(AEM_ERR_IO, "cannot write '" + p + "'");
+        }
+    }
+    const size_t n = (size_t)full_n[0] * full_n[1] * full_n[2];
+    k_full<<<dim3(cdiv(full_n[0], 32), cdiv(full_n[1], 8), full_n[2]), dim3(32, 8, 1), 0, st>>>(
+        g, full_n[0], full_n[1], full_n[2], d_fullp, F[0], F[1], F[2], F[3], F[4], F[5]);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 6; ++q)
+        HIPCHK(hipMemcpyAsync(h_full + q * n, d_full[q], n * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int32_t len = (int32_t)(n * sizeof(float));
+    for (int q = 0; q < 6; ++q)
+        if (std::fwrite(&len, 4, 1, full_f[q]) != 1 || std::fwrite(h_full + q * n, 4, n, full_f[q]) != n ||
+            std::fwrite(&len, 4, 1, full_f[q]) != 1)
+            throw aem::Error(AEM_ERR_IO, "writing a full-field record failed");
+}
+
+void aem_sim::enqueue_watch(hipStream_t s) {
+    const WatchSet ws{{F[0], F[1], F[2], F[3], F[4], F[5]}};
+    k_watch<<<1024, 256, 0, s>>>(ws, (long long)(plan->v.ncell / 4), d_watch);
+}
+
+void aem_sim::check_watch() {
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, d_watch, sizeof bad, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (bad)
+        throw aem::Error(AEM_ERR_NUMERIC, "non-finite field value (NaN or Inf) by step " + std::to_string(steps_done) +
+                                              ": the update is unstable (check the deck's Courant factor)");
 }
 
 namespace aem {
@@ -930,7 +1061,11 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
             } else {
                 s->enqueue_step();
             }
+            if (s->full_on) s->export_step();
+            if ((q + 1) % kWatchSteps == 0 && q + 1 < n_steps) s->enqueue_watch(s->st);
+            s->ra.pace(s->st, q);
         }
+        s->enqueue_watch(s->st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(s->ev1, s->st));
         HIPCHK(hipEventSynchronize(s->ev1));
@@ -938,6 +1073,7 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
         HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
         s->last_ms = ms / (float)n_steps;
         s->steps_done += n_steps;
+        s->check_watch();
     });
 }
 
@@ -1089,6 +1225,27 @@ aem_status aem_get_field(aem_sim* s, int32_t comp, float* out, int64_t n) {
     });
 }
 
+aem_status aem_get_video(aem_sim* s, int32_t comp, int64_t first_step, int64_t n_steps, float* out, int64_t n) {
+    if (!s || !out || comp < 0 || comp > 5) {
+        aem::set_error("aem_get_video: bad argument");
+        return AEM_ERR_STATE;
+    }
+    if (!s->vid_on) {
+        aem::set_error("aem_get_video: the deck exports no video slice");
+        return AEM_ERR_STATE;
+    }
+    const int64_t ps = (int64_t)s->vid.v1 * s->vid.v2;
+    if (first_step < 1 || n_steps < 0 || first_step + n_steps - 1 > s->vid.T || n != ps * n_steps) {
+        aem::set_error("aem_get_video: frames outside [1, time_steps] or n is not frames x slice size");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        HIPCHK(hipMemcpyAsync(out, s->vid.V[comp] + (first_step - 1) * ps, (size_t)n * sizeof(float),
+                              hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    });
+}
+
 aem_status aem_set_field(aem_sim* s, int32_t comp, const float* in, int64_t n) {
     if (!s || !in || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
         aem::set_error("aem_set_field: bad argument");
@@ -1205,6 +1362,7 @@ struct aem_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    RunAhead ra;
     float last_ms = 0.f;
 
     ~aem_batch() {
@@ -1213,6 +1371,7 @@ struct aem_batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        ra.destroy();
         for (void* p : owned) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -1228,6 +1387,7 @@ struct aem_batch {
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
+        ra.create();
         const int n = (int)m.size();
         std::vector<HArgs> ha(n);
         std::vector<EArgs> ea(n);
@@ -1370,6 +1530,10 @@ aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
                 aem::set_error("aem_batch_create: a simulation appears twice");
                 return AEM_ERR_STATE;
             }
+        if (sims[q]->full_on) {
+            aem::set_error("aem_batch_create: a member exports the full field every step; run it alone");
+            return AEM_ERR_UNSUPPORTED;
+        }
     }
     aem_batch* b = new (std::nothrow) aem_batch();
     if (!b) {
@@ -1417,7 +1581,11 @@ aem_status aem_batch_run(aem_batch* b, int64_t n_steps, int32_t use_graph) {
         for (int64_t q = 0; q < n_steps; ++q) {
             if (use_graph) HIPCHK(hipGraphLaunch(b->gexec, b->st));
             else b->enqueue(b->st);
+            if ((q + 1) % kWatchSteps == 0 && q + 1 < n_steps)
+                for (aem_sim* s : b->m) s->enqueue_watch(b->st);
+            b->ra.pace(b->st, q);
         }
+        for (aem_sim* s : b->m) s->enqueue_watch(b->st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(b->ev1, b->st));
         HIPCHK(hipEventSynchronize(b->ev1));
@@ -1425,6 +1593,7 @@ aem_status aem_batch_run(aem_batch* b, int64_t n_steps, int32_t use_graph) {
         HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
         b->last_ms = ms / (float)n_steps;
         for (aem_sim* s : b->m) s->steps_done += n_steps;
+        for (aem_sim* s : b->m) s->check_watch();
     });
 }
 
@@ -1570,6 +1739,15 @@ void cell_gradient(const aem::Plan& P, const AdjRec& r, const int lo[3], const i
 
 bool fresh(const aem_sim* s) { return s->steps_done == 0 && !s->gexec; }
 
+// A batch captured its members' kernel arguments (coefficient table, record
+// and objective state) when it was created; a member cannot gain a design
+// box, a record or an objective afterwards.
+bool refuse_batched(const aem_sim* s, const char* who) {
+    if (!s->batched) return false;
+    aem::set_error(std::string(who) + ": a batch steps this simulation; set it up before aem_batch_create");
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1579,6 +1757,7 @@ aem_status aem_objective_set(aem_sim* s, float theta_deg, float phi_deg, double 
         aem::set_error("aem_objective_set: bad argument");
         return AEM_ERR_STATE;
     }
+    if (refuse_batched(s, "aem_objective_set")) return AEM_ERR_STATE;
     const aem_plan_view& v = s->plan->v;
     if (!fresh(s) || v.n_angles <= 0) {
         aem::set_error("aem_objective_set: needs a deck with a far-field box and a sim that has not stepped");
@@ -1644,6 +1823,7 @@ aem_status aem_ffdft_set(aem_sim* s, int32_t n_ang, const float* theta_deg, cons
         aem::set_error("aem_ffdft_set: bad argument");
         return AEM_ERR_STATE;
     }
+    if (refuse_batched(s, "aem_ffdft_set")) return AEM_ERR_STATE;
     const aem_plan_view& v = s->plan->v;
     if (!fresh(s) || v.n_angles <= 0 || s->ff_on) {
         aem::set_error("aem_ffdft_set: needs a deck with a far-field box and a sim that has not stepped (once)");
@@ -1753,6 +1933,7 @@ aem_status aem_adjoint_record(aem_sim* s, const int32_t* lo, const int32_t* hi) 
         aem::set_error("aem_adjoint_record: null argument");
         return AEM_ERR_STATE;
     }
+    if (refuse_batched(s, "aem_adjoint_record")) return AEM_ERR_STATE;
     const aem_plan_view& v = s->plan->v;
     if (!fresh(s)) {
         aem::set_error("aem_adjoint_record: call before the first step");
@@ -1795,6 +1976,7 @@ void aem_sim::upload_design() {
 }
 
 aem_status aem_design_box(aem_sim* s, const int32_t* lo, const int32_t* hi) {
+    if (s && refuse_batched(s, "aem_design_box")) return AEM_ERR_STATE;
     if (s && (s->rec_on || s->des_on)) {
         aem::set_error("aem_design_box: the simulation already records a box");
         return AEM_ERR_STATE;
@@ -1908,7 +2090,10 @@ void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb
     HIPCHK(hipStreamEndCapture(s->st, &gr));
     HIPCHK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
     HIPCHK(hipEventRecord(s->ev0, s->st));
-    for (int n = 0; n < T; ++n) HIPCHK(hipGraphLaunch(gx, s->st));
+    for (int n = 0; n < T; ++n) {
+        HIPCHK(hipGraphLaunch(gx, s->st));
+        s->ra.pace(s->st, n);
+    }
     HIPCHK(hipEventRecord(s->ev1, s->st));
     HIPCHK(hipEventSynchronize(s->ev1));
     float ms = 0.f;

@@ -6,6 +6,7 @@
 // whitespace-separated column, consumes it unchanged.  Reals are printed in
 // the shortest form that round-trips FP32 (the reference's list-directed
 // output does the same), with an upper-case exponent.
+#include <algorithm>
 #include <charconv>
 #include <cmath>
 #include <cstdio>
@@ -41,6 +42,37 @@ struct File {
 void get(aem_sim* s, int kind, int idx, std::vector<float>& out) {
     aem_status st = aem_get_series(s, kind, idx, out.data(), (int64_t)out.size());
     if (st != AEM_OK) throw aem::Error(st, aem_last_error());
+}
+
+// <stem>_video_{E,H}{x,y,z}.bin: the whole (size_a, size_b, steps) array of
+// a component as ONE Fortran unformatted sequential record -- a 4-byte byte
+// count, the floats in Fortran order, the count again (f90:8788-8807, in the
+// reference's file order Ex, Hx, Ey, Hy, Ez, Hz).  Frames are copied from
+// the device in chunks.
+void write_video(aem_sim* s, const aem_plan_view& v, const std::string& stem) {
+    static const int order[6] = {0, 3, 1, 4, 2, 5};
+    static const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+    const int64_t ps = (int64_t)v.vid_size1 * v.vid_size2, T = v.time_steps;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(T, (64ll << 20) / std::max<int64_t>(ps * 4, 1)));
+    std::vector<float> buf((size_t)(chunk * ps));
+    for (int c : order) {
+        const std::string p = stem + "_video_" + names[c] + ".bin";
+        std::FILE* f = std::fopen(p.c_str(), "wb");
+        if (!f) throw aem::Error(AEM_ERR_IO, "cannot write '" + p + "'");
+        const int32_t len = (int32_t)(ps * T * 4);
+        bool ok = std::fwrite(&len, 4, 1, f) == 1;
+        for (int64_t n0 = 1; ok && n0 <= T; n0 += chunk) {
+            const int64_t m = std::min(chunk, T - n0 + 1);
+            aem_status st = aem_get_video(s, c, n0, m, buf.data(), m * ps);
+            if (st != AEM_OK) {
+                std::fclose(f);
+                throw aem::Error(st, aem_last_error());
+            }
+            ok = std::fwrite(buf.data(), 4, (size_t)(m * ps), f) == (size_t)(m * ps);
+        }
+        ok = ok && std::fwrite(&len, 4, 1, f) == 1;
+        if (std::fclose(f) != 0 || !ok) throw aem::Error(AEM_ERR_IO, "writing '" + p + "' failed");
+    }
 }
 
 }  // namespace
@@ -124,7 +156,8 @@ extern "C" aem_status aem_write_outputs(aem_sim* s, const char* path) {
             get(s, AEM_SERIES_FF_PC, 0, buf);
             for (int n = 0; n < T; ++n) f.line(fmt(buf[n]));
         }
-        std::string gname = name.size() > 4 ? name.substr(0, name.size() - 4) + "_geometry.bin" : name + "_geometry.bin";
-        aem::write_geometry(*plan, gname);
+        const std::string stem = name.size() > 4 ? name.substr(0, name.size() - 4) : name;
+        aem::write_geometry(*plan, stem + "_geometry.bin");
+        if (v.video_on) write_video(s, v, stem);
     });
 }

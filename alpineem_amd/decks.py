@@ -126,6 +126,7 @@ class Deck:
     phase_centre: Tuple[int, int, int] = (1, 1, 1)
     far_field: Sequence[Tuple[float, float]] = ()
     video: Optional[Tuple[str, int]] = None
+    export_all: bool = False                         # per-step full-field files (f90:913)
     filename: str = "data.dat"
     materials: list = field(default_factory=list)
     objects: list = field(default_factory=list)
@@ -148,7 +149,7 @@ class Deck:
             L += ["yes", self.video[0], _f(self.video[1])]
         else:
             L.append("no")
-        L += ["no", self.filename, _f(len(self.materials))]
+        L += ["yes" if self.export_all else "no", self.filename, _f(len(self.materials))]
         for m in self.materials:
             L += m.lines()
         if self.optional_geometry:
@@ -288,7 +289,8 @@ def population(count: int = 64, seed: int = 3, steps: int = 3000, edges=range(80
         dk = dipole(n=n, steps=steps, angles=19, arm=arm)
         nb = int(rng.integers(0, 5))
         for b in range(nb):
-            sz = [int(v) for v in rng.integers(4, 21, 3)]
+            # blocks fit between the 13-cell margins (only bites below 46 cells)
+            sz = [min(int(v), n - 26) for v in rng.integers(4, 21, 3)]
             st = [int(rng.integers(13, n - 12 - s)) for s in sz]
             dk.materials.append(Material(40 + b, (float(rng.uniform(1.0, 10.0)),) * 3, (0.0, 0.0, 0.0)))
             dk.objects.insert(0, Block(40 + b, tuple(st), tuple(sz)))

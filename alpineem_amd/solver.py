@@ -237,6 +237,17 @@ class Simulation:
         check(lib().aem_get_field(self._h, FIELDS[name], out.ctypes.data, v.ncell))
         return out.reshape(self.plan.shape)
 
+    def video(self, name: str, first_step: int = 1, n_steps: Optional[int] = None) -> np.ndarray:
+        """Frames [first_step, first_step + n_steps) of the deck's video
+        slice of one component, shape (n_steps, size_b, size_a)
+        (aem_get_video)."""
+        v = self.plan.view
+        if n_steps is None:
+            n_steps = v.time_steps - first_step + 1
+        out = np.zeros(n_steps * v.vid_size1 * v.vid_size2, np.float32)
+        check(lib().aem_get_video(self._h, FIELDS[name], int(first_step), int(n_steps), out.ctypes.data, out.size))
+        return out.reshape(n_steps, v.vid_size2, v.vid_size1)
+
     def set_field(self, name: str, values: np.ndarray) -> None:
         v = self.plan.view
         a = np.ascontiguousarray(values, np.float32).reshape(-1)

@@ -136,7 +136,14 @@ void aem_destroy(aem_sim* sim);
 
 /* Advance n_steps leapfrog steps (H phase, E phase, per-step outputs).  The
  * total may not exceed the deck's time_steps (output arrays are sized by
- * it).  use_graph!=0 replays a captured HIP graph of one step. */
+ * it).  use_graph!=0 replays a captured HIP graph of one step.
+ * A deck that exports the full field ("yes" after the video flag,
+ * f90:913) appends one record per step to <name>_full_{E,H}{x,y,z}.bin in
+ * the current directory (f90:4348-4355, 6992-7003).
+ * Every 256 steps and after the last one, every field value is checked:
+ * a NaN or infinity (an unstable deck, e.g. a Courant factor below the
+ * reference's 0.98, f90:1312) returns AEM_ERR_NUMERIC once the steps are
+ * done.  The reference has no such check. */
 aem_status aem_run(aem_sim* sim, int64_t n_steps, int32_t use_graph);
 
 /* Steps completed so far. */
@@ -160,6 +167,13 @@ aem_status aem_get_field(aem_sim* sim, int32_t comp, float* out, int64_t n);
 
 /* Overwrite one field component from host memory (ncell floats). */
 aem_status aem_set_field(aem_sim* sim, int32_t comp, const float* in, int64_t n);
+
+/* Frames [first_step, first_step + n_steps) of the video slice of one
+ * component (comp 0..5 = Ex..Hz), n = n_steps * vid_size1 * vid_size2
+ * floats, layout [step][size_b][size_a] -- the Ex_video...Hz_video arrays
+ * the reference fills every step and writes after the loop
+ * (fdtd_solver.f90:6833-6990, 8788-8807).  Needs a deck with video "yes". */
+aem_status aem_get_video(aem_sim* sim, int32_t comp, int64_t first_step, int64_t n_steps, float* out, int64_t n);
 
 /* Raw NTFF accumulators U/W (fdtd_solver.f90:7643-7927) as FP64:
  * which = 0..5 -> Ux,Uy,Uz,Wx,Wy,Wz; out has n_angles*len_ff entries
@@ -279,7 +293,9 @@ int32_t aem_batch_size(const aem_batch* batch);
 void aem_batch_destroy(aem_batch* batch);
 
 /* Write data.dat (positional format read by utility_scripts/post_processor.py)
- * and <name>_geometry.bin.  path NULL -> the deck's own file name. */
+ * and <name>_geometry.bin, and with video on <name>_video_{E,H}{x,y,z}.bin
+ * (one Fortran unformatted record each, f90:8788-8807).  path NULL -> the
+ * deck's own file name; <name> is path without its 4-character extension. */
 aem_status aem_write_outputs(aem_sim* sim, const char* path);
 
 /* Average device time per step of the last aem_run (ms), from HIP events

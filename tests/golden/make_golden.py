@@ -6,10 +6,15 @@ Runs ``oracle/_ref/fdtd_ref`` (the serial reference, compiled by
 the series of data.dat (incident, port voltages, far field, periodic-slab
 reflection/transmission, phase-centre incident, far-field time corrections,
 the time step) and, for decks that export a video slice, every 25th step of
-the six cell-centred slices.  Only outputs are stored, never reference code.
+the six cell-centred slices.  The SHA-256 of every binary file the run
+writes (_geometry.bin, _video_*.bin, _full_*.bin) is stored so that the
+device's files can be compared byte for byte, and for the decks in DAT_DECKS
+the reference's data.dat text itself is kept as tests/golden/dat/<deck>.dat.
+Only outputs are stored, never reference code.
 
 Usage:  python tests/golden/make_golden.py [deck-name ...]
 """
+import hashlib
 import os
 import shutil
 import subprocess
@@ -26,6 +31,18 @@ from alpineem_amd.datfile import read_dat, read_fortran_record_file  # noqa: E40
 
 REF = os.path.join(ROOT, "oracle", "_ref", "fdtd_ref")
 VIDEO_STRIDE = 25
+# decks whose data.dat is kept whole: antenna with far field, plane wave with
+# far field and phase-centre incident, several ports, periodic R/T, sheets
+# with Drude media, video and full-field export
+DAT_DECKS = ("dipole36", "sphere_oblique", "master_ports", "drude_slab", "c3_mini", "export_small")
+
+
+def sha256(path: str) -> str:
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
 
 
 def run_deck(name: str) -> None:
@@ -62,6 +79,12 @@ def run_deck(name: str) -> None:
         geo = np.fromfile(os.path.join(tmp, "data_geometry.bin"), np.float32)
         out["geometry_sum"] = np.float64(geo.astype(np.float64).sum())
         out["geometry_nnz"] = np.int64(np.count_nonzero(geo))
+        files = sorted(f for f in os.listdir(tmp) if f.endswith(".bin"))
+        out["bin_files"] = np.array(files)
+        out["bin_sha256"] = np.array([sha256(os.path.join(tmp, f)) for f in files])
+        if name in DAT_DECKS:
+            os.makedirs(os.path.join(HERE, "dat"), exist_ok=True)
+            shutil.copy(os.path.join(tmp, "data.dat"), os.path.join(HERE, "dat", name + ".dat"))
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         print(f"{name}: " + ", ".join(f"{k}{tuple(np.shape(v))}" for k, v in out.items()))
 

@@ -106,3 +106,41 @@ def test_batch_after_single_pass_steps():
     ref.run(100)
     for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
         np.testing.assert_array_equal(a.field(comp), ref.field(comp), err_msg=comp)
+
+
+def _design_sim(steps, eps):
+    d, lo, hi = decks.adjoint_deck(n=48, steps=steps, box=(6, 6, 4))
+    s = Simulation(Plan.from_text(d.text()), ntff="off")
+    s.design_box(lo, hi)
+    s.set_design_eps(eps)
+    return s, lo, hi
+
+
+def test_batched_member_refuses_late_setup(golden_dir):
+    """A batch captures its members' kernel arguments (coefficient table,
+    record and objective state) when it is created, so a design box, a
+    record, an objective or a far-field DFT asked for afterwards is refused
+    with a state error instead of stepping with stale arguments."""
+    d, lo, hi = decks.adjoint_deck(n=48, steps=40, box=(6, 6, 4))
+    a = Simulation(Plan.from_text(d.text()), ntff="off")
+    b = Batch([a, _make(golden_dir, "dipole36", "ordered", 0)])
+    for call in (lambda: a.design_box(lo, hi), lambda: a.record_design(lo, hi),
+                 lambda: a.set_objective(30.0, 0.0, 10e9), lambda: a.set_farfield_dft([(30.0, 0.0)], 1e9, 1e9, 2)):
+        with pytest.raises(RuntimeError, match="batch"):
+            call()
+    b.run(5)
+
+
+def test_design_box_member_batched_equals_solo(golden_dir):
+    """A member whose design box (per-cell permittivity) and objective were
+    set before the batch was created steps exactly as it does alone."""
+    eps = decks.design_permittivity((4, 6, 6), seed=5)
+    a, _, _ = _design_sim(60, eps)
+    a.set_objective(30.0, 0.0, 10e9)
+    s, _, _ = _design_sim(60, eps)
+    s.set_objective(30.0, 0.0, 10e9)
+    Batch([a, _make(golden_dir, "dipole36", "ordered", 0)]).run(60)
+    s.run(60)
+    for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        np.testing.assert_array_equal(a.field(comp), s.field(comp), err_msg=comp)
+    assert a.objective() == s.objective()

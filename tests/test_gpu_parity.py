@@ -67,10 +67,10 @@ def _series_sets(src, g, v):
 
 @pytest.mark.parametrize("name", DECKS)
 def test_series_match_reference(runs, name):
-    """Every series within rel-L2 1e-5 of the reference -- or, where the
-    reference's own FP32 rounding is larger (signals that cancel: dipole
-    pattern nulls, scattered fields), no further from it than the FP64
-    oracle is (the reference's distance from FP64 arithmetic)."""
+    """Every series within rel-L2 1e-5 of the reference, including signals
+    that nearly cancel (dipole pattern nulls, fields behind metal), where the
+    reference itself sits far further from FP64 arithmetic.  The distances
+    from the FP64 oracle are printed beside each error."""
     plan, sim, g = runs[name]
     v = plan.view
     print(f"\n[{name}] grid {v.nx}x{v.ny}x{v.nz}, {plan.time_steps} steps")
@@ -86,8 +86,8 @@ def test_series_match_reference(runs, name):
         exact = int(np.sum(got == ref))
         print(f"  {label:14s} relL2={r:.2e}  (reference vs FP64: {floors[label]:.2e}, device vs FP64: "
               f"{rel_l2(got, fp64[label]):.2e})  exact {exact}/{ref.size}")
-        if not (r <= TOL or r <= floors[label]):
-            bad.append((label, r, floors[label]))
+        if not r <= TOL:
+            bad.append((label, r))
     if "ff_time" in g:
         np.testing.assert_array_equal(sim.ff_time(), g["ff_time"])
     assert not bad, bad

@@ -1,0 +1,119 @@
+"""The batched GA population against the reference solver and against solo runs.
+
+Reduced members (tests/golden/decks/pop4_m*.txt: decks.population with seed
+4, cube edges 40-64, random arms and 0-4 dielectric blocks, 19 far-field
+angles, 300 steps) were run through the flang-built serial reference
+(tests/golden/make_golden.py).  Run as ONE batch on the device they must
+reproduce its outputs: bit for bit with the ordered FP32 far-field binning,
+within 1e-5 rel-L2 with the FP64 LDS-histogram binning that the population
+benchmark uses.  The full-size population (64 members, cubes 80-160, seed
+3) is checked batched against solo runs, field for field.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from alpineem_amd import decks
+from alpineem_amd.solver import Batch, Plan, Simulation
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [f"pop4_m{q}" for q in range(4)]
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _batch_run(golden_dir, ntff):
+    sims, fix = [], []
+    for name in MEMBERS:
+        plan = Plan.from_inputs(os.path.join(golden_dir, "decks", name + ".txt"))
+        sims.append(Simulation(plan, ntff=ntff))
+        fix.append(np.load(os.path.join(golden_dir, name + ".npz")))
+    steps = sims[0].plan.time_steps
+    assert all(s.plan.time_steps == steps for s in sims)
+    b = Batch(sims)
+    b.run(steps // 3)
+    b.run(steps - steps // 3)
+    for s in sims:
+        s.finalize()
+    return sims, fix
+
+
+def test_population_members_bit_exact_vs_reference_ordered(golden_dir):
+    """Ordered binning: every member's port voltage, incident wave, far-field
+    E_theta/E_phi at 19 angles and far-field time equal the reference's."""
+    sims, fix = _batch_run(golden_dir, "ordered")
+    for name, s, g in zip(MEMBERS, sims, fix):
+        v = s.plan.view
+        print(f"\n  {name}: {v.nx}^3, {v.n_angles} angles, {s.plan.time_steps} steps")
+        np.testing.assert_array_equal(s.series("incident"), g["incident"], err_msg=name)
+        np.testing.assert_array_equal(s.series("port_v", 0), g["port_v"][0], err_msg=name)
+        for a in range(v.n_angles):
+            np.testing.assert_array_equal(s.series("e_theta", a), g["e_theta"][a], err_msg=f"{name} E_theta {a}")
+            np.testing.assert_array_equal(s.series("e_phi", a), g["e_phi"][a], err_msg=f"{name} E_phi {a}")
+        np.testing.assert_array_equal(s.ff_time(), g["ff_time"], err_msg=name)
+
+
+def test_population_members_close_to_reference_histogram(golden_dir):
+    """FP64 LDS-histogram binning (the benchmark's mode): port voltage and
+    incident wave bit-exact, each angle's far-field vector within 1e-5."""
+    sims, fix = _batch_run(golden_dir, "histogram")
+    worst = 0.0
+    for name, s, g in zip(MEMBERS, sims, fix):
+        v = s.plan.view
+        np.testing.assert_array_equal(s.series("port_v", 0), g["port_v"][0], err_msg=name)
+        np.testing.assert_array_equal(s.series("incident"), g["incident"], err_msg=name)
+        for a in range(v.n_angles):
+            got = np.concatenate([s.series("e_theta", a), s.series("e_phi", a)])
+            ref = np.concatenate([g["e_theta"][a], g["e_phi"][a]])
+            r = rel_l2(got, ref)
+            worst = max(worst, r)
+            assert r <= TOL, (name, a, r)
+    print(f"\n  histogram far field: worst rel-L2 against the reference {worst:.2e}")
+
+
+def test_full_population_batched_equals_solo():
+    """The benchmark's 64-member population (cubes 80-160, seed 3, 19 angles,
+    histogram binning) for 50 steps as one batch: every member's fields and
+    port series equal its solo run bit for bit."""
+    steps = 50
+    pop = decks.population(count=64, seed=3, steps=steps)
+    batched = [Simulation(Plan.from_text(d.text()), ntff="histogram") for d in pop]
+    Batch(batched).run(steps)
+    for q, d in enumerate(pop):
+        solo = Simulation(Plan.from_text(d.text()), ntff="histogram")
+        solo.run(steps)
+        a = batched[q]
+        for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+            np.testing.assert_array_equal(a.field(comp), solo.field(comp), err_msg=f"member {q} {comp}")
+        np.testing.assert_array_equal(a.series("port_v", 0, steps), solo.series("port_v", 0, steps),
+                                      err_msg=f"member {q}")
+        solo.close()
+        a.close()
+    print(f"\n  64 members x {steps} steps: batched == solo")
+
+
+def test_single_pass_400cube_matches_split_launches():
+    """The benchmark's 400^3 deck: 20 single-pass steps (32-bit offsets,
+    the timed tile and chunk counts) equal 20 steps of the split H/E
+    launches bit for bit in every field."""
+    text = decks.roofline_deck(400, steps=20).text()
+    sp = Simulation(Plan.from_text(text))
+    assert sp.single_pass
+    sp.run(13)
+    sp.run(7, graph=False)
+    fields = {c: sp.field(c) for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")}
+    sp.close()
+    ref = Simulation(Plan.from_text(text), single_pass=False)
+    assert not ref.single_pass
+    ref.run(20)
+    assert np.any(fields["Ez"] != 0)
+    for comp, got in fields.items():
+        np.testing.assert_array_equal(got, ref.field(comp), err_msg=comp)
+    ref.close()

@@ -8,6 +8,7 @@ voltages, far-field E_theta/E_phi (including the FP32 retarded-time binning),
 periodic-slab reflection/transmission, the phase-centre incident and the
 cell-centred video slices of all six field components.
 """
+import hashlib
 import os
 
 import numpy as np
@@ -17,7 +18,7 @@ from alpineem_amd.solver import Plan
 from tests.oracle_ffi import Oracle
 
 DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini",
-         "sheets_xyz"]
+         "sheets_xyz", "pop4_m0", "pop4_m1", "pop4_m2", "pop4_m3", "export_small"]
 
 
 def _run(golden_dir, name, fp64=False):
@@ -56,9 +57,11 @@ def test_oracle_bit_exact_vs_reference(golden_dir, name):
                                               err_msg=f"video comp {c} step {(f + 1) * stride}")
 
 
-@pytest.mark.parametrize("name", ["dipole36", "master_ports", "sphere_oblique", "sheets_xyz"])
+@pytest.mark.parametrize("name", DECKS)
 def test_geometry_file_matches_reference(golden_dir, name, tmp_path):
-    plan, _, g = _run(golden_dir, name)
+    """_geometry.bin equals the reference's file byte for byte (SHA-256)."""
+    plan = Plan.from_inputs(os.path.join(golden_dir, "decks", name + ".txt"))
+    g = np.load(os.path.join(golden_dir, name + ".npz"))
     out = tmp_path / "geo.bin"
     plan.write_geometry(str(out))
     geo = np.fromfile(out, np.float32)
@@ -66,6 +69,8 @@ def test_geometry_file_matches_reference(golden_dir, name, tmp_path):
     assert geo.size == 4 * v.nx * v.ny * v.nz
     assert geo.astype(np.float64).sum() == g["geometry_sum"]
     assert np.count_nonzero(geo) == g["geometry_nnz"]
+    ref = dict(zip(g["bin_files"], g["bin_sha256"]))
+    assert hashlib.sha256(out.read_bytes()).hexdigest() == ref["data_geometry.bin"]
 
 
 @pytest.mark.parametrize("name", ["dipole36", "drude_slab", "c2_mini"])
