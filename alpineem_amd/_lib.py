@@ -17,7 +17,7 @@ STATUS_NAMES = {0: "AEM_OK", 1: "AEM_ERR_INPUT", 2: "AEM_ERR_IO", 3: "AEM_ERR_DE
                 5: "AEM_ERR_UNSUPPORTED", 6: "AEM_ERR_NUMERIC"}
 
 SERIES = {"port_v": 0, "port_i": 1, "incident": 2, "e_theta": 3, "e_phi": 4, "reflected": 5,
-          "transmitted": 6, "ff_pc": 7}
+          "transmitted": 6, "ff_pc": 7, "spice_v": 8, "spice_i": 9}
 FIELDS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 
 # (name, restype, argtypes) for every entry point of include/aem.h
@@ -26,6 +26,9 @@ SIGNATURES = [
     ("aem_version", C.c_char_p, []),
     ("aem_plan_from_inputs", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     ("aem_plan_from_text", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("aem_plan_options_init", None, [C.c_void_p]),
+    ("aem_plan_from_text_ex", C.c_int, [C.c_char_p, C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("aem_plan_from_inputs_ex", C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("aem_plan_view_get", C.c_void_p, [C.c_void_p]),
     ("aem_plan_write_geometry", C.c_int, [C.c_void_p, C.c_char_p]),
     ("aem_plan_destroy", None, [C.c_void_p]),
@@ -42,6 +45,7 @@ SIGNATURES = [
     ("aem_steps_done", C.c_int64, [C.c_void_p]),
     ("aem_sync", C.c_int, [C.c_void_p]),
     ("aem_finalize", C.c_int, [C.c_void_p]),
+    ("aem_port_exchange", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("aem_get_series", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_get_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_set_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
@@ -58,6 +62,7 @@ SIGNATURES = [
     ("aem_design_box", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("aem_set_design_eps", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_adjoint_design_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("aem_adjoint", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_adjoint_step", C.c_int, [C.c_void_p, C.c_int64]),
     ("aem_adjoint_get_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_adjoint_set_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
@@ -103,10 +108,17 @@ _IP = C.POINTER(C.c_int32)
 NTFF_BINNING = {"ordered": 0, "atomic": 1, "histogram": 2, "off": 3}
 
 
+class PlanOptions(C.Structure):
+    """aem_plan_options (include/aem_plan.h)."""
+    _fields_ = [("cpml_cells", C.c_int32), ("cpml_grading", C.c_int32), ("cpml_kappa_max", C.c_float),
+                ("cpml_alpha_max", C.c_float), ("cpml_reflection", C.c_float), ("tfsf_inset", C.c_int32),
+                ("ntff_gap", C.c_int32), ("kmax", C.c_int32)]
+
+
 class SimOptions(C.Structure):
     """aem_sim_options (include/aem.h)."""
     _fields_ = [("ntff_binning", C.c_int32), ("planes", C.c_int32), ("no_cpml_fusion", C.c_int32),
-                ("no_single_pass", C.c_int32)]
+                ("no_single_pass", C.c_int32), ("ntff_lists", C.c_int32)]
 
 
 class PortView(C.Structure):
@@ -116,6 +128,12 @@ class PortView(C.Structure):
                 ("kI", C.c_float), ("kV", C.c_float), ("kS", C.c_float),
                 ("cur_num", C.c_float), ("cur_den", C.c_float), ("gap_div", C.c_float),
                 ("excited", C.c_int32)]
+
+
+class SpiceView(C.Structure):
+    """aem_spice_view (include/aem_plan.h)."""
+    _fields_ = [("dir", C.c_int32), ("loc", C.c_int32 * 3), ("size", C.c_int32 * 3), ("v_bias", C.c_float),
+                ("cap", C.c_float), ("cur_num", C.c_float), ("cur_den", C.c_float), ("gap_div", C.c_float)]
 
 
 class PlanView(C.Structure):
@@ -155,4 +173,10 @@ class PlanView(C.Structure):
                 ("excitation", C.c_int32), ("min_steps", C.c_float),
                 ("video_on", C.c_int32), ("slice", C.c_int32), ("slice_location", C.c_int32),
                 ("vid_size1", C.c_int32), ("vid_size2", C.c_int32),
-                ("src_table", _FP), ("exp_half", C.c_float)]
+                ("src_table", _FP), ("exp_half", C.c_float),
+                ("n_spice", C.c_int32), ("n_spice_extra", C.c_int32), ("spice", C.POINTER(SpiceView)),
+                ("spice_names", C.c_char_p),
+                ("kmax", C.c_int32), ("mode_type", C.c_int32), ("k_direction", C.c_int32), ("k_axis", C.c_int32),
+                ("kpl_H", C.c_int32), ("kpl_E", C.c_int32), ("k_vec", C.c_float * 3), ("k_exc", C.c_float * 3),
+                ("f_adj", C.c_float), ("bloch_c", C.c_float * 3), ("bloch_s", C.c_float * 3),
+                ("src_table_im", C.POINTER(C.c_float)), ("ksrc", C.POINTER(C.c_float))]

@@ -23,19 +23,35 @@ const char* aem_last_error(void) { return g_err.c_str(); }
 
 const char* aem_version(void) { return "libaem 0.1 (gfx950)"; }
 
-aem_status aem_plan_from_text(const char* deck_text, const char* base_dir, aem_plan** out) {
+void aem_plan_options_init(aem_plan_options* opt) {
+    if (opt) aem::default_options(*opt);
+}
+
+aem_status aem_plan_from_text_ex(const char* deck_text, const char* base_dir, const aem_plan_options* opt,
+                                 aem_plan** out) {
     if (!deck_text || !out) {
         aem::set_error("aem_plan_from_text: null argument");
         return AEM_ERR_STATE;
     }
     *out = nullptr;
     return aem::guard([&]() {
-        aem::Plan* p = aem::build_plan(deck_text, base_dir ? base_dir : ".");
+        aem_plan_options o;
+        aem::default_options(o);
+        if (opt) o = *opt;
+        aem::Plan* p = aem::build_plan(deck_text, base_dir ? base_dir : ".", o);
         *out = reinterpret_cast<aem_plan*>(p);
     });
 }
 
+aem_status aem_plan_from_text(const char* deck_text, const char* base_dir, aem_plan** out) {
+    return aem_plan_from_text_ex(deck_text, base_dir, nullptr, out);
+}
+
 aem_status aem_plan_from_inputs(const char* inputs_path, aem_plan** out) {
+    return aem_plan_from_inputs_ex(inputs_path, nullptr, out);
+}
+
+aem_status aem_plan_from_inputs_ex(const char* inputs_path, const aem_plan_options* opt, aem_plan** out) {
     if (!inputs_path || !out) {
         aem::set_error("aem_plan_from_inputs: null argument");
         return AEM_ERR_STATE;
@@ -51,7 +67,7 @@ aem_status aem_plan_from_inputs(const char* inputs_path, aem_plan** out) {
     std::string path(inputs_path);
     size_t slash = path.find_last_of('/');
     std::string dir = slash == std::string::npos ? "." : path.substr(0, slash);
-    return aem_plan_from_text(ss.str().c_str(), dir.c_str(), out);
+    return aem_plan_from_text_ex(ss.str().c_str(), dir.c_str(), opt, out);
 }
 
 const aem_plan_view* aem_plan_view_get(const aem_plan* plan) {

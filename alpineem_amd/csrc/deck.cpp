@@ -130,9 +130,10 @@ int axis_of(const std::string& s, const char* what) {
 
 }  // namespace
 
-Deck parse_deck(const std::string& text) {
+Deck parse_deck(const std::string& text, bool kmax) {
     Reader r(text);
     Deck d;
+    d.kmax = kmax ? 1 : 0;
     int n3[3];
     r.getni(n3, 3, "grid size");                       // f90:728
     d.nx = n3[0]; d.ny = n3[1]; d.nz = n3[2];
@@ -161,7 +162,41 @@ Deck parse_deck(const std::string& text) {
 
     std::string exc = trim(r.line("excitation type"));  // f90:770 read(1,'(a)')
     if (exc.size() > 15) exc = trim(exc.substr(0, 15));
-    if (exc == "plane wave") {
+    if (kmax) {                                           // f90:832-882
+        if (exc == "plane wave") {
+            d.excitation = 1;
+            auto t = r.items(2, "plane wave amplitude, delay");
+            d.pw_amp = Reader::to_float(t[0], "amplitude");
+            d.pw_delay = Reader::to_float(t[1], "delay");
+            const std::string mt = trim(r.gets("mode type"));
+            if (mt == "TE") d.mode_type = 0;
+            else if (mt == "TM") d.mode_type = 1;
+            const std::string kd = trim(r.gets("propagation direction"));
+            if (kd == "-") d.k_direction = 0;
+            else if (kd == "+") d.k_direction = 1;
+            else throw Error(1, "ERROR: INVALID PROPOGATION DIRECTION DETECTED");
+        } else if (exc == "antenna") {
+            d.excitation = 2;
+            auto t = r.items(2, "antenna amplitude, delay");
+            d.ant_amp = Reader::to_float(t[0], "amplitude");
+            d.ant_delay = Reader::to_float(t[1], "delay");
+            d.exc_port = r.geti("excitation port");
+            if (d.exc_port == 0) { d.ant_amp = 0; d.ant_delay = 0; }
+        }
+        int a0 = -1, a1 = -1;
+        if (d.pbc[1] + d.pbc[2] == 2) { a0 = 1; a1 = 2; }
+        else if (d.pbc[0] + d.pbc[2] == 2) { a0 = 0; a1 = 2; }
+        else if (d.pbc[0] + d.pbc[1] == 2) { a0 = 0; a1 = 1; }
+        if (a0 >= 0) {
+            auto t = r.items(2, "k vector");
+            d.k_count[a0] = Reader::to_float(t[0], "k vector");
+            d.k_count[a1] = Reader::to_float(t[1], "k vector");
+        }
+        const int np = d.pbc[0] + d.pbc[1] + d.pbc[2];
+        if (np == 2 && d.is_mirror == 1)
+            throw Error(1, "ERROR: USER ADDED AN IGP CONDITION TO A UNIT CELL W/ 2 SETS OF PBC - THIS IS NOT NEEDED AND INVALID");
+        if (np == 1) throw Error(1, "ERROR: USER ONLY ADDED 1 PBC W/ KMAX - THIS IS INVALID");
+    } else if (exc == "plane wave") {
         d.excitation = 1;
         auto t = r.items(2, "plane wave amplitude, delay");   // f90:773
         d.pw_amp = Reader::to_float(t[0], "amplitude");
@@ -180,15 +215,15 @@ Deck parse_deck(const std::string& text) {
 
     // input checks of f90:809-829 (the pbc==3 case is only a warning there)
     int npbc = d.pbc[0] + d.pbc[1] + d.pbc[2];
-    if (npbc == 2 && d.is_mirror == 1)
+    if (!kmax && npbc == 2 && d.is_mirror == 1)
         throw Error(1, "ERROR: USER ADDED AN IGP CONDITION TO A UNIT CELL W/ 2 SETS OF PBC - THIS IS NOT NEEDED AND INVALID");
-    if (d.pbc[0] == 1 && d.pw_amp > 0)
+    if (!kmax && d.pbc[0] == 1 && d.pw_amp > 0)
         if (std::fmod(d.theta, 180.0f) != 0.0f && std::fmod(d.phi, 180.0f) != 90.0f)
             throw Error(1, "ERROR: INCIDENT PLANE WAVE W/ OBLIQUE ANGLE W/ RESPECT TO PBC DETECTED - USE KMAX");
-    if (d.pbc[1] == 1 && d.pw_amp > 0)
+    if (!kmax && d.pbc[1] == 1 && d.pw_amp > 0)
         if (std::fmod(d.theta, 180.0f) != 0.0f && std::fmod(d.phi, 180.0f) != 0.0f)
             throw Error(1, "ERROR: INCIDENT PLANE WAVE W/ OBLIQUE ANGLE W/ RESPECT TO PBC DETECTED - USE KMAX");
-    if (d.pbc[2] == 1 && d.pw_amp > 0)
+    if (!kmax && d.pbc[2] == 1 && d.pw_amp > 0)
         if (std::fmod(d.theta, 180.0f) != 90.0f)
             throw Error(1, "ERROR: INCIDENT PLANE WAVE W/ OBLIQUE ANGLE W/ RESPECT TO PBC DETECTED - USE KMAX");
 
@@ -323,6 +358,50 @@ Deck parse_deck(const std::string& text) {
         d.ports.push_back(p);
     }
     d.num_spice_ports = r.geti("number of spice ports");      // f90:1195
+    // The SPICE section (f90:1196-1286, compiled with use_spice_version):
+    // netlist name, then per port its type, direction, location, size, DC
+    // bias and names, then the additional nodes.  The reference reads the
+    // location, size and bias into a real array.
+    if (d.num_spice_ports > 0) {
+        d.netlist = trim(r.gets("SPICE netlist"));
+        for (int i = 0; i < d.num_spice_ports; ++i) {
+            SpiceIn p;
+            std::string pt = trim(r.gets("SPICE port type"));
+            auto geti_real = [&](int* v, int n, const char* what) {
+                auto t = r.items((size_t)n, what);
+                for (int q = 0; q < n; ++q) v[q] = (int)Reader::to_float(t[q], what);
+            };
+            if (pt == "basic") {
+                p.dir = axis_of(r.gets("SPICE port direction"), "SPICE port direction");
+                geti_real(p.loc, 3, "SPICE port location");
+                if (p.loc[0] == 1 || p.loc[1] == 1 || p.loc[2] == 1)
+                    throw Error(1, "ERROR: FDTD SOFTWARE DOES NOT CURRENTLY SUPPORT PLACING A SPICE LUMPED PORT AT A FIRST CUBE");
+                geti_real(p.size, 3, "SPICE port size");
+                p.v_bias = r.getf("SPICE port DC bias");
+                p.node = trim(r.gets("SPICE port node"));
+                p.isrc = trim(r.gets("SPICE port current source"));
+                p.cap = trim(r.gets("SPICE port capacitance"));
+                for (int a = 0; a < 3; ++a)                          // f90:1225-1235
+                    if (a != p.dir) p.size[a] += 1;
+            } else if (pt == "gridded") {
+                p.gridded = 1;
+                r.gets("SPICE gridded port direction");
+                geti_real(p.loc, 3, "SPICE port location");
+                if (p.loc[0] == 1 || p.loc[1] == 1 || p.loc[2] == 1)
+                    throw Error(1, "ERROR: FDTD SOFTWARE DOES NOT CURRENTLY SUPPORT PLACING A SPICE LUMPED PORT AT A FIRST CUBE");
+                geti_real(p.size, 2, "SPICE gridded port size");
+                r.gets("SPICE gridded feed file");
+                p.v_bias = r.getf("SPICE port DC bias");
+                p.node = trim(r.gets("SPICE port node"));
+                p.isrc = trim(r.gets("SPICE port current source"));
+            } else {
+                throw Error(1, "ERROR: unknown SPICE port type '" + pt + "'");
+            }
+            d.spice.push_back(p);
+        }
+        const int extra = r.geti("number of additional SPICE nodes");
+        for (int i = 0; i < extra; ++i) d.spice_extra.push_back(trim(r.gets("additional SPICE node")));
+    }
     return d;
 }
 

@@ -1177,6 +1177,34 @@ struct PortArgs {
     const int* step;
 };
 
+// Loop integral of H around a lumped gap of cells [loc, loc+size-1] along
+// axis dir, in the reference's loop order (f90:4820-4857, 5456-5496), before
+// the (I*num)/den scaling.
+__device__ __forceinline__ float gap_loop_current(const Grid& g, int dir, const int* loc, const int* size,
+                                                  const float* Hx, const float* Hy, const float* Hz, float dx,
+                                                  float dy, float dz) {
+    const int l1 = loc[0], l2 = loc[1], l3 = loc[2];
+    const int d1 = size[0] + l1 - 1, d2 = size[1] + l2 - 1, d3 = size[2] + l3 - 1;
+    float c = 0.0f;
+    if (dir == 0) {
+        for (int ii = l1; ii <= d1; ++ii) {
+            for (int jj = l2; jj <= d2; ++jj) c = c + dy * (Hy[off(g, ii, jj, l3 - 1)] - Hy[off(g, ii, jj, d3)]);
+            for (int kk = l3; kk <= d3; ++kk) c = c + dz * (Hz[off(g, ii, l2, kk)] - Hz[off(g, ii, d2 - 1, kk)]);
+        }
+    } else if (dir == 1) {
+        for (int jj = l2; jj <= d2; ++jj) {
+            for (int ii = l1; ii <= d1; ++ii) c = c + dx * (Hx[off(g, ii, jj, l3)] - Hx[off(g, ii, jj, d3 - 1)]);
+            for (int kk = l3; kk <= d3; ++kk) c = c + dz * (Hz[off(g, l1 - 1, jj, kk)] - Hz[off(g, d1, jj, kk)]);
+        }
+    } else {
+        for (int kk = l3; kk <= d3; ++kk) {
+            for (int ii = l1; ii <= d1; ++ii) c = c + dx * (Hx[off(g, ii, l2 - 1, kk)] - Hx[off(g, ii, d2, kk)]);
+            for (int jj = l2; jj <= d2; ++jj) c = c + dy * (Hy[off(g, l1, jj, kk)] - Hy[off(g, d1 - 1, jj, kk)]);
+        }
+    }
+    return c;
+}
+
 __device__ __forceinline__ void ports_h_body(const Grid& g, const PortDev* ports, int np, int M, float* V, float* I,
                                              float* Iout, int T, const float* Hx, const float* Hy, const float* Hz,
                                              float dx, float dy, float dz, const Pulse& pu, const int* step) {
@@ -1184,25 +1212,7 @@ __device__ __forceinline__ void ports_h_body(const Grid& g, const PortDev* ports
     if (threadIdx.x == 0) {
         for (int q = 0; q < np; ++q) {
             const PortDev pt = ports[q];
-            const int l1 = pt.loc[0], l2 = pt.loc[1], l3 = pt.loc[2];
-            const int d1 = pt.size[0] + l1 - 1, d2 = pt.size[1] + l2 - 1, d3 = pt.size[2] + l3 - 1;
-            float c = 0.0f;
-            if (pt.dir == 0) {
-                for (int ii = l1; ii <= d1; ++ii) {
-                    for (int jj = l2; jj <= d2; ++jj) c = c + dy * (Hy[off(g, ii, jj, l3 - 1)] - Hy[off(g, ii, jj, d3)]);
-                    for (int kk = l3; kk <= d3; ++kk) c = c + dz * (Hz[off(g, ii, l2, kk)] - Hz[off(g, ii, d2 - 1, kk)]);
-                }
-            } else if (pt.dir == 1) {
-                for (int jj = l2; jj <= d2; ++jj) {
-                    for (int ii = l1; ii <= d1; ++ii) c = c + dx * (Hx[off(g, ii, jj, l3)] - Hx[off(g, ii, jj, d3 - 1)]);
-                    for (int kk = l3; kk <= d3; ++kk) c = c + dz * (Hz[off(g, l1 - 1, jj, kk)] - Hz[off(g, d1, jj, kk)]);
-                }
-            } else {
-                for (int kk = l3; kk <= d3; ++kk) {
-                    for (int ii = l1; ii <= d1; ++ii) c = c + dx * (Hx[off(g, ii, l2 - 1, kk)] - Hx[off(g, ii, d2, kk)]);
-                    for (int jj = l2; jj <= d2; ++jj) c = c + dy * (Hy[off(g, l1, jj, kk)] - Hy[off(g, d1 - 1, jj, kk)]);
-                }
-            }
+            float c = gap_loop_current(g, pt.dir, pt.loc, pt.size, Hx, Hy, Hz, dx, dy, dz);
             c = c * pt.cur_num / pt.cur_den;
             I[(long long)q * M] = c;
             Iout[(long long)q * T + (n - 1)] = c;
@@ -1268,6 +1278,59 @@ __global__ void __launch_bounds__(256) k_ports_h_batch(const PortArgs* __restric
 __global__ void __launch_bounds__(256) k_ports_e_batch(const PortArgs* __restrict__ args) {
     const PortArgs& a = args[blockIdx.x];
     ports_e_body(a.g, a.ports, a.np, a.M, a.V, a.I, a.Vout, a.T, a.Ex, a.Ey, a.Ez, a.pu, a.step);
+}
+
+// ---------------------------------------------------------------------------
+// SPICE ports (f90:5435-5665 current, 6285-6520 gap field).  After the H
+// phase, k_spice_h forms each port's gap current as a lumped port does; a
+// host circuit answers with node voltages, and in the E phase (after the
+// lumped ports, before the separate CPML) k_spice_e hard-sets each gap to
+// (V - bias) / gap_div.  One thread: a few ports of a few cells.
+// ---------------------------------------------------------------------------
+struct SpiceDev {
+    int dir, loc[3], size[3];
+    float cur_num, cur_den, gap_div, bias;
+};
+
+struct SpiceArgs {
+    Grid g;
+    const SpiceDev* ports;
+    int np, nextra, T;
+    float* I;                  // this step's gap currents [np], read by the host
+    const float* Vnode;        // the host's node voltages [np + nextra]
+    float *Iout, *Vout;        // series [np][T] and [np + nextra][T]
+    float *Ex, *Ey, *Ez;
+    const float *Hx, *Hy, *Hz;
+    float dx, dy, dz;
+    const int* step;
+};
+
+__global__ void k_spice_h(SpiceArgs a) {
+    if (threadIdx.x != 0) return;
+    const int n = *a.step;
+    for (int q = 0; q < a.np; ++q) {
+        const SpiceDev pt = a.ports[q];
+        float c = gap_loop_current(a.g, pt.dir, pt.loc, pt.size, a.Hx, a.Hy, a.Hz, a.dx, a.dy, a.dz);
+        c = c * pt.cur_num / pt.cur_den;
+        a.I[q] = c;
+        a.Iout[(long long)q * a.T + (n - 1)] = c;
+    }
+}
+
+__global__ void k_spice_e(SpiceArgs a) {
+    if (threadIdx.x != 0) return;
+    const int n = *a.step;
+    for (int q = 0; q < a.np; ++q) {
+        const SpiceDev pt = a.ports[q];
+        const float vs = a.Vnode[q] - pt.bias;
+        float* F = pt.dir == 0 ? a.Ex : (pt.dir == 1 ? a.Ey : a.Ez);
+        const float val = 1.0f * vs / pt.gap_div;
+        for (int ii = pt.loc[0]; ii <= pt.loc[0] + pt.size[0] - 1; ++ii)
+            for (int jj = pt.loc[1]; jj <= pt.loc[1] + pt.size[1] - 1; ++jj)
+                for (int kk = pt.loc[2]; kk <= pt.loc[2] + pt.size[2] - 1; ++kk) F[off(a.g, ii, jj, kk)] = val;
+        a.Vout[(long long)q * a.T + (n - 1)] = vs;
+    }
+    for (int e = 0; e < a.nextra; ++e) a.Vout[(long long)(a.np + e) * a.T + (n - 1)] = a.Vnode[a.np + e];
 }
 
 // ---------------------------------------------------------------------------
@@ -1795,6 +1858,92 @@ __global__ void __launch_bounds__(256) k_ntff_exact(NtffFaces f, NtffBins bn, Nt
     if (own) A[bin] = acc;
 }
 
+// Ordered binning from contributor lists built at setup (aem_sim_options
+// ntff_lists).  A face cell's retarded time tfv does not change from step
+// to step, so the bins it can reach are fixed relative to n: offset o =
+// int(n + 0.5 + tfv - 1) - n (W: without the 0.5), up to the FP32 rounding
+// of that sum near a bin edge.  For every (angle, accumulator) the host
+// lists, per bin offset k, the cells whose o can be k or k - 1, in the
+// reference's loop order (face pairs x, y, z; rows v outer, u inner; low
+// then high face).  The bin's sum then walks its list, forming t, ii and
+// the weight with the reference's FP32 expressions and adding the
+// contribution when ii or ii + 1 is the bin, so every FP32 sum is the serial
+// reference's bit for bit.
+//  - Short lists: a wave owns 64 consecutive bins, one per lane, and reads
+//    the lists interleaved ([entry][lane]), padded to the wave's longest.
+//  - Long lists (a face whose cells share one retarded time, as when the
+//    direction is along an axis): a wave owns one bin; its lanes compute 64
+//    consecutive contributions at once and the sum takes them in lane
+//    order, so the serial chain waits on arithmetic, not on memory.
+// ---------------------------------------------------------------------------
+struct NtffList {
+    const uint2* ent;       // x: element index of the face cell (0xffffffff: padding), y: tfv bits
+    const int4* waves;      // x: accumulator row (arr * n_angles + angle), y: first bin offset,
+                            // z: first entry, w: entries per lane (>= 0) or -(entries of one long list)
+    const unsigned long long* skip;   // per wave of short lists: lanes whose bin a long-list wave owns
+    int n_waves;
+    const float* Mbase;     // M of every (face, slot) at (2 face + slot) * stride; Mo at + off_old
+    const float* Jbase;     // J, likewise; Jo at + off_old
+    unsigned off_old;
+    int face_shift;         // face of an element index: idx >> face_shift
+    float* acc;             // FP32 accumulators [6][n_angles][LL]: Ux,Uy,Uz,Wx,Wy,Wz
+};
+
+// Contribution of entry q to bin `bin` at step n (the reference's FP32
+// expressions); false when the cell's bins at this step are other ones.
+__device__ __forceinline__ bool list_term(const NtffList& L, const NtffBins& bn, uint2 q, bool isW, float nf,
+                                          int bin, float& out) {
+    if (q.x == 0xffffffffu) return false;
+    const float tfv = __uint_as_float(q.y);
+    const float t = isW ? (nf + tfv - 1.0f) : (nf + 0.5f + tfv - 1.0f);
+    const int ii = (int)t;
+    if (ii != bin && ii + 1 != bin) return false;
+    const float wt = t - (float)ii;
+    const float* B = isW ? L.Jbase : L.Mbase;
+    const float dd = B[q.x] - B[q.x + L.off_old];
+    const int ax = (int)(q.x >> L.face_shift) >> 1;
+    const float K = ax == 0 ? bn.K[0] : (ax == 1 ? bn.K[1] : bn.K[2]);
+    out = ii == bin ? (1.0f - wt) * K * dd : wt * K * dd;
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_ntff_list(NtffList L, NtffBins bn, const int* step) {
+    const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (w >= L.n_waves) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int4 d = L.waves[w];
+    const int arr = d.x / bn.n_angles;
+    const bool isW = arr >= 3;
+    const int n = *step;
+    const float nf = (float)n;
+    float* A = L.acc + (long long)d.x * bn.LL;
+    if (d.w >= 0) {                                   // 64 bins, one per lane
+        const int bin = n + d.y + lane;
+        const bool own = bin >= 0 && bin < bn.LL && !((L.skip[w] >> lane) & 1ull);
+        float acc = own ? A[bin] : 0.0f;
+        const uint2* e = L.ent + d.z + lane;
+        for (int p = 0; p < d.w; ++p, e += 64) {
+            float c;
+            if (list_term(L, bn, *e, isW, nf, bin, c)) acc = acc + c;
+        }
+        if (own) A[bin] = acc;
+        return;
+    }
+    // one long list: lanes form 64 contributions, the sum takes them in order
+    const int bin = n + d.y;
+    if (bin < 0 || bin >= bn.LL) return;
+    float acc = A[bin];
+    const int cnt = -d.w;
+    for (int p0 = 0; p0 < cnt; p0 += 64) {
+        float c = 0.0f;
+        const bool hit = p0 + lane < cnt && list_term(L, bn, L.ent[d.z + p0 + lane], isW, nf, bin, c);
+        const unsigned long long m = __ballot(hit);
+        for (int l = 0; l < 64; ++l)
+            if ((m >> l) & 1ull) acc = acc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), l));
+    }
+    if (lane == 0) A[bin] = acc;
+}
+
 // NTFF retarded-time binning through an LDS histogram (f90:7643-7927 with
 // FP64 accumulation).  One work-group per (angle, accumulator): the step's
 // contributions all fall in a window of bins [n + win[0], n + win[0] +
@@ -2047,12 +2196,13 @@ __global__ void k_plane_norm(int T, const double* refl, const double* trans, flo
 // t, t+256, ... with Kahan compensation, and the 256 partial sums are added
 // in a fixed tree order, so the result does not depend on scheduling.  The
 // phase f*dt*n is reduced to [0,1) before sincospi, which keeps it exact to
-// FP64 rounding for any series length.
+// FP64 rounding for any series length.  freq holds the frequencies in Hz;
+// out receives (re, im) per frequency.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_dft(const float* __restrict__ x, int n, const double* __restrict__ fdt,
-                                             double* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_dft(const float* __restrict__ x, int n, const double* __restrict__ freq,
+                                             double dt, double* __restrict__ out) {
     __shared__ double sr[256], si[256];
-    const double a = fdt[blockIdx.x];
+    const double a = freq[blockIdx.x] * dt;
     double re = 0.0, im = 0.0, cre = 0.0, cim = 0.0;
     for (int t = threadIdx.x; t < n; t += 256) {
         double p = a * (double)t;

@@ -67,6 +67,15 @@ struct PortIn {              // ports(i,:,:) (f90:1116-1193)
     int size[3] = {0, 0, 0}; // after the +1 on the transverse axes
 };
 
+struct SpiceIn {             // ports_spice(i,:,:) and names (f90:1195-1286)
+    int gridded = 0;
+    int dir = 2;
+    int loc[3] = {0, 0, 0};
+    int size[3] = {0, 0, 0}; // after the +1 on the transverse axes
+    float v_bias = 0;        // ports_spice(i,4,1): subtracted from the node voltage
+    std::string node, isrc, cap;
+};
+
 struct Deck {
     int nx = 0, ny = 0, nz = 0;
     float dx = 0, dy = 0, dz = 0;
@@ -77,6 +86,10 @@ struct Deck {
     int bnd[6] = {0, 0, 0, 0, 0, 0};
     int pbc[3] = {0, 0, 0};
     int is_mirror = 0, mirror_type = 0, mirror_height = 0;
+    int kmax = 0;                        // deck of the kmax build (complex fields, Bloch-periodic walls)
+    int mode_type = 0;                   // kmax plane wave: 0 TE (E source), 1 TM (H source)
+    int k_direction = 1;                 // kmax plane wave: 1 '+', 0 '-'
+    float k_count[3] = {0, 0, 0};        // kmax Bloch wavenumbers of the periodic axes (rad/m)
     int excitation = 0;                  // 0 none, 1 plane wave, 2 antenna
     float pw_amp = 0, pw_delay = 0;
     float theta = 0, phi = 0, pol = 0;   // degrees as read
@@ -99,13 +112,19 @@ struct Deck {
     std::vector<SheetIn> sheets;
     std::vector<PortIn> ports;
     int num_spice_ports = 0;
+    std::string netlist;                 // SPICE section, read by the SPICE build only
+    std::vector<SpiceIn> spice;
+    std::vector<std::string> spice_extra;   // additional node names
 };
 
-Deck parse_deck(const std::string& text);
+// kmax: the grammar of the reference's kmax build (fdtd_solver.f90:832-882
+// in place of 769-830).
+Deck parse_deck(const std::string& text, bool kmax = false);
 
 struct Plan {
     Deck deck;
     std::string base_dir;
+    aem_plan_options opt{};                       // setup constants the plan was built with
     aem_plan_view v{};
 
     std::vector<float> den[6];                    // ex ey ez hx hy hz
@@ -117,8 +136,12 @@ struct Plan {
     std::vector<float> sheet_sig[3], sheet_ep[3];
     std::vector<float> Ja[3], Jb[3];
     std::vector<aem_port_view> ports;
+    std::vector<aem_spice_view> spice;
+    std::string spice_names;
     std::vector<float> ang;
     std::vector<float> src;                       // [3][time_steps] source waveforms
+    std::vector<float> src_im;                    // kmax: their imaginary parts
+    std::vector<float> ksrc;                      // kmax: [4][time_steps] modal source factors
 
     // rasterised geometry kept for the _geometry.bin writer (f90:2355-2357)
     std::vector<float> geom;                      // (4, nx, ny, nz) Fortran order
@@ -128,7 +151,12 @@ struct Plan {
     std::vector<uint16_t> cell_mat;
 };
 
-Plan* build_plan(const std::string& deck_text, const std::string& base_dir);
+// The reference's setup constants (f90:34-39, 1686-1694).
+void default_options(aem_plan_options& o);
+// Throws Error(1, ...) for values outside the accepted ranges.
+void check_options(const aem_plan_options& o);
+
+Plan* build_plan(const std::string& deck_text, const std::string& base_dir, const aem_plan_options& opt);
 
 // Ga/Gb of every edge of the box [lo, hi+1] when the cells of [lo, hi] have
 // the per-axis relative permittivity eps3 ([3][cells of the box], x fastest)

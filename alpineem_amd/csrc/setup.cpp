@@ -74,18 +74,24 @@ inline float drude_b(float wp, float loss, float dt) {
     return (float)(num / (double)den);
 }
 
-// CPML profile of one face (f90:3405-3577).
-void cpml_face(int n, float sig_max, float alpha_max, float kappa_max, float dt,
+// x**m for a small integer m, by repeated products as the reference's
+// integer power evaluates it (m = 3: x*x*x).
+float ipow(float x, int m) {
+    float p = x;
+    for (int q = 1; q < m; ++q) p = p * x;
+    return p;
+}
+
+// CPML profile of one face (f90:3405-3577); m is the grading order.
+void cpml_face(int n, int m, float sig_max, float alpha_max, float kappa_max, float dt,
                std::vector<float>& be, std::vector<float>& ce,
                std::vector<float>& bh, std::vector<float>& ch, std::vector<float>& ke,
                std::vector<float>& kh) {
-    const int m = 3;
     be.assign(std::max(n, 0), 0.f); ce.assign(std::max(n, 0), 0.f); ke.assign(std::max(n, 0), 1.f);
     bh.assign(std::max(n - 1, 0), 0.f); ch.assign(std::max(n - 1, 0), 0.f); kh.assign(std::max(n - 1, 0), 1.f);
-    (void)m;
     for (int i = 1; i <= n; ++i) {
         float r = (float)(n - i) / ((float)n - 1.0f);
-        float r3 = r * r * r;                                    // x**m with m=3
+        float r3 = ipow(r, m);
         float sg = sig_max * r3;
         float al = alpha_max * (((float)i - 1.0f) / ((float)n - 1.0f));
         float ka = 1.0f + (kappa_max - 1.0f) * r3;
@@ -97,7 +103,7 @@ void cpml_face(int n, float sig_max, float alpha_max, float kappa_max, float dt,
     }
     for (int i = 1; i <= n - 1; ++i) {
         float r = ((float)(n - i) - 0.5f) / ((float)n - 1.0f);
-        float r3 = r * r * r;
+        float r3 = ipow(r, m);
         float sg = sig_max * r3;
         float al = alpha_max * (((float)i - 0.5f) / ((float)n - 1.0f));
         float ka = 1.0f + (kappa_max - 1.0f) * r3;
@@ -117,10 +123,35 @@ std::string read_file(const std::string& path) {
 
 }  // namespace
 
-Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
+void default_options(aem_plan_options& o) {
+    o.cpml_cells = 10;          // 11 surfaces (f90:34)
+    o.cpml_grading = 3;         // m (f90:35)
+    o.cpml_kappa_max = 7.0f;    // f90:1690
+    o.cpml_alpha_max = 0.03f;   // f90:1687
+    o.cpml_reflection = 1.0E-6f;// f90:1686
+    o.tfsf_inset = 3;           // buffer (f90:39)
+    o.ntff_gap = 2;             // ff_*low = *low - 2, ff_*high = *high + 1 (f90:1638-1643)
+    o.kmax = 0;
+}
+
+void check_options(const aem_plan_options& o) {
+    auto bad = [](const std::string& what) { throw Error(1, "ERROR: setup option " + what); };
+    if (o.cpml_cells < 2 || o.cpml_cells > 64) bad("cpml_cells must be 2..64");
+    if (o.cpml_grading < 1 || o.cpml_grading > 8) bad("cpml_grading must be 1..8");
+    if (!(o.cpml_kappa_max >= 1.0f) || !std::isfinite(o.cpml_kappa_max)) bad("cpml_kappa_max must be >= 1");
+    if (!(o.cpml_alpha_max >= 0.0f) || !std::isfinite(o.cpml_alpha_max)) bad("cpml_alpha_max must be >= 0");
+    if (!(o.cpml_reflection > 0.0f && o.cpml_reflection < 1.0f)) bad("cpml_reflection must be in (0, 1)");
+    if (o.tfsf_inset < 3 || o.tfsf_inset > 64) bad("tfsf_inset must be 3..64");
+    if (o.ntff_gap < 2 || o.ntff_gap > o.tfsf_inset - 1) bad("ntff_gap must be 2..tfsf_inset-1");
+    if (o.kmax != 0 && o.kmax != 1) bad("kmax must be 0 or 1");
+}
+
+Plan* build_plan(const std::string& deck_text, const std::string& base_dir, const aem_plan_options& opt) {
+    check_options(opt);
     Plan* P = new Plan();
+    P->opt = opt;
     try {
-        P->deck = parse_deck(deck_text);
+        P->deck = parse_deck(deck_text, opt.kmax != 0);
     } catch (...) {
         delete P;
         throw;
@@ -132,13 +163,22 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         if (d.is_mirror) throw Error(5, "infinite ground plane (IGP) decks are not supported by libaem");
         for (auto& p : d.ports)
             if (p.gridded) throw Error(5, "gridded ports are not supported by libaem");
-        // the non-SPICE reference build reads the count and ignores it
-        // (f90:1195-1286 are compiled only with use_spice_version); a deck that
-        // asks for SPICE ports is refused here rather than run without them
-        if (d.num_spice_ports > 0)
-            throw Error(5, "SPICE ports need the ngspice co-simulation build, which libaem does not provide");
-        if (d.pulse_type != 1 && d.pulse_type != 2)
+        // SPICE ports (the reference's use_spice_version build): basic ones
+        // exchange current and gap voltage with a host circuit every step
+        for (auto& p : d.spice)
+            if (p.gridded) throw Error(5, "gridded SPICE ports are not supported by libaem");
+        if (!d.kmax && d.pulse_type != 1 && d.pulse_type != 2)
             throw Error(1, "ERROR: pulse type must be 1 (Gaussian) or 2 (differentiated Gaussian)");
+        if (d.kmax) {
+            // parts of the kmax build outside this library: its far field
+            // (Bloch phase removal, f90:7202-7510), the E_special Bloch copies
+            // of thin sheets, SPICE pairs, and the exports
+            if (d.n_angles > 0) throw Error(5, "far-field angles of the kmax build are not supported by libaem");
+            if (!d.sheets.empty()) throw Error(5, "thin sheets in a kmax deck are not supported by libaem");
+            if (!d.spice.empty()) throw Error(5, "SPICE ports in a kmax deck are not supported by libaem");
+            if (d.video_on || d.export_all == "yes")
+                throw Error(5, "video and full-field exports of a kmax deck are not supported by libaem");
+        }
 
         // ---- grid and derived scalars (f90:1309-1397) ----------------------
         Grid g;
@@ -169,14 +209,15 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         v.M = (int)((float)d.time_steps * dt * kC / std::min(std::min(dx, dy), dz) / 2.0f + 100.0f);  // f90:1339
 
         int npml[6];
-        for (int a = 0; a < 3; ++a) npml[2 * a] = npml[2 * a + 1] = d.pbc[a] ? 0 : 11;  // f90:34,1342-1353
+        for (int a = 0; a < 3; ++a)                                    // f90:34,1342-1353
+            npml[2 * a] = npml[2 * a + 1] = d.pbc[a] ? 0 : opt.cpml_cells + 1;
         for (int f = 0; f < 6; ++f) v.npml[f] = npml[f];
 
-        if (d.video_on) {                                              // f90:1356-1367
-            if (d.slice == 0) { v.vid_size1 = NY - 1 - d.pbc[1]; v.vid_size2 = NZ - 1 - d.pbc[2]; }
-            if (d.slice == 1) { v.vid_size1 = NX - 1 - d.pbc[0]; v.vid_size2 = NZ - 1 - d.pbc[2]; }
-            if (d.slice == 2) { v.vid_size1 = NX - 1 - d.pbc[0]; v.vid_size2 = NY - 1 - d.pbc[1]; }
-        }
+        // f90:1356-1367: the slice sizes follow the slice axis (0 when video
+        // is off) whether or not video is on; data.dat prints them either way
+        if (d.slice == 0) { v.vid_size1 = NY - 1 - d.pbc[1]; v.vid_size2 = NZ - 1 - d.pbc[2]; }
+        if (d.slice == 1) { v.vid_size1 = NX - 1 - d.pbc[0]; v.vid_size2 = NZ - 1 - d.pbc[2]; }
+        if (d.slice == 2) { v.vid_size1 = NX - 1 - d.pbc[0]; v.vid_size2 = NY - 1 - d.pbc[1]; }
         v.video_on = d.video_on; v.slice = d.slice; v.slice_location = d.slice_location;
 
         const float theta = kPi / 180.0f * d.theta;                    // f90:1370-1372
@@ -191,7 +232,7 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         int N3[3] = {NX, NY, NZ};
         for (int a = 0; a < 3; ++a) {
             lo[a] = 1; hi[a] = N3[a] - 1;
-            if (!d.pbc[a]) { lo[a] = npml[2 * a] + 3; hi[a] = N3[a] - npml[2 * a + 1] - 3; }
+            if (!d.pbc[a]) { lo[a] = npml[2 * a] + opt.tfsf_inset; hi[a] = N3[a] - npml[2 * a + 1] - opt.tfsf_inset; }
             v.tf_lo[a] = lo[a]; v.tf_hi[a] = hi[a];
         }
         // plane-wave delays by incidence quadrant (f90:1451-1494)
@@ -226,10 +267,19 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         v.inv_c = 1.0f / kC;
         v.plane_wave_amp = d.pw_amp;
         v.has_tfsf = d.pw_amp > 0 ? 1 : 0;
+        if (d.kmax) {                                                  // f90:1506-1517
+            // both directions are launched from one plane: no TF/SF box
+            v.has_tfsf = 0;
+            for (int a = 0; a < 3; ++a) v.WE[a] = v.WH[a] = 0.0f;
+            if (d.mode_type == 0)
+                for (int a = 0; a < 3; ++a) v.WE[a] = 2.0f * d.pw_amp;
+            if (d.mode_type == 1)
+                for (int a = 0; a < 3; ++a) v.WH[a] = 2.0f * std::sqrt(kEp0 / kMu0) * d.pw_amp;
+        }
 
         for (int a = 0; a < 3; ++a) {                                  // f90:1638-1680
-            v.ff_lo[a] = d.pbc[a] ? 1 : lo[a] - 2;
-            v.ff_hi[a] = d.pbc[a] ? N3[a] - 2 : hi[a] + 1;
+            v.ff_lo[a] = d.pbc[a] ? 1 : lo[a] - opt.ntff_gap;
+            v.ff_hi[a] = d.pbc[a] ? N3[a] - 2 : hi[a] + opt.ntff_gap - 1;
         }
         {
             const float fpdc = 4.0f * kPi * dt * kC;
@@ -241,12 +291,25 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         // CPML maxima (f90:1686-1694)
         const float s_mu_ep = std::pow(kMu0 / kEp0 * 1.0f, 0.5f);
         float sig_max[3];
-        sig_max[0] = -std::log(1.0E-6f) * ((float)(3 + 1) / (dx * s_mu_ep)) / (float)(2 * npml[0]);
-        sig_max[1] = -std::log(1.0E-6f) * ((float)(3 + 1) / (dy * s_mu_ep)) / (float)(2 * npml[2]);
-        sig_max[2] = -std::log(1.0E-6f) * ((float)(3 + 1) / (dz * s_mu_ep)) / (float)(2 * npml[4]);
-        const float alpha_max = 0.03f, kappa_max = 7.0f;
+        const float lnR = std::log(opt.cpml_reflection);
+        const float m1 = (float)(opt.cpml_grading + 1);
+        sig_max[0] = -lnR * (m1 / (dx * s_mu_ep)) / (float)(2 * npml[0]);
+        sig_max[1] = -lnR * (m1 / (dy * s_mu_ep)) / (float)(2 * npml[2]);
+        sig_max[2] = -lnR * (m1 / (dz * s_mu_ep)) / (float)(2 * npml[4]);
+        float alpha_max = opt.cpml_alpha_max;
+        const float kappa_max = opt.cpml_kappa_max;
+        const float kmag = std::sqrt(d.k_count[1] * d.k_count[1] + d.k_count[2] * d.k_count[2] +
+                                     d.k_count[0] * d.k_count[0]);
+        if (d.kmax) {                                                  // f90:1696-1705
+            for (int a = 0; a < 3; ++a) sig_max[a] = sig_max[a] * 1.25f;
+            alpha_max = 0.03f + 0.9f * kmag / 377.0f;
+        }
 
-        if (d.pulse_type == 1) {                                       // f90:1711-1718
+        if (d.kmax) {                                                  // f90:1720-1724
+            v.spread = 4.0f / (2.0f * kPi * d.f_center);
+            v.t_spread = 5.0f * v.spread + d.pw_delay + d.ant_delay;
+            v.f_adj = (kmag * kC) / (2.0f * kPi) + d.f_center;
+        } else if (d.pulse_type == 1) {                                // f90:1711-1718
             v.spread = 1.0f / (2.0f * kPi * d.f_center);
             v.t_spread = 5.0f * v.spread + d.pw_delay + d.ant_delay;
         } else {
@@ -691,7 +754,7 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         std::vector<float> ke[6], kh[6];
         for (int f = 0; f < 6; ++f) {
             int ax = f / 2;
-            cpml_face(npml[f], sig_max[ax], alpha_max, kappa_max, dt, P->cpml[f][0], P->cpml[f][1],
+            cpml_face(npml[f], opt.cpml_grading, sig_max[ax], alpha_max, kappa_max, dt, P->cpml[f][0], P->cpml[f][1],
                       P->cpml[f][2], P->cpml[f][3], ke[f], kh[f]);
         }
         for (int ax = 0; ax < 3; ++ax) {
@@ -748,6 +811,47 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         }
         v.n_ports = (int32_t)P->ports.size();
 
+        // ---- SPICE ports (f90:1195-1286, grid capacitance f90:3800-3890) ----
+        P->spice_names = d.netlist + "\n";
+        for (size_t si = 0; si < d.spice.size(); ++si) {
+            const SpiceIn& in = d.spice[si];
+            aem_spice_view sv{};
+            sv.dir = in.dir;
+            for (int a = 0; a < 3; ++a) { sv.loc[a] = in.loc[a]; sv.size[a] = in.size[a]; }
+            for (int a = 0; a < 3; ++a) {
+                int hiq = in.loc[a] + in.size[a] - 1;
+                if (in.loc[a] < 2 || hiq > N3[a] - 1)
+                    throw Error(1, "ERROR: SPICE port " + std::to_string(si + 1) + " lies outside the grid");
+            }
+            sv.v_bias = in.v_bias;
+            const float n0 = (float)in.size[0], n1 = (float)in.size[1], n2 = (float)in.size[2];
+            if (in.dir == 0) { sv.cur_num = n1 * n2; sv.cur_den = n0; }
+            else if (in.dir == 1) { sv.cur_num = n0 * n2; sv.cur_den = n1; }
+            else { sv.cur_num = n0 * n1; sv.cur_den = n2; }
+            sv.gap_div = (float)in.size[in.dir] * dd[in.dir];
+            // parallel-plate capacitance of the gap from the average edge
+            // permittivity, normalised to the gap length as the current is
+            float cap = 0.0f;
+            for (int ii = in.loc[0]; ii <= in.loc[0] + in.size[0] - 1; ++ii)
+                for (int jj = in.loc[1]; jj <= in.loc[1] + in.size[1] - 1; ++jj)
+                    for (int kk = in.loc[2]; kk <= in.loc[2] + in.size[2] - 1; ++kk) {
+                        float eps, sig;
+                        bulk_avg(in.dir, ii, jj, kk, eps, sig);
+                        cap = cap + eps * kEp0;
+                    }
+            cap = cap / (n0 * n1 * n2);
+            if (in.dir == 0) cap = cap * (n1 * dy * n2 * dz) / (n0 * dx);
+            else if (in.dir == 1) cap = cap * (n0 * dx * n2 * dz) / (n1 * dy);
+            else cap = cap * (n0 * dx * n1 * dy) / (n2 * dz);
+            cap = cap / (float)in.size[in.dir];
+            sv.cap = cap;
+            P->spice.push_back(sv);
+            P->spice_names += in.node + "\n" + in.isrc + "\n" + in.cap + "\n";
+        }
+        for (auto& e : d.spice_extra) P->spice_names += e + "\n";
+        v.n_spice = (int32_t)P->spice.size();
+        v.n_spice_extra = v.n_spice > 0 ? (int32_t)d.spice_extra.size() : 0;
+
         // ---- far field -----------------------------------------------------
         v.n_angles = d.n_angles;
         v.ic = d.ic; v.jc = d.jc; v.kc = d.kc;
@@ -772,13 +876,85 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
         v.ps_c1 = -1.0f * std::cos(pol) * std::cos(phi) - std::sin(pol) * std::cos(theta) * std::sin(phi);
         v.ps_c2 = std::cos(pol) * std::sin(phi) - std::sin(pol) * std::cos(theta) * std::cos(phi);
 
+        // ---- kmax: source planes, mode weights, Bloch phases ----------------
+        v.kmax = d.kmax;
+        v.k_axis = -1;
+        v.mode_type = d.mode_type;
+        v.k_direction = d.k_direction;
+        for (int a = 0; a < 3; ++a) v.k_vec[a] = d.k_count[a];
+        if (d.kmax) {
+            int a = -1;
+            if (d.pbc[1] + d.pbc[2] == 2) a = 0;
+            else if (d.pbc[0] + d.pbc[2] == 2) a = 1;
+            else if (d.pbc[0] + d.pbc[1] == 2) a = 2;
+            v.k_axis = a;
+            if (a >= 0) {                                              // f90:1727-1806
+                if (d.k_direction == 1) { v.kpl_H = lo[a] - 1; v.kpl_E = lo[a]; }
+                else { v.kpl_H = hi[a]; v.kpl_E = hi[a]; }
+                // the periodic pair (p, q) in the reference's order: (y,z), (x,z), (x,y)
+                const int p = a == 0 ? 1 : 0, q = a == 2 ? 1 : 2;
+                const float kp = d.k_count[p], kq = d.k_count[q];
+                const float nrm = std::sqrt(kq * kq + kp * kp);
+                if (kp == 0.0f && kq != 0.0f) { v.k_exc[p] = 0.0f; v.k_exc[q] = kq / nrm; }
+                if (kp != 0.0f && kq == 0.0f) { v.k_exc[p] = kp / nrm; v.k_exc[q] = 0.0f; }
+                if (kp == 0.0f && kq == 0.0f) { v.k_exc[p] = 0.0f; v.k_exc[q] = 1.0f; }
+                if (kp != 0.0f && kq != 0.0f) { v.k_exc[p] = kp / nrm; v.k_exc[q] = kq / nrm; }
+            }
+            // E copies to the first cell carry exp(i k (N-2) d), H copies to
+            // the last cell exp(-i k (N-2) d) (f90:5387-5428, 6685-6736)
+            for (int b = 0; b < 3; ++b) {
+                const float th = d.k_count[b] * (float)(N3[b] - 2) * dd[b];
+                v.bloch_c[b] = std::cos(th);
+                v.bloch_s[b] = std::sin(th);
+            }
+            // modal source factors per step (Inc, f90:8860-8885): the Gaussian
+            // exp(-((t_spread - t + x d/c) / spread)^2 / 2) and the carrier
+            // phase 2 pi f_adj (t + x d / c), with t = (n-1) dt, x = 0 on the
+            // H plane and t = (n-1/2) dt, x = -1/2 on the E plane, d the cell
+            // size along the source axis
+            const int T = d.time_steps;
+            P->ksrc.assign((size_t)4 * T, 0.0f);
+            const float dn = a >= 0 ? dd[a] : 0.0f;
+            const float tpf = 2.0f * kPi * v.f_adj;
+            for (int n = 1; n <= T; ++n) {
+                for (int ph = 0; ph < 2; ++ph) {
+                    const float ctr = ph == 0 ? (float)n - 1.0f : (float)n - 0.5f;
+                    const float x = ph == 0 ? 0.0f : -0.5f;
+                    const float q = ((v.t_spread - (ctr * dt)) + 1.0f / kC * (x * dn)) / v.spread;
+                    P->ksrc[(size_t)(2 * ph) * T + n - 1] = std::exp(-0.5f * (q * q));
+                    P->ksrc[(size_t)(2 * ph + 1) * T + n - 1] = tpf * ((ctr * dt) + 1.0f / kC * (x * dn));
+                }
+            }
+        }
+
         // ---- time-only source waveforms (f90:5001, 6162, 8368) -------------
         // Evaluated here once so that the device uses the same FP32 values
         // (and the same libm exp) as the reference's serial loop.
         {
             const int T = d.time_steps;
             P->src.assign((size_t)3 * T, 0.0f);
-            if (d.exc_port > 0 && d.ant_amp > 0) {
+            P->src_im.assign((size_t)3 * T, 0.0f);
+            if (d.kmax) {
+                // complex E_or_V_1D of the kmax build (f90:8888-8900):
+                // amp exp(-((t_spread - (tc-1) dt - pos/c) / spread)^2 / 2)
+                // exp(i 2 pi f_adj ((tc-1) dt - pos/c))
+                auto eov = [&](float amp, float tc, float pos, size_t at) {
+                    const float g = ((v.t_spread - ((tc - 1.0f) * dt)) - 1.0f / kC * pos) / v.spread;
+                    const float m = amp * std::exp(-0.5f * (g * g));
+                    const float th = 2.0f * kPi * v.f_adj * (((tc - 1.0f) * dt) - 1.0f / kC * pos);
+                    P->src[at] = m * std::cos(th);
+                    P->src_im[at] = m * std::sin(th);
+                };
+                if (d.exc_port > 0 && d.ant_amp > 0) {
+                    const float dl = P->ports[d.exc_port - 1].dl;
+                    for (int n = 1; n <= T; ++n) {
+                        eov(d.ant_amp, (float)n + 0.0f, 0.0f * dl, (size_t)n - 1);
+                        eov(d.ant_amp, (float)n + 0.5f, 0.5f * dl, (size_t)T + n - 1);
+                    }
+                }
+                if (d.pw_amp > 0)
+                    for (int n = 1; n <= T; ++n) eov(d.pw_amp, (float)n + 0.0f, 0.0f, (size_t)2 * T + n - 1);
+            } else if (d.exc_port > 0 && d.ant_amp > 0) {
                 const float dl = P->ports[d.exc_port - 1].dl;
                 for (int n = 1; n <= T; ++n) {
                     P->src[n - 1] = e_or_v_1d(d.ant_amp, (float)n + 0.0f, 0.0f * dl, v.spread, v.t_spread,
@@ -813,8 +989,12 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir) {
             v.Jb[a] = P->Jb[a].empty() ? nullptr : P->Jb[a].data();
         }
         v.ports = P->ports.empty() ? nullptr : P->ports.data();
+        v.spice = P->spice.empty() ? nullptr : P->spice.data();
+        v.spice_names = P->spice_names.c_str();
         v.ang = P->ang.empty() ? nullptr : P->ang.data();
         v.src_table = P->src.data();
+        v.src_table_im = P->src_im.data();
+        v.ksrc = P->ksrc.empty() ? nullptr : P->ksrc.data();
     } catch (...) {
         delete P;
         throw;

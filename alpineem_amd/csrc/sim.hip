@@ -100,6 +100,12 @@ bool cpml_fusable(const aem_plan_view& v) {
         for (int a = 0; a < 3; ++a) { b[2 * a] = p.loc[a] - 1; b[2 * a + 1] = p.loc[a] + p.size[a]; }
         if (!box_inside(b)) return false;
     }
+    for (int q = 0; q < v.n_spice; ++q) {
+        const aem_spice_view& p = v.spice[q];
+        int b[6];
+        for (int a = 0; a < 3; ++a) { b[2 * a] = p.loc[a] - 1; b[2 * a + 1] = p.loc[a] + p.size[a]; }
+        if (!box_inside(b)) return false;
+    }
     if (v.has_tfsf) {
         int b[6];
         for (int a = 0; a < 3; ++a) { b[2 * a] = v.tf_lo[a] - 1; b[2 * a + 1] = v.tf_hi[a] + 1; }
@@ -133,7 +139,7 @@ bool cpml_fusable(const aem_plan_view& v) {
 // sheets, TF/SF, PBC or Drude media.  Ports qualify: their H phase only reads
 // H (fdtd_solver.f90:4917-4957) and their E phase follows the bulk E.
 bool single_pass_ok(const aem_plan_view& v) {
-    if (v.has_tfsf || v.has_drude) return false;
+    if (v.has_tfsf || v.has_drude || v.n_spice > 0) return false;
     for (int a = 0; a < 3; ++a)
         if (v.pbc[a] || v.n_sheet_planes[a] > 0) return false;
     return true;
@@ -163,14 +169,21 @@ enum : unsigned {
     kPartEBulk = 1u << 4,    // bulk E (+ fused CPML)
     kPartEPre = 1u << 5,     // E sheets, Drude, TF/SF
     kPartEPort = 1u << 6,    // ports, E phase
-    kPartEPost = 1u << 7,    // separate CPML E, PBC, E_old
+    kPartEPost = 1u << 7,    // separate CPML E, PBC
     kPartOutPre = 1u << 8,   // plane sums
     kPartNtff = 1u << 9,     // far-field currents and binning
     kPartOutPost = 1u << 10, // phase-centre samples
     kPartStep = 1u << 11,    // step counter
     kPartRecord = 1u << 12,  // design-box record for the adjoint (between H and E)
     kPartObjective = 1u << 13, // far-field objective accumulation
-    kPartAll = (1u << 14) - 1
+    kPartSpiceH = 1u << 14,  // SPICE gap currents (after the H phase)
+    kPartSpiceE = 1u << 15,  // SPICE gap fields (after the lumped ports' E phase)
+    kPartEOld = 1u << 16,    // E_old of the Drude boxes (after the PBC copies)
+    kPartAll = (1u << 17) - 1,
+    // a step split at the SPICE exchange: everything up to the gap currents,
+    // then the rest
+    kPartToExchange = kPartHBulk | kPartHPre | kPartHPort | kPartHPost | kPartRecord | kPartSpiceH,
+    kPartFromExchange = kPartAll & ~kPartToExchange
 };
 
 Launch make_launch(const std::vector<Box>& boxes, int c) {
@@ -256,6 +269,9 @@ struct aem_sim {
     NtffBins nb{};
     NtffExact nx_acc{};
     int win_max = 0;
+    NtffList nl{};                 // ordered binning by contributor lists (nl.n_waves > 0)
+    bool ntff_lists = false;       // ordered binning from contributor lists (aem_sim_options)
+    void build_ntff_lists(const std::vector<int>& win, unsigned stride);
     double *d_refl = nullptr, *d_trans = nullptr;
     float* d_pc = nullptr;
     float *d_incident = nullptr, *d_Eth = nullptr, *d_Eph = nullptr, *d_fftime = nullptr;
@@ -267,6 +283,13 @@ struct aem_sim {
     HArgs ha{};                    // bulk kernel arguments (fields, tables, CPML state)
     EArgs ea{};
     PortArgs pa{};                 // port kernel arguments
+    // SPICE co-simulation: gap currents out and node voltages in through
+    // pinned staging, once per step (aem_port_exchange)
+    SpiceArgs spa{};
+    SpiceDev* d_spice = nullptr;
+    float *d_spI = nullptr, *d_spV = nullptr, *d_spIout = nullptr, *d_spVout = nullptr;
+    float *h_spI = nullptr, *h_spV = nullptr;
+    bool x_pending = false;        // a step waits at the exchange for its node voltages
     NtffArgs na{};                 // far-field kernel arguments (batched form)
     int ntff_mode = AEM_NTFF_ORDERED;
     int ntff_mu = 0, ntff_mv = 0;  // largest face extents of the far-field box
@@ -319,7 +342,8 @@ struct aem_sim {
     RunAhead ra;                   // bounds the steps a replay loop queues
     float last_ms = 0.f;
     bool finalized = false;
-    double* d_dft = nullptr;   // aem_get_dft staging: f*dt, then (re,im) pairs
+    int dev = 0;                   // HIP device current at creation
+    double* d_dft = nullptr;   // aem_get_dft staging: frequencies, then (re,im) pairs
     int dft_cap = 0;
     // exports (f90:6833-7003): the video slice stays on the device for the
     // run; the full-field records go to their files after every step
@@ -342,6 +366,8 @@ struct aem_sim {
         for (auto& f : full_f)
             if (f) std::fclose(f);
         if (h_full) (void)hipHostFree(h_full);
+        if (h_spI) (void)hipHostFree(h_spI);
+        if (h_spV) (void)hipHostFree(h_spV);
         drop_graphs();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -385,7 +411,9 @@ void aem_sim::create(const aem_sim_options* opt) {
         }
         allow_fuse = opt->no_cpml_fusion == 0;
         allow_sp = opt->no_single_pass == 0;
+        ntff_lists = opt->ntff_lists != 0;
     }
+    HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0));
     HIPCHK(hipEventCreate(&ev1));
@@ -482,6 +510,23 @@ void aem_sim::create(const aem_sim_options* opt) {
     d_I = zeros<float>((size_t)np * v.M);
     d_Vout = zeros<float>((size_t)np * T);
     d_Iout = zeros<float>((size_t)np * T);
+    if (v.n_spice > 0) {
+        std::vector<SpiceDev> sd(v.n_spice);
+        for (int q = 0; q < v.n_spice; ++q) {
+            const aem_spice_view& p = v.spice[q];
+            sd[q].dir = p.dir;
+            for (int a = 0; a < 3; ++a) { sd[q].loc[a] = p.loc[a]; sd[q].size[a] = p.size[a]; }
+            sd[q].cur_num = p.cur_num; sd[q].cur_den = p.cur_den; sd[q].gap_div = p.gap_div; sd[q].bias = p.v_bias;
+        }
+        const size_t nv = (size_t)v.n_spice + v.n_spice_extra;
+        d_spice = up<SpiceDev>(sd.data(), sd.size());
+        d_spI = zeros<float>((size_t)v.n_spice);
+        d_spV = zeros<float>(nv);
+        d_spIout = zeros<float>((size_t)v.n_spice * T);
+        d_spVout = zeros<float>(nv * T);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_spI), (size_t)v.n_spice * sizeof(float)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_spV), nv * sizeof(float)));
+    }
     pu.src = up<float>(v.src_table, (size_t)3 * T);
     pu.T = T;
 
@@ -497,14 +542,34 @@ void aem_sim::create(const aem_sim_options* opt) {
     // NTFF
     if (v.n_angles > 0) {
         for (int a = 0; a < 3; ++a) { nf.lo[a] = v.ff_lo[a]; nf.hi[a] = v.ff_hi[a]; nf.pbc[a] = v.pbc[a]; }
+        // the face arrays of every (face, slot) in one allocation, each at a
+        // power-of-two stride, in blocks M, Mo, J, Jo (the list kernel reaches
+        // any of them from one base and an element index)
+        unsigned stride = 1;
         for (int f = 0; f < 6; ++f) {
             const int a = f / 2, b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
-            nf.fd0[f] = v.ff_hi[b] + 2;
             const size_t n = (size_t)(v.ff_hi[b] + 2) * (v.ff_hi[c] + 2);
+            while (stride < n) stride <<= 1;
+        }
+        float* fb = zeros<float>((size_t)48 * stride);
+        for (int f = 0; f < 6; ++f) {
+            const int a = f / 2, b = a == 0 ? 1 : 0;
+            nf.fd0[f] = v.ff_hi[b] + 2;
             for (int s = 0; s < 2; ++s) {
-                nf.M[f][s] = zeros<float>(n); nf.Mo[f][s] = zeros<float>(n);
-                nf.J[f][s] = zeros<float>(n); nf.Jo[f][s] = zeros<float>(n);
+                const size_t o = (size_t)(2 * f + s) * stride;
+                nf.M[f][s] = fb + o;
+                nf.Mo[f][s] = fb + (size_t)12 * stride + o;
+                nf.J[f][s] = fb + (size_t)24 * stride + o;
+                nf.Jo[f][s] = fb + (size_t)36 * stride + o;
             }
+        }
+        nl.Mbase = fb;
+        nl.Jbase = fb + (size_t)24 * stride;
+        nl.off_old = 12u * stride;
+        {
+            int sh = 1;
+            while ((1u << (sh - 1)) < stride) ++sh;
+            nl.face_shift = sh;           // (2 face + slot) * stride >> (log2 stride + 1) = face
         }
         nb.ang = up<float>(v.ang, (size_t)v.n_angles * 6);
         nb.n_angles = v.n_angles;
@@ -548,15 +613,19 @@ void aem_sim::create(const aem_sim_options* opt) {
                 win_max = std::max(win_max, win[2 * kk + 1]);
             }
             nx_acc.win = up<int>(win.data(), win.size());
-        }
-        if (nb.exact) {
-            for (int q = 0; q < 3; ++q) {
-                nb.Uf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
-                nb.Wf[q] = zeros<float>((size_t)v.n_angles * nb.LL);
-                nx_acc.acc[q] = nb.Uf[q];
-                nx_acc.acc[3 + q] = nb.Wf[q];
+            if (nb.exact) {
+                const size_t row = (size_t)v.n_angles * nb.LL;
+                nl.acc = zeros<float>(6 * row);
+                for (int q = 0; q < 3; ++q) {
+                    nb.Uf[q] = nl.acc + q * row;
+                    nb.Wf[q] = nl.acc + (3 + q) * row;
+                    nx_acc.acc[q] = nb.Uf[q];
+                    nx_acc.acc[3 + q] = nb.Wf[q];
+                }
+                if (ntff_lists) build_ntff_lists(win, stride);
             }
-        } else {
+        }
+        if (!nb.exact) {
             for (int q = 0; q < 3; ++q) {
                 nb.U[q] = zeros<double>((size_t)v.n_angles * nb.LL);
                 nb.W[q] = zeros<double>((size_t)v.n_angles * nb.LL);
@@ -565,7 +634,7 @@ void aem_sim::create(const aem_sim_options* opt) {
         // the histogram keeps one window of bins in LDS; larger windows bin
         // through global atomics
         if (ntff_mode == AEM_NTFF_HISTOGRAM && win_max > kHistMax) ntff_mode = AEM_NTFF_ATOMIC;
-        if (ntff_mode == AEM_NTFF_ORDERED)
+        if (ntff_mode == AEM_NTFF_ORDERED && nl.n_waves == 0)   // the row-scan kernel's limit
             for (int a = 0; a < 2; ++a)
                 if (!v.pbc[a] && 2 * (v.ff_hi[a] - v.ff_lo[a] + 1) > kExactCells)
                     throw aem::Error(AEM_ERR_UNSUPPORTED,
@@ -683,6 +752,8 @@ void aem_sim::bind() {
     pa = PortArgs{g, d_ports, v.n_ports, v.M, v.time_steps, d_V, d_I, d_Vout, d_Iout, F[0], F[1], F[2], F[3], F[4],
                   F[5], v.dx, v.dy, v.dz, pu, d_step};
     na = NtffArgs{g, nf, nb, nx_acc.win, F[0], F[1], F[2], F[3], F[4], F[5], d_step};
+    spa = SpiceArgs{g, d_spice, v.n_spice, v.n_spice_extra, v.time_steps, d_spI, d_spV, d_spIout, d_spVout,
+                    F[0], F[1], F[2], F[3], F[4], F[5], v.dx, v.dy, v.dz, d_step};
 }
 
 // After a single-pass step: the other buffer set now holds the fields.
@@ -812,6 +883,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Hy, Hx);
     }
 
+    if ((parts & kPartSpiceH) && v.n_spice > 0) k_spice_h<<<1, 64, 0, s>>>(spa);
+
     if ((parts & kPartRecord) && rec_on && !des_on)   // a design box records in k_e_design
         k_adj_record<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, Ex, Ey, Ez, Hx, Hy, Hz, d_den[0], d_den[1], d_den[2],
                                                           d_step);
@@ -846,6 +919,7 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         }
     }
     if ((parts & kPartEPort) && v.n_ports > 0) k_ports_e<<<1, 256, 0, s>>>(pa);
+    if ((parts & kPartSpiceE) && v.n_spice > 0) k_spice_e<<<1, 64, 0, s>>>(spa);
     if (parts & kPartEPost) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
@@ -868,10 +942,10 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[1].np), 256, 0, s>>>(sh[1], 1, v.NZ - 1, v.NX - 1);
             if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[0].np), 256, 0, s>>>(sh[0], 1, v.NZ - 1, v.NY - 1);
         }
-        if (v.has_drude)
-            for (int c = 0; c < 3; ++c)
-                if (db[c].nb > 0) k_save_eold<<<cdiv(db[c].nb, 256), 256, 0, s>>>(g, db[c], F[c]);
     }
+    if ((parts & kPartEOld) && v.has_drude)
+        for (int c = 0; c < 3; ++c)
+            if (db[c].nb > 0) k_save_eold<<<cdiv(db[c].nb, 256), 256, 0, s>>>(g, db[c], F[c]);
 
     // ---- per-step outputs ----
     if (mark) HIPCHK(hipEventRecord(mark[4], st));
@@ -892,7 +966,9 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         const dim3 gf(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6);
         k_ntff_currents<<<gf, bb, 0, s>>>(g, nf, Ex, Ey, Ez, Hx, Hy, Hz);
         if (ntff_mode == AEM_NTFF_ORDERED) {
-            if (win_max > 0)
+            if (nl.n_waves > 0)
+                k_ntff_list<<<cdiv(nl.n_waves, 4), 256, 0, s>>>(nl, nb, d_step);
+            else if (win_max > 0)
                 k_ntff_exact<<<dim3(cdiv(win_max, 256), 6, v.n_angles), 256, 0, s>>>(nf, nb, nx_acc, d_step);
         } else if (ntff_mode == AEM_NTFF_HISTOGRAM) {
             k_ntff_hist<<<dim3(v.n_angles, 6), 256, 0, s>>>(nf, nb, nx_acc.win, d_step);
@@ -951,6 +1027,102 @@ void aem_sim::export_step() {
         if (std::fwrite(&len, 4, 1, full_f[q]) != 1 || std::fwrite(h_full + q * n, 4, n, full_f[q]) != n ||
             std::fwrite(&len, 4, 1, full_f[q]) != 1)
             throw aem::Error(AEM_ERR_IO, "writing a full-field record failed");
+}
+
+// Contributor lists of the ordered far-field binning (k_ntff_list).  For
+// angle kk and accumulator arr (Ux,Uy,Uz,Wx,Wy,Wz) a face cell with retarded
+// time tfv lands at t = n + tfv - 0.5 (W: n + tfv - 1), evaluated in FP32 as
+// the reference does.  Its bin offset ii - n is floor of the exact value,
+// except that the FP32 rounding of t (at most one ulp of the largest t, here
+// doubled) can move it across a bin edge; the cell goes into the list of
+// every offset it can reach with either weight.  Lists keep the reference's
+// loop order (the order k_ntff_exact walks).  Larger than kListBytes in
+// total, the builder gives up and the row-scan kernel stays in use.  A
+// wave's descriptor: row, first bin offset, first entry, and the entries per
+// lane (64 short lists) or minus the length of one long list.
+void aem_sim::build_ntff_lists(const std::vector<int>& win, unsigned stride) {
+    constexpr size_t kListBytes = size_t(4) << 30;
+    constexpr size_t kLong = 192;     // entries from which a bin's list gets a wave of its own
+    const aem_plan_view& v = plan->v;
+    const float cdt = aem::kC * v.dt;
+    const double tmax = (double)v.time_steps + (double)v.len_ff + 8.0;
+    const double eps = 2.0 * std::ldexp(1.0, (int)std::ceil(std::log2(tmax)) - 23);
+    std::vector<int4> waves;
+    std::vector<unsigned long long> skip;
+    std::vector<uint2> ent;
+    std::vector<std::vector<uint2>> lists;
+    for (int kk = 0; kk < v.n_angles; ++kk) {
+        const float* an = v.ang + 6 * kk;
+        const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
+        const int wlo = win[2 * kk], wlen = win[2 * kk + 1];
+        for (int arr = 0; arr < 6; ++arr) {
+            const int comp = arr % 3;
+            const double shift = arr >= 3 ? -1.0 : -0.5;
+            lists.assign((size_t)wlen, {});
+            for (int a = 0; a < 3; ++a) {                     // face pairs in the reference's order
+                if (a == comp || v.pbc[a]) continue;
+                const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+                const int slot = comp == b ? 0 : 1;
+                for (int vv = v.ff_lo[c]; vv <= v.ff_hi[c]; ++vv)
+                    for (int uu = v.ff_lo[b]; uu <= v.ff_hi[b]; ++uu)
+                        for (int side = 0; side < 2; ++side) {
+                            const int pos = side ? v.ff_hi[a] : v.ff_lo[a];
+                            int i, j, k;
+                            if (a == 0) { i = pos; j = uu; k = vv; }
+                            else if (a == 1) { i = uu; j = pos; k = vv; }
+                            else { i = uu; j = vv; k = pos; }
+                            const float tfv = (v.r_relay - ((float)(i - v.ic) * v.dx * s1 * c2 +
+                                                            (float)(j - v.jc) * v.dy * s1 * s2 +
+                                                            (float)(k - v.kc) * v.dz * c1)) / cdt;
+                            const double x = (double)tfv + shift;
+                            const int olo = (int)std::floor(x - eps), ohi = (int)std::floor(x + eps);
+                            const int face = 2 * a + side;
+                            const unsigned idx = (unsigned)(2 * face + slot) * stride +
+                                                 (unsigned)(uu + nf.fd0[face] * vv);
+                            uint32_t tb;
+                            std::memcpy(&tb, &tfv, 4);
+                            for (int o = olo; o <= ohi + 1; ++o) {
+                                const int q = o - wlo;
+                                if (q < 0 || q >= wlen)
+                                    throw aem::Error(AEM_ERR_STATE, "far-field list outside its bin window");
+                                lists[q].push_back(make_uint2(idx, tb));
+                            }
+                        }
+            }
+            const int row = arr * v.n_angles + kk;
+            for (int w0 = 0; w0 < wlen; w0 += 64) {
+                // lists longer than kLong get a wave of their own; the rest
+                // of the 64 bins share one wave, one lane per bin
+                size_t m = 0;
+                unsigned long long sk = 0;
+                for (int l = 0; l < 64 && w0 + l < wlen; ++l) {
+                    const std::vector<uint2>& li = lists[w0 + l];
+                    if (li.size() > kLong) {
+                        sk |= 1ull << l;
+                        waves.push_back(make_int4(row, wlo + w0 + l, (int)ent.size(), -(int)li.size()));
+                        skip.push_back(0ull);
+                        ent.insert(ent.end(), li.begin(), li.end());
+                    } else {
+                        m = std::max(m, li.size());
+                    }
+                }
+                if ((ent.size() + 64 * m) * sizeof(uint2) > kListBytes) return;
+                waves.push_back(make_int4(row, wlo + w0, (int)ent.size(), (int)m));
+                skip.push_back(sk);
+                for (size_t p = 0; p < m; ++p)
+                    for (int l = 0; l < 64; ++l) {
+                        const int q = w0 + l;
+                        const bool in = q < wlen && !((sk >> l) & 1ull) && p < lists[q].size();
+                        ent.push_back(in ? lists[q][p] : make_uint2(0xffffffffu, 0u));
+                    }
+            }
+        }
+    }
+    if (ent.size() > 0x7fffffffull) return;
+    nl.ent = up<uint2>(ent.data(), ent.size());
+    nl.waves = up<int4>(waves.data(), waves.size());
+    nl.skip = up<unsigned long long>(skip.data(), skip.size());
+    nl.n_waves = (int)waves.size();
 }
 
 void aem_sim::enqueue_watch(hipStream_t s) {
@@ -1035,6 +1207,10 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
         aem::set_error("aem_run: simulation already finalized");
         return AEM_ERR_STATE;
     }
+    if (s->plan->v.n_spice > 0) {
+        aem::set_error("aem_run: the deck has SPICE ports; step it with aem_port_exchange");
+        return AEM_ERR_STATE;
+    }
     return aem::guard([&]() {
         if (n_steps == 0) return;
         if (use_graph && !s->gexec) {
@@ -1093,12 +1269,68 @@ aem_status aem_sync(aem_sim* s) {
     return aem::guard([&]() { HIPCHK(hipStreamSynchronize(s->st)); });
 }
 
+aem_status aem_port_exchange(aem_sim* s, const float* v_in, float* i_out) {
+    if (!s) {
+        aem::set_error("aem_port_exchange: null handle");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (v.n_spice == 0) {
+        aem::set_error("aem_port_exchange: the deck has no SPICE ports");
+        return AEM_ERR_STATE;
+    }
+    if (s->finalized) {
+        aem::set_error("aem_port_exchange: simulation already finalized");
+        return AEM_ERR_STATE;
+    }
+    if (s->x_pending && !v_in) {
+        aem::set_error("aem_port_exchange: a step waits for its node voltages (v_in)");
+        return AEM_ERR_STATE;
+    }
+    if (i_out && s->steps_done + (s->x_pending ? 1 : 0) >= v.time_steps) {
+        aem::set_error("aem_port_exchange: no step left (the deck's time_steps are done)");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        const size_t nv = (size_t)v.n_spice + v.n_spice_extra;
+        if (s->x_pending) {
+            // rest of the waiting step, from the gap fields on
+            std::memcpy(s->h_spV, v_in, nv * sizeof(float));
+            HIPCHK(hipMemcpyAsync(s->d_spV, s->h_spV, nv * sizeof(float), hipMemcpyHostToDevice, s->st));
+            s->enqueue_parts(nullptr, s->st, kPartFromExchange);
+            HIPCHK(hipGetLastError());
+            s->x_pending = false;
+            s->steps_done += 1;
+            if (s->full_on) s->export_step();
+            if (s->steps_done % kWatchSteps == 0 || s->steps_done == v.time_steps) {
+                s->enqueue_watch(s->st);
+                s->check_watch();
+            }
+        }
+        if (i_out) {
+            // next step up to its gap currents
+            s->enqueue_parts(nullptr, s->st, kPartToExchange);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(s->h_spI, s->d_spI, (size_t)v.n_spice * sizeof(float), hipMemcpyDeviceToHost, s->st));
+            HIPCHK(hipStreamSynchronize(s->st));
+            std::memcpy(i_out, s->h_spI, (size_t)v.n_spice * sizeof(float));
+            s->x_pending = true;
+        } else {
+            HIPCHK(hipStreamSynchronize(s->st));
+        }
+    });
+}
+
 aem_status aem_finalize(aem_sim* s) {
     if (!s) {
         aem::set_error("aem_finalize: null handle");
         return AEM_ERR_STATE;
     }
     if (s->finalized) return AEM_OK;
+    if (s->x_pending) {
+        aem::set_error("aem_finalize: a step waits at the SPICE exchange; finish it with aem_port_exchange");
+        return AEM_ERR_STATE;
+    }
     return aem::guard([&]() {
         const aem_plan_view& v = s->plan->v;
         const int T = v.time_steps;
@@ -1134,6 +1366,23 @@ aem_status aem_finalize(aem_sim* s) {
 
 namespace {
 // Device address of an output series (T floats), or null with the error set.
+// Series recorded step by step (valid for the steps run so far), as opposed
+// to the post-loop products.
+bool per_step_series(int32_t kind) {
+    return kind == AEM_SERIES_PORT_V || kind == AEM_SERIES_PORT_I || kind == AEM_SERIES_SPICE_V ||
+           kind == AEM_SERIES_SPICE_I;
+}
+
+// Frequencies (Hz) to the device buffer buf, the DFT of src (n samples) into
+// buf + n_freq, the (re, im) pairs back to re_im.  buf holds 3 n_freq doubles.
+void dft_into(aem_sim* s, const float* src, int n, const double* freqs_hz, int n_freq, double* buf, double* re_im) {
+    HIPCHK(hipMemcpyAsync(buf, freqs_hz, (size_t)n_freq * sizeof(double), hipMemcpyHostToDevice, s->st));
+    k_dft<<<n_freq, 256, 0, s->st>>>(src, n, buf, (double)s->plan->v.dt, buf + n_freq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(re_im, buf + n_freq, (size_t)n_freq * 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+}
+
 const float* series_src(const aem_sim* s, int32_t kind, int32_t idx) {
     const aem_plan_view& v = s->plan->v;
     const int T = v.time_steps;
@@ -1148,6 +1397,12 @@ const float* series_src(const aem_sim* s, int32_t kind, int32_t idx) {
             if (s->ntff_mode == AEM_NTFF_OFF) { aem::set_error("the time-domain far field is off (AEM_NTFF_OFF)"); return nullptr; }
             if (!s->finalized) { aem::set_error("call aem_finalize first"); return nullptr; }
             return (kind == AEM_SERIES_E_THETA ? s->d_Eth : s->d_Eph) + (size_t)idx * T;
+        case AEM_SERIES_SPICE_V:
+            if (idx < 0 || idx >= v.n_spice + v.n_spice_extra) { aem::set_error("SPICE port index out of range"); return nullptr; }
+            return s->d_spVout + (size_t)idx * T;
+        case AEM_SERIES_SPICE_I:
+            if (idx < 0 || idx >= v.n_spice) { aem::set_error("SPICE port index out of range"); return nullptr; }
+            return s->d_spIout + (size_t)idx * T;
         case AEM_SERIES_INCIDENT:
         case AEM_SERIES_REFLECTED:
         case AEM_SERIES_TRANSMITTED:
@@ -1193,7 +1448,7 @@ aem_status aem_get_dft(aem_sim* s, int32_t kind, int32_t idx, const double* freq
     if (n_freq == 0) return AEM_OK;
     const aem_plan_view& v = s->plan->v;
     // port series hold the steps run so far; the post-loop series all T
-    const int n = (kind == AEM_SERIES_PORT_V || kind == AEM_SERIES_PORT_I) ? (int)s->steps_done : v.time_steps;
+    const int n = per_step_series(kind) ? (int)s->steps_done : v.time_steps;
     return aem::guard([&]() {
         if (n_freq > s->dft_cap) {
             if (s->d_dft) HIPCHK(hipFree(s->d_dft));
@@ -1202,15 +1457,7 @@ aem_status aem_get_dft(aem_sim* s, int32_t kind, int32_t idx, const double* freq
             HIPCHK(hipMalloc(&s->d_dft, (size_t)n_freq * 3 * sizeof(double)));
             s->dft_cap = n_freq;
         }
-        std::vector<double> fdt((size_t)n_freq);
-        for (int q = 0; q < n_freq; ++q) fdt[q] = freqs_hz[q] * (double)v.dt;
-        double* d_fdt = s->d_dft;
-        double* d_out = s->d_dft + n_freq;
-        HIPCHK(hipMemcpyAsync(d_fdt, fdt.data(), fdt.size() * sizeof(double), hipMemcpyHostToDevice, s->st));
-        k_dft<<<n_freq, 256, 0, s->st>>>(src, n, d_fdt, d_out);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(re_im, d_out, (size_t)n_freq * 2 * sizeof(double), hipMemcpyDeviceToHost, s->st));
-        HIPCHK(hipStreamSynchronize(s->st));
+        dft_into(s, src, n, freqs_hz, n_freq, s->d_dft, re_im);
     });
 }
 
@@ -1500,7 +1747,7 @@ struct aem_batch {
         }
         each(kPartEPre);
         if (n_port) k_ports_e_batch<<<n_port, 256, 0, s>>>(d_pargs);
-        each(kPartEPost);
+        each(kPartEPost | kPartEOld);
         each(kPartOutPre);
         if (n_ntff) {
             k_ntff_currents_batch<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6 * n_ntff), bb, 0, s>>>(d_nargs);
@@ -1532,6 +1779,10 @@ aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
             }
         if (sims[q]->full_on) {
             aem::set_error("aem_batch_create: a member exports the full field every step; run it alone");
+            return AEM_ERR_UNSUPPORTED;
+        }
+        if (sims[q]->plan->v.n_spice > 0) {
+            aem::set_error("aem_batch_create: a member has SPICE ports; step it alone with aem_port_exchange");
             return AEM_ERR_UNSUPPORTED;
         }
     }
@@ -2059,11 +2310,20 @@ namespace {
 // The full backward sweep of J = |F|^2: dJ/dGa and dJ/dGb of every record
 // edge ([3][edges] each).  Clears the adjoint state, captures one backward
 // step as a graph and replays it time_steps times.
-void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb) {
+// The adjoint source weights a step's terms by 2 Re(conj(F) w), the
+// derivative of J = |F|^2.  A general real J(F) with dJ/dF = dJ/dRe F +
+// i dJ/dIm F (dj, two doubles) has dJ = Re(conj(dj) dF): the same source
+// with F replaced by dj / 2.
+void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb, const double* dj = nullptr) {
     const aem_plan_view& v = s->plan->v;
     s->ensure_adjoint();
     double F[2];
-    HIPCHK(hipMemcpy(F, s->obj.F, sizeof F, hipMemcpyDeviceToHost));
+    if (dj) {
+        F[0] = 0.5 * dj[0];
+        F[1] = 0.5 * dj[1];
+    } else {
+        HIPCHK(hipMemcpy(F, s->obj.F, sizeof F, hipMemcpyDeviceToHost));
+    }
     const long long ne = s->rec.nedge;
     std::vector<void*> tmp;
     double* dga = dalloc<double>(3 * ne, tmp);
@@ -2169,6 +2429,39 @@ aem_status aem_adjoint_design_gradient(aem_sim* s, double* grad, int64_t n) {
         backward_sweep(s, ha, hb);
         cell_gradient(*s->plan, s->rec, s->rec_lo, s->rec_hi, s->des_eps.data(), ha, hb, cells);
         std::memcpy(grad, cells.data(), cells.size() * sizeof(double));
+    });
+}
+
+aem_status aem_adjoint(aem_sim* s, const double* dJ_dF, double* grad, int64_t n) {
+    if (!s || !dJ_dF || !grad || n < 0) {
+        aem::set_error("aem_adjoint: bad argument");
+        return AEM_ERR_STATE;
+    }
+    if (!gradient_ready(s, "aem_adjoint")) return AEM_ERR_STATE;
+    const aem_plan_view& v = s->plan->v;
+    const int64_t want = s->des_on ? (int64_t)s->des_eps.size() / 3 : (int64_t)s->plan->deck.mats.size();
+    if (s->des_on ? n != want : n > want) {
+        aem::set_error("aem_adjoint: n must be the design box's cell count " + std::string(s->des_on ? "" : "or at most the deck's material count ") +
+                       "(" + std::to_string(want) + ")");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        std::vector<double> ha, hb, cells;
+        backward_sweep(s, ha, hb, dJ_dF);
+        cell_gradient(*s->plan, s->rec, s->rec_lo, s->rec_hi, s->des_on ? s->des_eps.data() : nullptr, ha, hb, cells);
+        if (s->des_on) {
+            std::memcpy(grad, cells.data(), cells.size() * sizeof(double));
+            return;
+        }
+        for (int64_t m = 0; m < n; ++m) grad[m] = 0.0;
+        size_t b = 0;
+        for (int k = s->rec_lo[2]; k <= s->rec_hi[2]; ++k)
+            for (int j = s->rec_lo[1]; j <= s->rec_hi[1]; ++j)
+                for (int i = s->rec_lo[0]; i <= s->rec_hi[0]; ++i, ++b) {
+                    const int m = s->plan->cell_mat[(size_t)(i - 1) +
+                                                    (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (k - 1))];
+                    if (m >= 1 && m <= n) grad[m - 1] += cells[b];
+                }
     });
 }
 

@@ -106,7 +106,7 @@ extern "C" aem_status aem_write_outputs(aem_sim* s, const char* path) {
         f.line("video size_b =  " + std::to_string(v.vid_size2));
         f.line("number of far field angles =  " + std::to_string(v.n_angles));
         f.line("number of internal ports =  " + std::to_string(v.n_ports));
-        f.line("number of spice ports =  0");
+        f.line("number of spice ports =  " + std::to_string(v.n_spice + v.n_spice_extra));
         f.line("simulation type " + std::to_string(v.pbc[0]) + " " + std::to_string(v.pbc[1]) + " " +
                std::to_string(v.pbc[2]));
         std::vector<float> buf(T);
@@ -142,6 +142,12 @@ extern "C" aem_status aem_write_outputs(aem_sim* s, const char* path) {
             f.line("recieved voltage at port  " + std::to_string(q + 1) + " w/ Z");
             f.line(fmt(std::sqrt(v.ports[q].L / v.ports[q].C)));
             get(s, AEM_SERIES_PORT_V, q, buf);
+            for (int n = 0; n < T; ++n) f.line(fmt(buf[n]));
+        }
+        for (int q = 0; q < v.n_spice + v.n_spice_extra; ++q) {              // f90:8619-8627 (SPICE build)
+            f.line("recieved voltage at spice port  " + std::to_string(q + 1) + " w/ Z");
+            f.line("1");
+            get(s, AEM_SERIES_SPICE_V, q, buf);
             for (int n = 0; n < T; ++n) f.line(fmt(buf[n]));
         }
         if (v.n_angles > 0) {

@@ -27,6 +27,7 @@ class DatFile:
     angles: List[tuple] = field(default_factory=list)
     port_v: List[np.ndarray] = field(default_factory=list)
     port_z: List[float] = field(default_factory=list)
+    spice_v: List[np.ndarray] = field(default_factory=list)   # SPICE build: per SPICE port / extra node
     ff_time: np.ndarray = None
     ff_pc: np.ndarray = None
 
@@ -65,6 +66,8 @@ def read_dat(path: str) -> DatFile:
             out.e_theta.append(np.array([r[0] for r in rows[1:T + 1]], np.float32))
         elif name.startswith("far field rE phi"):
             out.e_phi.append(np.array([r[0] for r in rows[1:T + 1]], np.float32))
+        elif name.startswith("recieved voltage at spice port"):
+            out.spice_v.append(np.array([r[0] for r in rows[1:T + 1]], np.float32))
         elif name.startswith("recieved voltage"):
             out.port_z.append(rows[0][0])
             out.port_v.append(np.array([r[0] for r in rows[1:T + 1]], np.float32))

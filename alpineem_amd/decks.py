@@ -110,6 +110,24 @@ class Port:
 
 
 @dataclass
+class SpicePort:
+    """Basic SPICE port of the SPICE build's deck section (f90:1195-1286):
+    gap axis, first cell, size as typed, DC bias and the circuit's names for
+    the port node, its current source and its capacitor."""
+    axis: str
+    start: Tuple[int, int, int]
+    size: Tuple[int, int, int] = (1, 1, 1)
+    bias: float = 0.0
+    node: str = "out"
+    source: str = "I"
+    capacitor: str = "Cp"
+
+    def lines(self) -> List[str]:
+        return ["basic", self.axis, _row(*self.start), _row(*self.size), _f(self.bias), self.node, self.source,
+                self.capacitor]
+
+
+@dataclass
 class Deck:
     cells: Tuple[int, int, int]
     cell_size: Tuple[float, float, float]
@@ -133,6 +151,9 @@ class Deck:
     sheets: list = field(default_factory=list)
     ports: list = field(default_factory=list)
     optional_geometry: Optional[str] = None
+    spice_ports: list = field(default_factory=list)
+    netlist: str = "fdtd_netlist.cir"
+    spice_extra_nodes: list = field(default_factory=list)
 
     def text(self) -> str:
         L = [_row(*self.cells), _row(*self.cell_size), _f(self.time_steps), _f(self.factor), _f(self.f_center),
@@ -165,7 +186,13 @@ class Deck:
         L.append(_f(len(self.ports)))
         for p in self.ports:
             L += p.lines()
-        L.append("0")                                  # SPICE ports
+        L.append(_f(len(self.spice_ports)))
+        if self.spice_ports:
+            L.append(self.netlist)
+            for p in self.spice_ports:
+                L += p.lines()
+            L.append(_f(len(self.spice_extra_nodes)))
+            L += list(self.spice_extra_nodes)
         return "\n".join(L) + "\n"
 
 

@@ -28,16 +28,36 @@ class Plan:
         self._h = handle
         self.view = _lib.PlanView.from_address(lib().aem_plan_view_get(handle))
 
+    @staticmethod
+    def options(**setup) -> Optional[_lib.PlanOptions]:
+        """aem_plan_options holding the reference's setup constants with the
+        given fields replaced (cpml_cells, cpml_grading, cpml_kappa_max,
+        cpml_alpha_max, cpml_reflection, tfsf_inset, ntff_gap); None when
+        nothing is replaced."""
+        if not setup:
+            return None
+        o = _lib.PlanOptions()
+        lib().aem_plan_options_init(C.byref(o))
+        for k, val in setup.items():
+            if k not in dict(_lib.PlanOptions._fields_):
+                raise TypeError(f"unknown setup option {k!r}")
+            setattr(o, k, val)
+        return o
+
     @classmethod
-    def from_inputs(cls, path: str) -> "Plan":
+    def from_inputs(cls, path: str, **setup) -> "Plan":
+        """Parse a deck file; keyword arguments replace the reference's setup
+        constants (see :meth:`options`)."""
         h = C.c_void_p()
-        check(lib().aem_plan_from_inputs(os.fspath(path).encode(), C.byref(h)))
+        o = cls.options(**setup)
+        check(lib().aem_plan_from_inputs_ex(os.fspath(path).encode(), C.byref(o) if o else None, C.byref(h)))
         return cls(h)
 
     @classmethod
-    def from_text(cls, text: str, base_dir: str = ".") -> "Plan":
+    def from_text(cls, text: str, base_dir: str = ".", **setup) -> "Plan":
         h = C.c_void_p()
-        check(lib().aem_plan_from_text(text.encode(), base_dir.encode(), C.byref(h)))
+        o = cls.options(**setup)
+        check(lib().aem_plan_from_text_ex(text.encode(), base_dir.encode(), C.byref(o) if o else None, C.byref(h)))
         return cls(h)
 
     @property
@@ -81,12 +101,14 @@ class Simulation:
     time-domain far field, e.g. for objective-only runs), ``planes``
     the z planes each bulk thread marches, ``fuse_cpml=False`` forces the
     separate CPML kernels, ``single_pass=False`` separate H and E bulk
-    launches (aem_sim_options in include/aem.h)."""
+    launches, ``ntff_lists=True`` ordered binning from contributor lists
+    instead of the row-scan kernel (aem_sim_options in include/aem.h)."""
 
     def __init__(self, plan: Plan, ntff: str = "ordered", planes: int = 0, fuse_cpml: bool = True,
-                 single_pass: bool = True):
+                 single_pass: bool = True, ntff_lists: bool = False):
         self.plan = plan
-        opt = _lib.SimOptions(_lib.NTFF_BINNING[ntff], int(planes), 0 if fuse_cpml else 1, 0 if single_pass else 1)
+        opt = _lib.SimOptions(_lib.NTFF_BINNING[ntff], int(planes), 0 if fuse_cpml else 1, 0 if single_pass else 1,
+                              1 if ntff_lists else 0)
         h = C.c_void_p()
         check(lib().aem_sim_create_ex(plan.handle, C.byref(opt), C.byref(h)))
         self._h = h
@@ -130,6 +152,37 @@ class Simulation:
 
     def finalize(self) -> None:
         check(lib().aem_finalize(self._h))
+
+    # -- SPICE co-simulation -------------------------------------------------------
+    @property
+    def spice_ports(self):
+        """The deck's SPICE ports and circuit names (plan view): a dict with
+        ``netlist``, ``ports`` (per port: dir, loc, size, bias, cap, node,
+        source, capacitor) and ``extra_nodes``."""
+        v = self.plan.view
+        names = (v.spice_names or b"").decode().split("\n")
+        ports = []
+        for q in range(v.n_spice):
+            sp = v.spice[q]
+            node, src, cap = names[1 + 3 * q: 4 + 3 * q]
+            ports.append(dict(dir=sp.dir, loc=tuple(sp.loc), size=tuple(sp.size), bias=sp.v_bias, cap=sp.cap,
+                              node=node, source=src, capacitor=cap))
+        extra = names[1 + 3 * v.n_spice: 1 + 3 * v.n_spice + v.n_spice_extra]
+        return dict(netlist=names[0] if v.n_spice else "", ports=ports, extra_nodes=extra)
+
+    def port_exchange(self, v_in=None, current: bool = True) -> Optional[np.ndarray]:
+        """One exchange with the host circuit (aem_port_exchange): finish the
+        step waiting for node voltages ``v_in`` (ports, then extra nodes; None
+        when no step waits), then, if ``current``, run the next step up to its
+        SPICE gap currents and return them."""
+        v = self.plan.view
+        vin = None if v_in is None else np.ascontiguousarray(v_in, np.float32).reshape(-1)
+        if vin is not None and vin.size != v.n_spice + v.n_spice_extra:
+            raise ValueError(f"v_in needs {v.n_spice + v.n_spice_extra} node voltages")
+        out = np.zeros(max(v.n_spice, 1), np.float32)
+        check(lib().aem_port_exchange(self._h, None if vin is None else vin.ctypes.data,
+                                      out.ctypes.data if current else None))
+        return out[:v.n_spice] if current else None
 
     # -- outputs -----------------------------------------------------------------
     def series(self, kind: str, idx: int = 0, n: Optional[int] = None) -> np.ndarray:
@@ -210,6 +263,18 @@ class Simulation:
         (or flat, x fastest)."""
         a = np.ascontiguousarray(eps, np.float32).reshape(-1)
         check(lib().aem_set_design_eps(self._h, a.ctypes.data, a.size))
+
+    def gradient(self, dJ_dF: complex, n_materials: int = 0) -> np.ndarray:
+        """Gradient of a general real objective J(F) of the far-field
+        objective F (aem_adjoint): ``dJ_dF`` = dJ/dRe F + i dJ/dIm F at the
+        run's F.  Per design cell (shape ``design_shape``) with a design box,
+        else per deck material 1..n_materials."""
+        g = np.array([complex(dJ_dF).real, complex(dJ_dF).imag], np.float64)
+        shape = getattr(self, "_design_shape", None)
+        n = int(np.prod(shape)) if shape else int(n_materials)
+        out = np.zeros(n, np.float64)
+        check(lib().aem_adjoint(self._h, g.ctypes.data, out.ctypes.data, n))
+        return out.reshape(shape) if shape else out
 
     def design_gradient(self) -> np.ndarray:
         """d|F|^2 / d eps_r of every design cell, shape ``design_shape``

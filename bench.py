@@ -127,7 +127,7 @@ def population_bench(steps: int, count: int = 64) -> dict:
 
 
 def adjoint_bench(steps: int) -> dict:
-    """The differentiable workload (decks.adjoint_deck): 160^3, 10-cell CPML,
+    """The differentiable workload (decks.adjoint_deck): 160^3, 12-cell CPML,
     x-polarised TF/SF differentiated Gaussian, a 40 x 40 x 20 design box of
     eps_r U[1, 4] (seed 5), objective |r E_theta(30 deg, 0, 10 GHz)|^2 by
     the frequency-domain transform (the time-domain far-field series are off
@@ -138,7 +138,7 @@ def adjoint_bench(steps: int) -> dict:
     from alpineem_amd.solver import Plan, Simulation
 
     d, lo, hi = decks.adjoint_deck(steps=steps)
-    plan = Plan.from_text(d.text())
+    plan = Plan.from_text(d.text(), cpml_cells=12)
     shape = tuple(hi[a] - lo[a] + 1 for a in (2, 1, 0))
     eps = decks.design_permittivity(shape, seed=5)
     plain = Simulation(plan, ntff="off")
@@ -164,6 +164,101 @@ def adjoint_bench(steps: int) -> dict:
             "gradient_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
             "forward_gcells": round(n * steps / t_plain / 1e9, 2), "J": J,
             "grad_norm": float(np.linalg.norm(g))}
+
+
+def configs_bench() -> dict:
+    """The BASELINE configurations with every output of each, timed end to
+    end on the device (wall clock around the stepping; setup excluded):
+
+    C1  100^3 radiating dipole (arms 20 cells, 1-cell gap), 2000 steps,
+        10-cell CPML with kappa_max 5 / alpha_max 0.05, 50 ohm port; port
+        V/I series, S11 at 101 frequencies over 0.5-8 GHz (device DFT) and
+        the far field at 37 theta angles (phi = 0) with the reference's
+        ordered FP32 binning (its outputs bit for bit).
+    C2  200^3 TF/SF PEC sphere of radius 40 cells, dx 2 mm, 4000 steps,
+        12-cell CPML, TF/SF box 15 cells inside the CPML, far-field box 5
+        cells outside it; monostatic far field (ordered binning) and the
+        running FP64/Kahan DFT far field at 19 bistatic angles x 201
+        frequencies.
+    C3  32 x 32 x 1200 periodic layered slab, five lossy anisotropic layers,
+        a 6-pole Drude slab and a 100 ohm/sq sheet, 16-cell CPML in z, 12000
+        steps; reflection/transmission plane sums and their spectra at 401
+        frequencies.
+    R5  the 400^3 throughput deck with the survey's performance CPML
+        (kappa_max 5, alpha_max 0.05), 200 steps."""
+    from alpineem_amd import decks
+    from alpineem_amd.solver import Plan, Simulation
+
+    out = {}
+
+    def timed(name, plan, sim, prep=None, post=None):
+        t0 = time.perf_counter()
+        sim.run()
+        step_s = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        if post:
+            post(sim)
+        post_s = time.perf_counter() - t1
+        v = plan.view
+        cells = v.nx * v.ny * v.nz
+        out[name] = {"cells": cells, "steps": plan.time_steps, "step_s": round(step_s, 4),
+                     "gcells": round(cells * plan.time_steps / step_s / 1e9, 3), "outputs_s": round(post_s, 4)}
+        print(f"[bench] {name}: {v.nx}x{v.ny}x{v.nz}, {plan.time_steps} steps, {out[name]['gcells']} Gcell/s "
+              f"(stepping {step_s:.3f} s, outputs {post_s:.3f} s)", file=sys.stderr, flush=True)
+        sim.close()
+
+    # C1
+    plan = Plan.from_text(decks.dipole(n=100, steps=2000, angles=37, arm=20).text(), cpml_kappa_max=5.0,
+                          cpml_alpha_max=0.05)
+    sim = Simulation(plan)
+
+    def c1_post(s):
+        s.finalize()
+        f = np.linspace(0.5e9, 8e9, 101)
+        _ = (s.dft("port_v", f) - s.dft("incident", f)) / s.dft("incident", f)
+        for a in range(plan.view.n_angles):
+            s.series("e_theta", a)
+    timed("C1", plan, sim, post=c1_post)
+    out["C1"]["outputs"] = "port V/I, S11 x101, far field 37 angles (ordered FP32 binning)"
+
+    # C2
+    plan = Plan.from_text(decks.pec_sphere(n=200, steps=4000, radius=40, angles=[(180.0, 0.0)]).text(),
+                          cpml_cells=12, tfsf_inset=15, ntff_gap=5)
+    sim = Simulation(plan)
+    sim.set_farfield_dft([(float(t), 0.0) for t in np.linspace(0.0, 180.0, 19)], 0.5e9, 0.05e9, 201)
+
+    def c2_post(s):
+        s.finalize()
+        s.series("e_theta", 0)
+        s.farfield_dft()
+    timed("C2", plan, sim, post=c2_post)
+    out["C2"]["outputs"] = "monostatic far field (ordered FP32 binning), running FP64 DFT far field 19 angles x 201"
+
+    # C3
+    plan = Plan.from_text(decks.layered_slab().text(), cpml_cells=16)
+    sim = Simulation(plan)
+
+    def c3_post(s):
+        s.finalize()
+        f = np.linspace(1e9, 100e9, 401)
+        s.dft("reflected", f)
+        s.dft("transmitted", f)
+    timed("C3", plan, sim, post=c3_post)
+    out["C3"]["outputs"] = "reflected/transmitted plane sums, spectra x401"
+
+    # performance CPML at 400^3
+    plan = Plan.from_text(decks.roofline_deck(400, 220).text(), cpml_kappa_max=5.0, cpml_alpha_max=0.05)
+    sim = Simulation(plan)
+    sim.run(20)
+    t0 = time.perf_counter()
+    sim.run(200)
+    el = time.perf_counter() - t0
+    out["R400_kappa5"] = {"cells": 400 ** 3, "steps": 200, "gcells": round(400 ** 3 * 200 / el / 1e9, 3),
+                          "single_pass": sim.single_pass}
+    print(f"[bench] 400^3 with kappa_max 5, alpha_max 0.05: {out['R400_kappa5']['gcells']} Gcell/s", file=sys.stderr,
+          flush=True)
+    sim.close()
+    return out
 
 
 def _short_name(kernel: str) -> str:
@@ -222,7 +317,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", action="store_true", help="also run a known-size device copy (counter scaling)")
     ap.add_argument("--no-single-pass", action="store_true",
                     help="separate H and E bulk launches instead of the single-pass core launch")
-    ap.add_argument("--population-steps", type=int, default=1000,
+    ap.add_argument("--configs", type=int, default=1, help="1: also time the other configurations with full outputs")
+    ap.add_argument("--population-steps", type=int, default=3000,
                     help="steps of the 64-member batched-population comparison (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the 160^3 design-gradient timing (0: skip)")
@@ -293,6 +389,7 @@ def main() -> int:
         print(f"[bench] adjoint: {adj['workload']}: forward {adj['forward_s']} s, recorded forward "
               f"{adj['recorded_forward_s']} s, adjoint {adj['adjoint_s']} s, gradient/forward "
               f"{adj['gradient_over_forward']}", file=sys.stderr, flush=True)
+    cfg = configs_bench() if args.configs else None
     cpu = None if args.no_cpu else cpu_baseline(n, args.cpu_steps)
     line = {
         "metric": "Gcell-updates/s at 400^3 FP32 (CPML, lumped port)",
@@ -314,6 +411,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "population": pop,
         "adjoint": adj,
+        "configs": cfg,
     }
     print(json.dumps(line), flush=True)
     return 0

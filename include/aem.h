@@ -55,7 +55,10 @@ typedef enum aem_series {
     AEM_SERIES_E_PHI = 4,         /* E_phi_out(angle,:) */
     AEM_SERIES_REFLECTED = 5,     /* E_reflected(:) normalised (f90:8342-8361) -- after aem_finalize */
     AEM_SERIES_TRANSMITTED = 6,   /* E_transmitted(:) */
-    AEM_SERIES_FF_PC = 7          /* incident at the far-field phase centre (f90:8636-8642) */
+    AEM_SERIES_FF_PC = 7,         /* incident at the far-field phase centre (f90:8636-8642) */
+    AEM_SERIES_SPICE_V = 8,       /* Spice_Voltage_out(idx,:) (f90:8250-8254): node voltage minus bias per
+                                     SPICE port, then the raw voltage of each additional node */
+    AEM_SERIES_SPICE_I = 9        /* the gap current handed to the circuit per SPICE port (f90:5456-5496) */
 } aem_series;
 
 typedef enum aem_field {
@@ -81,6 +84,23 @@ aem_status aem_plan_from_inputs(const char* inputs_path, aem_plan** out);
 
 /* Same, from deck text held in memory; base_dir resolves relative files. */
 aem_status aem_plan_from_text(const char* deck_text, const char* base_dir, aem_plan** out);
+
+/* Fill opt with the reference's setup constants (aem_plan_options in
+ * aem_plan.h: CPML thickness, grading, kappa/alpha maxima and reflection
+ * target, TF/SF inset, far-field box gap). */
+void aem_plan_options_init(aem_plan_options* opt);
+
+/* aem_plan_from_text / aem_plan_from_inputs with setup constants other than
+ * the reference's (NULL = the reference's).  No counterpart in the
+ * reference, where these are source constants (f90:34-39, 1380-1397,
+ * 1638-1643, 1686-1694).  Values outside the accepted range (cpml_cells
+ * 2..64, cpml_grading 1..8, kappa_max >= 1, alpha_max >= 0, 0 < reflection
+ * < 1, tfsf_inset 3..64, ntff_gap 2..tfsf_inset-1, which keeps the far-field
+ * box between the CPML and the TF/SF box) return AEM_ERR_INPUT; so does a grid
+ * too small to hold the boxes. */
+aem_status aem_plan_from_text_ex(const char* deck_text, const char* base_dir, const aem_plan_options* opt,
+                                 aem_plan** out);
+aem_status aem_plan_from_inputs_ex(const char* inputs_path, const aem_plan_options* opt, aem_plan** out);
 
 /* Read-only view of the plan (valid until aem_plan_destroy). */
 const aem_plan_view* aem_plan_view_get(const aem_plan* plan);
@@ -125,6 +145,9 @@ typedef struct aem_sim_options {
     int32_t planes;         /* z planes marched per thread in the bulk kernels; 0 = default (4) */
     int32_t no_cpml_fusion; /* 1: the separate CPML kernels after ports/TF/SF and one TF/SF launch per face pair */
     int32_t no_single_pass; /* 1: separate H and E bulk launches even where one single-pass launch qualifies */
+    int32_t ntff_lists;     /* 1: ordered binning from contributor lists built at creation instead of the
+                               per-bin scan of staged face rows (same results; faster where no bin collects
+                               thousands of cells) */
 } aem_sim_options;
 
 /* aem_sim_create with options (NULL = defaults). */
@@ -153,6 +176,24 @@ int64_t aem_steps_done(const aem_sim* sim);
 aem_status aem_sync(aem_sim* sim);
 
 /* ---- outputs ---------------------------------------------------------------- */
+
+/* SPICE co-simulation (the reference's use_spice_version build, whose
+ * circuit is ngspice driven from the solver: fdtd_solver.f90:5435-5665,
+ * 6285-6520, main_fdtd/circuit.F90:140-174).  A deck with SPICE ports is
+ * stepped by this call instead of aem_run, one step per call, and the host
+ * runs the circuit between calls:
+ *   - a call first finishes the step that waits at the exchange: the node
+ *     voltages v_in (n_spice + n_spice_extra floats, ports then extra nodes,
+ *     as the circuit reports them) set each gap field to (V - bias) / gap
+ *     after the lumped ports' E phase, and the rest of the step runs;
+ *   - then, when i_out is not NULL, it runs the next step's H phase and
+ *     returns the gap current of each SPICE port in i_out (n_spice floats),
+ *     and that step waits at the exchange.
+ * The first call passes v_in = NULL (no step waits); the last passes
+ * i_out = NULL.  Host staging is pinned memory owned by the sim.  The plan
+ * view lists the ports (aem_spice_view: location, bias, the grid capacitance
+ * the reference assigns to the port's capacitor) and the circuit names. */
+aem_status aem_port_exchange(aem_sim* sim, const float* v_in, float* i_out);
 
 /* Post-loop work (fdtd_solver.f90:8340-8534): plane-sum normalisation,
  * incident wave (analytic or the clear 1-D line), NTFF combine and output
@@ -256,6 +297,18 @@ aem_status aem_set_design_eps(aem_sim* sim, const float* eps, int64_t n);
 /* After the full run: dJ/d eps_r of every cell of the design box (x fastest,
  * n = the box's cell count), J = |F|^2 of the objective. */
 aem_status aem_adjoint_design_gradient(aem_sim* sim, double* grad, int64_t n);
+
+/* Gradient of a general real objective J(F) of the far-field objective F
+ * (aem_objective_set), given dJ_dF = {dJ/dRe F, dJ/dIm F} evaluated at the
+ * run's F (aem_objective_get): with a design box, dJ/d eps_r of each of its
+ * cells (n = the box's cell count, x fastest); otherwise dJ/d eps_r of the
+ * deck's materials 1..n (n at most their count) changed in the recorded box.
+ * J = |F|^2 is dJ_dF = 2F.  Replaces the gradient step of the reference's
+ * adjoint workflow (optimization_scripts/adjoint_optimization/
+ * adjoint_optimization.py:76-134), whose objective is a function of a
+ * frequency-domain quantity; the caller's J (a goal mismatch, a ratio)
+ * enters only through dJ_dF. */
+aem_status aem_adjoint(aem_sim* sim, const double* dJ_dF, double* grad, int64_t n);
 
 /* Low-level adjoint access: apply the transposed step (no source) n_steps
  * times to the adjoint state, and read/write its six field components

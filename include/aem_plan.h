@@ -50,6 +50,18 @@ typedef struct aem_port_view {
     int32_t excited;      /* 1 if this is excitation_port_number */
 } aem_port_view;
 
+/* One basic SPICE port: a lumped gap whose voltage a host circuit sets
+ * every step from the gap current (fdtd_solver.f90:1195-1286 input,
+ * 5435-5665 current and exchange, 6285-6520 gap field). */
+typedef struct aem_spice_view {
+    int32_t dir;                  /* gap field component 0..2 */
+    int32_t loc[3], size[3];      /* first cell; cells per axis (+1 on the transverse axes) */
+    float   v_bias;               /* DC bias subtracted from the node voltage */
+    float   cap;                  /* grid capacitance the reference assigns to the port's capacitor */
+    float   cur_num, cur_den;     /* current scale (I*cur_num)/cur_den */
+    float   gap_div;              /* gap E = V / gap_div */
+} aem_spice_view;
+
 typedef struct aem_plan_view {
     /* ---- grid ---------------------------------------------------------- */
     int32_t nx, ny, nz;       /* user cell counts as read from the deck */
@@ -155,7 +167,48 @@ typedef struct aem_plan_view {
      * (dl of the excited port; zeros when the source is absent). */
     const float *src_table;
     float   exp_half;             /* expf(0.5) as the host evaluates it (pulse type 2) */
+
+    /* ---- SPICE co-simulation ports (deck section f90:1195-1286) ---------- */
+    int32_t n_spice;              /* basic SPICE ports */
+    int32_t n_spice_extra;        /* additional circuit nodes reported beside them */
+    const aem_spice_view *spice;
+    const char *spice_names;      /* one name per line: the netlist file, then per port its
+                                     node, current source and capacitor, then the extra nodes */
+
+    /* ---- kmax build (plan option kmax; fdtd_solver.f90 under use_kmax_version):
+     * complex fields with Bloch-periodic walls, a modal plane source in place
+     * of TF/SF, TE/TM plane sums in place of the periodic-slab sums -------- */
+    int32_t kmax;                 /* 1: the kmax deck grammar and setup */
+    int32_t mode_type;            /* 0 TE (source on the H plane), 1 TM (source on the E plane) */
+    int32_t k_direction;          /* 1 '+', 0 '-' */
+    int32_t k_axis;               /* the non-periodic axis (source and sum planes), -1 without two PBC pairs */
+    int32_t kpl_H, kpl_E;         /* source planes k_pl_start_H / k_pl_start_E (f90:1727-1800) */
+    float   k_vec[3];             /* Bloch wavenumbers of the periodic axes, rad/m */
+    float   k_exc[3];             /* k_num_{x,y,z}_exception */
+    float   f_adj;                /* carrier of the kmax sources (f90:1722) */
+    float   bloch_c[3], bloch_s[3]; /* cos, sin of k_a (N_a - 2) d_a: an E copy to the first cell multiplies by
+                                     (c, s), an H copy to the last cell by (c, -s) (f90:5387-5428, 6685-6736) */
+    const float *src_table_im;    /* imaginary parts of the complex src_table waveforms */
+    const float *ksrc;            /* [4][time_steps]: the modal source's Gaussian factor and carrier phase
+                                     for the H plane (time n-1), then for the E plane (time n-1/2) */
 } aem_plan_view;
+
+/* Setup constants that the reference fixes in its source rather than reading
+ * from the deck.  aem_plan_options_init fills the reference's values; a plan
+ * built with them (or with no options) is the reference's bit for bit.  A
+ * caller changes fields after the init call. */
+typedef struct aem_plan_options {
+    int32_t cpml_cells;       /* CPML thickness in cells on each non-periodic face
+                                 (the reference's 11 surfaces = 10 cells, f90:34) */
+    int32_t cpml_grading;     /* polynomial order m of the sigma and kappa profiles (3, f90:35) */
+    float   cpml_kappa_max;   /* kappa at the outer surface (7.0, f90:1690) */
+    float   cpml_alpha_max;   /* CFS alpha at the inner surface, graded linearly (0.03, f90:1687) */
+    float   cpml_reflection;  /* R of sigma_max = -ln(R) (m+1) / (2 eta0 d n) (1e-6, f90:1686) */
+    int32_t tfsf_inset;       /* cells between the CPML and the TF/SF box ("buffer" = 3, f90:39, 1380-1397) */
+    int32_t ntff_gap;         /* far-field box faces this many cells below the TF/SF box's low
+                                 faces and one fewer above its high faces (2, f90:1638-1643) */
+    int32_t kmax;             /* 1: the deck is for the reference's kmax build (0) */
+} aem_plan_options;
 
 #ifdef __cplusplus
 }

@@ -5,9 +5,10 @@
  * of the product.
  *
  * Every region below restates one loop region of
- * main_fdtd/fdtd_solver.f90 (cited per function) for the non-kmax,
- * non-SPICE build, in the reference's sequential loop order and FP32
- * left-to-right operation order.  Build with -ffp-contract=off.
+ * main_fdtd/fdtd_solver.f90 (cited per function) for the non-kmax build,
+ * with the SPICE build's gap exchange (the circuit itself is the caller's),
+ * in the reference's sequential loop order and FP32 left-to-right operation
+ * order.  Build with -ffp-contract=off.
  *
  * Deliberate restatement choices (documented, value-preserving):
  *  - Fields use the common layout of include/aem_plan.h instead of six
@@ -59,6 +60,10 @@ struct oracle_state {
     /* ports */
     real *V, *I;               /* [port][M] and [port][M-1] (0-based j-1) */
     real *Vout, *Iout;         /* [port][T] */
+    /* SPICE ports: this step's gap currents, the node voltages less bias
+     * (ports, then extra nodes as given), and their series */
+    real *spI, *spV, *spIout, *spVout;
+    int x_pending;             /* a step waits at the exchange */
     /* NTFF */
     real *Mcur[NFACE][2], *Mold[NFACE][2], *Jcur[NFACE][2], *Jold[NFACE][2];
     int64_t fdim[NFACE][2];
@@ -158,6 +163,14 @@ oracle_state* oracle_create(const aem_plan_view* plan) {
     s->I = zalloc((int64_t)np * p->M);
     s->Vout = zalloc((int64_t)np * p->time_steps);
     s->Iout = zalloc((int64_t)np * p->time_steps);
+    {
+        int64_t ns = p->n_spice > 0 ? p->n_spice : 1, nv = (int64_t)p->n_spice + p->n_spice_extra;
+        if (nv < 1) nv = 1;
+        s->spI = zalloc(ns);
+        s->spV = zalloc(nv);
+        s->spIout = zalloc(ns * p->time_steps);
+        s->spVout = zalloc(nv * p->time_steps);
+    }
     if (p->n_angles > 0) {
         for (int f = 0; f < NFACE; ++f) {
             int a = f / 2;
@@ -197,6 +210,7 @@ void oracle_destroy(oracle_state* s) {
         }
     for (int a = 0; a < 3; ++a) { free(s->Es[a]); free(s->J[a]); free(s->Eold[a]); free(s->U[a]); free(s->W[a]); }
     free(s->V); free(s->I); free(s->Vout); free(s->Iout);
+    free(s->spI); free(s->spV); free(s->spIout); free(s->spVout);
     free(s->refl); free(s->trans);
     for (int q = 0; q < 6; ++q) { free(s->pc[q]); free(s->vid[q]); }
     free(s->incident); free(s->Eth); free(s->Eph); free(s->fftime);
@@ -387,6 +401,36 @@ static void h_ports(oracle_state* s, int n) {
             const aem_port_view* pt = &p->ports[q];
             I[1] = I[1] - R(pt->kI) * e_or_v(s, R(p->antenna_amp), R(n) + R(0.0), R(0.0) * R(pt->dl));
         }
+}
+
+/* SPICE gap currents after the H phase, f90:5442-5498 (basic ports): the
+ * loop integral of H around each gap, scaled as a lumped port's. */
+static void h_spice(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    for (int q = 0; q < p->n_spice; ++q) {
+        const aem_spice_view* pt = &p->spice[q];
+        int l1 = pt->loc[0], l2 = pt->loc[1], l3 = pt->loc[2];
+        int d1 = pt->size[0] + l1 - 1, d2 = pt->size[1] + l2 - 1, d3 = pt->size[2] + l3 - 1;
+        real c = R(0.0);
+        if (pt->dir == 0) {
+            for (int ii = l1; ii <= d1; ++ii) {
+                for (int jj = l2; jj <= d2; ++jj) c = c + R(p->dy) * (HY(ii, jj, l3 - 1) - HY(ii, jj, d3));
+                for (int kk = l3; kk <= d3; ++kk) c = c + R(p->dz) * (HZ(ii, l2, kk) - HZ(ii, d2 - 1, kk));
+            }
+        } else if (pt->dir == 1) {
+            for (int jj = l2; jj <= d2; ++jj) {
+                for (int ii = l1; ii <= d1; ++ii) c = c + R(p->dx) * (HX(ii, jj, l3) - HX(ii, jj, d3 - 1));
+                for (int kk = l3; kk <= d3; ++kk) c = c + R(p->dz) * (HZ(l1 - 1, jj, kk) - HZ(d1, jj, kk));
+            }
+        } else {
+            for (int kk = l3; kk <= d3; ++kk) {
+                for (int ii = l1; ii <= d1; ++ii) c = c + R(p->dx) * (HX(ii, l2 - 1, kk) - HX(ii, d2, kk));
+                for (int jj = l2; jj <= d2; ++jj) c = c + R(p->dy) * (HY(l1, jj, kk) - HY(d1 - 1, jj, kk));
+            }
+        }
+        s->spI[q] = c * R(pt->cur_num) / R(pt->cur_den);
+        s->spIout[(int64_t)q * p->time_steps + (n - 1)] = s->spI[q];
+    }
 }
 
 /* CPML H, f90:5227-5375 */
@@ -675,6 +719,25 @@ static void e_ports(oracle_state* s, int n) {
             for (int jj = l2; jj <= d2; ++jj)
                 for (int kk = l3; kk <= d3; ++kk) F[IDX(s, ii, jj, kk)] = val;
     }
+}
+
+/* SPICE gap fields, f90:6285-6330 (basic ports): E = 1.0*V/(n*delta) on the
+ * gap cells, with V = node voltage - DC bias (f90:5650-5656); the series
+ * record V, and the extra nodes' raw voltages (f90:8250-8254). */
+static void e_spice(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    for (int q = 0; q < p->n_spice; ++q) {
+        const aem_spice_view* pt = &p->spice[q];
+        int l1 = pt->loc[0], l2 = pt->loc[1], l3 = pt->loc[2];
+        int d1 = pt->size[0] + l1 - 1, d2 = pt->size[1] + l2 - 1, d3 = pt->size[2] + l3 - 1;
+        real* F = pt->dir == 0 ? s->Ex : (pt->dir == 1 ? s->Ey : s->Ez);
+        real val = R(1.0) * s->spV[q] / R(pt->gap_div);
+        for (int ii = l1; ii <= d1; ++ii)
+            for (int jj = l2; jj <= d2; ++jj)
+                for (int kk = l3; kk <= d3; ++kk) F[IDX(s, ii, jj, kk)] = val;
+    }
+    for (int q = 0; q < p->n_spice + p->n_spice_extra; ++q)
+        s->spVout[(int64_t)q * p->time_steps + (n - 1)] = s->spV[q];
 }
 
 /* CPML E, f90:6522-6670 */
@@ -1057,33 +1120,68 @@ static void out_phase_centre(oracle_state* s, int n) {
     }
 }
 
+/* The step splits at the SPICE exchange (f90:5435-5665): the H phase and
+ * the gap currents, then the E phase with the circuit's gap voltages and
+ * the per-step outputs. */
+static void step_to_exchange(oracle_state* s, int n) {
+    h_bulk(s);
+    h_sheets(s);
+    h_tfsf(s, n);
+    h_ports(s, n);
+    h_cpml(s);
+    h_pbc(s);
+    h_spice(s, n);
+}
+
+static void step_from_exchange(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    e_bulk(s);
+    e_sheets(s);
+    e_drude(s);
+    e_tfsf(s, n);
+    e_ports(s, n);
+    e_spice(s, n);
+    e_cpml(s);
+    e_pbc(s);
+    e_oldt(s);
+    out_video(s, n);
+    out_plane_sums(s, n);
+    if (p->n_angles > 0) {
+        out_ntff_currents(s);
+        out_ntff_bins(s, n);
+    }
+    for (int pi = 0; pi < p->n_ports; ++pi) s->Vout[(int64_t)pi * p->time_steps + (n - 1)] = s->V[(int64_t)pi * p->M + 1];
+    out_phase_centre(s, n);
+}
+
 int oracle_step(oracle_state* s, int nsteps) {
     const aem_plan_view* p = &s->p;
+    if (p->n_spice > 0) return -1;   /* stepped by oracle_port_exchange */
     for (int q = 0; q < nsteps; ++q) {
         if (s->n >= p->time_steps) return -1;
         const int n = ++s->n;
-        h_bulk(s);
-        h_sheets(s);
-        h_tfsf(s, n);
-        h_ports(s, n);
-        h_cpml(s);
-        h_pbc(s);
-        e_bulk(s);
-        e_sheets(s);
-        e_drude(s);
-        e_tfsf(s, n);
-        e_ports(s, n);
-        e_cpml(s);
-        e_pbc(s);
-        e_oldt(s);
-        out_video(s, n);
-        out_plane_sums(s, n);
-        if (p->n_angles > 0) {
-            out_ntff_currents(s);
-            out_ntff_bins(s, n);
-        }
-        for (int pi = 0; pi < p->n_ports; ++pi) s->Vout[(int64_t)pi * p->time_steps + (n - 1)] = s->V[(int64_t)pi * p->M + 1];
-        out_phase_centre(s, n);
+        step_to_exchange(s, n);
+        step_from_exchange(s, n);
+    }
+    return 0;
+}
+
+int oracle_port_exchange(oracle_state* s, const float* v_in, float* i_out) {
+    const aem_plan_view* p = &s->p;
+    if (p->n_spice == 0) return -1;
+    if (s->x_pending) {
+        if (!v_in) return -1;
+        for (int q = 0; q < p->n_spice; ++q) s->spV[q] = R(v_in[q]) - R(p->spice[q].v_bias);
+        for (int q = 0; q < p->n_spice_extra; ++q) s->spV[p->n_spice + q] = R(v_in[p->n_spice + q]);
+        step_from_exchange(s, s->n);
+        s->x_pending = 0;
+    }
+    if (i_out) {
+        if (s->n >= p->time_steps) return -1;
+        const int n = ++s->n;
+        step_to_exchange(s, n);
+        for (int q = 0; q < p->n_spice; ++q) i_out[q] = (float)s->spI[q];
+        s->x_pending = 1;
     }
     return 0;
 }
@@ -1203,6 +1301,8 @@ int oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int6
         case 8: case 9: case 10: case 11: case 12: case 13:
             for (int64_t q = 0; q < n; ++q) out[q] = (float)s->pc[kind - 8][q];
             return 0;
+        case 20: if (idx < 0 || idx >= p->n_spice + p->n_spice_extra) return -1; src = s->spVout + (int64_t)idx * T; break;
+        case 21: if (idx < 0 || idx >= p->n_spice) return -1; src = s->spIout + (int64_t)idx * T; break;
         default: return -1;
     }
     for (int64_t q = 0; q < n; ++q) out[q] = (float)src[q];

@@ -29,6 +29,9 @@ void          oracle_destroy(oracle_state* s);
 /* advance n steps; returns 0 or -1 when the deck's time_steps would be exceeded */
 int           oracle_step(oracle_state* s, int n);
 int           oracle_steps_done(const oracle_state* s);
+/* SPICE decks: one step per call, split at the gap exchange as
+ * aem_port_exchange (include/aem.h) splits it; returns -1 on misuse */
+int           oracle_port_exchange(oracle_state* s, const float* v_in, float* i_out);
 /* post-loop products (f90:8340-8534); call once after the last step */
 int           oracle_finalize(oracle_state* s);
 /* copy a field component (ncell entries of the common layout) as float */
@@ -37,7 +40,8 @@ int           oracle_get_field(const oracle_state* s, int comp, float* out, int6
 int           oracle_get_field64(const oracle_state* s, int comp, double* out, int64_t n);
 /* overwrite a field component (ncell entries) -- for initial-value tests */
 int           oracle_set_field(oracle_state* s, int comp, const float* in, int64_t n);
-/* copy a series (same kinds as aem_series in include/aem.h) as float */
+/* copy a series (kinds 0..7 as aem_series in include/aem.h; 20/21 = SPICE
+ * voltages/currents, which are aem_series 8/9) as float */
 int           oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int64_t n);
 /* raw NTFF accumulators as double: which 0..5 = Ux,Uy,Uz,Wx,Wy,Wz */
 int           oracle_get_ntff_raw(const oracle_state* s, int which, double* out, int64_t n);

