@@ -53,6 +53,7 @@ SIGNATURES = [
     ("aem_get_ntff_raw", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_get_ff_time", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_get_dft", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+    ("aem_get_dft_noalloc", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
     ("aem_objective_set", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_double, C.c_int32]),
     ("aem_objective_get", C.c_int, [C.c_void_p, C.c_void_p]),
     ("aem_ffdft_set", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int32]),

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1383,6 +1384,35 @@ void dft_into(aem_sim* s, const float* src, int n, const double* freqs_hz, int n
     HIPCHK(hipStreamSynchronize(s->st));
 }
 
+// Static scratch of aem_get_dft_noalloc, for callers that cannot allocate
+// device memory: every device the library runs on holds one copy
+// (kDftScratchFreqs frequencies x 3 doubles = 384 KiB) until the process
+// ends.  In host code the array's name is not a device address; each
+// device's address comes from hipGetSymbolAddress, looked up once with that
+// device current.  All calls on one device share its copy, so concurrent
+// calls on one device race.
+constexpr int kDftScratchFreqs = 16384;
+static __device__ __align__(256) double dft_scratch[kDftScratchFreqs * 3];
+
+double* dft_scratch_ptr(int dev) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* ptr[kMaxDevices] = {};
+    if (dev < 0 || dev >= kMaxDevices) throw aem::Error(AEM_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(dft_scratch));
+        if (cur != dev) HIPCHK(hipSetDevice(cur));
+        HIPCHK(e);
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
 const float* series_src(const aem_sim* s, int32_t kind, int32_t idx) {
     const aem_plan_view& v = s->plan->v;
     const int T = v.time_steps;
@@ -1459,6 +1489,24 @@ aem_status aem_get_dft(aem_sim* s, int32_t kind, int32_t idx, const double* freq
         }
         dft_into(s, src, n, freqs_hz, n_freq, s->d_dft, re_im);
     });
+}
+
+aem_status aem_get_dft_noalloc(aem_sim* s, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
+                               double* re_im) {
+    if (!s || !freqs_hz || !re_im || n_freq < 0) {
+        aem::set_error("aem_get_dft_noalloc: bad argument");
+        return AEM_ERR_STATE;
+    }
+    if (n_freq > kDftScratchFreqs) {
+        aem::set_error("aem_get_dft_noalloc: " + std::to_string(n_freq) + " frequencies exceed the static scratch of " +
+                       std::to_string(kDftScratchFreqs) + "; use aem_get_dft");
+        return AEM_ERR_STATE;
+    }
+    const float* src = series_src(s, kind, idx);
+    if (!src) return AEM_ERR_STATE;
+    if (n_freq == 0) return AEM_OK;
+    const int n = per_step_series(kind) ? (int)s->steps_done : s->plan->v.time_steps;
+    return aem::guard([&]() { dft_into(s, src, n, freqs_hz, n_freq, dft_scratch_ptr(s->dev), re_im); });
 }
 
 aem_status aem_get_field(aem_sim* s, int32_t comp, float* out, int64_t n) {

@@ -237,6 +237,17 @@ aem_status aem_get_ff_time(aem_sim* sim, float* out, int64_t n);
 aem_status aem_get_dft(aem_sim* sim, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
                        double* re_im);
 
+/* aem_get_dft without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as aem_get_dft, bit for bit; at most 16384
+ * frequencies (more return AEM_ERR_STATE).  The staging lives in a static
+ * device array of 384 KiB, one per device, held until the process ends and
+ * shared by every call on that device: calls that overlap on one device
+ * race. */
+aem_status aem_get_dft_noalloc(aem_sim* sim, int32_t kind, int32_t idx, const double* freqs_hz, int32_t n_freq,
+                               double* re_im);
+
 /* ---- differentiation --------------------------------------------------------
  * Replaces the reference's adjoint workflow
  * (optimization_scripts/adjoint_optimization/adjoint_optimization.py:55-134,
