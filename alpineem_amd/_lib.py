@@ -17,8 +17,9 @@ STATUS_NAMES = {0: "AEM_OK", 1: "AEM_ERR_INPUT", 2: "AEM_ERR_IO", 3: "AEM_ERR_DE
                 5: "AEM_ERR_UNSUPPORTED", 6: "AEM_ERR_NUMERIC"}
 
 SERIES = {"port_v": 0, "port_i": 1, "incident": 2, "e_theta": 3, "e_phi": 4, "reflected": 5,
-          "transmitted": 6, "ff_pc": 7, "spice_v": 8, "spice_i": 9}
+          "transmitted": 6, "ff_pc": 7, "spice_v": 8, "spice_i": 9, "reflected_tm": 10, "transmitted_tm": 11}
 FIELDS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
+IMAG = 0x100    # AEM_IMAG: the imaginary part of a kmax simulation's series or field
 
 # (name, restype, argtypes) for every entry point of include/aem.h
 SIGNATURES = [

@@ -6,7 +6,9 @@ runs every time step on the GPU in the deck's directory and writes
 video and full-field files there, printing the reference's progress and
 timing lines (fdtd_solver.f90:724, 4366-4368, 8313-8316).  Deck errors print
 the reference's ``ERROR: ...`` message and exit with status 1, as does a run
-stopped by the non-finite field check.
+stopped by the non-finite field check.  ``--kmax`` reads the deck as the
+reference's kmax executable does (its -Duse_kmax_version build) and writes
+that build's data.dat.
 """
 from __future__ import annotations
 
@@ -19,16 +21,18 @@ from .solver import Plan, Simulation
 
 
 def main(argv=None) -> int:
-    argv = sys.argv[1:] if argv is None else argv
-    if not argv:
-        print("usage: python -m alpineem_amd.cli <inputs.txt>", file=sys.stderr)
+    argv = list(sys.argv[1:] if argv is None else argv)
+    kmax = "--kmax" in argv
+    argv = [a for a in argv if a != "--kmax"]
+    if len(argv) != 1:
+        print("usage: python -m alpineem_amd.cli [--kmax] <inputs.txt>", file=sys.stderr)
         return 2
     deck = os.path.abspath(argv[0])
     cwd = os.getcwd()
     # the reference runs in the deck's directory and writes its files there
     os.chdir(os.path.dirname(deck))
     try:
-        return _run(deck)
+        return _run(deck, kmax)
     except AemError as e:
         print(str(e).split(": ", 1)[-1])
         return 1
@@ -36,10 +40,10 @@ def main(argv=None) -> int:
         os.chdir(cwd)
 
 
-def _run(deck: str) -> int:
+def _run(deck: str, kmax: bool = False) -> int:
     t0 = time.time()
     print("Reading in the inputs file and setting up the simulation...")
-    plan = Plan.from_inputs(deck)
+    plan = Plan.from_inputs(deck, kmax=1) if kmax else Plan.from_inputs(deck)
     sim = Simulation(plan)
     print(f"Setup is finished. Time to complete was {time.time() - t0:.3f} seconds.")
     print("Starting main FDTD algorithm...")

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace aemk {
 
 struct Grid {
@@ -30,6 +32,34 @@ struct Grid {
 
 __device__ __forceinline__ long long off(const Grid& g, int i, int j, int k) {
     return (long long)(i - 1) + (long long)g.PX * ((long long)(j - 1) + (long long)g.NY * (long long)(k - 1));
+}
+
+// Division of a field quantity by a real constant d.  The ordinary build
+// divides reals: x / d.  The kmax build's fields are complex, and Fortran
+// divides a complex by a real as complex by complex; its flang build does
+// that with the C99 routine __divsc3, which for a divisor (d, 0) computes,
+// with e = logb(d) and d' = d 2^-e, each part as ((x d') / (d' d')) 2^-e.
+// That is not x / d to the last bit, so a kmax simulation (cplx = 1) uses it.
+struct RDiv {
+    float d;       // the divisor
+    float ds;      // d 2^-e
+    float dd;      // ds * ds, rounded
+    int e;         // logb(d)
+    int cplx;
+};
+
+inline RDiv make_rdiv(float d, bool cplx) {
+    RDiv r{d, 0.0f, 0.0f, 0, cplx ? 1 : 0};
+    if (cplx) {
+        r.e = ilogbf(d);
+        r.ds = scalbnf(d, -r.e);
+        r.dd = r.ds * r.ds;
+    }
+    return r;
+}
+
+__device__ __forceinline__ float rdiv(float x, const RDiv& r) {
+    return r.cplx ? ldexpf((x * r.ds) / r.dd, -r.e) : x / r.d;
 }
 
 // ---------------------------------------------------------------------------
@@ -761,7 +791,7 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
 
 // H, slab normal to x.  Threads: (t, j, k); t enumerates low then high layers.
 __global__ void k_h_cpml_x(Grid g, Cpml p, float* Hy, float* Hz, const float* Ey, const float* Ez, float DB,
-                           float dx) {
+                           RDiv dx) {
     const int nl = max(p.n[0] - 1, 0), nh = max(p.n[1] - 1, 0), nt = nl + nh;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)nt * (g.NY - 1) * (g.NZ - 1);
@@ -777,16 +807,16 @@ __global__ void k_h_cpml_x(Grid g, Cpml p, float* Hy, float* Hz, const float* Ey
     const long long o = off(g, i, j, k);
     const float b = p.b[f][q - 1], c = p.c[f][q - 1];
     float* ps = p.psi[f][0];
-    ps[po] = b * ps[po] + c * (Ez[o + 1] - Ez[o]) / dx;
+    ps[po] = b * ps[po] + rdiv(c * (Ez[o + 1] - Ez[o]), dx);
     Hy[o] = Hy[o] + DB * ps[po];
     ps = p.psi[f][1];
-    ps[po] = b * ps[po] + c * (Ey[o] - Ey[o + 1]) / dx;
+    ps[po] = b * ps[po] + rdiv(c * (Ey[o] - Ey[o + 1]), dx);
     Hz[o] = Hz[o] + DB * ps[po];
 }
 
 // H, slab normal to y.  Threads: (i, t, k).
 __global__ void k_h_cpml_y(Grid g, Cpml p, float* Hx, float* Hz, const float* Ex, const float* Ez, float DB,
-                           float dy) {
+                           RDiv dy) {
     const int nl = max(p.n[2] - 1, 0), nh = max(p.n[3] - 1, 0), nt = nl + nh;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)(g.NX - 1) * nt * (g.NZ - 1);
@@ -802,16 +832,16 @@ __global__ void k_h_cpml_y(Grid g, Cpml p, float* Hx, float* Hz, const float* Ex
     const long long o = off(g, i, j, k);
     const float b = p.b[f][q - 1], c = p.c[f][q - 1];
     float* ps = p.psi[f][0];
-    ps[po] = b * ps[po] + c * (Ez[o] - Ez[o + g.PX]) / dy;
+    ps[po] = b * ps[po] + rdiv(c * (Ez[o] - Ez[o + g.PX]), dy);
     Hx[o] = Hx[o] + DB * ps[po];
     ps = p.psi[f][1];
-    ps[po] = b * ps[po] + c * (Ex[o + g.PX] - Ex[o]) / dy;
+    ps[po] = b * ps[po] + rdiv(c * (Ex[o + g.PX] - Ex[o]), dy);
     Hz[o] = Hz[o] + DB * ps[po];
 }
 
 // H, slab normal to z.  Threads: (i, j, t).
 __global__ void k_h_cpml_z(Grid g, Cpml p, float* Hx, float* Hy, const float* Ex, const float* Ey, float DB,
-                           float dz) {
+                           RDiv dz) {
     const int nl = max(p.n[4] - 1, 0), nh = max(p.n[5] - 1, 0), nt = nl + nh;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)(g.NX - 1) * (g.NY - 1) * nt;
@@ -826,10 +856,10 @@ __global__ void k_h_cpml_z(Grid g, Cpml p, float* Hx, float* Hy, const float* Ex
     const long long o = off(g, i, j, k);
     const float b = p.b[f][q - 1], c = p.c[f][q - 1];
     float* ps = p.psi[f][0];
-    ps[po] = b * ps[po] + c * (Ey[o + g.plane] - Ey[o]) / dz;
+    ps[po] = b * ps[po] + rdiv(c * (Ey[o + g.plane] - Ey[o]), dz);
     Hx[o] = Hx[o] + DB * ps[po];
     ps = p.psi[f][1];
-    ps[po] = b * ps[po] + c * (Ex[o] - Ex[o + g.plane]) / dz;
+    ps[po] = b * ps[po] + rdiv(c * (Ex[o] - Ex[o + g.plane]), dz);
     Hy[o] = Hy[o] + DB * ps[po];
 }
 
@@ -842,7 +872,7 @@ __device__ __forceinline__ float gb_at(const IdxT* idx, const float* tup, long l
 // Layers: low i = 2..n1 (q = i), high i = NX+1-n2..NX-1 (q = NX+1-i).
 template <typename IdxT>
 __global__ void k_e_cpml_x(Grid g, Cpml p, float* Ey, float* Ez, const float* Hy, const float* Hz,
-                           const IdxT* idx, const float* tup, float dx) {
+                           const IdxT* idx, const float* tup, RDiv dx) {
     const int nl = max(p.n[0] - 1, 0), nh = max(p.n[1] - 1, 0), nt = nl + nh;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)nt * (g.NY - 1) * (g.NZ - 1);
@@ -859,12 +889,12 @@ __global__ void k_e_cpml_x(Grid g, Cpml p, float* Ey, float* Ez, const float* Hy
     const float b = p.b[f][q - 1], c = p.c[f][q - 1];
     if (k >= 2) {
         float* ps = p.psi[f][0];
-        ps[po] = b * ps[po] + c * (Hz[o - 1] - Hz[o]) / dx;
+        ps[po] = b * ps[po] + rdiv(c * (Hz[o - 1] - Hz[o]), dx);
         Ey[o] = Ey[o] + gb_at(idx, tup, o, 1) * ps[po];
     }
     if (j >= 2) {
         float* ps = p.psi[f][1];
-        ps[po] = b * ps[po] + c * (Hy[o] - Hy[o - 1]) / dx;
+        ps[po] = b * ps[po] + rdiv(c * (Hy[o] - Hy[o - 1]), dx);
         Ez[o] = Ez[o] + gb_at(idx, tup, o, 2) * ps[po];
     }
 }
@@ -872,7 +902,7 @@ __global__ void k_e_cpml_x(Grid g, Cpml p, float* Ey, float* Ez, const float* Hy
 // E, slab normal to y: Ex (i 1..NX-1, k 2..NZ-1) and Ez (i 2..NX-1, k 1..NZ-1).
 template <typename IdxT>
 __global__ void k_e_cpml_y(Grid g, Cpml p, float* Ex, float* Ez, const float* Hx, const float* Hz,
-                           const IdxT* idx, const float* tup, float dy) {
+                           const IdxT* idx, const float* tup, RDiv dy) {
     const int nl = max(p.n[2] - 1, 0), nh = max(p.n[3] - 1, 0), nt = nl + nh;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)(g.NX - 1) * nt * (g.NZ - 1);
@@ -889,12 +919,12 @@ __global__ void k_e_cpml_y(Grid g, Cpml p, float* Ex, float* Ez, const float* Hx
     const float b = p.b[f][q - 1], c = p.c[f][q - 1];
     if (k >= 2) {
         float* ps = p.psi[f][0];
-        ps[po] = b * ps[po] + c * (Hz[o] - Hz[o - g.PX]) / dy;
+        ps[po] = b * ps[po] + rdiv(c * (Hz[o] - Hz[o - g.PX]), dy);
         Ex[o] = Ex[o] + gb_at(idx, tup, o, 0) * ps[po];
     }
     if (i >= 2) {
         float* ps = p.psi[f][1];
-        ps[po] = b * ps[po] + c * (Hx[o - g.PX] - Hx[o]) / dy;
+        ps[po] = b * ps[po] + rdiv(c * (Hx[o - g.PX] - Hx[o]), dy);
         Ez[o] = Ez[o] + gb_at(idx, tup, o, 2) * ps[po];
     }
 }
@@ -902,7 +932,7 @@ __global__ void k_e_cpml_y(Grid g, Cpml p, float* Ex, float* Ez, const float* Hx
 // E, slab normal to z: Ex (i 1..NX-1, j 2..NY-1) and Ey (i 2..NX-1, j 1..NY-1).
 template <typename IdxT>
 __global__ void k_e_cpml_z(Grid g, Cpml p, float* Ex, float* Ey, const float* Hx, const float* Hy,
-                           const IdxT* idx, const float* tup, float dz) {
+                           const IdxT* idx, const float* tup, RDiv dz) {
     const int nl = max(p.n[4] - 1, 0), nh = max(p.n[5] - 1, 0), nt = nl + nh;
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long tot = (long long)(g.NX - 1) * (g.NY - 1) * nt;
@@ -918,12 +948,12 @@ __global__ void k_e_cpml_z(Grid g, Cpml p, float* Ex, float* Ey, const float* Hx
     const float b = p.b[f][q - 1], c = p.c[f][q - 1];
     if (j >= 2) {
         float* ps = p.psi[f][0];
-        ps[po] = b * ps[po] + c * (Hy[o - g.plane] - Hy[o]) / dz;
+        ps[po] = b * ps[po] + rdiv(c * (Hy[o - g.plane] - Hy[o]), dz);
         Ex[o] = Ex[o] + gb_at(idx, tup, o, 0) * ps[po];
     }
     if (i >= 2) {
         float* ps = p.psi[f][1];
-        ps[po] = b * ps[po] + c * (Hx[o] - Hx[o - g.plane]) / dz;
+        ps[po] = b * ps[po] + rdiv(c * (Hx[o] - Hx[o - g.plane]), dz);
         Ey[o] = Ey[o] + gb_at(idx, tup, o, 1) * ps[po];
     }
 }
@@ -1153,6 +1183,7 @@ struct PortDev {
     int dir, loc[3], size[3];
     float kI, kV, kS, dl, cur_num, cur_den, gap_div;
     int excited;
+    RDiv cdv, gdv;     // the divisions by cur_den and gap_div
 };
 
 // Time-only source waveforms E_or_V_1D() (f90:8844-8857), evaluated once per
@@ -1213,7 +1244,7 @@ __device__ __forceinline__ void ports_h_body(const Grid& g, const PortDev* ports
         for (int q = 0; q < np; ++q) {
             const PortDev pt = ports[q];
             float c = gap_loop_current(g, pt.dir, pt.loc, pt.size, Hx, Hy, Hz, dx, dy, dz);
-            c = c * pt.cur_num / pt.cur_den;
+            c = rdiv(c * pt.cur_num, pt.cdv);
             I[(long long)q * M] = c;
             Iout[(long long)q * T + (n - 1)] = c;
         }
@@ -1256,7 +1287,7 @@ __device__ __forceinline__ void ports_e_body(const Grid& g, const PortDev* ports
             const PortDev pt = ports[q];
             const float v2 = V[(long long)q * M + 1];
             float* F = pt.dir == 0 ? Ex : (pt.dir == 1 ? Ey : Ez);
-            const float val = 1.0f * v2 / pt.gap_div;
+            const float val = rdiv(1.0f * v2, pt.gdv);
             for (int ii = pt.loc[0]; ii <= pt.loc[0] + pt.size[0] - 1; ++ii)
                 for (int jj = pt.loc[1]; jj <= pt.loc[1] + pt.size[1] - 1; ++jj)
                     for (int kk = pt.loc[2]; kk <= pt.loc[2] + pt.size[2] - 1; ++kk) F[off(g, ii, jj, kk)] = val;
@@ -2178,11 +2209,11 @@ __global__ void __launch_bounds__(256) k_ntff_combine(NtffBins bn, int L, int T,
 }
 
 // Periodic-slab normalisation (f90:8342-8361) to FP32.
-__global__ void k_plane_norm(int T, const double* refl, const double* trans, float den, float* rf, float* tf) {
+__global__ void k_plane_norm(int T, const double* refl, const double* trans, RDiv den, float* rf, float* tf) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= T) return;
-    rf[n] = (float)refl[n] / den;
-    tf[n] = (float)trans[n] / den;
+    rf[n] = rdiv((float)refl[n], den);
+    tf[n] = rdiv((float)trans[n], den);
 }
 
 // ---------------------------------------------------------------------------

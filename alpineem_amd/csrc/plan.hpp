@@ -88,7 +88,7 @@ struct Deck {
     int is_mirror = 0, mirror_type = 0, mirror_height = 0;
     int kmax = 0;                        // deck of the kmax build (complex fields, Bloch-periodic walls)
     int mode_type = 0;                   // kmax plane wave: 0 TE (E source), 1 TM (H source)
-    int k_direction = 1;                 // kmax plane wave: 1 '+', 0 '-'
+    int k_direction = 0;                 // kmax plane wave: 1 '+', 0 '-' (antenna decks keep 0, f90:189)
     float k_count[3] = {0, 0, 0};        // kmax Bloch wavenumbers of the periodic axes (rad/m)
     int excitation = 0;                  // 0 none, 1 plane wave, 2 antenna
     float pw_amp = 0, pw_delay = 0;

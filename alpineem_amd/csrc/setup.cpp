@@ -178,6 +178,10 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir, cons
             if (!d.spice.empty()) throw Error(5, "SPICE ports in a kmax deck are not supported by libaem");
             if (d.video_on || d.export_all == "yes")
                 throw Error(5, "video and full-field exports of a kmax deck are not supported by libaem");
+            // with three periodic pairs the reference launches and sums on
+            // every axis into the same series (f90:4558-4625, 7105-7190)
+            if (d.pbc[0] + d.pbc[1] + d.pbc[2] == 3)
+                throw Error(5, "a kmax deck with three periodic pairs is not supported by libaem");
         }
 
         // ---- grid and derived scalars (f90:1309-1397) ----------------------
@@ -963,7 +967,7 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir, cons
                                                           v.t_spread, d.pulse_type, dt);
                 }
             }
-            if (d.pw_amp > 0)
+            if (d.pw_amp > 0 && !d.kmax)
                 for (int n = 1; n <= T; ++n)
                     P->src[(size_t)2 * T + n - 1] = e_or_v_1d(d.pw_amp, (float)n + 0.0f, 0.0f * dx, v.spread,
                                                               v.t_spread, d.pulse_type, dt);

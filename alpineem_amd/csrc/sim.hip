@@ -15,11 +15,13 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/aem.h"
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "kernels_kmax.hpp"
 #include "kernels_out.hpp"
 #include "plan.hpp"
 
@@ -114,6 +116,13 @@ bool cpml_fusable(const aem_plan_view& v) {
             if (v.pbc[a]) { b[2 * a] = lo[a]; b[2 * a + 1] = hi[a]; }   // periodic axes carry no slab
         if (!box_inside(b)) return false;
     }
+    if (v.kmax && v.k_axis >= 0) {   // the modal source planes, across the periodic axes
+        int b[6];
+        for (int a = 0; a < 3; ++a) { b[2 * a] = lo[a]; b[2 * a + 1] = hi[a]; }
+        b[2 * v.k_axis] = std::min(v.kpl_H, v.kpl_E) - 1;
+        b[2 * v.k_axis + 1] = std::max(v.kpl_H, v.kpl_E) + 1;
+        if (!box_inside(b)) return false;
+    }
     for (int a = 0; a < 3; ++a)
         for (int s = 0; s < v.n_sheet_planes[a]; ++s) {
             const int P = v.sheet_dim[a][0], Q = v.sheet_dim[a][1];
@@ -140,7 +149,7 @@ bool cpml_fusable(const aem_plan_view& v) {
 // sheets, TF/SF, PBC or Drude media.  Ports qualify: their H phase only reads
 // H (fdtd_solver.f90:4917-4957) and their E phase follows the bulk E.
 bool single_pass_ok(const aem_plan_view& v) {
-    if (v.has_tfsf || v.has_drude || v.n_spice > 0) return false;
+    if (v.has_tfsf || v.has_drude || v.n_spice > 0 || v.kmax) return false;
     for (int a = 0; a < 3; ++a)
         if (v.pbc[a] || v.n_sheet_planes[a] > 0) return false;
     return true;
@@ -274,12 +283,26 @@ struct aem_sim {
     bool ntff_lists = false;       // ordered binning from contributor lists (aem_sim_options)
     void build_ntff_lists(const std::vector<int>& win, unsigned stride);
     double *d_refl = nullptr, *d_trans = nullptr;
+    double *d_refl2 = nullptr, *d_trans2 = nullptr;     // kmax: the TM sums (d_refl, d_trans hold TE)
+    float *d_reflf2 = nullptr, *d_transf2 = nullptr;
+    // kmax (complex fields): this simulation holds the real part and kim the
+    // imaginary part, a second simulation of the same plan stepped with it on
+    // this stream; the parts meet only in the Bloch copies of the periodic
+    // walls and in the modal plane source
+    aem_sim* kim = nullptr;
+    bool kim_part = false;         // this simulation is the imaginary part of a kmax pair
+    KmaxSrc kmH{}, kmE{};
+    KmaxTab ktab{};                // the modal source as host-built increments (ktab.ncell > 0)
+    void kmax_setup();
+    void kmax_table();
+    void enqueue_kmax_step(hipStream_t s);
     float* d_pc = nullptr;
     float *d_incident = nullptr, *d_Eth = nullptr, *d_Eph = nullptr, *d_fftime = nullptr;
     float *d_reflf = nullptr, *d_transf = nullptr, *d_ffpc = nullptr, *d_scratch = nullptr;
     Tfsf tf{};
     Pulse pu{};
     bool fuse_cpml = false;        // CPML corrections folded into the bulk kernels
+    RDiv rdv[3] = {};              // the CPML kernels' divisions by dx, dy, dz
     std::vector<Launch> bulk;      // bulk launches of each phase; [0] is the core when fused
     HArgs ha{};                    // bulk kernel arguments (fields, tables, CPML state)
     EArgs ea{};
@@ -364,6 +387,7 @@ struct aem_sim {
 
     ~aem_sim() {
         if (st) (void)hipStreamSynchronize(st);
+        delete kim;
         for (auto& f : full_f)
             if (f) std::fclose(f);
         if (h_full) (void)hipHostFree(h_full);
@@ -396,7 +420,10 @@ struct aem_sim {
     void create(const aem_sim_options* opt);
     // one full step; a single-pass step leaves the fields in the other buffer
     // set and flips the host's pointers to it
-    void enqueue_step(hipEvent_t* mark = nullptr) { enqueue_parts(mark, st, kPartAll); }
+    void enqueue_step(hipEvent_t* mark = nullptr) {
+        if (kim) enqueue_kmax_step(st);
+        else enqueue_parts(mark, st, kPartAll);
+    }
     void enqueue_parts(hipEvent_t* mark, hipStream_t s, unsigned parts);
     template <class IdxT>
     void enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts);
@@ -504,6 +531,8 @@ void aem_sim::create(const aem_sim_options* opt) {
         for (int a = 0; a < 3; ++a) { d.loc[a] = p.loc[a]; d.size[a] = p.size[a]; }
         d.kI = p.kI; d.kV = p.kV; d.kS = p.kS; d.dl = p.dl;
         d.cur_num = p.cur_num; d.cur_den = p.cur_den; d.gap_div = p.gap_div;
+        d.cdv = make_rdiv(p.cur_den, v.kmax != 0);
+        d.gdv = make_rdiv(p.gap_div, v.kmax != 0);
         d.excited = p.excited;
     }
     d_ports = up<PortDev>(pd.data(), pd.size());
@@ -528,7 +557,7 @@ void aem_sim::create(const aem_sim_options* opt) {
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_spI), (size_t)v.n_spice * sizeof(float)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_spV), nv * sizeof(float)));
     }
-    pu.src = up<float>(v.src_table, (size_t)3 * T);
+    pu.src = up<float>(kim_part ? v.src_table_im : v.src_table, (size_t)3 * T);
     pu.T = T;
 
     // TF/SF
@@ -651,6 +680,12 @@ void aem_sim::create(const aem_sim_options* opt) {
     d_trans = zeros<double>(T);
     d_reflf = zeros<float>(T);
     d_transf = zeros<float>(T);
+    if (v.kmax) {
+        d_refl2 = zeros<double>(T);
+        d_trans2 = zeros<double>(T);
+        d_reflf2 = zeros<float>(T);
+        d_transf2 = zeros<float>(T);
+    }
     d_pc = zeros<float>((size_t)6 * (T + 1));
     d_ffpc = zeros<float>(T);
     d_incident = zeros<float>(T);
@@ -686,7 +721,12 @@ void aem_sim::create(const aem_sim_options* opt) {
         full_on = true;
     }
 
-    fuse_cpml = allow_fuse && cpml_fusable(v);
+    // (a kmax pair runs the separate CPML kernels: they carry the complex
+    // division of its psi updates)
+    fuse_cpml = allow_fuse && !v.kmax && cpml_fusable(v);
+    rdv[0] = make_rdiv(v.dx, v.kmax != 0);
+    rdv[1] = make_rdiv(v.dy, v.kmax != 0);
+    rdv[2] = make_rdiv(v.dz, v.kmax != 0);
     bulk = bulk_launches(v, fuse_cpml, planes);
     for (int f = 0; f < 6; ++f)
         for (int q = 0; q < 2; ++q) {
@@ -738,6 +778,13 @@ void aem_sim::create(const aem_sim_options* opt) {
     }
     bind();
     HIPCHK(hipStreamSynchronize(st));
+    if (v.kmax && !kim_part) {
+        kim = new aem_sim();
+        kim->plan = plan;
+        kim->kim_part = true;
+        kim->create(opt);
+        kmax_setup();
+    }
 }
 
 // Kernel argument blocks from the current field and psi pointers.  In place,
@@ -767,6 +814,227 @@ void aem_sim::flip() {
         }
     par ^= 1;
     bind();
+}
+
+// Modal source descriptors of a kmax pair (the H plane for TE, the E plane
+// for TM), set once both parts exist.  The per-step Gaussian factor and
+// carrier phase of each plane come from the plan's tables.
+void aem_sim::kmax_setup() {
+    const aem_plan_view& v = plan->v;
+    if (v.k_axis < 0) return;
+    const int T = v.time_steps;
+    const float* ks = up<float>(v.ksrc, (size_t)4 * T);
+    const float dd[3] = {v.dx, v.dy, v.dz};
+    for (int e = 0; e < 2; ++e) {
+        KmaxSrc& k = e ? kmE : kmH;
+        k.g = g;
+        k.plane = e ? v.kpl_E : v.kpl_H;
+        for (int a = 0; a < 3; ++a) {
+            k.lo[a] = v.tf_lo[a];
+            k.hi[a] = v.tf_hi[a];
+            k.w[a] = e ? v.WH[a] : v.WE[a];
+            k.kexc[a] = v.k_exc[a];
+            k.kv[a] = v.k_vec[a];
+            k.d[a] = dd[a];
+        }
+        k.c0 = e ? v.dt / (aem::kEp0 * dd[v.k_axis]) : v.dt / (aem::kMu0 * dd[v.k_axis]);
+        k.gss = ks + (size_t)(2 * e) * T;
+        k.ph = ks + (size_t)(2 * e + 1) * T;
+        const int o = e ? 0 : 3;
+        k.xr = F[o]; k.yr = F[o + 1]; k.zr = F[o + 2];
+        k.xi = kim->F[o]; k.yi = kim->F[o + 1]; k.zi = kim->F[o + 2];
+        k.step = d_step;
+    }
+    kmax_table();
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+// The modal source's increments c0 Inc k_exception of every step, built on
+// the host with the formulas of k_kmax_src_h / k_kmax_src_e and the host
+// libm's sincosf (what the reference's CEXP evaluates), when the table fits
+// kKmaxTabBytes; larger decks evaluate Inc on the device instead, whose
+// cosine and sine are correctly rounded and so differ from the reference's
+// in the last bit for about 1% of the arguments.
+constexpr size_t kKmaxTabBytes = size_t(2) << 30;
+
+void aem_sim::kmax_table() {
+    const aem_plan_view& v = plan->v;
+    const int A = v.k_axis;
+    const bool te = A >= 0 && v.mode_type == 0 && v.plane_wave_amp > 0;
+    const bool tm = A >= 0 && v.mode_type == 1 && v.plane_wave_amp > 0;
+    if (!te && !tm) return;
+    struct Cell { long long o; float w, kd, exc; };
+    std::vector<Cell> grp[2];
+    int fld[2];
+    const float dx = v.dx, dy = v.dy, dz = v.dz;
+    const float kx = v.k_vec[0], ky = v.k_vec[1], kz = v.k_vec[2];
+    const float* W = te ? v.WE : v.WH;
+    auto at = [&](int i, int j, int k) {
+        return (long long)(i - 1) + (long long)v.PX * ((long long)(j - 1) + (long long)v.NY * (long long)(k - 1));
+    };
+    auto h = [](int q) { return (float)q + 0.5f; };
+    if (te) {
+        const int pl = v.kpl_H;
+        const int* lo = v.tf_lo;
+        const int* hi = v.tf_hi;
+        if (A == 0) {
+            fld[0] = 5; fld[1] = 4;   // Hz, Hy
+            for (int k = lo[2]; k <= hi[2]; ++k)
+                for (int j = lo[1]; j <= hi[1]; ++j) {
+                    grp[0].push_back({at(pl, j, k), W[1], ky * (h(j) * dy) + kz * ((float)k * dz), v.k_exc[2]});
+                    grp[1].push_back({at(pl, j, k), W[2], ky * ((float)j * dy) + kz * (h(k) * dz), v.k_exc[1]});
+                }
+        } else if (A == 1) {
+            fld[0] = 5; fld[1] = 3;   // Hz, Hx
+            for (int k = lo[2]; k <= hi[2]; ++k)
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    grp[0].push_back({at(i, pl, k), W[0], kx * (h(i) * dx) + kz * ((float)k * dz), v.k_exc[2]});
+                    grp[1].push_back({at(i, pl, k), W[2], kx * ((float)i * dx) + kz * (h(k) * dz), v.k_exc[0]});
+                }
+        } else {
+            fld[0] = 3; fld[1] = 4;   // Hx, Hy
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    grp[0].push_back({at(i, j, pl), W[1], ky * (h(j) * dy) + kx * ((float)i * dx), v.k_exc[0]});
+                    grp[1].push_back({at(i, j, pl), W[0], ky * ((float)j * dy) + kx * (h(i) * dx), v.k_exc[1]});
+                }
+        }
+    } else {
+        const int pl = v.kpl_E;
+        if (A == 0) {
+            fld[0] = 2; fld[1] = 1;   // Ez, Ey
+            for (int k = 1; k <= v.NZ - 1; ++k)
+                for (int j = 2; j <= v.NY - 1; ++j)
+                    grp[0].push_back({at(pl, j, k), W[1], ky * ((float)j * dy) + kz * (h(k) * dz), v.k_exc[2]});
+            for (int k = 2; k <= v.NZ - 1; ++k)
+                for (int j = 1; j <= v.NY - 1; ++j)
+                    grp[1].push_back({at(pl, j, k), W[2], ky * (h(j) * dy) + kz * ((float)k * dz), v.k_exc[1]});
+        } else if (A == 1) {
+            fld[0] = 2; fld[1] = 0;   // Ez, Ex
+            for (int k = 1; k <= v.NZ - 1; ++k)
+                for (int i = 2; i <= v.NX - 1; ++i)
+                    grp[0].push_back({at(i, pl, k), W[0], kx * ((float)i * dx) + kz * (h(k) * dz), v.k_exc[2]});
+            for (int k = 2; k <= v.NZ - 1; ++k)
+                for (int i = 1; i <= v.NX - 1; ++i)
+                    grp[1].push_back({at(i, pl, k), W[2], kx * (h(i) * dx) + kz * ((float)k * dz), v.k_exc[0]});
+        } else {
+            fld[0] = 0; fld[1] = 1;   // Ex, Ey
+            for (int j = 2; j <= v.NY - 1; ++j)
+                for (int i = 1; i <= v.NX - 1; ++i)
+                    grp[0].push_back({at(i, j, pl), W[1], ky * ((float)j * dy) + kx * (h(i) * dx), v.k_exc[0]});
+            for (int j = 1; j <= v.NY - 1; ++j)
+                for (int i = 2; i <= v.NX - 1; ++i)
+                    grp[1].push_back({at(i, j, pl), W[0], ky * (h(j) * dy) + kx * ((float)i * dx), v.k_exc[1]});
+        }
+    }
+    const int T = v.time_steps;
+    const int n0 = (int)grp[0].size(), nc = n0 + (int)grp[1].size();
+    if (nc == 0 || (size_t)T * 2 * nc * sizeof(float) > kKmaxTabBytes) return;
+    std::vector<Cell> cells(grp[0]);
+    cells.insert(cells.end(), grp[1].begin(), grp[1].end());
+    const float dd = A == 0 ? dx : (A == 1 ? dy : dz);
+    const float c0 = te ? v.dt / (aem::kMu0 * dd) : v.dt / (aem::kEp0 * dd);
+    const float* gss = v.ksrc + (size_t)(te ? 0 : 2) * T;
+    const float* ph = v.ksrc + (size_t)(te ? 1 : 3) * T;
+    std::vector<long long> offs(nc);
+    for (int q = 0; q < nc; ++q) offs[q] = cells[q].o;
+    ktab.off = up<long long>(offs.data(), offs.size());
+    float* dinc = dalloc<float>((size_t)T * 2 * nc, owned);
+    // chunks of steps, rows split over host threads (each writes its own rows)
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)T, (size_t(64) << 20) / (2 * nc * sizeof(float))));
+    std::vector<float> buf((size_t)chunk * 2 * nc);
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (int n0c = 1; n0c <= T; n0c += chunk) {
+        const int m = std::min(chunk, T - n0c + 1);
+        auto work = [&](int t0) {
+            for (int r = t0; r < m; r += nth) {
+                const int n = n0c + r;
+                const float gs = gss[n - 1], tht = ph[n - 1];
+                float* row = buf.data() + (size_t)r * 2 * nc;
+                for (int q = 0; q < nc; ++q) {
+                    const Cell& cl = cells[q];
+                    float sn, cs;
+                    sincosf(tht - cl.kd, &sn, &cs);
+                    const float amp = cl.w * gs;
+                    row[q] = c0 * (amp * cs) * cl.exc;
+                    row[nc + q] = c0 * (amp * sn) * cl.exc;
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool) th.join();
+        HIPCHK(hipMemcpy(dinc + (size_t)(n0c - 1) * 2 * nc, buf.data(), (size_t)m * 2 * nc * sizeof(float),
+                         hipMemcpyHostToDevice));
+    }
+    ktab.inc = dinc;
+    ktab.n0 = n0;
+    ktab.ncell = nc;
+    ktab.r0 = F[fld[0]]; ktab.i0 = kim->F[fld[0]];
+    ktab.r1 = F[fld[1]]; ktab.i1 = kim->F[fld[1]];
+    ktab.step = d_step;
+}
+
+// One step of a kmax pair on stream s: both parts run each phase of the
+// ordinary step; the modal source follows the H (TE) or E (TM) sheets/Drude
+// updates where the TF/SF source would be (f90:4558-4625, 5857-5948), and
+// the Bloch copies replace the periodic copies after the CPML corrections
+// (f90:5377-5433, 6672-6741).
+void aem_sim::enqueue_kmax_step(hipStream_t s) {
+    const aem_plan_view& v = plan->v;
+    aem_sim* im = kim;
+    const dim3 bb(32, 8);
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    const int a = v.k_axis;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;   // the source plane's axes
+    const bool te = a >= 0 && v.mode_type == 0 && v.plane_wave_amp > 0;
+    const bool tm = a >= 0 && v.mode_type == 1 && v.plane_wave_amp > 0;
+    auto both = [&](unsigned parts) {
+        enqueue_parts(nullptr, s, parts);
+        im->enqueue_parts(nullptr, s, parts);
+    };
+    // Bloch copies of one phase, x then y then z; the H copy to the last
+    // cell multiplies by exp(-i th), the E copy to the first by exp(+i th)
+    auto bloch = [&](bool e) {
+        for (int q = 0; q < 3; ++q) {
+            if (!v.pbc[q]) continue;
+            const int u = q == 0 ? 1 : 0, w = q == 2 ? 1 : 2;
+            const dim3 gp(cdiv(N[u] - 1, 32), cdiv(N[w] - 1, 8));
+            // (Hz, Hy) / (Ez, Ey) on x, (z, x) on y, (y, x) on z
+            const int ca = q == 2 ? 1 : 2, cb = q == 0 ? 1 : 0;
+            const int o = e ? 0 : 3;
+            if (e)
+                k_kmax_pbc<true><<<gp, bb, 0, s>>>(g, q, v.bloch_c[q], v.bloch_s[q], F[o + ca], im->F[o + ca],
+                                                    F[o + cb], im->F[o + cb]);
+            else
+                k_kmax_pbc<false><<<gp, bb, 0, s>>>(g, q, v.bloch_c[q], -v.bloch_s[q], F[o + ca], im->F[o + ca],
+                                                     F[o + cb], im->F[o + cb]);
+        }
+    };
+    both(kPartHBulk | kPartHPre);
+    if (te && ktab.ncell > 0) {
+        k_kmax_src_tab<<<cdiv(ktab.ncell, 256), 256, 0, s>>>(ktab);
+    } else if (te) {
+        const dim3 gs(cdiv(kmH.hi[b] - kmH.lo[b] + 1, 32), cdiv(kmH.hi[c] - kmH.lo[c] + 1, 8));
+        if (a == 0) k_kmax_src_h<0><<<gs, bb, 0, s>>>(kmH);
+        else if (a == 1) k_kmax_src_h<1><<<gs, bb, 0, s>>>(kmH);
+        else k_kmax_src_h<2><<<gs, bb, 0, s>>>(kmH);
+    }
+    both(kPartHPort | kPartHPost);
+    bloch(false);
+    both(kPartEBulk | kPartEPre);
+    if (tm && ktab.ncell > 0) {
+        k_kmax_src_tab<<<cdiv(ktab.ncell, 256), 256, 0, s>>>(ktab);
+    } else if (tm) {
+        const dim3 gs(cdiv(N[b] - 1, 32), cdiv(N[c] - 1, 8));
+        if (a == 0) k_kmax_src_e<0><<<gs, bb, 0, s>>>(kmE);
+        else if (a == 1) k_kmax_src_e<1><<<gs, bb, 0, s>>>(kmE);
+        else k_kmax_src_e<2><<<gs, bb, 0, s>>>(kmE);
+    }
+    both(kPartEPort | kPartEPost);
+    bloch(true);
+    both(kPartEOld | kPartOutPre | kPartNtff | kPartOutPost | kPartStep);
 }
 
 void aem_sim::drop_graphs() {
@@ -876,12 +1144,13 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
         const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
-        if (!fuse_cpml && tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, s>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, v.dx);
-        if (!fuse_cpml && ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, s>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, v.dy);
-        if (!fuse_cpml && tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, s>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, v.dz);
-        if (v.pbc[0]) k_h_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 0, Hz, Hy);
-        if (v.pbc[1]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 1, Hz, Hx);
-        if (v.pbc[2]) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Hy, Hx);
+        if (!fuse_cpml && tx > 0) k_h_cpml_x<<<cdiv(tx, 256), 256, 0, s>>>(g, cpH, Hy, Hz, Ey, Ez, v.DB, rdv[0]);
+        if (!fuse_cpml && ty > 0) k_h_cpml_y<<<cdiv(ty, 256), 256, 0, s>>>(g, cpH, Hx, Hz, Ex, Ez, v.DB, rdv[1]);
+        if (!fuse_cpml && tz > 0) k_h_cpml_z<<<cdiv(tz, 256), 256, 0, s>>>(g, cpH, Hx, Hy, Ex, Ey, v.DB, rdv[2]);
+        // (a kmax pair makes its Bloch copies across both parts)
+        if (v.pbc[0] && !v.kmax) k_h_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 0, Hz, Hy);
+        if (v.pbc[1] && !v.kmax) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 1, Hz, Hx);
+        if (v.pbc[2] && !v.kmax) k_h_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Hy, Hx);
     }
 
     if ((parts & kPartSpiceH) && v.n_spice > 0) k_spice_h<<<1, 64, 0, s>>>(spa);
@@ -925,20 +1194,20 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
         const long long tx = (long long)(std::max(v.npml[0] - 1, 0) + std::max(v.npml[1] - 1, 0)) * (v.NY - 1) * (v.NZ - 1);
         const long long ty = (long long)(v.NX - 1) * (std::max(v.npml[2] - 1, 0) + std::max(v.npml[3] - 1, 0)) * (v.NZ - 1);
         const long long tz = (long long)(v.NX - 1) * (v.NY - 1) * (std::max(v.npml[4] - 1, 0) + std::max(v.npml[5] - 1, 0));
-        if (!fuse_cpml && tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, s>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, v.dx);
-        if (!fuse_cpml && ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, s>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, v.dy);
-        if (!fuse_cpml && tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, s>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, v.dz);
-        if (v.pbc[0]) {
+        if (!fuse_cpml && tx > 0) k_e_cpml_x<IdxT><<<cdiv(tx, 256), 256, 0, s>>>(g, cpE, Ey, Ez, Hy, Hz, idx, d_tup, rdv[0]);
+        if (!fuse_cpml && ty > 0) k_e_cpml_y<IdxT><<<cdiv(ty, 256), 256, 0, s>>>(g, cpE, Ex, Ez, Hx, Hz, idx, d_tup, rdv[1]);
+        if (!fuse_cpml && tz > 0) k_e_cpml_z<IdxT><<<cdiv(tz, 256), 256, 0, s>>>(g, cpE, Ex, Ey, Hx, Hy, idx, d_tup, rdv[2]);
+        if (v.pbc[0] && !v.kmax) {
             k_e_pbc<<<dim3(cdiv(v.NY - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 0, Ez, Ey);
             if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[2].np), 256, 0, s>>>(sh[2], 0, v.NX - 1, v.NY - 1);
             if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[1].np), 256, 0, s>>>(sh[1], 0, v.NX - 1, v.NZ - 1);
         }
-        if (v.pbc[1]) {
+        if (v.pbc[1] && !v.kmax) {
             k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NZ - 1, 8)), bb, 0, s>>>(g, 1, Ez, Ex);
             if (sh[2].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[2].np), 256, 0, s>>>(sh[2], 1, v.NY - 1, v.NX - 1);
             if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NZ - 1, 256), sh[0].np), 256, 0, s>>>(sh[0], 0, v.NY - 1, v.NZ - 1);
         }
-        if (v.pbc[2]) {
+        if (v.pbc[2] && !v.kmax) {
             k_e_pbc<<<dim3(cdiv(v.NX - 1, 32), cdiv(v.NY - 1, 8)), bb, 0, s>>>(g, 2, Ey, Ex);
             if (sh[1].np) k_sheet_pbc<<<dim3(cdiv(v.NX - 1, 256), sh[1].np), 256, 0, s>>>(sh[1], 1, v.NZ - 1, v.NX - 1);
             if (sh[0].np) k_sheet_pbc<<<dim3(cdiv(v.NY - 1, 256), sh[0].np), 256, 0, s>>>(sh[0], 1, v.NZ - 1, v.NY - 1);
@@ -950,7 +1219,20 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
 
     // ---- per-step outputs ----
     if (mark) HIPCHK(hipEventRecord(mark[4], st));
-    if ((parts & kPartOutPre) && v.plane_sum_axis >= 0) {
+    if ((parts & kPartOutPre) && v.kmax && v.k_axis >= 0) {
+        // TE and TM sums of the kmax build (f90:7105-7190): TE = Ea k1 - Eb k2, TM = Ea k2 + Eb k1
+        const int ax = v.k_axis;
+        const int lo = v.tf_lo[ax], hi = v.tf_hi[ax];
+        const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+        const float* Ea = ax == 2 ? Ex : Ez;
+        const float* Eb = ax == 0 ? Ey : (ax == 1 ? Ex : Ey);
+        // (x normal: Ez, Ey with k_y, k_z; y normal: Ez, Ex with k_x, k_z; z normal: Ex, Ey with k_y, k_x)
+        const float k1 = ax == 2 ? v.k_exc[1] : v.k_exc[p], k2 = ax == 2 ? v.k_exc[0] : v.k_exc[q];
+        k_plane_sums<<<1, 256, 0, s>>>(g, ax, lo, hi, v.ff_lo[p], v.ff_hi[p], v.ff_lo[q], v.ff_hi[q], Ea, Eb, k1, k2,
+                                        1.0f, 1, d_refl, d_trans, d_step);
+        k_plane_sums<<<1, 256, 0, s>>>(g, ax, lo, hi, v.ff_lo[p], v.ff_hi[p], v.ff_lo[q], v.ff_hi[q], Ea, Eb, k2, k1,
+                                        1.0f, 0, d_refl2, d_trans2, d_step);
+    } else if ((parts & kPartOutPre) && v.plane_sum_axis >= 0) {
         const int ax = v.plane_sum_axis;
         const int lo = v.tf_lo[ax], hi = v.tf_hi[ax];
         if (ax == 0)
@@ -1129,6 +1411,7 @@ void aem_sim::build_ntff_lists(const std::vector<int>& win, unsigned stride) {
 void aem_sim::enqueue_watch(hipStream_t s) {
     const WatchSet ws{{F[0], F[1], F[2], F[3], F[4], F[5]}};
     k_watch<<<1024, 256, 0, s>>>(ws, (long long)(plan->v.ncell / 4), d_watch);
+    if (kim) kim->enqueue_watch(s);
 }
 
 void aem_sim::check_watch() {
@@ -1138,6 +1421,7 @@ void aem_sim::check_watch() {
     if (bad)
         throw aem::Error(AEM_ERR_NUMERIC, "non-finite field value (NaN or Inf) by step " + std::to_string(steps_done) +
                                               ": the update is unstable (check the deck's Courant factor)");
+    if (kim) kim->check_watch();
 }
 
 namespace aem {
@@ -1250,6 +1534,7 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
         HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
         s->last_ms = ms / (float)n_steps;
         s->steps_done += n_steps;
+        if (s->kim) s->kim->steps_done = s->steps_done;
         s->check_watch();
     });
 }
@@ -1332,6 +1617,12 @@ aem_status aem_finalize(aem_sim* s) {
         aem::set_error("aem_finalize: a step waits at the SPICE exchange; finish it with aem_port_exchange");
         return AEM_ERR_STATE;
     }
+    if (s->kim) {   // the imaginary part of a kmax pair: its steps ran on this simulation's stream
+        const aem_status st = aem::guard([&]() { HIPCHK(hipStreamSynchronize(s->st)); });
+        if (st != AEM_OK) return st;
+        const aem_status sk = aem_finalize(s->kim);
+        if (sk != AEM_OK) return sk;
+    }
     return aem::guard([&]() {
         const aem_plan_view& v = s->plan->v;
         const int T = v.time_steps;
@@ -1340,7 +1631,11 @@ aem_status aem_finalize(aem_sim* s) {
             if (v.plane_sum_axis == 0) den = ((float)v.NY - 2.0f) * ((float)v.NZ - 2.0f);
             else if (v.plane_sum_axis == 1) den = ((float)v.NX - 2.0f) * ((float)v.NZ - 2.0f);
             else den = ((float)v.NY - 2.0f) * ((float)v.NX - 2.0f);
-            k_plane_norm<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->d_refl, s->d_trans, den, s->d_reflf, s->d_transf);
+            const RDiv dv = make_rdiv(den, v.kmax != 0);
+            k_plane_norm<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->d_refl, s->d_trans, dv, s->d_reflf, s->d_transf);
+            if (v.kmax)   // the TM sums (f90:8411-8435)
+                k_plane_norm<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->d_refl2, s->d_trans2, dv, s->d_reflf2,
+                                                                s->d_transf2);
         }
         if (v.plane_wave_amp > 0)
             k_plane_incident<<<cdiv(T, 256), 256, 0, s->st>>>(T, s->pu, s->d_incident);
@@ -1414,9 +1709,18 @@ double* dft_scratch_ptr(int dev) {
 }
 
 const float* series_src(const aem_sim* s, int32_t kind, int32_t idx) {
+    if (kind & AEM_IMAG) {
+        if (!s->kim) { aem::set_error("AEM_IMAG selects the imaginary part of a kmax simulation"); return nullptr; }
+        return series_src(s->kim, kind & ~AEM_IMAG, idx);
+    }
     const aem_plan_view& v = s->plan->v;
     const int T = v.time_steps;
     switch (kind) {
+        case AEM_SERIES_REFLECTED_TM:
+        case AEM_SERIES_TRANSMITTED_TM:
+            if (!v.kmax) { aem::set_error("the TM sums belong to kmax simulations"); return nullptr; }
+            if (!s->finalized) { aem::set_error("call aem_finalize first"); return nullptr; }
+            return kind == AEM_SERIES_REFLECTED_TM ? s->d_reflf2 : s->d_transf2;
         case AEM_SERIES_PORT_V:
         case AEM_SERIES_PORT_I:
             if (idx < 0 || idx >= v.n_ports) { aem::set_error("port index out of range"); return nullptr; }
@@ -1478,7 +1782,7 @@ aem_status aem_get_dft(aem_sim* s, int32_t kind, int32_t idx, const double* freq
     if (n_freq == 0) return AEM_OK;
     const aem_plan_view& v = s->plan->v;
     // port series hold the steps run so far; the post-loop series all T
-    const int n = per_step_series(kind) ? (int)s->steps_done : v.time_steps;
+    const int n = per_step_series(kind & ~AEM_IMAG) ? (int)s->steps_done : v.time_steps;
     return aem::guard([&]() {
         if (n_freq > s->dft_cap) {
             if (s->d_dft) HIPCHK(hipFree(s->d_dft));
@@ -1505,11 +1809,19 @@ aem_status aem_get_dft_noalloc(aem_sim* s, int32_t kind, int32_t idx, const doub
     const float* src = series_src(s, kind, idx);
     if (!src) return AEM_ERR_STATE;
     if (n_freq == 0) return AEM_OK;
-    const int n = per_step_series(kind) ? (int)s->steps_done : s->plan->v.time_steps;
+    const int n = per_step_series(kind & ~AEM_IMAG) ? (int)s->steps_done : s->plan->v.time_steps;
     return aem::guard([&]() { dft_into(s, src, n, freqs_hz, n_freq, dft_scratch_ptr(s->dev), re_im); });
 }
 
+// The part of a kmax pair that holds a field component OR-ed with AEM_IMAG.
+static aem_sim* field_part(aem_sim* s, int32_t& comp) {
+    if (!s || !(comp & AEM_IMAG)) return s;
+    comp &= ~AEM_IMAG;
+    return s->kim;
+}
+
 aem_status aem_get_field(aem_sim* s, int32_t comp, float* out, int64_t n) {
+    s = field_part(s, comp);
     if (!s || !out || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
         aem::set_error("aem_get_field: bad argument");
         return AEM_ERR_STATE;
@@ -1542,6 +1854,7 @@ aem_status aem_get_video(aem_sim* s, int32_t comp, int64_t first_step, int64_t n
 }
 
 aem_status aem_set_field(aem_sim* s, int32_t comp, const float* in, int64_t n) {
+    s = field_part(s, comp);
     if (!s || !in || comp < 0 || comp > 5 || n < 0 || n > s->plan->v.ncell) {
         aem::set_error("aem_set_field: bad argument");
         return AEM_ERR_STATE;
@@ -1599,6 +1912,10 @@ aem_status aem_time_phases(aem_sim* s, int64_t n_steps, float* ms, int32_t n_ms)
     if (!s || !ms || n_ms < 5 || n_steps <= 0 || s->steps_done + n_steps > s->plan->v.time_steps || s->finalized) {
         aem::set_error("aem_time_phases: bad argument or not enough steps left");
         return AEM_ERR_STATE;
+    }
+    if (s->kim) {
+        aem::set_error("aem_time_phases: not available for a kmax simulation");
+        return AEM_ERR_UNSUPPORTED;
     }
     return aem::guard([&]() {
         hipEvent_t ev[6];
@@ -1833,6 +2150,10 @@ aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
             aem::set_error("aem_batch_create: a member has SPICE ports; step it alone with aem_port_exchange");
             return AEM_ERR_UNSUPPORTED;
         }
+        if (sims[q]->kim) {
+            aem::set_error("aem_batch_create: a member is a kmax (complex-field) simulation; run it alone");
+            return AEM_ERR_UNSUPPORTED;
+        }
     }
     aem_batch* b = new (std::nothrow) aem_batch();
     if (!b) {
@@ -1983,6 +2304,7 @@ const char* adjoint_unsupported(const aem_plan_view& v) {
     if (v.n_sheet_planes[0] + v.n_sheet_planes[1] + v.n_sheet_planes[2] > 0) return "thin sheets";
     if (v.has_drude) return "Drude media";
     if (v.pbc[0] || v.pbc[1] || v.pbc[2]) return "periodic boundaries";
+    if (v.kmax) return "the kmax build";
     return nullptr;
 }
 

@@ -44,6 +44,69 @@ void get(aem_sim* s, int kind, int idx, std::vector<float>& out) {
     if (st != AEM_OK) throw aem::Error(st, aem_last_error());
 }
 
+// T lines "re im" of a kmax simulation's complex series
+void complex_lines(File& f, aem_sim* s, int kind, int idx, int T) {
+    std::vector<float> re(T), im(T);
+    get(s, kind, idx, re);
+    get(s, kind | AEM_IMAG, idx, im);
+    for (int n = 0; n < T; ++n) f.line(fmt(re[n]) + " " + fmt(im[n]));
+}
+
+// data.dat of the kmax build (f90:8648-8785): the same header with "N/A"
+// for the minimum steps and pulse type, the k vector and mode records,
+// complex series as "re im" lines, and the TE/TM plane sums, whose 'Refl'
+// records hold the sums of the plane on the launch side for k_direction 1
+// and of the far plane for k_direction 0.
+void write_kmax(File& f, aem_sim* s, const aem_plan_view& v, const aem::Deck& d) {
+    const int T = v.time_steps;
+    f.line("x steps =  " + std::to_string(v.nx));
+    f.line("y steps =  " + std::to_string(v.ny));
+    f.line("z steps =  " + std::to_string(v.nz));
+    f.line("delta x,y,z =  " + fmt(v.dx) + " " + fmt(v.dy) + " " + fmt(v.dz));
+    f.line("time step =  " + fmt(v.dt));
+    f.line("number of time steps =  " + std::to_string(T));
+    f.line("minimum time steps needed N/A");
+    f.line("freq center =  " + fmt(v.f_center));
+    f.line("pulse type = N/A");
+    f.line("slice =  " + std::to_string(v.slice));
+    f.line("slice location =  " + std::to_string(v.slice_location));
+    f.line("video size_a =  " + std::to_string(v.vid_size1));
+    f.line("video size_b =  " + std::to_string(v.vid_size2));
+    f.line("number of far field angles =  " + std::to_string(v.n_angles));
+    f.line("number of internal ports =  " + std::to_string(v.n_ports));
+    f.line("number of spice ports =  0");
+    f.line("simulation type " + std::to_string(v.pbc[0]) + " " + std::to_string(v.pbc[1]) + " " +
+           std::to_string(v.pbc[2]) + " " + std::to_string(v.k_direction));
+    f.line("simulation is k vector method w/ k1,k2:");
+    if (v.k_axis >= 0) {
+        const int p = v.k_axis == 0 ? 1 : 0, q = v.k_axis == 2 ? 1 : 2;
+        f.line(fmt(v.k_vec[p]) + " " + fmt(v.k_vec[q]));
+    }
+    f.line("mode type is 0 for TE and 1 for TM: " + std::to_string(v.mode_type));
+    if (d.excitation == 1) f.line("incident E-field wave: (not used in post processor bc r(k))");
+    if (d.excitation == 2) f.line("incident voltage wave at port  " + std::to_string(d.exc_port));
+    complex_lines(f, s, AEM_SERIES_INCIDENT, 0, T);
+    const bool fwd = v.k_direction == 1;
+    const int sec[4] = {fwd ? AEM_SERIES_REFLECTED : AEM_SERIES_TRANSMITTED,
+                        fwd ? AEM_SERIES_REFLECTED_TM : AEM_SERIES_TRANSMITTED_TM,
+                        fwd ? AEM_SERIES_TRANSMITTED : AEM_SERIES_REFLECTED,
+                        fwd ? AEM_SERIES_TRANSMITTED_TM : AEM_SERIES_REFLECTED_TM};
+    const char* lab[4] = {"TE Refl Wave:", "TM Refl Wave:", "TE Trans Wave:", "TM Trans Wave:"};
+    for (int q = 0; q < 4; ++q) {
+        f.line(lab[q]);
+        if (v.k_axis >= 0) {
+            complex_lines(f, s, sec[q], 0, T);
+        } else {
+            for (int n = 0; n < T; ++n) f.line("0 0");   // no periodic pair: the sums stay zero
+        }
+    }
+    for (int q = 0; q < v.n_ports; ++q) {
+        f.line("recieved voltage at port  " + std::to_string(q + 1) + " w/ Z");
+        f.line(fmt(std::sqrt(v.ports[q].L / v.ports[q].C)));
+        complex_lines(f, s, AEM_SERIES_PORT_V, q, T);
+    }
+}
+
 // <stem>_video_{E,H}{x,y,z}.bin: the whole (size_a, size_b, steps) array of
 // a component as ONE Fortran unformatted sequential record -- a 4-byte byte
 // count, the floats in Fortran order, the count again (f90:8788-8807, in the
@@ -91,6 +154,11 @@ extern "C" aem_status aem_write_outputs(aem_sim* s, const char* path) {
         if (fs != AEM_OK) throw aem::Error(fs, aem_last_error());
         File f(name);
         const aem::Deck& d = plan->deck;
+        if (v.kmax) {
+            write_kmax(f, s, v, d);
+            aem::write_geometry(*plan, (name.size() > 4 ? name.substr(0, name.size() - 4) : name) + "_geometry.bin");
+            return;
+        }
         f.line("x steps =  " + std::to_string(v.nx));
         f.line("y steps =  " + std::to_string(v.ny));
         f.line("z steps =  " + std::to_string(v.nz));

@@ -5,6 +5,11 @@ Layout of the reference writer (``main_fdtd/fdtd_solver.f90:8536-8645``): a
 wave, periodic reflection/transmission, far field per angle, port voltages,
 far-field time corrections, phase-centre incident).  The positions match what
 ``utility_scripts/post_processor.py:43-160`` consumes.
+
+The kmax build's file (f90:8648-8785) has the same header, then the k vector
+and mode records, complex series (real and imaginary part per line) and the
+TE/TM reflected and transmitted waves in place of the periodic R/T; its
+series are returned as complex64 arrays and ``kmax`` is set.
 """
 from __future__ import annotations
 
@@ -30,6 +35,13 @@ class DatFile:
     spice_v: List[np.ndarray] = field(default_factory=list)   # SPICE build: per SPICE port / extra node
     ff_time: np.ndarray = None
     ff_pc: np.ndarray = None
+    kmax: bool = False
+    k_vec: tuple = None            # (k1, k2) of the two periodic axes
+    mode_type: int = -1            # 0 TE, 1 TM
+    te_refl: np.ndarray = None     # kmax TE/TM waves, as labelled by the writer (f90:8690-8730)
+    tm_refl: np.ndarray = None
+    te_trans: np.ndarray = None
+    tm_trans: np.ndarray = None
 
 
 def read_dat(path: str) -> DatFile:
@@ -54,6 +66,29 @@ def read_dat(path: str) -> DatFile:
             continue
         secs[-1][1].append(vals)
     T = out.time_steps
+    out.kmax = any(name.startswith("simulation is k vector") for name, _ in secs)
+    if out.kmax:
+        def cplx(rows):
+            return np.array([complex(r[0], r[1]) for r in rows], np.complex64)
+        for name, rows in secs:
+            if name.startswith("simulation is k vector"):
+                out.k_vec = tuple(rows[0]) if rows else None
+            elif name.startswith("mode type is"):
+                out.mode_type = int(name.split(":")[-1])
+            elif name.startswith("incident voltage") or name.startswith("incident E-field"):
+                out.incident = cplx(rows)
+            elif name.startswith("TE Refl"):
+                out.te_refl = cplx(rows)
+            elif name.startswith("TM Refl"):
+                out.tm_refl = cplx(rows)
+            elif name.startswith("TE Trans"):
+                out.te_trans = cplx(rows)
+            elif name.startswith("TM Trans"):
+                out.tm_trans = cplx(rows)
+            elif name.startswith("recieved voltage"):
+                out.port_z.append(rows[0][0])
+                out.port_v.append(cplx(rows[1:T + 1]))
+        return out
     for name, rows in secs:
         if name.startswith("incident voltage") or name.startswith("incident E-field"):
             out.incident = np.array([r[0] for r in rows], np.float32)

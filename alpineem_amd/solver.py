@@ -185,11 +185,17 @@ class Simulation:
         return out[:v.n_spice] if current else None
 
     # -- outputs -----------------------------------------------------------------
-    def series(self, kind: str, idx: int = 0, n: Optional[int] = None) -> np.ndarray:
+    def series(self, kind: str, idx: int = 0, n: Optional[int] = None, imag: bool = False) -> np.ndarray:
+        """An output series; ``imag`` selects the imaginary part of a kmax
+        simulation's complex series (AEM_IMAG)."""
         n = self.plan.time_steps if n is None else n
         out = np.zeros(n, np.float32)
-        check(lib().aem_get_series(self._h, SERIES[kind], idx, out.ctypes.data, n))
+        check(lib().aem_get_series(self._h, SERIES[kind] | (_lib.IMAG if imag else 0), idx, out.ctypes.data, n))
         return out
+
+    def cseries(self, kind: str, idx: int = 0, n: Optional[int] = None) -> np.ndarray:
+        """A kmax simulation's complex series (real and imaginary part) as complex64."""
+        return (self.series(kind, idx, n) + 1j * self.series(kind, idx, n, imag=True)).astype(np.complex64)
 
     def dft(self, kind: str, freqs_hz, idx: int = 0) -> np.ndarray:
         """Complex spectrum of an output series at ``freqs_hz``, computed on
@@ -296,10 +302,12 @@ class Simulation:
         a = np.ascontiguousarray(values, np.float32).reshape(-1)
         check(lib().aem_adjoint_set_field(self._h, FIELDS[name], a.ctypes.data, a.size))
 
-    def field(self, name: str) -> np.ndarray:
+    def field(self, name: str, imag: bool = False) -> np.ndarray:
+        """A field component in the common layout; ``imag`` selects the
+        imaginary part of a kmax simulation's complex field (AEM_IMAG)."""
         v = self.plan.view
         out = np.zeros(v.ncell, np.float32)
-        check(lib().aem_get_field(self._h, FIELDS[name], out.ctypes.data, v.ncell))
+        check(lib().aem_get_field(self._h, FIELDS[name] | (_lib.IMAG if imag else 0), out.ctypes.data, v.ncell))
         return out.reshape(self.plan.shape)
 
     def video(self, name: str, first_step: int = 1, n_steps: Optional[int] = None) -> np.ndarray:
@@ -313,12 +321,12 @@ class Simulation:
         check(lib().aem_get_video(self._h, FIELDS[name], int(first_step), int(n_steps), out.ctypes.data, out.size))
         return out.reshape(n_steps, v.vid_size2, v.vid_size1)
 
-    def set_field(self, name: str, values: np.ndarray) -> None:
+    def set_field(self, name: str, values: np.ndarray, imag: bool = False) -> None:
         v = self.plan.view
         a = np.ascontiguousarray(values, np.float32).reshape(-1)
         if a.size != v.ncell:
             raise ValueError(f"expected {v.ncell} values, got {a.size}")
-        check(lib().aem_set_field(self._h, FIELDS[name], a.ctypes.data, a.size))
+        check(lib().aem_set_field(self._h, FIELDS[name] | (_lib.IMAG if imag else 0), a.ctypes.data, a.size))
 
     def ntff_raw(self, which: int) -> np.ndarray:
         v = self.plan.view

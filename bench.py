@@ -166,7 +166,7 @@ def adjoint_bench(steps: int) -> dict:
             "grad_norm": float(np.linalg.norm(g))}
 
 
-def configs_bench() -> dict:
+def configs_bench(only: str = None) -> dict:
     """The BASELINE configurations with every output of each, timed end to
     end on the device (wall clock around the stepping; setup excluded):
 
@@ -185,7 +185,8 @@ def configs_bench() -> dict:
         steps; reflection/transmission plane sums and their spectra at 401
         frequencies.
     R5  the 400^3 throughput deck with the survey's performance CPML
-        (kappa_max 5, alpha_max 0.05), 200 steps."""
+        (kappa_max 5, alpha_max 0.05), 200 steps.
+    ``only`` runs one of them ("C1", "C2", "C3", "R5")."""
     from alpineem_amd import decks
     from alpineem_amd.solver import Plan, Simulation
 
@@ -207,57 +208,61 @@ def configs_bench() -> dict:
               f"(stepping {step_s:.3f} s, outputs {post_s:.3f} s)", file=sys.stderr, flush=True)
         sim.close()
 
-    # C1
-    plan = Plan.from_text(decks.dipole(n=100, steps=2000, angles=37, arm=20).text(), cpml_kappa_max=5.0,
-                          cpml_alpha_max=0.05)
-    sim = Simulation(plan)
+    def want(name):
+        return only is None or only == name
 
-    def c1_post(s):
-        s.finalize()
-        f = np.linspace(0.5e9, 8e9, 101)
-        _ = (s.dft("port_v", f) - s.dft("incident", f)) / s.dft("incident", f)
-        for a in range(plan.view.n_angles):
-            s.series("e_theta", a)
-    timed("C1", plan, sim, post=c1_post)
-    out["C1"]["outputs"] = "port V/I, S11 x101, far field 37 angles (ordered FP32 binning)"
+    if want("C1"):
+        plan = Plan.from_text(decks.dipole(n=100, steps=2000, angles=37, arm=20).text(), cpml_kappa_max=5.0,
+                              cpml_alpha_max=0.05)
+        sim = Simulation(plan)
 
-    # C2
-    plan = Plan.from_text(decks.pec_sphere(n=200, steps=4000, radius=40, angles=[(180.0, 0.0)]).text(),
-                          cpml_cells=12, tfsf_inset=15, ntff_gap=5)
-    sim = Simulation(plan)
-    sim.set_farfield_dft([(float(t), 0.0) for t in np.linspace(0.0, 180.0, 19)], 0.5e9, 0.05e9, 201)
+        def c1_post(s):
+            s.finalize()
+            f = np.linspace(0.5e9, 8e9, 101)
+            _ = (s.dft("port_v", f) - s.dft("incident", f)) / s.dft("incident", f)
+            for a in range(plan.view.n_angles):
+                s.series("e_theta", a)
+        timed("C1", plan, sim, post=c1_post)
+        out["C1"]["outputs"] = "port V/I, S11 x101, far field 37 angles (ordered FP32 binning)"
 
-    def c2_post(s):
-        s.finalize()
-        s.series("e_theta", 0)
-        s.farfield_dft()
-    timed("C2", plan, sim, post=c2_post)
-    out["C2"]["outputs"] = "monostatic far field (ordered FP32 binning), running FP64 DFT far field 19 angles x 201"
+    if want("C2"):
+        plan = Plan.from_text(decks.pec_sphere(n=200, steps=4000, radius=40, angles=[(180.0, 0.0)]).text(),
+                              cpml_cells=12, tfsf_inset=15, ntff_gap=5)
+        sim = Simulation(plan)
+        sim.set_farfield_dft([(float(t), 0.0) for t in np.linspace(0.0, 180.0, 19)], 0.5e9, 0.05e9, 201)
 
-    # C3
-    plan = Plan.from_text(decks.layered_slab().text(), cpml_cells=16)
-    sim = Simulation(plan)
+        def c2_post(s):
+            s.finalize()
+            s.series("e_theta", 0)
+            s.farfield_dft()
+        timed("C2", plan, sim, post=c2_post)
+        out["C2"]["outputs"] = ("monostatic far field (ordered FP32 binning), running FP64 DFT far field 19 angles "
+                                "x 201")
 
-    def c3_post(s):
-        s.finalize()
-        f = np.linspace(1e9, 100e9, 401)
-        s.dft("reflected", f)
-        s.dft("transmitted", f)
-    timed("C3", plan, sim, post=c3_post)
-    out["C3"]["outputs"] = "reflected/transmitted plane sums, spectra x401"
+    if want("C3"):
+        plan = Plan.from_text(decks.layered_slab().text(), cpml_cells=16)
+        sim = Simulation(plan)
 
-    # performance CPML at 400^3
-    plan = Plan.from_text(decks.roofline_deck(400, 220).text(), cpml_kappa_max=5.0, cpml_alpha_max=0.05)
-    sim = Simulation(plan)
-    sim.run(20)
-    t0 = time.perf_counter()
-    sim.run(200)
-    el = time.perf_counter() - t0
-    out["R400_kappa5"] = {"cells": 400 ** 3, "steps": 200, "gcells": round(400 ** 3 * 200 / el / 1e9, 3),
-                          "single_pass": sim.single_pass}
-    print(f"[bench] 400^3 with kappa_max 5, alpha_max 0.05: {out['R400_kappa5']['gcells']} Gcell/s", file=sys.stderr,
-          flush=True)
-    sim.close()
+        def c3_post(s):
+            s.finalize()
+            f = np.linspace(1e9, 100e9, 401)
+            s.dft("reflected", f)
+            s.dft("transmitted", f)
+        timed("C3", plan, sim, post=c3_post)
+        out["C3"]["outputs"] = "reflected/transmitted plane sums, spectra x401"
+
+    if want("R5"):   # performance CPML at 400^3
+        plan = Plan.from_text(decks.roofline_deck(400, 220).text(), cpml_kappa_max=5.0, cpml_alpha_max=0.05)
+        sim = Simulation(plan)
+        sim.run(20)
+        t0 = time.perf_counter()
+        sim.run(200)
+        el = time.perf_counter() - t0
+        out["R400_kappa5"] = {"cells": 400 ** 3, "steps": 200, "gcells": round(400 ** 3 * 200 / el / 1e9, 3),
+                              "single_pass": sim.single_pass}
+        print(f"[bench] 400^3 with kappa_max 5, alpha_max 0.05: {out['R400_kappa5']['gcells']} Gcell/s",
+              file=sys.stderr, flush=True)
+        sim.close()
     return out
 
 
@@ -322,11 +327,18 @@ def main() -> int:
                     help="steps of the 64-member batched-population comparison (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the 160^3 design-gradient timing (0: skip)")
+    ap.add_argument("--leg", choices=["C1", "C2", "C3", "R5"], default=None,
+                    help="run only this configuration leg, as the first and only simulation of the process, and "
+                         "print its figures (one workload per profiled process)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this solver runs on one GPU; runs over several GPUs (z-slab sharding with one "
               "halo plane per half-step) are not built", file=sys.stderr)
         return 2
+
+    if args.leg:
+        print(json.dumps({"leg": args.leg, "configs": configs_bench(args.leg)}), flush=True)
+        return 0
 
     from alpineem_amd import decks
     from alpineem_amd.solver import Plan, Simulation

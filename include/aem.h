@@ -43,7 +43,7 @@ typedef enum aem_status {
     AEM_ERR_IO = 2,           /* file missing or unwritable */
     AEM_ERR_DEVICE = 3,       /* HIP runtime error */
     AEM_ERR_STATE = 4,        /* call out of order, bad handle or argument */
-    AEM_ERR_UNSUPPORTED = 5,  /* deck feature outside this library (kmax, SPICE, gridded ports, IGP) */
+    AEM_ERR_UNSUPPORTED = 5,  /* deck feature outside this library (gridded ports, IGP, kmax far field) */
     AEM_ERR_NUMERIC = 6       /* non-finite field detected by the watchdog */
 } aem_status;
 
@@ -58,12 +58,22 @@ typedef enum aem_series {
     AEM_SERIES_FF_PC = 7,         /* incident at the far-field phase centre (f90:8636-8642) */
     AEM_SERIES_SPICE_V = 8,       /* Spice_Voltage_out(idx,:) (f90:8250-8254): node voltage minus bias per
                                      SPICE port, then the raw voltage of each additional node */
-    AEM_SERIES_SPICE_I = 9        /* the gap current handed to the circuit per SPICE port (f90:5456-5496) */
+    AEM_SERIES_SPICE_I = 9,       /* the gap current handed to the circuit per SPICE port (f90:5456-5496) */
+    /* kmax build (plan option kmax): AEM_SERIES_REFLECTED / _TRANSMITTED hold the TE sums
+     * E_reflected_TE / E_transmitted_TE (f90:7105-7190, normalised f90:8411-8435), these the TM sums */
+    AEM_SERIES_REFLECTED_TM = 10,
+    AEM_SERIES_TRANSMITTED_TM = 11
 } aem_series;
 
 typedef enum aem_field {
     AEM_EX = 0, AEM_EY = 1, AEM_EZ = 2, AEM_HX = 3, AEM_HY = 4, AEM_HZ = 5
 } aem_field;
+
+/* A kmax simulation's fields and series are complex (the reference's kmax
+ * build declares them complex): a series kind or field component OR-ed with
+ * AEM_IMAG selects the imaginary part, without it the real part.  For any
+ * other simulation AEM_IMAG is an error (AEM_ERR_STATE). */
+#define AEM_IMAG 0x100
 
 typedef struct aem_plan aem_plan;   /* host-side setup result (no GPU needed) */
 typedef struct aem_sim  aem_sim;    /* device state of one simulation */

@@ -6,7 +6,11 @@
  *
  * Every region below restates one loop region of
  * main_fdtd/fdtd_solver.f90 (cited per function) for the non-kmax build,
- * with the SPICE build's gap exchange (the circuit itself is the caller's),
+ * with the SPICE build's gap exchange (the circuit itself is the caller's)
+ * and the kmax build (complex fields, Bloch-periodic walls, modal plane
+ * source: a kmax plan makes two states, the real and the imaginary part,
+ * stepped together by oracle_step; its sources are evaluated here from the
+ * plan's scalars, not from the setup's tables),
  * in the reference's sequential loop order and FP32 left-to-right operation
  * order.  Build with -ffp-contract=off.
  *
@@ -35,10 +39,22 @@ typedef ORACLE_REAL real;
 #if defined(ORACLE_FP64)
 #define EXP(x) exp(x)
 #define SQRT(x) sqrt(x)
+#define COS(x) cos(x)
+#define SIN(x) sin(x)
+#define ILOGB(x) ilogb(x)
+#define SCALBN(x, e) scalbn(x, e)
 #else
 #define EXP(x) expf(x)
 #define SQRT(x) sqrtf(x)
+#define COS(x) cosf(x)
+#define SIN(x) sinf(x)
+#define ILOGB(x) ilogbf(x)
+#define SCALBN(x, e) scalbnf(x, e)
 #endif
+/* pi, mu_0, eps_0 as the reference declares them (FP32, f90:24-26, 44) */
+#define PI_F 3.14159265358979323846f
+#define MU0_F 1.25663706E-6f
+#define EP0_F 8.85418782E-12f
 
 #define NFACE 6
 
@@ -77,6 +93,11 @@ struct oracle_state {
     real *incident, *Eth, *Eph;
     real *fftime;
     int finalized;
+    /* kmax: the TM plane sums (refl/trans hold TE), which part this state
+     * is (0 real, 1 imaginary) and, on the real part, the imaginary part */
+    real *refl2, *trans2;
+    int part;
+    struct oracle_state* kim;
 };
 
 /* ---- helpers ----------------------------------------------------------- */
@@ -93,6 +114,19 @@ static real* zalloc(int64_t n) {
     if (n <= 0) n = 1;
     return (real*)calloc((size_t)n, sizeof(real));
 }
+
+/* A complex value over a real one in the kmax build.  Fortran divides a
+ * complex by a real as complex by complex (the real promoted, imaginary part
+ * 0), and the flang build does that with the C99 complex division routine
+ * __divsc3: with c' = scalbn(c, -logb c) it forms (x c') / (c' c') and scales
+ * back, which is not x / c to the last bit.  Each part of the complex value
+ * goes through this formula on its own (the imaginary divisor part is 0). */
+static inline real cdivr(real x, real c) {
+    const int e = ILOGB(c);
+    const real cs = SCALBN(c, -e);
+    return SCALBN((x * cs) / (cs * cs), -e);
+}
+#define DIVR(st, x, d) ((st)->p.kmax ? cdivr((x), (d)) : (x) / (d))
 
 static inline int tidx(const oracle_state* s, int64_t o) {
     return s->p.idx_bytes == 1 ? (int)s->p.tuple_idx8[o] : (int)s->p.tuple_idx16[o];
@@ -112,17 +146,23 @@ static real inc_fn(const oracle_state* s, real w, real xf, real yf, real zf, rea
     return R(-1.0) * w * arg * EXP(R(0.5)) * EXP(R(-0.5) * (arg * arg));
 }
 
-/* E_or_V_1D() of f90:8844-8857 */
+/* E_or_V_1D() of f90:8844-8857; the kmax build's complex one (f90:8888-8900),
+ * amp exp(-x^2/2) exp(i 2 pi f_adj ((tc-1) dt - pos/c)), gives this state's part */
 static real e_or_v(const oracle_state* s, real amp, real tc, real pos) {
     const aem_plan_view* p = &s->p;
     real x = ((R(p->t_spread) - ((tc - R(1.0)) * R(p->dt))) - R(p->inv_c) * pos) / R(p->spread);
+    if (p->kmax) {
+        const real m = amp * EXP(R(-0.5) * (x * x));
+        const real th = R(2.0) * R(PI_F) * R(p->f_adj) * (((tc - R(1.0)) * R(p->dt)) - R(p->inv_c) * pos);
+        return s->part ? m * SIN(th) : m * COS(th);
+    }
     if (p->pulse_type == 1) return amp * EXP(R(-0.5) * (x * x));
     return amp * R(-1.0) * x * EXP(R(0.5)) * EXP(R(-0.5) * (x * x));
 }
 
 /* ---- lifetime ------------------------------------------------------------ */
 
-oracle_state* oracle_create(const aem_plan_view* plan) {
+static oracle_state* create_state(const aem_plan_view* plan) {
     oracle_state* s = (oracle_state*)calloc(1, sizeof(oracle_state));
     if (!s) return NULL;
     s->p = *plan;
@@ -197,11 +237,30 @@ oracle_state* oracle_create(const aem_plan_view* plan) {
     s->Eth = zalloc((int64_t)(p->n_angles > 0 ? p->n_angles : 1) * p->time_steps);
     s->Eph = zalloc((int64_t)(p->n_angles > 0 ? p->n_angles : 1) * p->time_steps);
     s->fftime = zalloc(p->n_angles > 0 ? p->n_angles : 1);
+    if (p->kmax) {
+        s->refl2 = zalloc(p->time_steps);
+        s->trans2 = zalloc(p->time_steps);
+    }
+    return s;
+}
+
+oracle_state* oracle_create(const aem_plan_view* plan) {
+    oracle_state* s = create_state(plan);
+    if (s && plan->kmax) {
+        s->kim = create_state(plan);
+        if (!s->kim) {
+            oracle_destroy(s);
+            return NULL;
+        }
+        s->kim->part = 1;
+    }
     return s;
 }
 
 void oracle_destroy(oracle_state* s) {
     if (!s) return;
+    oracle_destroy(s->kim);
+    free(s->refl2); free(s->trans2);
     for (int q = 0; q < 6; ++q) free(s->F[q]);
     for (int f = 0; f < NFACE; ++f)
         for (int c = 0; c < 2; ++c) {
@@ -386,7 +445,7 @@ static void h_ports(oracle_state* s, int n) {
                 for (int jj = l2; jj <= d2; ++jj) c = c + R(p->dy) * (HY(l1, jj, kk) - HY(d1 - 1, jj, kk));
             }
         }
-        s->I[(int64_t)q * M + 0] = c * R(pt->cur_num) / R(pt->cur_den);
+        s->I[(int64_t)q * M + 0] = DIVR(s, c * R(pt->cur_num), R(pt->cur_den));
         s->Iout[(int64_t)q * p->time_steps + (n - 1)] = s->I[(int64_t)q * M + 0];
     }
     for (int q = 0; q < p->n_ports; ++q) {
@@ -445,13 +504,13 @@ static void h_cpml(oracle_state* s) {
         for (int i = 1; i <= NX - 1; ++i) {
             for (int j = 1; j <= ny1 - 1; ++j) {
                 real* ps = psiH_at(s, 2, 0, j, i, k);
-                *ps = R(p->bh[2][j - 1]) * *ps + R(p->ch[2][j - 1]) * (EZ(i, j, k) - EZ(i, j + 1, k)) / dy;
+                *ps = R(p->bh[2][j - 1]) * *ps + DIVR(s, R(p->ch[2][j - 1]) * (EZ(i, j, k) - EZ(i, j + 1, k)), dy);
                 HX(i, j, k) = HX(i, j, k) + DB * *ps;
             }
             for (int j = NY + 1 - ny2; j <= NY - 1; ++j) {
                 int jj = NY - j;
                 real* ps = psiH_at(s, 3, 0, jj, i, k);
-                *ps = R(p->bh[3][jj - 1]) * *ps + R(p->ch[3][jj - 1]) * (EZ(i, j, k) - EZ(i, j + 1, k)) / dy;
+                *ps = R(p->bh[3][jj - 1]) * *ps + DIVR(s, R(p->ch[3][jj - 1]) * (EZ(i, j, k) - EZ(i, j + 1, k)), dy);
                 HX(i, j, k) = HX(i, j, k) + DB * *ps;
             }
         }
@@ -460,13 +519,13 @@ static void h_cpml(oracle_state* s) {
         for (int i = 1; i <= NX - 1; ++i) {
             for (int k = 1; k <= nz1 - 1; ++k) {
                 real* ps = psiH_at(s, 4, 0, k, i, j);
-                *ps = R(p->bh[4][k - 1]) * *ps + R(p->ch[4][k - 1]) * (EY(i, j, k + 1) - EY(i, j, k)) / dz;
+                *ps = R(p->bh[4][k - 1]) * *ps + DIVR(s, R(p->ch[4][k - 1]) * (EY(i, j, k + 1) - EY(i, j, k)), dz);
                 HX(i, j, k) = HX(i, j, k) + DB * *ps;
             }
             for (int k = NZ + 1 - nz2; k <= NZ - 1; ++k) {
                 int kk = NZ - k;
                 real* ps = psiH_at(s, 5, 0, kk, i, j);
-                *ps = R(p->bh[5][kk - 1]) * *ps + R(p->ch[5][kk - 1]) * (EY(i, j, k + 1) - EY(i, j, k)) / dz;
+                *ps = R(p->bh[5][kk - 1]) * *ps + DIVR(s, R(p->ch[5][kk - 1]) * (EY(i, j, k + 1) - EY(i, j, k)), dz);
                 HX(i, j, k) = HX(i, j, k) + DB * *ps;
             }
         }
@@ -475,13 +534,13 @@ static void h_cpml(oracle_state* s) {
         for (int j = 1; j <= NY - 1; ++j) {
             for (int i = 1; i <= nx1 - 1; ++i) {
                 real* ps = psiH_at(s, 0, 0, i, j, k);
-                *ps = R(p->bh[0][i - 1]) * *ps + R(p->ch[0][i - 1]) * (EZ(i + 1, j, k) - EZ(i, j, k)) / dx;
+                *ps = R(p->bh[0][i - 1]) * *ps + DIVR(s, R(p->ch[0][i - 1]) * (EZ(i + 1, j, k) - EZ(i, j, k)), dx);
                 HY(i, j, k) = HY(i, j, k) + DB * *ps;
             }
             for (int i = NX + 1 - nx2; i <= NX - 1; ++i) {
                 int ii = NX - i;
                 real* ps = psiH_at(s, 1, 0, ii, j, k);
-                *ps = R(p->bh[1][ii - 1]) * *ps + R(p->ch[1][ii - 1]) * (EZ(i + 1, j, k) - EZ(i, j, k)) / dx;
+                *ps = R(p->bh[1][ii - 1]) * *ps + DIVR(s, R(p->ch[1][ii - 1]) * (EZ(i + 1, j, k) - EZ(i, j, k)), dx);
                 HY(i, j, k) = HY(i, j, k) + DB * *ps;
             }
         }
@@ -490,13 +549,13 @@ static void h_cpml(oracle_state* s) {
         for (int i = 1; i <= NX - 1; ++i) {
             for (int k = 1; k <= nz1 - 1; ++k) {
                 real* ps = psiH_at(s, 4, 1, k, i, j);
-                *ps = R(p->bh[4][k - 1]) * *ps + R(p->ch[4][k - 1]) * (EX(i, j, k) - EX(i, j, k + 1)) / dz;
+                *ps = R(p->bh[4][k - 1]) * *ps + DIVR(s, R(p->ch[4][k - 1]) * (EX(i, j, k) - EX(i, j, k + 1)), dz);
                 HY(i, j, k) = HY(i, j, k) + DB * *ps;
             }
             for (int k = NZ + 1 - nz2; k <= NZ - 1; ++k) {
                 int kk = NZ - k;
                 real* ps = psiH_at(s, 5, 1, kk, i, j);
-                *ps = R(p->bh[5][kk - 1]) * *ps + R(p->ch[5][kk - 1]) * (EX(i, j, k) - EX(i, j, k + 1)) / dz;
+                *ps = R(p->bh[5][kk - 1]) * *ps + DIVR(s, R(p->ch[5][kk - 1]) * (EX(i, j, k) - EX(i, j, k + 1)), dz);
                 HY(i, j, k) = HY(i, j, k) + DB * *ps;
             }
         }
@@ -505,13 +564,13 @@ static void h_cpml(oracle_state* s) {
         for (int j = 1; j <= NY - 1; ++j) {
             for (int i = 1; i <= nx1 - 1; ++i) {
                 real* ps = psiH_at(s, 0, 1, i, j, k);
-                *ps = R(p->bh[0][i - 1]) * *ps + R(p->ch[0][i - 1]) * (EY(i, j, k) - EY(i + 1, j, k)) / dx;
+                *ps = R(p->bh[0][i - 1]) * *ps + DIVR(s, R(p->ch[0][i - 1]) * (EY(i, j, k) - EY(i + 1, j, k)), dx);
                 HZ(i, j, k) = HZ(i, j, k) + DB * *ps;
             }
             for (int i = NX + 1 - nx2; i <= NX - 1; ++i) {
                 int ii = NX - i;
                 real* ps = psiH_at(s, 1, 1, ii, j, k);
-                *ps = R(p->bh[1][ii - 1]) * *ps + R(p->ch[1][ii - 1]) * (EY(i, j, k) - EY(i + 1, j, k)) / dx;
+                *ps = R(p->bh[1][ii - 1]) * *ps + DIVR(s, R(p->ch[1][ii - 1]) * (EY(i, j, k) - EY(i + 1, j, k)), dx);
                 HZ(i, j, k) = HZ(i, j, k) + DB * *ps;
             }
         }
@@ -520,13 +579,13 @@ static void h_cpml(oracle_state* s) {
         for (int i = 1; i <= NX - 1; ++i) {
             for (int j = 1; j <= ny1 - 1; ++j) {
                 real* ps = psiH_at(s, 2, 1, j, i, k);
-                *ps = R(p->bh[2][j - 1]) * *ps + R(p->ch[2][j - 1]) * (EX(i, j + 1, k) - EX(i, j, k)) / dy;
+                *ps = R(p->bh[2][j - 1]) * *ps + DIVR(s, R(p->ch[2][j - 1]) * (EX(i, j + 1, k) - EX(i, j, k)), dy);
                 HZ(i, j, k) = HZ(i, j, k) + DB * *ps;
             }
             for (int j = NY + 1 - ny2; j <= NY - 1; ++j) {
                 int jj = NY - j;
                 real* ps = psiH_at(s, 3, 1, jj, i, k);
-                *ps = R(p->bh[3][jj - 1]) * *ps + R(p->ch[3][jj - 1]) * (EX(i, j + 1, k) - EX(i, j, k)) / dy;
+                *ps = R(p->bh[3][jj - 1]) * *ps + DIVR(s, R(p->ch[3][jj - 1]) * (EX(i, j + 1, k) - EX(i, j, k)), dy);
                 HZ(i, j, k) = HZ(i, j, k) + DB * *ps;
             }
         }
@@ -714,7 +773,7 @@ static void e_ports(oracle_state* s, int n) {
         int l1 = pt->loc[0], l2 = pt->loc[1], l3 = pt->loc[2];
         int d1 = pt->size[0] + l1 - 1, d2 = pt->size[1] + l2 - 1, d3 = pt->size[2] + l3 - 1;
         real* F = pt->dir == 0 ? s->Ex : (pt->dir == 1 ? s->Ey : s->Ez);
-        real val = R(1.0) * V[1] / R(pt->gap_div);
+        real val = DIVR(s, R(1.0) * V[1], R(pt->gap_div));
         for (int ii = l1; ii <= d1; ++ii)
             for (int jj = l2; jj <= d2; ++jj)
                 for (int kk = l3; kk <= d3; ++kk) F[IDX(s, ii, jj, kk)] = val;
@@ -751,13 +810,13 @@ static void e_cpml(oracle_state* s) {
         for (int i = 1; i <= NX - 1; ++i) {
             for (int j = 2; j <= ny1; ++j) {
                 real* ps = psiE_at(s, 2, 0, j, i, k);
-                *ps = R(p->be[2][j - 1]) * *ps + R(p->ce[2][j - 1]) * (HZ(i, j, k) - HZ(i, j - 1, k)) / dy;
+                *ps = R(p->be[2][j - 1]) * *ps + DIVR(s, R(p->ce[2][j - 1]) * (HZ(i, j, k) - HZ(i, j - 1, k)), dy);
                 EX(i, j, k) = EX(i, j, k) + GB(s, 0, IDX(s, i, j, k)) * *ps;
             }
             for (int j = NY + 1 - ny2; j <= NY - 1; ++j) {
                 int jj = NY + 1 - j;
                 real* ps = psiE_at(s, 3, 0, jj, i, k);
-                *ps = R(p->be[3][jj - 1]) * *ps + R(p->ce[3][jj - 1]) * (HZ(i, j, k) - HZ(i, j - 1, k)) / dy;
+                *ps = R(p->be[3][jj - 1]) * *ps + DIVR(s, R(p->ce[3][jj - 1]) * (HZ(i, j, k) - HZ(i, j - 1, k)), dy);
                 EX(i, j, k) = EX(i, j, k) + GB(s, 0, IDX(s, i, j, k)) * *ps;
             }
         }
@@ -766,13 +825,13 @@ static void e_cpml(oracle_state* s) {
         for (int i = 1; i <= NX - 1; ++i) {
             for (int k = 2; k <= nz1; ++k) {
                 real* ps = psiE_at(s, 4, 0, k, i, j);
-                *ps = R(p->be[4][k - 1]) * *ps + R(p->ce[4][k - 1]) * (HY(i, j, k - 1) - HY(i, j, k)) / dz;
+                *ps = R(p->be[4][k - 1]) * *ps + DIVR(s, R(p->ce[4][k - 1]) * (HY(i, j, k - 1) - HY(i, j, k)), dz);
                 EX(i, j, k) = EX(i, j, k) + GB(s, 0, IDX(s, i, j, k)) * *ps;
             }
             for (int k = NZ + 1 - nz2; k <= NZ - 1; ++k) {
                 int kk = NZ + 1 - k;
                 real* ps = psiE_at(s, 5, 0, kk, i, j);
-                *ps = R(p->be[5][kk - 1]) * *ps + R(p->ce[5][kk - 1]) * (HY(i, j, k - 1) - HY(i, j, k)) / dz;
+                *ps = R(p->be[5][kk - 1]) * *ps + DIVR(s, R(p->ce[5][kk - 1]) * (HY(i, j, k - 1) - HY(i, j, k)), dz);
                 EX(i, j, k) = EX(i, j, k) + GB(s, 0, IDX(s, i, j, k)) * *ps;
             }
         }
@@ -781,13 +840,13 @@ static void e_cpml(oracle_state* s) {
         for (int j = 1; j <= NY - 1; ++j) {
             for (int i = 2; i <= nx1; ++i) {
                 real* ps = psiE_at(s, 0, 0, i, j, k);
-                *ps = R(p->be[0][i - 1]) * *ps + R(p->ce[0][i - 1]) * (HZ(i - 1, j, k) - HZ(i, j, k)) / dx;
+                *ps = R(p->be[0][i - 1]) * *ps + DIVR(s, R(p->ce[0][i - 1]) * (HZ(i - 1, j, k) - HZ(i, j, k)), dx);
                 EY(i, j, k) = EY(i, j, k) + GB(s, 1, IDX(s, i, j, k)) * *ps;
             }
             for (int i = NX + 1 - nx2; i <= NX - 1; ++i) {
                 int ii = NX + 1 - i;
                 real* ps = psiE_at(s, 1, 0, ii, j, k);
-                *ps = R(p->be[1][ii - 1]) * *ps + R(p->ce[1][ii - 1]) * (HZ(i - 1, j, k) - HZ(i, j, k)) / dx;
+                *ps = R(p->be[1][ii - 1]) * *ps + DIVR(s, R(p->ce[1][ii - 1]) * (HZ(i - 1, j, k) - HZ(i, j, k)), dx);
                 EY(i, j, k) = EY(i, j, k) + GB(s, 1, IDX(s, i, j, k)) * *ps;
             }
         }
@@ -796,13 +855,13 @@ static void e_cpml(oracle_state* s) {
         for (int i = 2; i <= NX - 1; ++i) {
             for (int k = 2; k <= nz1; ++k) {
                 real* ps = psiE_at(s, 4, 1, k, i, j);
-                *ps = R(p->be[4][k - 1]) * *ps + R(p->ce[4][k - 1]) * (HX(i, j, k) - HX(i, j, k - 1)) / dz;
+                *ps = R(p->be[4][k - 1]) * *ps + DIVR(s, R(p->ce[4][k - 1]) * (HX(i, j, k) - HX(i, j, k - 1)), dz);
                 EY(i, j, k) = EY(i, j, k) + GB(s, 1, IDX(s, i, j, k)) * *ps;
             }
             for (int k = NZ + 1 - nz2; k <= NZ - 1; ++k) {
                 int kk = NZ + 1 - k;
                 real* ps = psiE_at(s, 5, 1, kk, i, j);
-                *ps = R(p->be[5][kk - 1]) * *ps + R(p->ce[5][kk - 1]) * (HX(i, j, k) - HX(i, j, k - 1)) / dz;
+                *ps = R(p->be[5][kk - 1]) * *ps + DIVR(s, R(p->ce[5][kk - 1]) * (HX(i, j, k) - HX(i, j, k - 1)), dz);
                 EY(i, j, k) = EY(i, j, k) + GB(s, 1, IDX(s, i, j, k)) * *ps;
             }
         }
@@ -811,13 +870,13 @@ static void e_cpml(oracle_state* s) {
         for (int j = 2; j <= NY - 1; ++j) {
             for (int i = 2; i <= nx1; ++i) {
                 real* ps = psiE_at(s, 0, 1, i, j, k);
-                *ps = R(p->be[0][i - 1]) * *ps + R(p->ce[0][i - 1]) * (HY(i, j, k) - HY(i - 1, j, k)) / dx;
+                *ps = R(p->be[0][i - 1]) * *ps + DIVR(s, R(p->ce[0][i - 1]) * (HY(i, j, k) - HY(i - 1, j, k)), dx);
                 EZ(i, j, k) = EZ(i, j, k) + GB(s, 2, IDX(s, i, j, k)) * *ps;
             }
             for (int i = NX + 1 - nx2; i <= NX - 1; ++i) {
                 int ii = NX + 1 - i;
                 real* ps = psiE_at(s, 1, 1, ii, j, k);
-                *ps = R(p->be[1][ii - 1]) * *ps + R(p->ce[1][ii - 1]) * (HY(i, j, k) - HY(i - 1, j, k)) / dx;
+                *ps = R(p->be[1][ii - 1]) * *ps + DIVR(s, R(p->ce[1][ii - 1]) * (HY(i, j, k) - HY(i - 1, j, k)), dx);
                 EZ(i, j, k) = EZ(i, j, k) + GB(s, 2, IDX(s, i, j, k)) * *ps;
             }
         }
@@ -826,13 +885,13 @@ static void e_cpml(oracle_state* s) {
         for (int i = 2; i <= NX - 1; ++i) {
             for (int j = 2; j <= ny1; ++j) {
                 real* ps = psiE_at(s, 2, 1, j, i, k);
-                *ps = R(p->be[2][j - 1]) * *ps + R(p->ce[2][j - 1]) * (HX(i, j - 1, k) - HX(i, j, k)) / dy;
+                *ps = R(p->be[2][j - 1]) * *ps + DIVR(s, R(p->ce[2][j - 1]) * (HX(i, j - 1, k) - HX(i, j, k)), dy);
                 EZ(i, j, k) = EZ(i, j, k) + GB(s, 2, IDX(s, i, j, k)) * *ps;
             }
             for (int j = NY + 1 - ny2; j <= NY - 1; ++j) {
                 int jj = NY + 1 - j;
                 real* ps = psiE_at(s, 3, 1, jj, i, k);
-                *ps = R(p->be[3][jj - 1]) * *ps + R(p->ce[3][jj - 1]) * (HX(i, j - 1, k) - HX(i, j, k)) / dy;
+                *ps = R(p->be[3][jj - 1]) * *ps + DIVR(s, R(p->ce[3][jj - 1]) * (HX(i, j - 1, k) - HX(i, j, k)), dy);
                 EZ(i, j, k) = EZ(i, j, k) + GB(s, 2, IDX(s, i, j, k)) * *ps;
             }
         }
@@ -1120,6 +1179,264 @@ static void out_phase_centre(oracle_state* s, int n) {
     }
 }
 
+/* ---- kmax build -------------------------------------------------------------- */
+
+/* Inc() of the kmax build, f90:8860-8885: w exp(-g^2/2) exp(i (2 pi f_adj
+ * (t + x d/c) - (k1 p1 + k2 p2))), g = (t_spread - t + x d / c) / spread,
+ * t = cf dt; the complex value as (*re, *im). */
+static void inc_kmax(const aem_plan_view* p, real w, real xf, real k1, real p1, real k2, real p2, real cf, real dn,
+                     real* re, real* im) {
+    const real g = (R(p->t_spread) - (cf * R(p->dt)) + R(p->inv_c) * (xf * dn)) / R(p->spread);
+    const real a1 = w * EXP(R(-0.5) * (g * g));
+    const real th = R(2.0) * R(PI_F) * R(p->f_adj) * ((cf * R(p->dt)) + R(p->inv_c) * (xf * dn)) - (k1 * p1 + k2 * p2);
+    *re = a1 * COS(th);
+    *im = a1 * SIN(th);
+}
+
+/* F += c0 Inc exc on both parts of a kmax pair */
+#define KADD(FLD, o, c0, ir, ii, exc)                                        \
+    do {                                                                     \
+        s->FLD[o] = s->FLD[o] + (c0) * (ir) * (exc);                         \
+        m->FLD[o] = m->FLD[o] + (c0) * (ii) * (exc);                         \
+    } while (0)
+
+/* TE source on the H plane, f90:4558-4625 (time n-1, x = 0) */
+static void kmax_src_h(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    oracle_state* m = s->kim;
+    if (!(p->mode_type == 0 && p->plane_wave_amp > 0)) return;
+    const real cf = R(n) - R(1.0), x0 = R(0.0);
+    const real kx = R(p->k_vec[0]), ky = R(p->k_vec[1]), kz = R(p->k_vec[2]);
+    const real ex = R(p->k_exc[0]), ey = R(p->k_exc[1]), ez = R(p->k_exc[2]);
+    const real dx = R(p->dx), dy = R(p->dy), dz = R(p->dz);
+    const int kp = p->kpl_H;
+    real ir, ii;
+    if (p->pbc[1] + p->pbc[2] == 2) {
+        const real c0 = R(p->dt) / (R(MU0_F) * dx);
+        for (int k = p->tf_lo[2]; k <= p->tf_hi[2]; ++k)
+            for (int j = p->tf_lo[1]; j <= p->tf_hi[1]; ++j) {
+                const int64_t o = IDX(s, kp, j, k);
+                inc_kmax(p, R(p->WE[1]), x0, ky, (R(j) + R(0.5)) * dy, kz, R(k) * dz, cf, dx, &ir, &ii);
+                KADD(Hz, o, c0, ir, ii, ez);
+                inc_kmax(p, R(p->WE[2]), x0, ky, R(j) * dy, kz, (R(k) + R(0.5)) * dz, cf, dx, &ir, &ii);
+                KADD(Hy, o, c0, ir, ii, ey);
+            }
+    }
+    if (p->pbc[0] + p->pbc[2] == 2) {
+        const real c0 = R(p->dt) / (R(MU0_F) * dy);
+        for (int k = p->tf_lo[2]; k <= p->tf_hi[2]; ++k)
+            for (int i = p->tf_lo[0]; i <= p->tf_hi[0]; ++i) {
+                const int64_t o = IDX(s, i, kp, k);
+                inc_kmax(p, R(p->WE[0]), x0, kx, (R(i) + R(0.5)) * dx, kz, R(k) * dz, cf, dy, &ir, &ii);
+                KADD(Hz, o, c0, ir, ii, ez);
+                inc_kmax(p, R(p->WE[2]), x0, kx, R(i) * dx, kz, (R(k) + R(0.5)) * dz, cf, dy, &ir, &ii);
+                KADD(Hx, o, c0, ir, ii, ex);
+            }
+    }
+    if (p->pbc[1] + p->pbc[0] == 2) {
+        const real c0 = R(p->dt) / (R(MU0_F) * dz);
+        for (int j = p->tf_lo[1]; j <= p->tf_hi[1]; ++j)
+            for (int i = p->tf_lo[0]; i <= p->tf_hi[0]; ++i) {
+                const int64_t o = IDX(s, i, j, kp);
+                inc_kmax(p, R(p->WE[1]), x0, ky, (R(j) + R(0.5)) * dy, kx, R(i) * dx, cf, dz, &ir, &ii);
+                KADD(Hx, o, c0, ir, ii, ex);
+                inc_kmax(p, R(p->WE[0]), x0, ky, R(j) * dy, kx, (R(i) + R(0.5)) * dx, cf, dz, &ir, &ii);
+                KADD(Hy, o, c0, ir, ii, ey);
+            }
+    }
+}
+
+/* TM source on the E plane, f90:5857-5948 (time n-1/2, x = -1/2) */
+static void kmax_src_e(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    oracle_state* m = s->kim;
+    if (!(p->mode_type == 1 && p->plane_wave_amp > 0)) return;
+    const int NX = p->NX, NY = p->NY, NZ = p->NZ;
+    const real cf = R(n) - R(0.5), xh = R(-0.5);
+    const real kx = R(p->k_vec[0]), ky = R(p->k_vec[1]), kz = R(p->k_vec[2]);
+    const real ex = R(p->k_exc[0]), ey = R(p->k_exc[1]), ez = R(p->k_exc[2]);
+    const real dx = R(p->dx), dy = R(p->dy), dz = R(p->dz);
+    const int kp = p->kpl_E;
+    real ir, ii;
+    if (p->pbc[1] + p->pbc[2] == 2) {
+        const real c0 = R(p->dt) / (R(EP0_F) * dx);
+        for (int k = 1; k <= NZ - 1; ++k)
+            for (int j = 2; j <= NY - 1; ++j) {
+                inc_kmax(p, R(p->WH[1]), xh, ky, R(j) * dy, kz, (R(k) + R(0.5)) * dz, cf, dx, &ir, &ii);
+                KADD(Ez, IDX(s, kp, j, k), c0, ir, ii, ez);
+            }
+        for (int k = 2; k <= NZ - 1; ++k)
+            for (int j = 1; j <= NY - 1; ++j) {
+                inc_kmax(p, R(p->WH[2]), xh, ky, (R(j) + R(0.5)) * dy, kz, R(k) * dz, cf, dx, &ir, &ii);
+                KADD(Ey, IDX(s, kp, j, k), c0, ir, ii, ey);
+            }
+    }
+    if (p->pbc[0] + p->pbc[2] == 2) {
+        const real c0 = R(p->dt) / (R(EP0_F) * dy);
+        for (int k = 1; k <= NZ - 1; ++k)
+            for (int i = 2; i <= NX - 1; ++i) {
+                inc_kmax(p, R(p->WH[0]), xh, kx, R(i) * dx, kz, (R(k) + R(0.5)) * dz, cf, dy, &ir, &ii);
+                KADD(Ez, IDX(s, i, kp, k), c0, ir, ii, ez);
+            }
+        for (int k = 2; k <= NZ - 1; ++k)
+            for (int i = 1; i <= NX - 1; ++i) {
+                inc_kmax(p, R(p->WH[2]), xh, kx, (R(i) + R(0.5)) * dx, kz, R(k) * dz, cf, dy, &ir, &ii);
+                KADD(Ex, IDX(s, i, kp, k), c0, ir, ii, ex);
+            }
+    }
+    if (p->pbc[1] + p->pbc[0] == 2) {
+        const real c0 = R(p->dt) / (R(EP0_F) * dz);
+        for (int j = 2; j <= NY - 1; ++j)
+            for (int i = 1; i <= NX - 1; ++i) {
+                inc_kmax(p, R(p->WH[1]), xh, ky, R(j) * dy, kx, (R(i) + R(0.5)) * dx, cf, dz, &ir, &ii);
+                KADD(Ex, IDX(s, i, j, kp), c0, ir, ii, ex);
+            }
+        for (int j = 1; j <= NY - 1; ++j)
+            for (int i = 2; i <= NX - 1; ++i) {
+                inc_kmax(p, R(p->WH[0]), xh, ky, (R(j) + R(0.5)) * dy, kx, R(i) * dx, cf, dz, &ir, &ii);
+                KADD(Ey, IDX(s, i, j, kp), c0, ir, ii, ey);
+            }
+    }
+}
+
+/* dst = src * (c + i sn) on both parts */
+#define BLOCH(FLD, od, os, c, sn)                                            \
+    do {                                                                     \
+        const real a_ = s->FLD[os], b_ = m->FLD[os];                         \
+        s->FLD[od] = a_ * (c) - b_ * (sn);                                   \
+        m->FLD[od] = a_ * (sn) + b_ * (c);                                   \
+    } while (0)
+
+/* Bloch copies of H, f90:5387-5428: the last cell from the first times
+ * exp(-i k (N-2) d) */
+static void kmax_h_pbc(oracle_state* s) {
+    const aem_plan_view* p = &s->p;
+    oracle_state* m = s->kim;
+    const int NX = p->NX, NY = p->NY, NZ = p->NZ;
+    if (p->pbc[0]) {
+        const real th = R(p->k_vec[0]) * R(NX - 2) * R(p->dx), c = COS(-th), sn = SIN(-th);
+        for (int k = 1; k <= NZ - 1; ++k)
+            for (int j = 1; j <= NY - 1; ++j) {
+                BLOCH(Hz, IDX(s, NX - 1, j, k), IDX(s, 1, j, k), c, sn);
+                BLOCH(Hy, IDX(s, NX - 1, j, k), IDX(s, 1, j, k), c, sn);
+            }
+    }
+    if (p->pbc[1]) {
+        const real th = R(p->k_vec[1]) * R(NY - 2) * R(p->dy), c = COS(-th), sn = SIN(-th);
+        for (int k = 1; k <= NZ - 1; ++k)
+            for (int i = 1; i <= NX - 1; ++i) {
+                BLOCH(Hz, IDX(s, i, NY - 1, k), IDX(s, i, 1, k), c, sn);
+                BLOCH(Hx, IDX(s, i, NY - 1, k), IDX(s, i, 1, k), c, sn);
+            }
+    }
+    if (p->pbc[2]) {
+        const real th = R(p->k_vec[2]) * R(NZ - 2) * R(p->dz), c = COS(-th), sn = SIN(-th);
+        for (int j = 1; j <= NY - 1; ++j)
+            for (int i = 1; i <= NX - 1; ++i) {
+                BLOCH(Hy, IDX(s, i, j, NZ - 1), IDX(s, i, j, 1), c, sn);
+                BLOCH(Hx, IDX(s, i, j, NZ - 1), IDX(s, i, j, 1), c, sn);
+            }
+    }
+}
+
+/* Bloch copies of E, f90:6685-6736: the first cell from the last times
+ * exp(+i k (N-2) d).  (The J_source copies of f90:6742-6806 write entries
+ * that are never read, as in the ordinary build.) */
+static void kmax_e_pbc(oracle_state* s) {
+    const aem_plan_view* p = &s->p;
+    oracle_state* m = s->kim;
+    const int NX = p->NX, NY = p->NY, NZ = p->NZ;
+    if (p->pbc[0]) {
+        const real th = R(p->k_vec[0]) * R(NX - 2) * R(p->dx), c = COS(th), sn = SIN(th);
+        for (int k = 1; k <= NZ - 1; ++k)
+            for (int j = 1; j <= NY - 1; ++j) {
+                BLOCH(Ez, IDX(s, 1, j, k), IDX(s, NX - 1, j, k), c, sn);
+                BLOCH(Ey, IDX(s, 1, j, k), IDX(s, NX - 1, j, k), c, sn);
+            }
+    }
+    if (p->pbc[1]) {
+        const real th = R(p->k_vec[1]) * R(NY - 2) * R(p->dy), c = COS(th), sn = SIN(th);
+        for (int k = 1; k <= NZ - 1; ++k)
+            for (int i = 1; i <= NX - 1; ++i) {
+                BLOCH(Ez, IDX(s, i, 1, k), IDX(s, i, NY - 1, k), c, sn);
+                BLOCH(Ex, IDX(s, i, 1, k), IDX(s, i, NY - 1, k), c, sn);
+            }
+    }
+    if (p->pbc[2]) {
+        const real th = R(p->k_vec[2]) * R(NZ - 2) * R(p->dz), c = COS(th), sn = SIN(th);
+        for (int j = 1; j <= NY - 1; ++j)
+            for (int i = 1; i <= NX - 1; ++i) {
+                BLOCH(Ey, IDX(s, i, j, 1), IDX(s, i, j, NZ - 1), c, sn);
+                BLOCH(Ex, IDX(s, i, j, 1), IDX(s, i, j, NZ - 1), c, sn);
+            }
+    }
+}
+
+/* TE/TM plane sums of one part, f90:7105-7190 */
+static void kmax_plane_sums(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    const real ex = R(p->k_exc[0]), ey = R(p->k_exc[1]), ez = R(p->k_exc[2]);
+    real* rte = &s->refl[n - 1];
+    real* rtm = &s->refl2[n - 1];
+    real* tte = &s->trans[n - 1];
+    real* ttm = &s->trans2[n - 1];
+    if (p->pbc[1] + p->pbc[2] == 2) {
+        const int xl = p->tf_lo[0], xh = p->tf_hi[0];
+        for (int k = p->ff_lo[2]; k <= p->ff_hi[2]; ++k)
+            for (int j = p->ff_lo[1]; j <= p->ff_hi[1]; ++j) {
+                *rte = *rte + (EZ(xl, j, k) * ey - EY(xl, j, k) * ez);
+                *rtm = *rtm + (EZ(xl, j, k) * ez + EY(xl, j, k) * ey);
+                *tte = *tte + (EZ(xh, j, k) * ey - EY(xh, j, k) * ez);
+                *ttm = *ttm + (EZ(xh, j, k) * ez + EY(xh, j, k) * ey);
+            }
+    }
+    if (p->pbc[0] + p->pbc[2] == 2) {
+        const int yl = p->tf_lo[1], yh = p->tf_hi[1];
+        for (int k = p->ff_lo[2]; k <= p->ff_hi[2]; ++k)
+            for (int i = p->ff_lo[0]; i <= p->ff_hi[0]; ++i) {
+                *rte = *rte + (EZ(i, yl, k) * ex - EX(i, yl, k) * ez);
+                *rtm = *rtm + (EZ(i, yl, k) * ez + EX(i, yl, k) * ex);
+                *tte = *tte + (EZ(i, yh, k) * ex - EX(i, yh, k) * ez);
+                *ttm = *ttm + (EZ(i, yh, k) * ez + EX(i, yh, k) * ex);
+            }
+    }
+    if (p->pbc[1] + p->pbc[0] == 2) {
+        const int zl = p->tf_lo[2], zh = p->tf_hi[2];
+        for (int j = p->ff_lo[1]; j <= p->ff_hi[1]; ++j)
+            for (int i = p->ff_lo[0]; i <= p->ff_hi[0]; ++i) {
+                *rte = *rte + (EX(i, j, zl) * ey - EY(i, j, zl) * ex);
+                *rtm = *rtm + (EX(i, j, zl) * ex + EY(i, j, zl) * ey);
+                *tte = *tte + (EX(i, j, zh) * ey - EY(i, j, zh) * ex);
+                *ttm = *ttm + (EX(i, j, zh) * ex + EY(i, j, zh) * ey);
+            }
+    }
+}
+
+/* One kmax step of the pair (s = real part): each part's ordinary updates,
+ * the modal source after the sheets/Drude updates, the Bloch copies in
+ * place of the periodic copies. */
+static void kmax_step(oracle_state* s, int n) {
+    oracle_state* m = s->kim;
+    oracle_state* part[2] = {s, m};
+    for (int q = 0; q < 2; ++q) { h_bulk(part[q]); h_sheets(part[q]); }
+    kmax_src_h(s, n);
+    for (int q = 0; q < 2; ++q) { h_ports(part[q], n); h_cpml(part[q]); }
+    kmax_h_pbc(s);
+    for (int q = 0; q < 2; ++q) { e_bulk(part[q]); e_sheets(part[q]); e_drude(part[q]); }
+    kmax_src_e(s, n);
+    for (int q = 0; q < 2; ++q) { e_ports(part[q], n); e_cpml(part[q]); }
+    kmax_e_pbc(s);
+    for (int q = 0; q < 2; ++q) {
+        oracle_state* t = part[q];
+        const aem_plan_view* p = &t->p;
+        e_oldt(t);
+        kmax_plane_sums(t, n);
+        for (int pi = 0; pi < p->n_ports; ++pi)
+            t->Vout[(int64_t)pi * p->time_steps + (n - 1)] = t->V[(int64_t)pi * p->M + 1];
+        t->n = n;
+    }
+}
+
 /* The step splits at the SPICE exchange (f90:5435-5665): the H phase and
  * the gap currents, then the E phase with the circuit's gap voltages and
  * the per-step outputs. */
@@ -1160,6 +1477,10 @@ int oracle_step(oracle_state* s, int nsteps) {
     for (int q = 0; q < nsteps; ++q) {
         if (s->n >= p->time_steps) return -1;
         const int n = ++s->n;
+        if (s->kim) {
+            kmax_step(s, n);
+            continue;
+        }
         step_to_exchange(s, n);
         step_from_exchange(s, n);
     }
@@ -1195,13 +1516,16 @@ int oracle_finalize(oracle_state* s) {
     const int T = p->time_steps;
     if (s->finalized) return 0;
     s->finalized = 1;
-    /* normalisation of the plane sums (f90:8342-8361) */
+    if (s->kim) oracle_finalize(s->kim);
+    /* normalisation of the plane sums (f90:8342-8361; kmax TE and TM, f90:8411-8435) */
     if (p->plane_sum_axis >= 0) {
         real den;
         if (p->plane_sum_axis == 0) den = (R(p->NY) - R(2.0)) * (R(p->NZ) - R(2.0));
         else if (p->plane_sum_axis == 1) den = (R(p->NX) - R(2.0)) * (R(p->NZ) - R(2.0));
         else den = (R(p->NY) - R(2.0)) * (R(p->NX) - R(2.0));
-        for (int n = 0; n < T; ++n) { s->refl[n] = s->refl[n] / den; s->trans[n] = s->trans[n] / den; }
+        for (int n = 0; n < T; ++n) { s->refl[n] = DIVR(s, s->refl[n], den); s->trans[n] = DIVR(s, s->trans[n], den); }
+        if (p->kmax)
+            for (int n = 0; n < T; ++n) { s->refl2[n] = cdivr(s->refl2[n], den); s->trans2[n] = cdivr(s->trans2[n], den); }
     }
     /* incident wave (f90:8366-8408) */
     if (p->plane_wave_amp > 0)
@@ -1261,25 +1585,41 @@ int oracle_finalize(oracle_state* s) {
 
 /* ---- accessors -------------------------------------------------------------- */
 
+/* the part of a kmax pair a component or series kind OR-ed with 0x100 selects */
+#define ORACLE_IMAG 0x100
+static const oracle_state* part_of(const oracle_state* s, int* sel) {
+    if (!(*sel & ORACLE_IMAG)) return s;
+    *sel &= ~ORACLE_IMAG;
+    return s->kim;
+}
+
 int oracle_get_field(const oracle_state* s, int comp, float* out, int64_t n) {
-    if (comp < 0 || comp > 5 || n > s->nc) return -1;
+    s = part_of(s, &comp);
+    if (!s || comp < 0 || comp > 5 || n > s->nc) return -1;
     for (int64_t q = 0; q < n; ++q) out[q] = (float)s->F[comp][q];
     return 0;
 }
 
 int oracle_get_field64(const oracle_state* s, int comp, double* out, int64_t n) {
-    if (comp < 0 || comp > 5 || n > s->nc) return -1;
+    s = part_of(s, &comp);
+    if (!s || comp < 0 || comp > 5 || n > s->nc) return -1;
     for (int64_t q = 0; q < n; ++q) out[q] = (double)s->F[comp][q];
     return 0;
 }
 
 int oracle_set_field(oracle_state* s, int comp, const float* in, int64_t n) {
-    if (comp < 0 || comp > 5 || n > s->nc) return -1;
+    if (comp & ORACLE_IMAG) {
+        s = s->kim;
+        comp &= ~ORACLE_IMAG;
+    }
+    if (!s || comp < 0 || comp > 5 || n > s->nc) return -1;
     for (int64_t q = 0; q < n; ++q) s->F[comp][q] = (real)in[q];
     return 0;
 }
 
 int oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int64_t n) {
+    s = part_of(s, &kind);
+    if (!s) return -1;
     const aem_plan_view* p = &s->p;
     const int T = p->time_steps;
     if (n > T) return -1;
@@ -1303,6 +1643,8 @@ int oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int6
             return 0;
         case 20: if (idx < 0 || idx >= p->n_spice + p->n_spice_extra) return -1; src = s->spVout + (int64_t)idx * T; break;
         case 21: if (idx < 0 || idx >= p->n_spice) return -1; src = s->spIout + (int64_t)idx * T; break;
+        case 22: if (!p->kmax) return -1; src = s->refl2; break;
+        case 23: if (!p->kmax) return -1; src = s->trans2; break;
         default: return -1;
     }
     for (int64_t q = 0; q < n; ++q) out[q] = (float)src[q];

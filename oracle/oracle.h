@@ -41,7 +41,10 @@ int           oracle_get_field64(const oracle_state* s, int comp, double* out, i
 /* overwrite a field component (ncell entries) -- for initial-value tests */
 int           oracle_set_field(oracle_state* s, int comp, const float* in, int64_t n);
 /* copy a series (kinds 0..7 as aem_series in include/aem.h; 20/21 = SPICE
- * voltages/currents, which are aem_series 8/9) as float */
+ * voltages/currents, which are aem_series 8/9; 22/23 = the kmax TM sums,
+ * aem_series 10/11) as float.  For a kmax plan (two parts, real and
+ * imaginary) a kind or field component OR-ed with 0x100 selects the
+ * imaginary part in every accessor. */
 int           oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int64_t n);
 /* raw NTFF accumulators as double: which 0..5 = Ux,Uy,Uz,Wx,Wy,Wz */
 int           oracle_get_ntff_raw(const oracle_state* s, int which, double* out, int64_t n);

@@ -12,6 +12,11 @@ device's files can be compared byte for byte, and for the decks in DAT_DECKS
 the reference's data.dat text itself is kept as tests/golden/dat/<deck>.dat.
 Only outputs are stored, never reference code.
 
+Decks named kmax_* run through ``oracle/_ref/fdtd_ref_kmax`` (the kmax build,
+``make -C oracle ref-kmax``); their series are complex (complex64 arrays):
+incident, port voltages and the TE/TM reflected and transmitted waves as the
+writer labels them (f90:8648-8785).
+
 Usage:  python tests/golden/make_golden.py [deck-name ...]
 """
 import hashlib
@@ -30,6 +35,7 @@ sys.path.insert(0, ROOT)
 from alpineem_amd.datfile import read_dat, read_fortran_record_file  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "fdtd_ref")
+REF_KMAX = os.path.join(ROOT, "oracle", "_ref", "fdtd_ref_kmax")
 VIDEO_STRIDE = 25
 # decks whose data.dat is kept whole: antenna with far field, plane wave with
 # far field and phase-centre incident, several ports, periodic R/T, sheets
@@ -49,9 +55,13 @@ def run_deck(name: str) -> None:
     deck = os.path.join(HERE, "decks", name + ".txt")
     with tempfile.TemporaryDirectory() as tmp:
         shutil.copy(deck, os.path.join(tmp, "inputs.txt"))
-        subprocess.run([REF, "inputs.txt"], cwd=tmp, check=True, stdout=subprocess.DEVNULL)
+        ref = REF_KMAX if name.startswith("kmax_") else REF
+        subprocess.run([ref, "inputs.txt"], cwd=tmp, check=True, stdout=subprocess.DEVNULL)
         d = read_dat(os.path.join(tmp, "data.dat"))
         out = {"dt": np.float32(d.dt), "time_steps": np.int32(d.time_steps)}
+        if d.kmax:
+            for k in ("te_refl", "tm_refl", "te_trans", "tm_trans"):
+                out[k] = getattr(d, k)
         if d.incident is not None:
             out["incident"] = d.incident
         if d.reflected is not None:
@@ -96,6 +106,8 @@ def main(argv):
                            if f.endswith(".txt") and not f.endswith("_example.txt"))
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: run 'make -C oracle ref' first")
+    if any(n.startswith("kmax_") for n in names) and not os.path.exists(REF_KMAX):
+        sys.exit(f"{REF_KMAX} missing: run 'make -C oracle ref-kmax' first")
     for n in names:
         run_deck(n)
 

@@ -37,7 +37,8 @@ def load(fp64: bool = False) -> C.CDLL:
 
 
 SERIES = {"port_v": 0, "port_i": 1, "incident": 2, "e_theta": 3, "e_phi": 4, "reflected": 5,
-          "transmitted": 6, "ff_pc": 7, "spice_v": 20, "spice_i": 21}
+          "transmitted": 6, "ff_pc": 7, "spice_v": 20, "spice_i": 21, "reflected_tm": 22, "transmitted_tm": 23}
+IMAG = 0x100    # the imaginary part of a kmax plan's series or field
 FIELDS = {"Ex": 0, "Ey": 1, "Ez": 2, "Hx": 3, "Hy": 4, "Hz": 5}
 
 
@@ -68,17 +69,22 @@ class Oracle:
             raise RuntimeError("oracle: SPICE exchange out of order")
         return i[:n_spice] if n_spice else None
 
-    def series(self, kind: str, idx: int = 0, n: int = None) -> np.ndarray:
+    def series(self, kind: str, idx: int = 0, n: int = None, imag: bool = False) -> np.ndarray:
         n = self.plan.time_steps if n is None else n
         out = np.zeros(n, np.float32)
-        if self.L.oracle_get_series(self.h, SERIES[kind], idx, out.ctypes.data, n) != 0:
+        if self.L.oracle_get_series(self.h, SERIES[kind] | (IMAG if imag else 0), idx, out.ctypes.data, n) != 0:
             raise ValueError(kind)
         return out
 
-    def field(self, name: str) -> np.ndarray:
+    def cseries(self, kind: str, idx: int = 0) -> np.ndarray:
+        """A kmax plan's complex series as complex64."""
+        return self.series(kind, idx) + 1j * self.series(kind, idx, imag=True).astype(np.complex64)
+
+    def field(self, name: str, imag: bool = False) -> np.ndarray:
         v = self.plan.view
         out = np.zeros(v.ncell, np.float32)
-        self.L.oracle_get_field(self.h, FIELDS[name], out.ctypes.data, v.ncell)
+        if self.L.oracle_get_field(self.h, FIELDS[name] | (IMAG if imag else 0), out.ctypes.data, v.ncell) != 0:
+            raise ValueError(name)
         return out.reshape(self.plan.shape)
 
     def field64(self, name: str) -> np.ndarray:
@@ -88,9 +94,9 @@ class Oracle:
         self.L.oracle_get_field64(self.h, FIELDS[name], out.ctypes.data, v.ncell)
         return out.reshape(self.plan.shape)
 
-    def set_field(self, name: str, values) -> None:
+    def set_field(self, name: str, values, imag: bool = False) -> None:
         a = np.ascontiguousarray(values, np.float32).reshape(-1)
-        if self.L.oracle_set_field(self.h, FIELDS[name], a.ctypes.data, a.size) != 0:
+        if self.L.oracle_set_field(self.h, FIELDS[name] | (IMAG if imag else 0), a.ctypes.data, a.size) != 0:
             raise ValueError(name)
 
     def ntff_raw(self, which: int) -> np.ndarray:
