@@ -120,7 +120,7 @@ class PlanOptions(C.Structure):
 class SimOptions(C.Structure):
     """aem_sim_options (include/aem.h)."""
     _fields_ = [("ntff_binning", C.c_int32), ("planes", C.c_int32), ("no_cpml_fusion", C.c_int32),
-                ("no_single_pass", C.c_int32), ("ntff_lists", C.c_int32)]
+                ("no_single_pass", C.c_int32), ("ntff_row_scan", C.c_int32)]
 
 
 class PortView(C.Structure):

@@ -1921,22 +1921,30 @@ struct NtffList {
 };
 
 // Contribution of entry q to bin `bin` at step n (the reference's FP32
-// expressions); false when the cell's bins at this step are other ones.
-__device__ __forceinline__ bool list_term(const NtffList& L, const NtffBins& bn, uint2 q, bool isW, float nf,
-                                          int bin, float& out) {
-    if (q.x == 0xffffffffu) return false;
+// expressions), without branches: the data
+// loads are unconditional (a padding or non-contributing entry reads element
+// 0), so the loads of a batch of entries are all in flight together; `hit`
+// says whether the value counts.
+__device__ __forceinline__ float list_term_nb(const NtffList& L, const NtffBins& bn, uint2 q, bool isW, float nf,
+                                              int bin, bool& hit) {
     const float tfv = __uint_as_float(q.y);
     const float t = isW ? (nf + tfv - 1.0f) : (nf + 0.5f + tfv - 1.0f);
     const int ii = (int)t;
-    if (ii != bin && ii + 1 != bin) return false;
+    hit = q.x != 0xffffffffu && (ii == bin || ii + 1 == bin);
+    const unsigned e = hit ? q.x : 0u;
     const float wt = t - (float)ii;
     const float* B = isW ? L.Jbase : L.Mbase;
-    const float dd = B[q.x] - B[q.x + L.off_old];
-    const int ax = (int)(q.x >> L.face_shift) >> 1;
+    const float dd = B[e] - B[e + L.off_old];
+    const int ax = (int)(e >> L.face_shift) >> 1;
     const float K = ax == 0 ? bn.K[0] : (ax == 1 ? bn.K[1] : bn.K[2]);
-    out = ii == bin ? (1.0f - wt) * K * dd : wt * K * dd;
-    return true;
+    return ii == bin ? (1.0f - wt) * K * dd : wt * K * dd;
 }
+
+// Entries per batch: their loads are issued together, then their terms are
+// added in list order (the list walk is bound by load latency otherwise, with
+// about one wave per SIMD in flight).
+constexpr int kListBatch = 8;
+constexpr int kLongBatch = 4;   // 64-entry chunks of a long list formed together
 
 __global__ void __launch_bounds__(256) k_ntff_list(NtffList L, NtffBins bn, const int* step) {
     const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -1953,24 +1961,53 @@ __global__ void __launch_bounds__(256) k_ntff_list(NtffList L, NtffBins bn, cons
         const bool own = bin >= 0 && bin < bn.LL && !((L.skip[w] >> lane) & 1ull);
         float acc = own ? A[bin] : 0.0f;
         const uint2* e = L.ent + d.z + lane;
-        for (int p = 0; p < d.w; ++p, e += 64) {
-            float c;
-            if (list_term(L, bn, *e, isW, nf, bin, c)) acc = acc + c;
+        int p = 0;
+        for (; p + kListBatch <= d.w; p += kListBatch) {
+            uint2 q[kListBatch];
+#pragma unroll
+            for (int u = 0; u < kListBatch; ++u) q[u] = e[(long long)(p + u) * 64];
+            float c[kListBatch];
+            bool hit[kListBatch];
+#pragma unroll
+            for (int u = 0; u < kListBatch; ++u) c[u] = list_term_nb(L, bn, q[u], isW, nf, bin, hit[u]);
+#pragma unroll
+            for (int u = 0; u < kListBatch; ++u)
+                if (hit[u]) acc = acc + c[u];
+        }
+        for (; p < d.w; ++p) {
+            bool hit;
+            const float c = list_term_nb(L, bn, e[(long long)p * 64], isW, nf, bin, hit);
+            if (hit) acc = acc + c;
         }
         if (own) A[bin] = acc;
         return;
     }
-    // one long list: lanes form 64 contributions, the sum takes them in order
+    // one long list: lanes form 64 contributions at a time (kLongBatch
+    // chunks together), the sum takes them in order
     const int bin = n + d.y;
     if (bin < 0 || bin >= bn.LL) return;
     float acc = A[bin];
     const int cnt = -d.w;
-    for (int p0 = 0; p0 < cnt; p0 += 64) {
-        float c = 0.0f;
-        const bool hit = p0 + lane < cnt && list_term(L, bn, L.ent[d.z + p0 + lane], isW, nf, bin, c);
-        const unsigned long long m = __ballot(hit);
-        for (int l = 0; l < 64; ++l)
-            if ((m >> l) & 1ull) acc = acc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), l));
+    for (int p0 = 0; p0 < cnt; p0 += 64 * kLongBatch) {
+        float c[kLongBatch];
+        bool hit[kLongBatch];
+#pragma unroll
+        for (int u = 0; u < kLongBatch; ++u) {
+            const int p = p0 + u * 64 + lane;
+            const uint2 q = p < cnt ? L.ent[d.z + p] : make_uint2(0xffffffffu, 0u);
+            c[u] = list_term_nb(L, bn, q, isW, nf, bin, hit[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kLongBatch; ++u) {
+            const unsigned long long m = __ballot(hit[u]);
+            if (m == ~0ull) {   // every entry counts (the usual case): no per-entry test
+#pragma unroll
+                for (int l = 0; l < 64; ++l) acc = acc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[u]), l));
+            } else {
+                for (int l = 0; l < 64; ++l)
+                    if ((m >> l) & 1ull) acc = acc + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c[u]), l));
+            }
+        }
     }
     if (lane == 0) A[bin] = acc;
 }

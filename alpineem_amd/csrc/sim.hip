@@ -280,7 +280,7 @@ struct aem_sim {
     NtffExact nx_acc{};
     int win_max = 0;
     NtffList nl{};                 // ordered binning by contributor lists (nl.n_waves > 0)
-    bool ntff_lists = false;       // ordered binning from contributor lists (aem_sim_options)
+    bool ntff_lists = true;        // ordered binning from contributor lists (else the row scan)
     void build_ntff_lists(const std::vector<int>& win, unsigned stride);
     double *d_refl = nullptr, *d_trans = nullptr;
     double *d_refl2 = nullptr, *d_trans2 = nullptr;     // kmax: the TM sums (d_refl, d_trans hold TE)
@@ -439,7 +439,7 @@ void aem_sim::create(const aem_sim_options* opt) {
         }
         allow_fuse = opt->no_cpml_fusion == 0;
         allow_sp = opt->no_single_pass == 0;
-        ntff_lists = opt->ntff_lists != 0;
+        ntff_lists = opt->ntff_row_scan == 0;
     }
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));

@@ -101,14 +101,14 @@ class Simulation:
     time-domain far field, e.g. for objective-only runs), ``planes``
     the z planes each bulk thread marches, ``fuse_cpml=False`` forces the
     separate CPML kernels, ``single_pass=False`` separate H and E bulk
-    launches, ``ntff_lists=True`` ordered binning from contributor lists
-    instead of the row-scan kernel (aem_sim_options in include/aem.h)."""
+    launches, ``ntff_lists=False`` ordered binning by the row-scan kernel
+    instead of the contributor lists (aem_sim_options in include/aem.h)."""
 
     def __init__(self, plan: Plan, ntff: str = "ordered", planes: int = 0, fuse_cpml: bool = True,
-                 single_pass: bool = True, ntff_lists: bool = False):
+                 single_pass: bool = True, ntff_lists: bool = True):
         self.plan = plan
         opt = _lib.SimOptions(_lib.NTFF_BINNING[ntff], int(planes), 0 if fuse_cpml else 1, 0 if single_pass else 1,
-                              1 if ntff_lists else 0)
+                              0 if ntff_lists else 1)
         h = C.c_void_p()
         check(lib().aem_sim_create_ex(plan.handle, C.byref(opt), C.byref(h)))
         self._h = h

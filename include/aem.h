@@ -155,9 +155,9 @@ typedef struct aem_sim_options {
     int32_t planes;         /* z planes marched per thread in the bulk kernels; 0 = default (4) */
     int32_t no_cpml_fusion; /* 1: the separate CPML kernels after ports/TF/SF and one TF/SF launch per face pair */
     int32_t no_single_pass; /* 1: separate H and E bulk launches even where one single-pass launch qualifies */
-    int32_t ntff_lists;     /* 1: ordered binning from contributor lists built at creation instead of the
-                               per-bin scan of staged face rows (same results; faster where no bin collects
-                               thousands of cells) */
+    int32_t ntff_row_scan;  /* 1: ordered binning by the per-bin scan of staged face rows instead of the
+                               contributor lists built at creation (same results; the lists are the default
+                               and fall back to the scan when they would exceed 4 GiB) */
 } aem_sim_options;
 
 /* aem_sim_create with options (NULL = defaults). */
