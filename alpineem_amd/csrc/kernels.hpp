@@ -489,12 +489,33 @@ struct SpArgs {
     const float *Ex, *Ey, *Ez, *Hx, *Hy, *Hz;   // fields at the start of the step
     float *Exo, *Eyo, *Ezo, *Hxo, *Hyo, *Hzo;   // fields at its end
     const void* idx;                            // uint8_t or uint16_t per IdxT
+    const uint32_t* idx4;                       // uint8_t indices packed 4 z planes per word (IdxT uint8_t)
     const float* tup;
+    int n_tup;                                  // coefficient tuples (at most 256 with uint8_t indices)
     const float *dex, *dey, *dez, *dhx, *dhy, *dhz;
     float DB;
     Cpml cpH, cpE;                              // x faces: psi read, pso written
     float dx, dy, dz;
 };
+
+// With uint8_t indices the single pass reads the index of four z planes
+// with one dword load (word (i, j, q) holds the bytes of planes 4q+1..4q+4)
+// and the coefficient tuples from LDS: one byte load and six table loads
+// per cell and plane took 6% of the launch at 400^3.
+constexpr int kSpTupMax = 256;
+
+// The packed copy from the byte indices (after they change on the device).
+__global__ void k_pack_idx4(const uint8_t* idx, uint32_t* idx4, long long plane, int nz) {
+    const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= plane * ((nz + 3) / 4)) return;
+    const long long q = w / plane, c = w % plane;
+    uint32_t v = 0u;
+    for (int s = 0; s < 4; ++s) {
+        const long long k = 4 * q + s;
+        if (k < nz) v |= (uint32_t)idx[k * plane + c] << (8 * s);
+    }
+    idx4[w] = v;
+}
 
 // Field access by 32-bit byte offset from a uniform base: one VGPR per
 // distinct offset, shared by all arrays (global loads with a scalar base).
@@ -519,6 +540,13 @@ constexpr int kSpRows = 2;   // y rows marched together by one wave
 template <typename IdxT, int C>
 __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
     constexpr int R = kSpRows;
+    constexpr bool kPacked = sizeof(IdxT) == 1;
+    __shared__ float stup[kPacked ? kSpTupMax * 6 : 1];
+    if (kPacked) {   // before any wave leaves: every wave reaches the barrier
+        const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+        for (int q = tid; q < a.n_tup * 6; q += (int)(blockDim.x * blockDim.y)) stup[q] = a.tup[q];
+        __syncthreads();
+    }
     const Grid& g = a.g;
     const float* __restrict__ Ex = a.Ex;
     const float* __restrict__ Ey = a.Ey;
@@ -616,16 +644,24 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
     const uint32_t IB = (uint32_t)sizeof(IdxT), ZI = (uint32_t)g.plane * IB;
     const uint32_t PI = (uint32_t)g.PX * IB;
     uint32_t oi = (uint32_t)off(g, ic, jw, k0) * IB;
+    const uint32_t ow = (uint32_t)off(g, ic, jw, 1), PW = (uint32_t)g.PX, QW = (uint32_t)g.plane;
+    uint32_t wd[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         exk[r] = eyk[r] = exc[r] = hxm[r] = hym[r] = 0.f;
         t[r] = tn[r] = 0;
+        wd[r] = 0u;
         if (r < nr) {
             const uint32_t o = ob + r * PB;
             exk[r] = ldb(Ex, o);
             eyk[r] = ldb(Ey, o);
             exc[r] = ldb(Ex, oc + r * PB);
-            t[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
+            if (kPacked) {
+                wd[r] = a.idx4[ow + r * PW + (uint32_t)((k0 - 1) >> 2) * QW];
+                t[r] = (int)((wd[r] >> (8 * ((k0 - 1) & 3))) & 0xffu) * 6;
+            } else {
+                t[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
+            }
             if (k0 >= 2) {                    // Hx, Hy of the plane below the chunk
                 const int j = jw + r;
                 const uint32_t om = o - ZB;
@@ -667,9 +703,17 @@ __global__ void __launch_bounds__(64 * kSY) k_step_sp(SpArgs a, Box b) {
             excy[r] = ldb(Ex, c + PB);
             hyc0[r] = ldb(Hy, c);
             hzc0[r] = ldb(Hz, c);
+            if (kPacked) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) ga[r][q] = tup[t[r] + q];
-            tn[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
+                for (int q = 0; q < 6; ++q) ga[r][q] = stup[t[r] + q];
+                // index of plane k+1 (0-based k): a new word starts every four planes
+                if (k < k1 && (k & 3) == 0) wd[r] = a.idx4[ow + r * PW + (uint32_t)(k >> 2) * QW];
+                tn[r] = (int)((wd[r] >> (8 * (k & 3))) & 0xffu) * 6;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) ga[r][q] = tup[t[r] + q];
+                tn[r] = (int)*reinterpret_cast<const IdxT*>(idx + oi + r * PI) * 6;
+            }
             const uint32_t pr = prow + (uint32_t)r;
             sh0[r] = sh1[r] = se0[r] = se1[r] = sc0[r] = sc1[r] = 0.f;
             if ((C & kCx) && fxh >= 0) {

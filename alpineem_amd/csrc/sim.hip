@@ -268,6 +268,8 @@ struct aem_sim {
     float* F[6] = {};              // fields now (Ex, Ey, Ez, Hx, Hy, Hz)
     float* F2[6] = {};             // second field set of the single-pass step
     void* d_idx = nullptr;
+    uint32_t* d_idx4 = nullptr;    // uint8_t indices packed four z planes per word (single pass)
+    int n_tup = 0;                 // coefficient tuples on the device (a design box adds one)
     float* d_tup = nullptr;
     float* d_den[6] = {};
     Cpml cpH{}, cpE{};
@@ -457,6 +459,7 @@ void aem_sim::create(const aem_sim_options* opt) {
     if (v.idx_bytes == 1) d_idx = up<uint8_t>(v.tuple_idx8, nc);
     else d_idx = up<uint16_t>(v.tuple_idx16, nc);
     d_tup = up<float>(v.tuples, (size_t)v.n_tuple * 6);
+    n_tup = v.n_tuple;
     const int dl[6] = {v.NX - 1, v.NY - 1, v.NZ - 1, v.NX - 1, v.NY - 1, v.NZ - 1};
     const float* dsrc[6] = {v.den_ex, v.den_ey, v.den_ez, v.den_hx, v.den_hy, v.den_hz};
     for (int q = 0; q < 6; ++q) d_den[q] = up<float>(dsrc[q], std::max(dl[q], 1));
@@ -759,6 +762,14 @@ void aem_sim::create(const aem_sim_options* opt) {
         // registers) and the two x-slab tiles of each row took 5-12% longer,
         // in sequence or on a second stream beside them
         sp_parts.push_back({sp_box, (int)kCx});
+        if (v.idx_bytes == 1) {   // word (cell of plane 1, q) = bytes of planes 4q+1 .. 4q+4
+            const size_t pl = (size_t)v.plane, nq = (size_t)(v.NZ + 3) / 4;
+            std::vector<uint32_t> w(pl * nq, 0u);
+            for (size_t k = 0; k < (size_t)v.NZ; ++k)
+                for (size_t c = 0; c < pl; ++c)
+                    w[(k >> 2) * pl + c] |= (uint32_t)v.tuple_idx8[k * pl + c] << (8 * (k & 3));
+            d_idx4 = up<uint32_t>(w.data(), w.size());
+        }
         for (int q = 0; q < 6; ++q) F2[q] = zeros<float>(nc + v.plane);
         {
             const size_t nt0 = (size_t)std::max(v.npml[0] - 1, 0), nt1 = (size_t)std::max(v.npml[1] - 1, 0);
@@ -1093,8 +1104,8 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
     // launch when sp --, [1,2] the rest of H)
     if (mark) HIPCHK(hipEventRecord(mark[0], st));
     if (sp) {
-        const SpArgs sa{g, Ex, Ey, Ez, Hx, Hy, Hz, F2[0], F2[1], F2[2], F2[3], F2[4], F2[5], d_idx, d_tup,
-                        d_den[0], d_den[1], d_den[2], d_den[3], d_den[4], d_den[5], v.DB, cpH, cpE, v.dx, v.dy, v.dz};
+        const SpArgs sa{g, Ex, Ey, Ez, Hx, Hy, Hz, F2[0], F2[1], F2[2], F2[3], F2[4], F2[5], d_idx, d_idx4, d_tup,
+                        n_tup, d_den[0], d_den[1], d_den[2], d_den[3], d_den[4], d_den[5], v.DB, cpH, cpE, v.dx, v.dy, v.dz};
         for (const auto& part : sp_parts) {
             const Box& b = part.first;
             const unsigned nblk = cdiv(b.ni, 64) * cdiv(b.nj, kSY * kSpRows) * cdiv(b.nk, b.kc);
@@ -2639,9 +2650,15 @@ aem_status aem_design_box(aem_sim* s, const int32_t* lo, const int32_t* hi) {
         HIPCHK(hipStreamSynchronize(s->st));
         s->d_tup = tup;
         s->ea.tup = tup;
-        if (v.idx_bytes == 1)
+        s->n_tup = nt + 1;
+        if (v.idx_bytes == 1) {
             k_set_idx<uint8_t><<<cdiv(d.nedge, 256), 256, 0, s->st>>>(s->g, d, static_cast<uint8_t*>(s->d_idx), (uint8_t)nt);
-        else
+            if (s->d_idx4) {   // the single pass reads the packed copy
+                const long long nw = (long long)v.plane * ((v.NZ + 3) / 4);
+                k_pack_idx4<<<cdiv(nw, 256), 256, 0, s->st>>>(static_cast<const uint8_t*>(s->d_idx), s->d_idx4,
+                                                                (long long)v.plane, v.NZ);
+            }
+        } else
             k_set_idx<uint16_t><<<cdiv(d.nedge, 256), 256, 0, s->st>>>(s->g, d, static_cast<uint16_t*>(s->d_idx),
                                                                        (uint16_t)nt);
         HIPCHK(hipGetLastError());
