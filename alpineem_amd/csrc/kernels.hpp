@@ -1523,6 +1523,39 @@ __global__ void k_save_eold(Grid g, DrudeBox b, const float* E) {
     b.Eold[tid] = E[off(g, i, j, k)];
 }
 
+// Transposed Drude update of one component: the adjoint of k_drude followed
+// by k_save_eold, on the adjoint state after step n, before the transposed
+// bulk E update.  Forward, per cell: E2 = E1 - gb/2 sum_q (1 + Ja_q) J_q,
+// J_q' = Ja_q J_q + Jb_q (E2 + Eold), and Eold' = E2 at the end of the step.
+// Here b.J holds a_q, the adjoint of J_q, and b.Eold holds o, the adjoint of
+// Eold:
+//   s = sum_q Jb_q a_q,   lambda_E2 = lambda_E + o + s,
+//   a_q <- Ja_q a_q - gb/2 (1 + Ja_q) lambda_E2,   o <- s,
+// and lambda_E1 = lambda_E2 with mu = gb lambda_E1 beside it.
+template <typename IdxT>
+__global__ void k_adj_drude(Grid g, DrudeBox b, int comp, int P, float* lE, float* mE, const IdxT* idx,
+                            const float* tup) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= b.nb) return;
+    const int bx = b.hi[0] - b.lo[0] + 1, by = b.hi[1] - b.lo[1] + 1;
+    const int i = b.lo[0] + (int)(tid % bx);
+    const int j = b.lo[1] + (int)((tid / bx) % by);
+    const int k = b.lo[2] + (int)(tid / ((long long)bx * by));
+    const long long o = off(g, i, j, k);
+    float s = 0.0f;
+    for (int q = 0; q < P; ++q) s = s + b.Jb[q * b.nb + tid] * b.J[q * b.nb + tid];
+    const float gb = gb_at(idx, tup, o, comp);
+    const float le = lE[o] + b.Eold[tid] + s;
+    lE[o] = le;
+    mE[o] = gb * le;
+    const float h = gb / 2.0f * le;
+    for (int q = 0; q < P; ++q) {
+        const float ja = b.Ja[q * b.nb + tid];
+        b.J[q * b.nb + tid] = ja * b.J[q * b.nb + tid] - (1.0f + ja) * h;
+    }
+    b.Eold[tid] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Periodic boundary copies (H f90:5377-5433, E f90:6672-6741).  One launch per
 // axis, in x, y, z order; the special E of sheet planes is copied alongside.
@@ -3163,6 +3196,117 @@ __global__ void k_adj_nu_adjust(Grid g, const float* lE, int comp, float sign, c
             if (e_slot_comp(A, sl) != comp || !e_slot_ok(A, sl, i, j, k)) continue;
             float* nu = lpe.psi[f][sl];
             nu[po] = nu[po] + sign * (tup[(int)idx[o] * 6 + 2 * comp + 1] * lE[o]);
+        }
+    }
+}
+
+// Adjoint of the lumped ports (k_ports_h, k_ports_e).  Forward, per port and
+// step, with the sources left out (they are affine):
+//   H phase: I_0 = L(H) num/den, L the gap's loop integral;
+//            I_i -= kI (V_{i+1} - V_i), i = 1..M-2
+//   E phase: V_i -= kV (I_i - I_{i-1}), i = 1..M-2;  E = V_1 / gap_div on the gap
+// and the series record V_1 after the E phase and I_0 after the H phase.
+// lV, lI ([np][M]) are the adjoints of the line; wV, wI ([np][T], or null)
+// are dJ/dV_1 and dJ/dI_0 of every step.  One thread runs the ports one
+// after another, in a fixed order (no two ports race on a shared H cell);
+// the lines are a few hundred nodes.
+struct AdjPorts {
+    const PortDev* ports;
+    int np, M, T;
+    float *lV, *lI;
+    const double *wV, *wI;
+};
+
+// Transposed E phase (before the transposed bulk update): dJ/dV_1 of step n
+// and the gap cells' lambda_E / gap_div into lambda_V_1; the gap cells' own
+// update is overwritten, so their lambda_E and mu become 0; then the V
+// recursion transposed into lambda_I.
+__global__ void k_adj_ports_e(Grid g, AdjPorts a, float* lEx, float* lEy, float* lEz, float* mEx, float* mEy,
+                              float* mEz, const int* step) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int n = *step, M = a.M;
+    for (int q = 0; q < a.np; ++q) {
+        const PortDev pt = a.ports[q];
+        float* L = pt.dir == 0 ? lEx : (pt.dir == 1 ? lEy : lEz);
+        float* Mu = pt.dir == 0 ? mEx : (pt.dir == 1 ? mEy : mEz);
+        float acc = 0.0f;
+        for (int ii = pt.loc[0]; ii <= pt.loc[0] + pt.size[0] - 1; ++ii)
+            for (int jj = pt.loc[1]; jj <= pt.loc[1] + pt.size[1] - 1; ++jj)
+                for (int kk = pt.loc[2]; kk <= pt.loc[2] + pt.size[2] - 1; ++kk) {
+                    const long long o = off(g, ii, jj, kk);
+                    acc = acc + L[o];
+                    L[o] = 0.0f;
+                    Mu[o] = 0.0f;
+                }
+        float* lV = a.lV + (long long)q * M;
+        float* lI = a.lI + (long long)q * M;
+        float v1 = lV[1] + acc / pt.gap_div;
+        if (a.wV) v1 = v1 + (float)a.wV[(long long)q * a.T + (n - 1)];
+        lV[1] = v1;
+        const float kV = pt.kV;
+        for (int i = 1; i <= M - 2; ++i) {
+            lI[i] = lI[i] - kV * lV[i];
+            lI[i - 1] = lI[i - 1] + kV * lV[i];
+        }
+    }
+}
+
+// Transposed H phase (between the transposed E and H bulk updates): the I
+// recursion transposed into lambda_V, dJ/dI_0 of step n, and lambda_I_0
+// num/den spread over the H of the gap's loop in gap_loop_current's terms;
+// I_0 is overwritten every step, so lambda_I_0 becomes 0.
+__global__ void k_adj_ports_h(Grid g, AdjPorts a, float* lHx, float* lHy, float* lHz, float dx, float dy, float dz,
+                              const int* step) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int n = *step, M = a.M;
+    for (int q = 0; q < a.np; ++q) {
+        const PortDev pt = a.ports[q];
+        float* lV = a.lV + (long long)q * M;
+        float* lI = a.lI + (long long)q * M;
+        const float kI = pt.kI;
+        for (int i = 1; i <= M - 2; ++i) {
+            lV[i + 1] = lV[i + 1] - kI * lI[i];
+            lV[i] = lV[i] + kI * lI[i];
+        }
+        float i0 = lI[0];
+        if (a.wI) i0 = i0 + (float)a.wI[(long long)q * a.T + (n - 1)];
+        lI[0] = 0.0f;
+        const float w = i0 * pt.cur_num / pt.cur_den;
+        const int l1 = pt.loc[0], l2 = pt.loc[1], l3 = pt.loc[2];
+        const int d1 = pt.size[0] + l1 - 1, d2 = pt.size[1] + l2 - 1, d3 = pt.size[2] + l3 - 1;
+        if (pt.dir == 0) {
+            for (int ii = l1; ii <= d1; ++ii) {
+                for (int jj = l2; jj <= d2; ++jj) {
+                    lHy[off(g, ii, jj, l3 - 1)] += w * dy;
+                    lHy[off(g, ii, jj, d3)] -= w * dy;
+                }
+                for (int kk = l3; kk <= d3; ++kk) {
+                    lHz[off(g, ii, l2, kk)] += w * dz;
+                    lHz[off(g, ii, d2 - 1, kk)] -= w * dz;
+                }
+            }
+        } else if (pt.dir == 1) {
+            for (int jj = l2; jj <= d2; ++jj) {
+                for (int ii = l1; ii <= d1; ++ii) {
+                    lHx[off(g, ii, jj, l3)] += w * dx;
+                    lHx[off(g, ii, jj, d3 - 1)] -= w * dx;
+                }
+                for (int kk = l3; kk <= d3; ++kk) {
+                    lHz[off(g, l1 - 1, jj, kk)] += w * dz;
+                    lHz[off(g, d1, jj, kk)] -= w * dz;
+                }
+            }
+        } else {
+            for (int kk = l3; kk <= d3; ++kk) {
+                for (int ii = l1; ii <= d1; ++ii) {
+                    lHx[off(g, ii, l2 - 1, kk)] += w * dx;
+                    lHx[off(g, ii, d2, kk)] -= w * dx;
+                }
+                for (int jj = l2; jj <= d2; ++jj) {
+                    lHy[off(g, l1, jj, kk)] += w * dy;
+                    lHy[off(g, d1 - 1, jj, kk)] -= w * dy;
+                }
+            }
         }
     }
 }

@@ -351,6 +351,7 @@ struct aem_sim {
     void upload_design();
     float* lam[9] = {};            // adjoint fields and mu = gb lambda_E (allocated at the first backward step)
     Cpml lpE{}, lpH{};             // adjoint CPML states
+    DrudeBox adb[3]{};             // adjoint Drude states: J holds lambda_J, Eold lambda_Eold
     int* d_bstep = nullptr;
     float* d_srcD = nullptr;       // adjoint source: [6][4][mv][mu] increments of a step
     float* d_desm = nullptr;       // design transpose: [3][nedge] gb lambda_E
@@ -2268,6 +2269,12 @@ void aem_sim::ensure_adjoint() {
             lpH.psi[f][s] = zeros<float>(nt * trans);
         }
     }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c) {
+            adb[c] = db[c];
+            adb[c].J = zeros<float>((size_t)db[c].nb * v.n_poles);
+            adb[c].Eold = zeros<float>((size_t)db[c].nb);
+        }
     d_bstep = zeros<int>(1);
     d_srcD = zeros<float>((size_t)24 * std::max(ntff_mu, 1) * std::max(ntff_mv, 1));
     d_desm = zeros<float>((size_t)3 * std::max(des.nedge, 1LL));
@@ -2290,8 +2297,16 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
                                                                                         d_bstep);
         k_adj_source_gather<IdxT><<<dim3(cdiv(ntff_mu + 1, 32), cdiv(ntff_mv + 1, 8), 12), 256, 0, s>>>(
             g, obj, d_srcD, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, d_tup);
-        if (rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
     }
+    // lambda_E of the step's end -> lambda_E of the bulk update's output
+    if (v.has_drude) {
+        float* lE[3] = {lEx, lEy, lEz};
+        float* mE[3] = {mEx, mEy, mEz};
+        for (int c = 0; c < 3; ++c)
+            if (adb[c].nb > 0)
+                k_adj_drude<IdxT><<<cdiv(adb[c].nb, 256), 256, 0, s>>>(g, adb[c], c, v.n_poles, lE[c], mE[c], idx, d_tup);
+    }
+    if (objective && rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
     if (des_on) {
         k_adj_design_m<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, lEx, lEy, lEz, d_desm);
         const long long nh = (long long)(des.n[0] + 1) * (des.n[1] + 1) * (des.n[2] + 1);
@@ -2309,11 +2324,36 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
 
 namespace {
 
+// The Drude box of component c as cells [lo, hi] per axis; false when empty.
+bool drude_box(const aem_plan_view& v, int c, int lo[3], int hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = v.dbox[c][2 * a];
+        hi[a] = v.dbox[c][2 * a + 1];
+        if (lo[a] > hi[a]) return false;
+    }
+    return true;
+}
+
+// The transposed Drude step covers boxes clear of the CPML slabs and the
+// outer boundary by a cell: every component in them is updated by the bulk
+// loop and by no psi term.
+bool drude_interior(const aem_plan_view& v) {
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    for (int c = 0; c < 3; ++c) {
+        int lo[3], hi[3];
+        if (!drude_box(v, c, lo, hi)) continue;
+        for (int a = 0; a < 3; ++a)
+            if (lo[a] < std::max(v.npml[2 * a], 0) + 2 || hi[a] > N[a] - std::max(v.npml[2 * a + 1], 0) - 2)
+                return false;
+    }
+    return true;
+}
+
 // Features whose update is not part of the transposed step.
 const char* adjoint_unsupported(const aem_plan_view& v) {
     if (v.n_ports > 0) return "lumped ports";
     if (v.n_sheet_planes[0] + v.n_sheet_planes[1] + v.n_sheet_planes[2] > 0) return "thin sheets";
-    if (v.has_drude) return "Drude media";
+    if (v.has_drude && !drude_interior(v)) return "Drude media within a cell of the CPML or the outer boundary";
     if (v.pbc[0] || v.pbc[1] || v.pbc[2]) return "periodic boundaries";
     if (v.kmax) return "the kmax build";
     return nullptr;
@@ -2584,6 +2624,18 @@ aem_status aem_adjoint_record(aem_sim* s, const int32_t* lo, const int32_t* hi) 
             return AEM_ERR_STATE;
         }
     }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c) {
+            // the record and design updates assume the plain E update on their edges [lo, hi+1]
+            int dl[3], dh[3];
+            if (!drude_box(v, c, dl, dh)) continue;
+            bool meet = true;
+            for (int a = 0; a < 3; ++a) meet = meet && lo[a] <= dh[a] && hi[a] + 1 >= dl[a];
+            if (meet) {
+                aem::set_error("aem_adjoint_record: design box must not meet the Drude media's bounding box");
+                return AEM_ERR_UNSUPPORTED;
+            }
+        }
     return aem::guard([&]() {
         AdjRec& r = s->rec;
         for (int a = 0; a < 3; ++a) {
@@ -2727,6 +2779,12 @@ void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb
             if (nt * trans) HIPCHK(hipMemsetAsync(s->lpH.psi[f][q], 0, nt * trans * 4, s->st));
         }
     }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c)
+            if (s->adb[c].nb > 0) {
+                HIPCHK(hipMemsetAsync(s->adb[c].J, 0, (size_t)s->adb[c].nb * v.n_poles * 4, s->st));
+                HIPCHK(hipMemsetAsync(s->adb[c].Eold, 0, (size_t)s->adb[c].nb * 4, s->st));
+            }
     const int T = v.time_steps;
     HIPCHK(hipMemcpyAsync(s->d_bstep, &T, sizeof(int), hipMemcpyHostToDevice, s->st));
     hipGraph_t gr = nullptr;

@@ -644,3 +644,107 @@ def test_general_objective_gradient():
         g_fd = (J[0] - J[1]) / (e32[0] - e32[1])
         print(f"cell {c}: adjoint {g[c]:.9e} finite difference {g_fd:.9e}")
         assert abs(g[c] - g_fd) <= 1e-3 * abs(g_fd)
+
+
+# ---------------------------------------------------------------------------
+# Drude media in the transposed step (fdtd_solver.f90:5782-5854, 6812-6826)
+# ---------------------------------------------------------------------------
+DRUDE_POLES = [((3e10, 3e10, 3e10), (1e10, 1e10, 1e10)), ((2e10, 2.2e10, 1.8e10), (5e9, 6e9, 4e9))]
+
+
+def drude_dot_deck():
+    """The dot-product deck with a two-pole Drude block clear of the CPML."""
+    d = dot_deck()
+    d.materials.append(decks.Material(7, (1.5, 2.0, 1.2), (0.01, 0.0, 0.02), DRUDE_POLES))
+    d.objects.append(decks.Block(7, (15, 14, 15), (3, 4, 3)))
+    return d
+
+
+def drude_scatter_deck(eps=(4.0, 4.5, 5.0, 3.5)):
+    """The plane-wave scatterer with a Drude slab above the design voxels
+    (cells 17..22 x 17..22 x 23..24, clear of the record box's edges)."""
+    d = scatter_deck(eps)
+    d.materials.append(decks.Material(21, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), DRUDE_POLES))
+    d.objects.append(decks.Block(21, (17, 17, 23), (6, 6, 2)))
+    return d
+
+
+@pytest.mark.parametrize("steps", [1, 4])
+def test_drude_transposed_step_dot_product(steps):
+    """<lam, A x> = <A^T lam, x> with the Drude current and E_old as state:
+    over several steps the current's recursion is transposed too."""
+    plan = Plan.from_text(drude_dot_deck().text())
+    assert plan.view.has_drude
+    m = masks(plan)
+    rng = np.random.default_rng(41 + steps)
+    x = {n: np.where(m[n], rng.standard_normal(plan.shape), 0.0).astype(np.float32) for n in NAMES}
+    lam = {n: np.where(m[n], rng.standard_normal(plan.shape), 0.0).astype(np.float32) for n in NAMES}
+    sim = Simulation(plan)
+    for n in NAMES:
+        sim.set_field(n, x[n])
+    sim.run(steps, graph=False)
+    y = {n: sim.field(n).astype(np.float64) for n in NAMES}
+    for n in NAMES:
+        sim.set_adjoint_field(n, lam[n])
+    sim.adjoint_step(steps)
+    mu = {n: sim.adjoint_field(n).astype(np.float64) for n in NAMES}
+    terms_l = np.concatenate([(lam[n] * y[n]).ravel() for n in NAMES])
+    terms_r = np.concatenate([(np.where(m[n], mu[n], 0.0) * x[n]).ravel() for n in NAMES])
+    lhs, rhs = terms_l.sum(), terms_r.sum()
+    scale = math.sqrt((terms_l ** 2).sum() + (terms_r ** 2).sum())
+    print(f"drude steps={steps} <lam,Ax>={lhs:.9e} <A^T lam,x>={rhs:.9e} rel={(lhs - rhs) / scale:.3e}")
+    assert abs(lhs - rhs) <= 1e-5 * scale
+    sim.close()
+
+
+def test_drude_eps_gradient_matches_finite_difference():
+    """d|F|^2/d eps_r of the four design materials with a Drude slab in the
+    scattered field: the adjoint equals FP64 central differences of the
+    oracle to 1e-3, and the slab changes F (the transposed Drude step is on
+    the gradient's path)."""
+    plan = Plan.from_text(drude_scatter_deck().text())
+    sim = Simulation(plan)
+    sim.set_objective(THETA, PHI, FREQ, "theta")
+    sim.record_design(DESIGN_LO, DESIGN_HI)
+    sim.run()
+    F = sim.objective()
+    g_adj = sim.eps_gradient(4)
+    sim.close()
+    plain = Simulation(Plan.from_text(scatter_deck().text()))
+    plain.set_objective(THETA, PHI, FREQ, "theta")
+    plain.run()
+    F0 = plain.objective()
+    plain.close()
+    print(f"F with the slab {F:.6e}, without {F0:.6e}")
+    assert abs(F - F0) > 0.05 * abs(F0)
+    eps0 = [4.0, 4.5, 5.0, 3.5]
+    h = 0.05
+    g_fd = np.zeros(4)
+    for q in range(4):
+        J, e32 = [], []
+        for sgn in (1, -1):
+            eps = list(eps0)
+            eps[q] += sgn * h
+            e32.append(float(np.float32(eps[q])))
+            p = Plan.from_text(drude_scatter_deck(tuple(eps)).text())
+            J.append(abs(oracle_objective(p, True, THETA, PHI, FREQ, "theta")) ** 2)
+        g_fd[q] = (J[0] - J[1]) / (e32[0] - e32[1])
+    print("adjoint", g_adj, "\nfinite difference", g_fd)
+    assert np.linalg.norm(g_adj - g_fd) <= 1e-3 * np.linalg.norm(g_fd)
+    assert np.all(np.abs(g_adj - g_fd) <= 2e-3 * np.abs(g_fd) + 1e-4 * np.abs(g_fd).max())
+
+
+def test_drude_design_box_refusals():
+    """A design box meeting the Drude media's bounding box, and Drude media
+    at the CPML, are refused with AEM_ERR_UNSUPPORTED."""
+    from alpineem_amd._lib import AemError
+    sim = Simulation(Plan.from_text(drude_scatter_deck().text()))
+    with pytest.raises(AemError, match="Drude"):
+        sim.design_box((18, 18, 21), (20, 20, 22))
+    sim.close()
+    d = drude_dot_deck()
+    d.objects[-1] = decks.Block(7, (4, 14, 15), (3, 4, 3))
+    sim = Simulation(Plan.from_text(d.text()))
+    with pytest.raises(AemError, match="Drude"):
+        sim.design_box((15, 15, 15), (16, 16, 16))
+    sim.close()
