@@ -352,12 +352,14 @@ struct aem_sim {
     float* lam[9] = {};            // adjoint fields and mu = gb lambda_E (allocated at the first backward step)
     Cpml lpE{}, lpH{};             // adjoint CPML states
     DrudeBox adb[3]{};             // adjoint Drude states: J holds lambda_J, Eold lambda_Eold
+    float *d_lV = nullptr, *d_lI = nullptr;   // adjoint lumped-port lines [np][M]
     int* d_bstep = nullptr;
     float* d_srcD = nullptr;       // adjoint source: [6][4][mv][mu] increments of a step
     float* d_desm = nullptr;       // design transpose: [3][nedge] gb lambda_E
     void ensure_adjoint();
     template <class IdxT>
-    void enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, double* dga, double* dgb);
+    void enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, const double* wV, const double* wI,
+                            double* dga, double* dgb);
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph2 = nullptr;   // single-pass steps: the step from the other buffer set
@@ -2275,6 +2277,10 @@ void aem_sim::ensure_adjoint() {
             adb[c].J = zeros<float>((size_t)db[c].nb * v.n_poles);
             adb[c].Eold = zeros<float>((size_t)db[c].nb);
         }
+    if (v.n_ports > 0) {
+        d_lV = zeros<float>((size_t)v.n_ports * v.M);
+        d_lI = zeros<float>((size_t)v.n_ports * v.M);
+    }
     d_bstep = zeros<int>(1);
     d_srcD = zeros<float>((size_t)24 * std::max(ntff_mu, 1) * std::max(ntff_mv, 1));
     d_desm = zeros<float>((size_t)3 * std::max(des.nedge, 1LL));
@@ -2283,9 +2289,12 @@ void aem_sim::ensure_adjoint() {
 
 // One step of the adjoint sweep (the adjoint state after step n -> after
 // step n-1).  With `objective`, first adds the source of J = |F|^2 at step n
-// and accumulates the design-edge derivatives.
+// of the far-field objective; wV, wI (device [np][T], or null) add dJ/dV and
+// dJ/dI of the port series at step n; with dga, dgb it accumulates the
+// design-edge derivatives.
 template <class IdxT>
-void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, double* dga, double* dgb) {
+void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, double Fi, const double* wV,
+                                 const double* wI, double* dga, double* dgb) {
     const aem_plan_view& v = plan->v;
     const IdxT* idx = static_cast<const IdxT*>(d_idx);
     float *lEx = lam[0], *lEy = lam[1], *lEz = lam[2], *lHx = lam[3], *lHy = lam[4], *lHz = lam[5];
@@ -2298,7 +2307,9 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
         k_adj_source_gather<IdxT><<<dim3(cdiv(ntff_mu + 1, 32), cdiv(ntff_mv + 1, 8), 12), 256, 0, s>>>(
             g, obj, d_srcD, lEx, lEy, lEz, lHx, lHy, lHz, mEx, mEy, mEz, idx, d_tup);
     }
+    const AdjPorts ap{d_ports, v.n_ports, v.M, v.time_steps, d_lV, d_lI, wV, wI};
     // lambda_E of the step's end -> lambda_E of the bulk update's output
+    if (v.n_ports > 0) k_adj_ports_e<<<1, 64, 0, s>>>(g, ap, lEx, lEy, lEz, mEx, mEy, mEz, d_bstep);
     if (v.has_drude) {
         float* lE[3] = {lEx, lEy, lEz};
         float* mE[3] = {mEx, mEy, mEz};
@@ -2306,7 +2317,7 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
             if (adb[c].nb > 0)
                 k_adj_drude<IdxT><<<cdiv(adb[c].nb, 256), 256, 0, s>>>(g, adb[c], c, v.n_poles, lE[c], mE[c], idx, d_tup);
     }
-    if (objective && rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
+    if (dga && rec_on) k_adj_grad<<<cdiv(rec.nedge, 256), 256, 0, s>>>(g, rec, lEx, lEy, lEz, dga, dgb, d_bstep);
     if (des_on) {
         k_adj_design_m<<<cdiv(des.nedge, 256), 256, 0, s>>>(g, des, lEx, lEy, lEz, d_desm);
         const long long nh = (long long)(des.n[0] + 1) * (des.n[1] + 1) * (des.n[2] + 1);
@@ -2316,6 +2327,7 @@ void aem_sim::enqueue_backward_t(hipStream_t s, bool objective, double Fr, doubl
     const dim3 bt(64, 4, 1);
     k_adj_h<<<dim3(cdiv(v.NX - 1, 64), cdiv(v.NY - 1, 4), v.NZ - 1), bt, 0, s>>>(
         g, lHx, lHy, lHz, mEx, mEy, mEz, d_den[0], d_den[1], d_den[2], cpE, lpE, cpH, lpH, v.DB, v.dx, v.dy, v.dz);
+    if (v.n_ports > 0) k_adj_ports_h<<<1, 64, 0, s>>>(g, ap, lHx, lHy, lHz, v.dx, v.dy, v.dz, d_bstep);
     k_adj_e<IdxT><<<dim3(cdiv(v.NX, 64), cdiv(v.NY, 4), v.NZ), bt, 0, s>>>(
         g, lEx, lEy, lEz, mEx, mEy, mEz, lHx, lHy, lHz, idx, d_tup, d_den[3], d_den[4], d_den[5], cpE, lpE, cpH, lpH,
         v.DB, v.dx, v.dy, v.dz);
@@ -2349,9 +2361,42 @@ bool drude_interior(const aem_plan_view& v) {
     return true;
 }
 
+// The cells a lumped port touches: its gap [loc, loc+size-1] and the H of
+// the loop around it, within [loc-1, loc+size-1] per axis.
+void port_reach(const aem_port_view& p, int lo[3], int hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = p.loc[a] - 1;
+        hi[a] = p.loc[a] + p.size[a] - 1;
+    }
+}
+
+// The transposed port step covers ports clear of the CPML slabs and the
+// outer boundary by a cell (no psi term and no fixed component in reach)
+// and of the Drude boxes (the gap's overwrite follows the Drude update).
+const char* ports_unsupported(const aem_plan_view& v) {
+    const int N[3] = {v.NX, v.NY, v.NZ};
+    for (int q = 0; q < v.n_ports; ++q) {
+        int lo[3], hi[3];
+        port_reach(v.ports[q], lo, hi);
+        for (int a = 0; a < 3; ++a)
+            if (lo[a] < std::max(v.npml[2 * a], 0) + 2 || hi[a] > N[a] - std::max(v.npml[2 * a + 1], 0) - 2)
+                return "lumped ports within a cell of the CPML or the outer boundary";
+        if (v.has_drude)
+            for (int c = 0; c < 3; ++c) {
+                int dl[3], dh[3];
+                if (!drude_box(v, c, dl, dh)) continue;
+                bool meet = true;
+                for (int a = 0; a < 3; ++a) meet = meet && lo[a] <= dh[a] && hi[a] >= dl[a];
+                if (meet) return "lumped ports inside the Drude media's bounding box";
+            }
+    }
+    return nullptr;
+}
+
 // Features whose update is not part of the transposed step.
 const char* adjoint_unsupported(const aem_plan_view& v) {
-    if (v.n_ports > 0) return "lumped ports";
+    if (v.n_spice > 0) return "SPICE ports";
+    if (const char* why = ports_unsupported(v)) return why;
     if (v.n_sheet_planes[0] + v.n_sheet_planes[1] + v.n_sheet_planes[2] > 0) return "thin sheets";
     if (v.has_drude && !drude_interior(v)) return "Drude media within a cell of the CPML or the outer boundary";
     if (v.pbc[0] || v.pbc[1] || v.pbc[2]) return "periodic boundaries";
@@ -2636,6 +2681,17 @@ aem_status aem_adjoint_record(aem_sim* s, const int32_t* lo, const int32_t* hi) 
                 return AEM_ERR_UNSUPPORTED;
             }
         }
+    for (int q = 0; q < v.n_ports; ++q) {
+        // the gap's overwrite and the loop integral stay outside the design edges
+        int pl[3], ph[3];
+        port_reach(v.ports[q], pl, ph);
+        bool meet = true;
+        for (int a = 0; a < 3; ++a) meet = meet && lo[a] <= ph[a] && hi[a] + 1 >= pl[a];
+        if (meet) {
+            aem::set_error("aem_adjoint_record: design box must not meet a lumped port's gap or loop");
+            return AEM_ERR_UNSUPPORTED;
+        }
+    }
     return aem::guard([&]() {
         AdjRec& r = s->rec;
         for (int a = 0; a < 3; ++a) {
@@ -2752,19 +2808,28 @@ namespace {
 // The adjoint source weights a step's terms by 2 Re(conj(F) w), the
 // derivative of J = |F|^2.  A general real J(F) with dJ/dF = dJ/dRe F +
 // i dJ/dIm F (dj, two doubles) has dJ = Re(conj(dj) dF): the same source
-// with F replaced by dj / 2.
-void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb, const double* dj = nullptr) {
+// with F replaced by dj / 2.  With port series sources (wV, wI: host
+// [np][T] or null) the far-field source is off unless dj is given.
+void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb, const double* dj = nullptr,
+                    const double* wV = nullptr, const double* wI = nullptr) {
     const aem_plan_view& v = s->plan->v;
     s->ensure_adjoint();
-    double F[2];
+    const bool ports_src = wV || wI;
+    const bool ff_src = s->obj_on && (dj || !ports_src);
+    double F[2] = {0.0, 0.0};
     if (dj) {
         F[0] = 0.5 * dj[0];
         F[1] = 0.5 * dj[1];
-    } else {
+    } else if (ff_src) {
         HIPCHK(hipMemcpy(F, s->obj.F, sizeof F, hipMemcpyDeviceToHost));
     }
     const long long ne = s->rec.nedge;
     std::vector<void*> tmp;
+    const size_t nw = (size_t)v.n_ports * v.time_steps;
+    double* d_wV = wV ? dalloc<double>(nw, tmp) : nullptr;
+    double* d_wI = wI ? dalloc<double>(nw, tmp) : nullptr;
+    if (wV) HIPCHK(hipMemcpyAsync(d_wV, wV, nw * sizeof(double), hipMemcpyHostToDevice, s->st));
+    if (wI) HIPCHK(hipMemcpyAsync(d_wI, wI, nw * sizeof(double), hipMemcpyHostToDevice, s->st));
     double* dga = dalloc<double>(3 * ne, tmp);
     double* dgb = dalloc<double>(3 * ne, tmp);
     HIPCHK(hipMemsetAsync(dga, 0, 3 * ne * sizeof(double), s->st));
@@ -2785,13 +2850,17 @@ void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb
                 HIPCHK(hipMemsetAsync(s->adb[c].J, 0, (size_t)s->adb[c].nb * v.n_poles * 4, s->st));
                 HIPCHK(hipMemsetAsync(s->adb[c].Eold, 0, (size_t)s->adb[c].nb * 4, s->st));
             }
+    if (v.n_ports > 0) {
+        HIPCHK(hipMemsetAsync(s->d_lV, 0, (size_t)v.n_ports * v.M * 4, s->st));
+        HIPCHK(hipMemsetAsync(s->d_lI, 0, (size_t)v.n_ports * v.M * 4, s->st));
+    }
     const int T = v.time_steps;
     HIPCHK(hipMemcpyAsync(s->d_bstep, &T, sizeof(int), hipMemcpyHostToDevice, s->st));
     hipGraph_t gr = nullptr;
     hipGraphExec_t gx = nullptr;
     HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-    if (v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, true, F[0], F[1], dga, dgb);
-    else s->enqueue_backward_t<uint16_t>(s->st, true, F[0], F[1], dga, dgb);
+    if (v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, ff_src, F[0], F[1], d_wV, d_wI, dga, dgb);
+    else s->enqueue_backward_t<uint16_t>(s->st, ff_src, F[0], F[1], d_wV, d_wI, dga, dgb);
     HIPCHK(hipStreamEndCapture(s->st, &gr));
     HIPCHK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
     HIPCHK(hipEventRecord(s->ev0, s->st));
@@ -2910,6 +2979,47 @@ aem_status aem_adjoint(aem_sim* s, const double* dJ_dF, double* grad, int64_t n)
     });
 }
 
+aem_status aem_adjoint_series(aem_sim* s, const double* dJ_dV, const double* dJ_dI, double* grad, int64_t n) {
+    if (!s || (!dJ_dV && !dJ_dI) || !grad || n < 0) {
+        aem::set_error("aem_adjoint_series: bad argument");
+        return AEM_ERR_STATE;
+    }
+    const aem_plan_view& v = s->plan->v;
+    if (v.n_ports == 0) {
+        aem::set_error("aem_adjoint_series: the deck has no lumped port");
+        return AEM_ERR_STATE;
+    }
+    if (!s->rec_on || s->steps_done != v.time_steps) {
+        aem::set_error("aem_adjoint_series: needs a design record and a completed run");
+        return AEM_ERR_STATE;
+    }
+    const int64_t want = s->des_on ? (int64_t)s->des_eps.size() / 3 : (int64_t)s->plan->deck.mats.size();
+    if (s->des_on ? n != want : n > want) {
+        aem::set_error("aem_adjoint_series: n must be the design box's cell count " +
+                       std::string(s->des_on ? "" : "or at most the deck's material count ") + "(" +
+                       std::to_string(want) + ")");
+        return AEM_ERR_STATE;
+    }
+    return aem::guard([&]() {
+        std::vector<double> ha, hb, cells;
+        backward_sweep(s, ha, hb, nullptr, dJ_dV, dJ_dI);
+        cell_gradient(*s->plan, s->rec, s->rec_lo, s->rec_hi, s->des_on ? s->des_eps.data() : nullptr, ha, hb, cells);
+        if (s->des_on) {
+            std::memcpy(grad, cells.data(), cells.size() * sizeof(double));
+            return;
+        }
+        for (int64_t m = 0; m < n; ++m) grad[m] = 0.0;
+        size_t b = 0;
+        for (int k = s->rec_lo[2]; k <= s->rec_hi[2]; ++k)
+            for (int j = s->rec_lo[1]; j <= s->rec_hi[1]; ++j)
+                for (int i = s->rec_lo[0]; i <= s->rec_hi[0]; ++i, ++b) {
+                    const int m = s->plan->cell_mat[(size_t)(i - 1) +
+                                                    (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (k - 1))];
+                    if (m >= 1 && m <= n) grad[m - 1] += cells[b];
+                }
+    });
+}
+
 aem_status aem_adjoint_step(aem_sim* s, int64_t n_steps) {
     if (!s || n_steps < 0) {
         aem::set_error("aem_adjoint_step: bad argument");
@@ -2922,8 +3032,9 @@ aem_status aem_adjoint_step(aem_sim* s, int64_t n_steps) {
     return aem::guard([&]() {
         s->ensure_adjoint();
         for (int64_t n = 0; n < n_steps; ++n) {
-            if (s->plan->v.idx_bytes == 1) s->enqueue_backward_t<uint8_t>(s->st, false, 0, 0, nullptr, nullptr);
-            else s->enqueue_backward_t<uint16_t>(s->st, false, 0, 0, nullptr, nullptr);
+            if (s->plan->v.idx_bytes == 1)
+                s->enqueue_backward_t<uint8_t>(s->st, false, 0, 0, nullptr, nullptr, nullptr, nullptr);
+            else s->enqueue_backward_t<uint16_t>(s->st, false, 0, 0, nullptr, nullptr, nullptr, nullptr);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s->st));

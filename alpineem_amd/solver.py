@@ -282,6 +282,43 @@ class Simulation:
         check(lib().aem_adjoint(self._h, g.ctypes.data, out.ctypes.data, n))
         return out.reshape(shape) if shape else out
 
+    def series_gradient(self, dJ_dV=None, dJ_dI=None, n_materials: int = 0) -> np.ndarray:
+        """Gradient of a general real objective J of the lumped-port series
+        (aem_adjoint_series): ``dJ_dV``, ``dJ_dI`` of shape (n_ports,
+        time_steps) hold dJ/dV and dJ/dI of every port and step (either may be
+        None).  Per design cell with a design box, else per deck material
+        1..n_materials."""
+        v = self.plan.view
+        T, npt = int(v.time_steps), int(v.n_ports)
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            if a.shape != (npt, T):
+                raise ValueError(f"series derivatives must have shape {(npt, T)}, got {a.shape}")
+            return a
+        wv, wi = arr(dJ_dV), arr(dJ_dI)
+        shape = getattr(self, "_design_shape", None)
+        n = int(np.prod(shape)) if shape else int(n_materials)
+        out = np.zeros(n, np.float64)
+        check(lib().aem_adjoint_series(self._h, None if wv is None else wv.ctypes.data,
+                                       None if wi is None else wi.ctypes.data, out.ctypes.data, n))
+        return out.reshape(shape) if shape else out
+
+    def port_gradient(self, port: int, freq_hz: float, dJ_dG: complex, n_materials: int = 0) -> np.ndarray:
+        """Gradient of a real J(G) of one port's voltage spectrum
+        G = dft('port_v', [freq_hz], port) given dJ_dG = dJ/dRe G + i dJ/dIm G:
+        dJ/dV(step n+1) = Re(conj(dJ_dG) exp(-2 pi i f n dt))."""
+        v = self.plan.view
+        T = int(v.time_steps)
+        n = np.arange(T, dtype=np.float64)
+        p = float(freq_hz) * float(v.dt) * n
+        p -= np.floor(p)
+        w = np.zeros((int(v.n_ports), T), np.float64)
+        w[port] = np.real(np.conj(complex(dJ_dG)) * np.exp(-2j * np.pi * p))
+        return self.series_gradient(dJ_dV=w, n_materials=n_materials)
+
     def design_gradient(self) -> np.ndarray:
         """d|F|^2 / d eps_r of every design cell, shape ``design_shape``
         (aem_adjoint_design_gradient); after the full run."""

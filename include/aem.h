@@ -331,6 +331,23 @@ aem_status aem_adjoint_design_gradient(aem_sim* sim, double* grad, int64_t n);
  * enters only through dJ_dF. */
 aem_status aem_adjoint(aem_sim* sim, const double* dJ_dF, double* grad, int64_t n);
 
+/* Gradient of a general real objective J of the lumped-port series
+ * (AEM_SERIES_PORT_V, AEM_SERIES_PORT_I: one value per step and port),
+ * given dJ_dV[q * time_steps + n] = dJ/dV_q(step n+1) and dJ_dI likewise
+ * (either may be null), evaluated on the run's series; grad and n as for
+ * aem_adjoint.  The far-field objective, if set, does not enter.  The
+ * transposed step includes the ports' 1-D lines, the gap fields and the
+ * loop currents (fdtd_solver.f90:4800-5013, 6120-6223).  The reference's
+ * adjoint workflow (adjoint_optimization.py:76-80) minimises
+ * J = |S11_goal - S11|^2 with S11 = V(f0)/Vinc(f0) - 1 (post_processor.py:
+ * 255-262), V(f0) = sum_n V(step n+1) exp(-2 pi i f0 n dt) (aem_get_dft):
+ * dJ/dV(step n+1) = Re(conj(g) exp(-2 pi i f0 n dt)) with
+ * g = 2 (S11 - S11_goal) / conj(Vinc(f0)).  Requires a design record
+ * (aem_adjoint_record or aem_design_box) and a completed run; ports must lie
+ * a cell clear of the CPML slabs, of the outer boundary and of Drude media
+ * (AEM_ERR_UNSUPPORTED at aem_adjoint_record otherwise). */
+aem_status aem_adjoint_series(aem_sim* sim, const double* dJ_dV, const double* dJ_dI, double* grad, int64_t n);
+
 /* Low-level adjoint access: apply the transposed step (no source) n_steps
  * times to the adjoint state, and read/write its six field components
  * (layout of aem_get_field).  The adjoint CPML state starts at zero. */

@@ -1651,6 +1651,21 @@ int oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int6
     return 0;
 }
 
+int oracle_get_series64(const oracle_state* s, int kind, int idx, double* out, int64_t n) {
+    const aem_plan_view* p = &s->p;
+    const int T = p->time_steps;
+    if (n > T) return -1;
+    const real* src = NULL;
+    switch (kind) {
+        case 0: if (idx < 0 || idx >= p->n_ports) return -1; src = s->Vout + (int64_t)idx * T; break;
+        case 1: if (idx < 0 || idx >= p->n_ports) return -1; src = s->Iout + (int64_t)idx * T; break;
+        case 2: src = s->incident; break;
+        default: return -1;
+    }
+    for (int64_t q = 0; q < n; ++q) out[q] = (double)src[q];
+    return 0;
+}
+
 int oracle_get_ntff_raw(const oracle_state* s, int which, double* out, int64_t n) {
     const aem_plan_view* p = &s->p;
     if (which < 0 || which > 5 || p->n_angles <= 0) return -1;

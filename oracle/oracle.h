@@ -46,6 +46,9 @@ int           oracle_set_field(oracle_state* s, int comp, const float* in, int64
  * imaginary) a kind or field component OR-ed with 0x100 selects the
  * imaginary part in every accessor. */
 int           oracle_get_series(const oracle_state* s, int kind, int idx, float* out, int64_t n);
+/* the port and incident series (kinds 0..2) in double: the FP64 build's
+ * full precision, for finite differences */
+int           oracle_get_series64(const oracle_state* s, int kind, int idx, double* out, int64_t n);
 /* raw NTFF accumulators as double: which 0..5 = Ux,Uy,Uz,Wx,Wy,Wz */
 int           oracle_get_ntff_raw(const oracle_state* s, int which, double* out, int64_t n);
 /* far-field phase correction time per angle (after finalize) */

@@ -28,6 +28,7 @@ def load(fp64: bool = False) -> C.CDLL:
         L.oracle_get_field64.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_set_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_get_series.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64]
+        L.oracle_get_series64.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_get_ntff_raw.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
         L.oracle_get_ff_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_get_video.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64]
@@ -73,6 +74,13 @@ class Oracle:
         n = self.plan.time_steps if n is None else n
         out = np.zeros(n, np.float32)
         if self.L.oracle_get_series(self.h, SERIES[kind] | (IMAG if imag else 0), idx, out.ctypes.data, n) != 0:
+            raise ValueError(kind)
+        return out
+
+    def series64(self, kind: str, idx: int = 0) -> np.ndarray:
+        """A port or incident series in double (the FP64 build's precision)."""
+        out = np.zeros(self.plan.time_steps, np.float64)
+        if self.L.oracle_get_series64(self.h, SERIES[kind], idx, out.ctypes.data, out.size) != 0:
             raise ValueError(kind)
         return out
 

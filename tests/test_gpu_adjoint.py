@@ -748,3 +748,179 @@ def test_drude_design_box_refusals():
     with pytest.raises(AemError, match="Drude"):
         sim.design_box((15, 15, 15), (16, 16, 16))
     sim.close()
+
+
+# ---------------------------------------------------------------------------
+# lumped ports in the transposed step, and the port-series objective
+# (fdtd_solver.f90:4800-5013, 6120-6223; adjoint_optimization.py:76-80)
+# ---------------------------------------------------------------------------
+def port_dot_deck():
+    """The dot-product deck with two lumped ports (z and x gaps) clear of
+    the CPML; the lines start at rest."""
+    d = dot_deck()
+    d.ports = [decks.Port("z", 50.0, (16, 16, 16), (0, 0, 1)), decks.Port("x", 75.0, (14, 18, 17), (2, 0, 0))]
+    return d
+
+
+@pytest.mark.parametrize("steps", [1, 5])
+def test_port_transposed_step_dot_product(steps):
+    """<lam, A x> = <A^T lam, x> with the ports' lines, loop currents and
+    gap fields in the step: from one step on, the gap field couples the
+    field state to the line and the line back to H."""
+    plan = Plan.from_text(port_dot_deck().text())
+    assert plan.view.n_ports == 2
+    m = masks(plan)
+    rng = np.random.default_rng(53 + steps)
+    x = {n: np.where(m[n], rng.standard_normal(plan.shape), 0.0).astype(np.float32) for n in NAMES}
+    lam = {n: np.where(m[n], rng.standard_normal(plan.shape), 0.0).astype(np.float32) for n in NAMES}
+    sim = Simulation(plan)
+    for n in NAMES:
+        sim.set_field(n, x[n])
+    sim.run(steps, graph=False)
+    y = {n: sim.field(n).astype(np.float64) for n in NAMES}
+    for n in NAMES:
+        sim.set_adjoint_field(n, lam[n])
+    sim.adjoint_step(steps)
+    mu = {n: sim.adjoint_field(n).astype(np.float64) for n in NAMES}
+    terms_l = np.concatenate([(lam[n] * y[n]).ravel() for n in NAMES])
+    terms_r = np.concatenate([(np.where(m[n], mu[n], 0.0) * x[n]).ravel() for n in NAMES])
+    lhs, rhs = terms_l.sum(), terms_r.sum()
+    scale = math.sqrt((terms_l ** 2).sum() + (terms_r ** 2).sum())
+    print(f"ports steps={steps} <lam,Ax>={lhs:.9e} <A^T lam,x>={rhs:.9e} rel={(lhs - rhs) / scale:.3e}")
+    assert abs(lhs - rhs) <= 1e-5 * scale
+    sim.close()
+
+
+PORT_LO, PORT_HI = (26, 23, 22), (28, 25, 26)     # 3 x 3 x 5 design cells beside the dipole's gap
+PORT_F0 = 3e9
+S11_GOAL = 0.6 * np.exp(0.4j)
+
+
+def port_deck(eps_cells=None):
+    """The 48^3 dipole (50 ohm port, arms of 10 cells) with a lossy
+    dielectric block (eps 2, sigma 0.05) filling the design box; with
+    ``eps_cells`` every box cell is its own material of that permittivity."""
+    d = decks.dipole(n=48, steps=900, angles=0, arm=10)
+    size = tuple(PORT_HI[a] - PORT_LO[a] + 1 for a in range(3))
+    d.materials.append(decks.Material(30, (2.0, 2.0, 2.0), (0.05, 0.05, 0.05)))
+    d.objects.append(decks.Block(30, PORT_LO, size))
+    if eps_cells is not None:
+        q = 0
+        for k in range(size[2]):
+            for j in range(size[1]):
+                for i in range(size[0]):
+                    e = float(eps_cells[k, j, i])
+                    d.materials.append(decks.Material(100 + q, (e, e, e), (0.05, 0.05, 0.05)))
+                    d.objects.append(decks.Block(100 + q, (PORT_LO[0] + i, PORT_LO[1] + j, PORT_LO[2] + k),
+                                                 (1, 1, 1)))
+                    q += 1
+    return d
+
+
+def port_eps(seed=8):
+    shape = tuple(PORT_HI[a] - PORT_LO[a] + 1 for a in (2, 1, 0))
+    return np.random.default_rng(seed).uniform(2.0, 8.0, shape).astype(np.float32)
+
+
+def spectrum(x, f, dt):
+    n = np.arange(len(x), dtype=np.float64)
+    p = f * dt * n
+    return complex(np.sum(x * np.exp(-2j * np.pi * (p - np.floor(p)))))
+
+
+def oracle_port_objectives(plan):
+    """FP64 oracle run: J_s11 = |goal - S11|^2 at f0 (S11 = V/Vinc - 1) and
+    J_i = |I(f0)|^2 of the port current, with V, I, Vinc in double."""
+    orc = Oracle(plan, fp64=True)
+    orc.step(plan.time_steps)
+    orc.finalize()
+    dt = float(plan.view.dt)
+    V = spectrum(orc.series64("port_v", 0), PORT_F0, dt)
+    I = spectrum(orc.series64("port_i", 0), PORT_F0, dt)
+    vinc = spectrum(orc.series64("incident", 0), PORT_F0, dt)
+    s11 = V / vinc - 1
+    return abs(S11_GOAL - s11) ** 2, abs(I) ** 2
+
+
+def test_port_s11_gradient_matches_finite_difference():
+    """The reference workflow's objective J = |S11_goal - S11|^2 at 3 GHz
+    (adjoint_optimization.py:76-80), and J = |I(f0)|^2 of the port current,
+    differentiated per design cell through the transposed port lines: the
+    adjoint equals FP64 central differences of the oracle to 1e-3 on four
+    cells."""
+    eps = port_eps()
+    plan = Plan.from_text(port_deck().text())
+    sim = Simulation(plan, ntff="off")
+    sim.design_box(PORT_LO, PORT_HI)
+    sim.set_design_eps(eps)
+    sim.run()
+    sim.finalize()
+    V = complex(sim.dft("port_v", [PORT_F0], 0)[0])
+    I = complex(sim.dft("port_i", [PORT_F0], 0)[0])
+    vinc = complex(sim.dft("incident", [PORT_F0], 0)[0])
+    s11 = V / vinc - 1
+    print(f"S11(3 GHz) = {s11:.6f}")
+    g_s11 = sim.port_gradient(0, PORT_F0, 2 * (s11 - S11_GOAL) / np.conj(vinc))
+    # J = |I|^2 through the current series: dJ/dI(step n+1) = Re(conj(2 I) e^{-i w n dt})
+    T = plan.time_steps
+    ph = PORT_F0 * float(plan.view.dt) * np.arange(T, dtype=np.float64)
+    wI = np.real(np.conj(2 * I) * np.exp(-2j * np.pi * (ph - np.floor(ph))))[None, :]
+    g_i = sim.series_gradient(dJ_dI=wI)
+    sim.close()
+    rng = np.random.default_rng(31)
+    cells = [tuple(int(v) for v in np.unravel_index(q, eps.shape)) for q in rng.choice(eps.size, 4, replace=False)]
+    h = 0.05
+    for c in cells:
+        Js, Ji, e32 = [], [], []
+        for sgn in (1, -1):
+            e = eps.copy()
+            e[c] = np.float32(e[c] + sgn * h)
+            e32.append(float(e[c]))
+            js, ji = oracle_port_objectives(Plan.from_text(port_deck(e).text()))
+            Js.append(js)
+            Ji.append(ji)
+        fd_s = (Js[0] - Js[1]) / (e32[0] - e32[1])
+        fd_i = (Ji[0] - Ji[1]) / (e32[0] - e32[1])
+        print(f"cell {c}: S11 adjoint {g_s11[c]:.9e} fd {fd_s:.9e} | I adjoint {g_i[c]:.9e} fd {fd_i:.9e}")
+        assert abs(g_s11[c] - fd_s) <= 1e-3 * abs(fd_s)
+        assert abs(g_i[c] - fd_i) <= 1e-3 * abs(fd_i)
+
+
+def test_port_s11_autograd_wrapper():
+    """torch.autograd: (goal - port_s11(eps)).abs()**2 and its gradient equal
+    the direct API's."""
+    torch = pytest.importorskip("torch")
+    from alpineem_amd.autograd import PortProblem, port_s11
+
+    eps0 = port_eps()
+    plan = Plan.from_text(port_deck().text())
+    prob = PortProblem(plan, PORT_LO, PORT_HI, PORT_F0, port=0)
+    eps = torch.tensor(eps0, dtype=torch.float64, requires_grad=True)
+    s11 = port_s11(eps, prob)
+    J = (torch.tensor(S11_GOAL, dtype=torch.complex128) - s11).abs() ** 2
+    J.backward()
+    sim = prob.simulate(eps0)
+    V = complex(sim.dft("port_v", [PORT_F0], 0)[0])
+    vinc = complex(sim.dft("incident", [PORT_F0], 0)[0])
+    s = V / vinc - 1
+    g = sim.port_gradient(0, PORT_F0, 2 * (s - S11_GOAL) / np.conj(vinc))
+    sim.close()
+    assert complex(s11.item()) == pytest.approx(s, rel=1e-12)
+    assert J.item() == pytest.approx(abs(S11_GOAL - s) ** 2, rel=1e-12)
+    assert np.allclose(eps.grad.numpy(), g, rtol=1e-9, atol=0)
+
+
+def test_port_adjoint_refusals():
+    """A design box meeting a port's gap or loop, and a port at the CPML,
+    are refused with AEM_ERR_UNSUPPORTED; SPICE decks stay refused."""
+    from alpineem_amd._lib import AemError
+    sim = Simulation(Plan.from_text(port_deck().text()))
+    with pytest.raises(AemError, match="port"):
+        sim.design_box((24, 23, 22), (26, 25, 26))
+    sim.close()
+    d = port_dot_deck()
+    d.ports = [decks.Port("z", 50.0, (16, 16, 5), (0, 0, 1))]
+    sim = Simulation(Plan.from_text(d.text()))
+    with pytest.raises(AemError, match="port"):
+        sim.design_box((16, 16, 16), (17, 17, 17))
+    sim.close()
