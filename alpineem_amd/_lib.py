@@ -182,4 +182,8 @@ class PlanView(C.Structure):
                 ("kmax", C.c_int32), ("mode_type", C.c_int32), ("k_direction", C.c_int32), ("k_axis", C.c_int32),
                 ("kpl_H", C.c_int32), ("kpl_E", C.c_int32), ("k_vec", C.c_float * 3), ("k_exc", C.c_float * 3),
                 ("f_adj", C.c_float), ("bloch_c", C.c_float * 3), ("bloch_s", C.c_float * 3),
-                ("src_table_im", C.POINTER(C.c_float)), ("ksrc", C.POINTER(C.c_float))]
+                ("src_table_im", C.POINTER(C.c_float)), ("ksrc", C.POINTER(C.c_float)),
+                ("mirror_type", C.c_int32), ("wall", C.c_float * 6), ("delay_m", C.c_int32 * 3),
+                ("WE_m", C.c_float * 3), ("WH_m", C.c_float * 3), ("s_thm", C.c_float), ("c_thm", C.c_float),
+                ("s_phm", C.c_float), ("c_phm", C.c_float), ("use_m", C.c_float * 3), ("off_m", C.c_float * 3),
+                ("sgn_M", C.c_float * 3), ("sgn_J", C.c_float * 3)]

@@ -1014,6 +1014,7 @@ struct Tfsf {
     float dx, dy, dz, dt, inv_c, spread, t_spread;
     float exp_half;            // expf(0.5) as evaluated on the host
     int pulse_type;
+    float wall[6];             // face factors: 0 on an infinite ground plane's face (f90:4627-4790)
 };
 
 // FP32 exp rounded from an FP64 evaluation: the correctly rounded value
@@ -1057,32 +1058,32 @@ __device__ __forceinline__ void h_tfsf_body(const Grid& g, const Tfsf& p, int fa
     if (face == 0) {  // u = i, v = k
         const int i = xl + a, k = zl + b;
         if (k <= zh && i <= xh - 1 + px) {
-            Hz[off(g, i, yl - 1, k)] = Hz[off(g, i, yl - 1, k)] - p.kH[1] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)yl, (float)k, cf) * 1.0f;
-            Hz[off(g, i, yh, k)] = Hz[off(g, i, yh, k)] + p.kH[1] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)yh, (float)k, cf) * 1.0f;
+            Hz[off(g, i, yl - 1, k)] = Hz[off(g, i, yl - 1, k)] - p.kH[1] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)yl, (float)k, cf) * p.wall[2];
+            Hz[off(g, i, yh, k)] = Hz[off(g, i, yh, k)] + p.kH[1] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)yh, (float)k, cf) * p.wall[3];
         }
         if (k <= zh - 1 + pz && i <= xh) {
-            Hx[off(g, i, yl - 1, k)] = Hx[off(g, i, yl - 1, k)] + p.kH[1] * inc_fn(p, p.WE[2], (float)i, (float)yl, (float)k + 0.5f, cf) * 1.0f;
-            Hx[off(g, i, yh, k)] = Hx[off(g, i, yh, k)] - p.kH[1] * inc_fn(p, p.WE[2], (float)i, (float)yh, (float)k + 0.5f, cf) * 1.0f;
+            Hx[off(g, i, yl - 1, k)] = Hx[off(g, i, yl - 1, k)] + p.kH[1] * inc_fn(p, p.WE[2], (float)i, (float)yl, (float)k + 0.5f, cf) * p.wall[2];
+            Hx[off(g, i, yh, k)] = Hx[off(g, i, yh, k)] - p.kH[1] * inc_fn(p, p.WE[2], (float)i, (float)yh, (float)k + 0.5f, cf) * p.wall[3];
         }
     } else if (face == 1) {  // u = i, v = j
         const int i = xl + a, j = yl + b;
         if (j <= yh && i <= xh - 1 + px) {
-            Hy[off(g, i, j, zl - 1)] = Hy[off(g, i, j, zl - 1)] + p.kH[2] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)j, (float)zl, cf) * 1.0f;
-            Hy[off(g, i, j, zh)] = Hy[off(g, i, j, zh)] - p.kH[2] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)j, (float)zh, cf) * 1.0f;
+            Hy[off(g, i, j, zl - 1)] = Hy[off(g, i, j, zl - 1)] + p.kH[2] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)j, (float)zl, cf) * p.wall[4];
+            Hy[off(g, i, j, zh)] = Hy[off(g, i, j, zh)] - p.kH[2] * inc_fn(p, p.WE[0], (float)i + 0.5f, (float)j, (float)zh, cf) * p.wall[5];
         }
         if (j <= yh - 1 + py && i <= xh) {
-            Hx[off(g, i, j, zl - 1)] = Hx[off(g, i, j, zl - 1)] - p.kH[2] * inc_fn(p, p.WE[1], (float)i, (float)j + 0.5f, (float)zl, cf) * 1.0f;
-            Hx[off(g, i, j, zh)] = Hx[off(g, i, j, zh)] + p.kH[2] * inc_fn(p, p.WE[1], (float)i, (float)j + 0.5f, (float)zh, cf) * 1.0f;
+            Hx[off(g, i, j, zl - 1)] = Hx[off(g, i, j, zl - 1)] - p.kH[2] * inc_fn(p, p.WE[1], (float)i, (float)j + 0.5f, (float)zl, cf) * p.wall[4];
+            Hx[off(g, i, j, zh)] = Hx[off(g, i, j, zh)] + p.kH[2] * inc_fn(p, p.WE[1], (float)i, (float)j + 0.5f, (float)zh, cf) * p.wall[5];
         }
     } else {  // u = j, v = k
         const int j = yl + a, k = zl + b;
         if (k <= zh && j <= yh - 1 + py) {
-            Hz[off(g, xl - 1, j, k)] = Hz[off(g, xl - 1, j, k)] + p.kH[0] * inc_fn(p, p.WE[1], (float)xl, (float)j + 0.5f, (float)k, cf) * 1.0f;
-            Hz[off(g, xh, j, k)] = Hz[off(g, xh, j, k)] - p.kH[0] * inc_fn(p, p.WE[1], (float)xh, (float)j + 0.5f, (float)k, cf) * 1.0f;
+            Hz[off(g, xl - 1, j, k)] = Hz[off(g, xl - 1, j, k)] + p.kH[0] * inc_fn(p, p.WE[1], (float)xl, (float)j + 0.5f, (float)k, cf) * p.wall[0];
+            Hz[off(g, xh, j, k)] = Hz[off(g, xh, j, k)] - p.kH[0] * inc_fn(p, p.WE[1], (float)xh, (float)j + 0.5f, (float)k, cf) * p.wall[1];
         }
         if (k <= zh - 1 + pz && j <= yh) {
-            Hy[off(g, xl - 1, j, k)] = Hy[off(g, xl - 1, j, k)] - p.kH[0] * inc_fn(p, p.WE[2], (float)xl, (float)j, (float)k + 0.5f, cf) * 1.0f;
-            Hy[off(g, xh, j, k)] = Hy[off(g, xh, j, k)] + p.kH[0] * inc_fn(p, p.WE[2], (float)xh, (float)j, (float)k + 0.5f, cf) * 1.0f;
+            Hy[off(g, xl - 1, j, k)] = Hy[off(g, xl - 1, j, k)] - p.kH[0] * inc_fn(p, p.WE[2], (float)xl, (float)j, (float)k + 0.5f, cf) * p.wall[0];
+            Hy[off(g, xh, j, k)] = Hy[off(g, xh, j, k)] + p.kH[0] * inc_fn(p, p.WE[2], (float)xh, (float)j, (float)k + 0.5f, cf) * p.wall[1];
         }
     }
 }
@@ -1097,32 +1098,32 @@ __global__ void k_e_tfsf(Grid g, Tfsf p, int face, float* Ex, float* Ey, float* 
     if (face == 0) {  // y faces: u = i (from xl), v = k (from zl)
         const int i = xl + a, k = zl + b;
         if (k <= zh - 1 + pz && i >= xl + px && i <= xh) {
-            Ez[off(g, i, yl, k)] = Ez[off(g, i, yl, k)] + p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yl - 0.5f, (float)k + 0.5f, cf) * 1.0f;
-            Ez[off(g, i, yh, k)] = Ez[off(g, i, yh, k)] - p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yh + 0.5f, (float)k + 0.5f, cf) * 1.0f;
+            Ez[off(g, i, yl, k)] = Ez[off(g, i, yl, k)] + p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yl - 0.5f, (float)k + 0.5f, cf) * p.wall[2];
+            Ez[off(g, i, yh, k)] = Ez[off(g, i, yh, k)] - p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yh + 0.5f, (float)k + 0.5f, cf) * p.wall[3];
         }
         if (k >= zl + pz && k <= zh && i <= xh - 1 + px) {
-            Ex[off(g, i, yl, k)] = Ex[off(g, i, yl, k)] - p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yl - 0.5f, (float)k, cf) * 1.0f;
-            Ex[off(g, i, yh, k)] = Ex[off(g, i, yh, k)] + p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yh + 0.5f, (float)k, cf) * 1.0f;
+            Ex[off(g, i, yl, k)] = Ex[off(g, i, yl, k)] - p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yl - 0.5f, (float)k, cf) * p.wall[2];
+            Ex[off(g, i, yh, k)] = Ex[off(g, i, yh, k)] + p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yh + 0.5f, (float)k, cf) * p.wall[3];
         }
     } else if (face == 1) {  // z faces: u = i, v = j
         const int i = xl + a, j = yl + b;
         if (j <= yh - 1 + py && i >= xl + px && i <= xh) {
-            Ey[off(g, i, j, zl)] = Ey[off(g, i, j, zl)] - p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zl - 0.5f, cf) * 1.0f;
-            Ey[off(g, i, j, zh)] = Ey[off(g, i, j, zh)] + p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zh + 0.5f, cf) * 1.0f;
+            Ey[off(g, i, j, zl)] = Ey[off(g, i, j, zl)] - p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zl - 0.5f, cf) * p.wall[4];
+            Ey[off(g, i, j, zh)] = Ey[off(g, i, j, zh)] + p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zh + 0.5f, cf) * p.wall[5];
         }
         if (j >= yl + py && j <= yh && i <= xh - 1 + px) {
-            Ex[off(g, i, j, zl)] = Ex[off(g, i, j, zl)] + p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zl - 0.5f, cf) * 1.0f;
-            Ex[off(g, i, j, zh)] = Ex[off(g, i, j, zh)] - p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zh + 0.5f, cf) * 1.0f;
+            Ex[off(g, i, j, zl)] = Ex[off(g, i, j, zl)] + p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zl - 0.5f, cf) * p.wall[4];
+            Ex[off(g, i, j, zh)] = Ex[off(g, i, j, zh)] - p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zh + 0.5f, cf) * p.wall[5];
         }
     } else {  // x faces: u = j, v = k
         const int j = yl + a, k = zl + b;
         if (k <= zh - 1 + pz && j >= yl + py && j <= yh) {
-            Ez[off(g, xl, j, k)] = Ez[off(g, xl, j, k)] - p.kE[0] * inc_fn(p, p.WH[1], (float)xl - 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f;
-            Ez[off(g, xh, j, k)] = Ez[off(g, xh, j, k)] + p.kE[0] * inc_fn(p, p.WH[1], (float)xh + 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f;
+            Ez[off(g, xl, j, k)] = Ez[off(g, xl, j, k)] - p.kE[0] * inc_fn(p, p.WH[1], (float)xl - 0.5f, (float)j, (float)k + 0.5f, cf) * p.wall[0];
+            Ez[off(g, xh, j, k)] = Ez[off(g, xh, j, k)] + p.kE[0] * inc_fn(p, p.WH[1], (float)xh + 0.5f, (float)j, (float)k + 0.5f, cf) * p.wall[1];
         }
         if (k >= zl + pz && k <= zh && j <= yh - 1 + py) {
-            Ey[off(g, xl, j, k)] = Ey[off(g, xl, j, k)] + p.kE[0] * inc_fn(p, p.WH[2], (float)xl - 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f;
-            Ey[off(g, xh, j, k)] = Ey[off(g, xh, j, k)] - p.kE[0] * inc_fn(p, p.WH[2], (float)xh + 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f;
+            Ey[off(g, xl, j, k)] = Ey[off(g, xl, j, k)] + p.kE[0] * inc_fn(p, p.WH[2], (float)xl - 0.5f, (float)j + 0.5f, (float)k, cf) * p.wall[0];
+            Ey[off(g, xh, j, k)] = Ey[off(g, xh, j, k)] - p.kE[0] * inc_fn(p, p.WH[2], (float)xh + 0.5f, (float)j + 0.5f, (float)k, cf) * p.wall[1];
         }
     }
 }
@@ -1144,17 +1145,17 @@ __global__ void k_e_tfsf3(Grid g, Tfsf p, float* Ex, float* Ey, float* Ez, const
     const int b = blockIdx.y * blockDim.y + threadIdx.y;
     // x-face terms (face 2) of Ez and Ey at the low (xl) or high (xh) face
     auto ez_x = [&](int x, int j, int k) {
-        return x == xl ? -(p.kE[0] * inc_fn(p, p.WH[1], (float)xl - 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f)
-                       : p.kE[0] * inc_fn(p, p.WH[1], (float)xh + 0.5f, (float)j, (float)k + 0.5f, cf) * 1.0f;
+        return x == xl ? -(p.kE[0] * inc_fn(p, p.WH[1], (float)xl - 0.5f, (float)j, (float)k + 0.5f, cf) * p.wall[0])
+                       : p.kE[0] * inc_fn(p, p.WH[1], (float)xh + 0.5f, (float)j, (float)k + 0.5f, cf) * p.wall[1];
     };
     auto ey_x = [&](int x, int j, int k) {
-        return x == xl ? p.kE[0] * inc_fn(p, p.WH[2], (float)xl - 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f
-                       : -(p.kE[0] * inc_fn(p, p.WH[2], (float)xh + 0.5f, (float)j + 0.5f, (float)k, cf) * 1.0f);
+        return x == xl ? p.kE[0] * inc_fn(p, p.WH[2], (float)xl - 0.5f, (float)j + 0.5f, (float)k, cf) * p.wall[0]
+                       : -(p.kE[0] * inc_fn(p, p.WH[2], (float)xh + 0.5f, (float)j + 0.5f, (float)k, cf) * p.wall[1]);
     };
     // z-face term (face 1) of Ex at the low (zl) or high (zh) face
     auto ex_z = [&](int i, int j, int z) {
-        return z == zl ? p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zl - 0.5f, cf) * 1.0f
-                       : -(p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zh + 0.5f, cf) * 1.0f);
+        return z == zl ? p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zl - 0.5f, cf) * p.wall[4]
+                       : -(p.kE[2] * inc_fn(p, p.WH[1], (float)i + 0.5f, (float)j, (float)zh + 0.5f, cf) * p.wall[5]);
     };
     if (face == 0) {
         if (!f0) return;
@@ -1162,22 +1163,22 @@ __global__ void k_e_tfsf3(Grid g, Tfsf p, float* Ex, float* Ey, float* Ez, const
         if (k <= zh - 1 + pz && i >= xl + px && i <= xh) {
             const bool e2 = f2 && (i == xl || i == xh);
             long long o = off(g, i, yl, k);
-            float e = Ez[o] + p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yl - 0.5f, (float)k + 0.5f, cf) * 1.0f;
+            float e = Ez[o] + p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yl - 0.5f, (float)k + 0.5f, cf) * p.wall[2];
             if (e2) e = e + ez_x(i, yl, k);
             Ez[o] = e;
             o = off(g, i, yh, k);
-            e = Ez[o] - p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yh + 0.5f, (float)k + 0.5f, cf) * 1.0f;
+            e = Ez[o] - p.kE[1] * inc_fn(p, p.WH[0], (float)i, (float)yh + 0.5f, (float)k + 0.5f, cf) * p.wall[3];
             if (e2) e = e + ez_x(i, yh, k);
             Ez[o] = e;
         }
         if (k >= zl + pz && k <= zh && i <= xh - 1 + px) {
             const bool e1 = f1 && (k == zl || k == zh);
             long long o = off(g, i, yl, k);
-            float e = Ex[o] - p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yl - 0.5f, (float)k, cf) * 1.0f;
+            float e = Ex[o] - p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yl - 0.5f, (float)k, cf) * p.wall[2];
             if (e1) e = e + ex_z(i, yl, k);
             Ex[o] = e;
             o = off(g, i, yh, k);
-            e = Ex[o] + p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yh + 0.5f, (float)k, cf) * 1.0f;
+            e = Ex[o] + p.kE[1] * inc_fn(p, p.WH[2], (float)i + 0.5f, (float)yh + 0.5f, (float)k, cf) * p.wall[3];
             if (e1) e = e + ex_z(i, yh, k);
             Ex[o] = e;
         }
@@ -1187,11 +1188,11 @@ __global__ void k_e_tfsf3(Grid g, Tfsf p, float* Ex, float* Ey, float* Ez, const
         if (j <= yh - 1 + py && i >= xl + px && i <= xh) {
             const bool e2 = f2 && (i == xl || i == xh);
             long long o = off(g, i, j, zl);
-            float e = Ey[o] - p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zl - 0.5f, cf) * 1.0f;
+            float e = Ey[o] - p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zl - 0.5f, cf) * p.wall[4];
             if (e2) e = e + ey_x(i, j, zl);
             Ey[o] = e;
             o = off(g, i, j, zh);
-            e = Ey[o] + p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zh + 0.5f, cf) * 1.0f;
+            e = Ey[o] + p.kE[2] * inc_fn(p, p.WH[0], (float)i, (float)j + 0.5f, (float)zh + 0.5f, cf) * p.wall[5];
             if (e2) e = e + ey_x(i, j, zh);
             Ey[o] = e;
         }
@@ -1663,6 +1664,7 @@ struct NtffFaces {
     int fd0[6];            // row length of each face array
     int lo[3], hi[3];
     int pbc[3];
+    float wall[6];         // face factors (0 on an infinite ground plane's face, f90:7223-7325)
 };
 
 __device__ __forceinline__ void ntff_currents_body(const Grid& g, const NtffFaces& f, int face, const float* Ex,
@@ -1677,29 +1679,30 @@ __device__ __forceinline__ void ntff_currents_body(const Grid& g, const NtffFace
     const bool low = (face & 1) == 0;
     const int pos = low ? f.lo[a] : f.hi[a];
     const float s = low ? 1.0f : -1.0f;      // sign flip between low and high faces
+    const float wl = f.wall[face];           // 0 on an infinite ground plane's face
     const long long fo = (long long)u + (long long)f.fd0[face] * v;
     float m0, m1, j0, j1;
     if (a == 0) {
         const int i = pos, j = u, k = v;
         const long long o = off(g, i, j, k);
-        m0 = (-1.0f * s) * (Ez[o] + Ez[o + g.PX] + Ez[o + 1] + Ez[o + 1 + g.PX]) / 4.0f * 1.0f;
-        m1 = s * (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f * 1.0f;
-        j0 = s * (Hz[o] + Hz[o + g.plane]) / 2.0f * 1.0f;
-        j1 = (-1.0f * s) * (Hy[o] + Hy[o + g.PX]) / 2.0f * 1.0f;
+        m0 = (-1.0f * s) * (Ez[o] + Ez[o + g.PX] + Ez[o + 1] + Ez[o + 1 + g.PX]) / 4.0f * wl;
+        m1 = s * (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f * wl;
+        j0 = s * (Hz[o] + Hz[o + g.plane]) / 2.0f * wl;
+        j1 = (-1.0f * s) * (Hy[o] + Hy[o + g.PX]) / 2.0f * wl;
     } else if (a == 1) {
         const int i = u, j = pos, k = v;
         const long long o = off(g, i, j, k);
-        m0 = s * (Ez[o] + Ez[o + 1] + Ez[o + g.PX] + Ez[o + 1 + g.PX]) / 4.0f * 1.0f;
-        m1 = (-1.0f * s) * (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f * 1.0f;
-        j0 = (-1.0f * s) * (Hz[o] + Hz[o + g.plane]) / 2.0f * 1.0f;
-        j1 = s * (Hx[o] + Hx[o + 1]) / 2.0f * 1.0f;
+        m0 = s * (Ez[o] + Ez[o + 1] + Ez[o + g.PX] + Ez[o + 1 + g.PX]) / 4.0f * wl;
+        m1 = (-1.0f * s) * (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f * wl;
+        j0 = (-1.0f * s) * (Hz[o] + Hz[o + g.plane]) / 2.0f * wl;
+        j1 = s * (Hx[o] + Hx[o + 1]) / 2.0f * wl;
     } else {
         const int i = u, j = v, k = pos;
         const long long o = off(g, i, j, k);
-        m0 = (-1.0f * s) * (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f * 1.0f;
-        m1 = s * (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f * 1.0f;
-        j0 = s * (Hy[o] + Hy[o + g.PX]) / 2.0f * 1.0f;
-        j1 = (-1.0f * s) * (Hx[o] + Hx[o + 1]) / 2.0f * 1.0f;
+        m0 = (-1.0f * s) * (Ey[o] + Ey[o + 1] + Ey[o + g.plane] + Ey[o + 1 + g.plane]) / 4.0f * wl;
+        m1 = s * (Ex[o] + Ex[o + g.PX] + Ex[o + g.plane] + Ex[o + g.PX + g.plane]) / 4.0f * wl;
+        j0 = s * (Hy[o] + Hy[o + g.PX]) / 2.0f * wl;
+        j1 = (-1.0f * s) * (Hx[o] + Hx[o + 1]) / 2.0f * wl;
     }
     f.Mo[face][0][fo] = f.M[face][0][fo];
     f.M[face][0][fo] = m0;
@@ -2008,13 +2011,15 @@ __device__ __forceinline__ float list_term_nb(const NtffList& L, const NtffBins&
     const float t = isW ? (nf + tfv - 1.0f) : (nf + 0.5f + tfv - 1.0f);
     const int ii = (int)t;
     hit = q.x != 0xffffffffu && (ii == bin || ii + 1 == bin);
-    const unsigned e = hit ? q.x : 0u;
+    // the top bit marks an image current of sign -1 (infinite ground plane)
+    const unsigned e = hit ? (q.x & 0x7fffffffu) : 0u;
+    const float sg = (hit && (q.x >> 31)) ? -1.0f : 1.0f;
     const float wt = t - (float)ii;
     const float* B = isW ? L.Jbase : L.Mbase;
     const float dd = B[e] - B[e + L.off_old];
     const int ax = (int)(e >> L.face_shift) >> 1;
     const float K = ax == 0 ? bn.K[0] : (ax == 1 ? bn.K[1] : bn.K[2]);
-    return ii == bin ? (1.0f - wt) * K * dd : wt * K * dd;
+    return (ii == bin ? (1.0f - wt) * K * dd : wt * K * dd) * sg;
 }
 
 // Entries per batch: their loads are issued together, then their terms are

@@ -160,7 +160,7 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir, cons
     Deck& d = P->deck;
     aem_plan_view& v = P->v;
     try {
-        if (d.is_mirror) throw Error(5, "infinite ground plane (IGP) decks are not supported by libaem");
+        if (d.is_mirror && d.kmax) throw Error(5, "an infinite ground plane (IGP) in a kmax deck is not supported by libaem");
         for (auto& p : d.ports)
             if (p.gridded) throw Error(5, "gridded ports are not supported by libaem");
         // SPICE ports (the reference's use_spice_version build): basic ones
@@ -237,24 +237,55 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir, cons
         for (int a = 0; a < 3; ++a) {
             lo[a] = 1; hi[a] = N3[a] - 1;
             if (!d.pbc[a]) { lo[a] = npml[2 * a] + opt.tfsf_inset; hi[a] = N3[a] - npml[2 * a + 1] - opt.tfsf_inset; }
-            v.tf_lo[a] = lo[a]; v.tf_hi[a] = hi[a];
         }
-        // plane-wave delays by incidence quadrant (f90:1451-1494)
-        int xd = 0, yd = 0, zd = 0;
-        const float hp = kPi / 2.0f;
-        if (theta >= 0.0f && theta <= hp) {
-            if (phi >= 0.0f && phi <= hp) { xd = lo[0]; yd = lo[1]; zd = lo[2]; }
-            if (phi > hp && phi <= kPi) { xd = hi[0]; yd = lo[1]; zd = lo[2]; }
-            if (phi > kPi && phi <= 3.0f * kPi / 2.0f) { xd = hi[0]; yd = hi[1]; zd = lo[2]; }
-            if (phi > 3.0f * kPi / 2.0f && phi < 2.0f * kPi) { xd = lo[0]; yd = hi[1]; zd = lo[2]; }
+        // infinite ground plane (f90:1402-1445): the TF/SF face on the ground
+        // moves to the plane (low side: the cell above it) and loses its
+        // corrections; that side's CPML shrinks to one surface (low) or none
+        // (high), which the sigma maximum below then sees (f90:1686-1688)
+        const int igp = d.is_mirror ? d.mirror_type : 0;
+        v.mirror_type = igp;
+        for (int f = 0; f < 6; ++f) v.wall[f] = 1.0f;
+        for (int a = 0; a < 3; ++a) {
+            v.use_m[a] = 0.0f; v.off_m[a] = 0.0f;
+            v.sgn_M[a] = 1.0f; v.sgn_J[a] = 1.0f;
         }
-        if (theta > hp && theta <= kPi) {
-            if (phi >= 0.0f && phi <= hp) { xd = lo[0]; yd = lo[1]; zd = hi[2]; }
-            if (phi > hp && phi <= kPi) { xd = hi[0]; yd = lo[1]; zd = hi[2]; }
-            if (phi > kPi && phi <= 3.0f * kPi / 2.0f) { xd = hi[0]; yd = hi[1]; zd = hi[2]; }
-            if (phi > 3.0f * kPi / 2.0f && phi < 2.0f * kPi) { xd = lo[0]; yd = hi[1]; zd = hi[2]; }
+        if (igp >= 1 && igp <= 6) {
+            const int a = (igp - 1) / 2;
+            const bool low = (igp & 1) == 1;
+            if (low) {
+                lo[a] = d.mirror_height + 1;
+                npml[2 * a] = 1;
+                v.off_m[a] = 2.0f * (float)lo[a];
+            } else {
+                hi[a] = d.mirror_height;
+                npml[2 * a + 1] = 0;
+                v.off_m[a] = 2.0f * (float)hi[a];
+            }
+            v.wall[2 * a + (low ? 0 : 1)] = 0.0f;
+            v.use_m[a] = 1.0f;
+            for (int f = 0; f < 6; ++f) v.npml[f] = npml[f];
         }
-        v.delay[0] = xd; v.delay[1] = yd; v.delay[2] = zd;
+        for (int a = 0; a < 3; ++a) { v.tf_lo[a] = lo[a]; v.tf_hi[a] = hi[a]; }
+        // plane-wave delays by incidence quadrant (f90:1451-1494; the mirrored
+        // wave's f90:1519-1565)
+        auto quadrant_delays = [&](float th, float ph, int* dl) {
+            const float hp = kPi / 2.0f;
+            if (th >= 0.0f && th <= hp) {
+                if (ph >= 0.0f && ph <= hp) { dl[0] = lo[0]; dl[1] = lo[1]; dl[2] = lo[2]; }
+                if (ph > hp && ph <= kPi) { dl[0] = hi[0]; dl[1] = lo[1]; dl[2] = lo[2]; }
+                if (ph > kPi && ph <= 3.0f * kPi / 2.0f) { dl[0] = hi[0]; dl[1] = hi[1]; dl[2] = lo[2]; }
+                if (ph > 3.0f * kPi / 2.0f && ph < 2.0f * kPi) { dl[0] = lo[0]; dl[1] = hi[1]; dl[2] = lo[2]; }
+            }
+            if (th > hp && th <= kPi) {
+                if (ph >= 0.0f && ph <= hp) { dl[0] = lo[0]; dl[1] = lo[1]; dl[2] = hi[2]; }
+                if (ph > hp && ph <= kPi) { dl[0] = hi[0]; dl[1] = lo[1]; dl[2] = hi[2]; }
+                if (ph > kPi && ph <= 3.0f * kPi / 2.0f) { dl[0] = hi[0]; dl[1] = hi[1]; dl[2] = hi[2]; }
+                if (ph > 3.0f * kPi / 2.0f && ph < 2.0f * kPi) { dl[0] = lo[0]; dl[1] = hi[1]; dl[2] = hi[2]; }
+            }
+        };
+        int dl[3] = {0, 0, 0};
+        quadrant_delays(theta, phi, dl);
+        for (int a = 0; a < 3; ++a) v.delay[a] = dl[a];
         const float A = d.pw_amp;                                      // f90:1499-1504
         const float s_ep_mu = std::sqrt(kEp0 / kMu0);
         v.WH[0] = A * (std::sin(pol) * std::sin(phi) + std::cos(pol) * std::cos(theta) * std::cos(phi)) * s_ep_mu;
@@ -263,6 +294,35 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir, cons
         v.WE[0] = A * (std::cos(pol) * std::sin(phi) - std::sin(pol) * std::cos(theta) * std::cos(phi));
         v.WE[1] = A * (-1.0f * std::cos(pol) * std::cos(phi) - std::sin(pol) * std::cos(theta) * std::sin(phi));
         v.WE[2] = A * (std::sin(pol) * std::sin(theta));
+        // the mirrored plane wave (f90:773-798 angles, 1375-1376, 1519-1585
+        // delays, 1590-1634 weights and image-current signs)
+        for (int a = 0; a < 3; ++a) { v.delay_m[a] = 0; v.WE_m[a] = 0.0f; v.WH_m[a] = 0.0f; }
+        v.s_thm = v.c_thm = v.s_phm = v.c_phm = 0.0f;
+        if (igp >= 1 && igp <= 6) {
+            const int a = (igp - 1) / 2;
+            float thm = 0.0f, phm = 0.0f;
+            if (d.excitation == 1) {   // the angles are read with a plane wave only
+                if (a == 0) { thm = d.theta; phm = 180.0f - d.phi; }
+                else if (a == 1) { thm = d.theta; phm = 360.0f - d.phi; }
+                else { thm = 180.0f - d.theta; phm = d.phi; }
+            }
+            thm = kPi / 180.0f * thm;
+            phm = kPi / 180.0f * phm;
+            v.s_thm = std::sin(thm); v.c_thm = std::cos(thm);
+            v.s_phm = std::sin(phm); v.c_phm = std::cos(phm);
+            int dm[3] = {0, 0, 0};
+            quadrant_delays(thm, phm, dm);
+            const int h = d.mirror_height;
+            dm[a] = h + (h - ((igp & 1) ? hi[a] : lo[a]));
+            for (int q = 0; q < 3; ++q) {
+                v.delay_m[q] = dm[q];
+                // E tangential to the ground flips, H normal to it flips
+                v.WE_m[q] = q == a ? v.WE[q] : -1.0f * v.WE[q];
+                v.WH_m[q] = q == a ? -1.0f * v.WH[q] : v.WH[q];
+                if (q != a) v.sgn_J[q] = -1.0f;
+            }
+            v.sgn_M[a] = -1.0f;
+        }
         const float dd[3] = {dx, dy, dz};
         for (int a = 0; a < 3; ++a) {
             v.kHinc[a] = dt / (kMu0 * dd[a]);
@@ -284,6 +344,13 @@ Plan* build_plan(const std::string& deck_text, const std::string& base_dir, cons
         for (int a = 0; a < 3; ++a) {                                  // f90:1638-1680
             v.ff_lo[a] = d.pbc[a] ? 1 : lo[a] - opt.ntff_gap;
             v.ff_hi[a] = d.pbc[a] ? N3[a] - 2 : hi[a] + opt.ntff_gap - 1;
+        }
+        if (igp >= 1 && igp <= 6) {   // the faces stop short of the ground plane (f90:1646-1669)
+            const int a = (igp - 1) / 2;
+            if (!d.pbc[a]) {
+                if (igp & 1) v.ff_lo[a] = lo[a] + 1;
+                else v.ff_hi[a] = hi[a] - 2;
+            }
         }
         {
             const float fpdc = 4.0f * kPi * dt * kC;

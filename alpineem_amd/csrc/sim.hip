@@ -155,6 +155,13 @@ bool single_pass_ok(const aem_plan_view& v) {
     return true;
 }
 
+// Image position of a far-field face cell across an infinite ground plane
+// along axis a, in the reference's FP32 form i (1 - use) + (offset - i) use
+// (f90:7957-7959); i itself off the plane's axis.
+int mirror_pos(const aem_plan_view& v, int a, int i) {
+    return (int)((float)i * (1.0f - v.use_m[a]) + (v.off_m[a] - (float)i) * v.use_m[a]);
+}
+
 // One bulk launch: a set of boxes and the slab tests compiled into its kernel.
 struct Launch {
     BoxSet bs;
@@ -302,6 +309,7 @@ struct aem_sim {
     float *d_incident = nullptr, *d_Eth = nullptr, *d_Eph = nullptr, *d_fftime = nullptr;
     float *d_reflf = nullptr, *d_transf = nullptr, *d_ffpc = nullptr, *d_scratch = nullptr;
     Tfsf tf{};
+    Tfsf tfm{};                    // an infinite ground plane's mirrored plane wave (v.mirror_type > 0)
     Pulse pu{};
     bool fuse_cpml = false;        // CPML corrections folded into the bulk kernels
     RDiv rdv[3] = {};              // the CPML kernels' divisions by dx, dy, dz
@@ -574,10 +582,15 @@ void aem_sim::create(const aem_sim_options* opt) {
     tf.s_th = v.s_th; tf.c_th = v.c_th; tf.s_ph = v.s_ph; tf.c_ph = v.c_ph;
     tf.dx = v.dx; tf.dy = v.dy; tf.dz = v.dz; tf.dt = v.dt; tf.inv_c = v.inv_c;
     tf.spread = v.spread; tf.t_spread = v.t_spread; tf.pulse_type = v.pulse_type; tf.exp_half = v.exp_half;
+    for (int f = 0; f < 6; ++f) tf.wall[f] = v.wall[f];
+    tfm = tf;   // the mirrored wave: its weights, angles and delays, the same faces
+    for (int a = 0; a < 3; ++a) { tfm.WE[a] = v.WE_m[a]; tfm.WH[a] = v.WH_m[a]; tfm.delay[a] = v.delay_m[a]; }
+    tfm.s_th = v.s_thm; tfm.c_th = v.c_thm; tfm.s_ph = v.s_phm; tfm.c_ph = v.c_phm;
 
     // NTFF
     if (v.n_angles > 0) {
         for (int a = 0; a < 3; ++a) { nf.lo[a] = v.ff_lo[a]; nf.hi[a] = v.ff_hi[a]; nf.pbc[a] = v.pbc[a]; }
+        for (int f = 0; f < 6; ++f) nf.wall[f] = v.wall[f];
         // the face arrays of every (face, slot) in one allocation, each at a
         // power-of-two stride, in blocks M, Mo, J, Jo (the list kernel reaches
         // any of them from one base and an element index)
@@ -624,6 +637,7 @@ void aem_sim::create(const aem_sim_options* opt) {
                 const float* an = v.ang + 6 * kk;
                 const float s1 = an[2], c1 = an[3], s2 = an[4], c2 = an[5];
                 float tmin = 1e30f, tmax = -1e30f;
+                for (int m = 0; m < (v.mirror_type > 0 ? 2 : 1); ++m)   // with a ground plane, the image cells too
                 for (int a = 0; a < 3; ++a) {
                     if (v.pbc[a]) continue;
                     const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
@@ -635,6 +649,7 @@ void aem_sim::create(const aem_sim_options* opt) {
                                 if (a == 0) { i = pos; j = uu; k = vv; }
                                 else if (a == 1) { i = uu; j = pos; k = vv; }
                                 else { i = uu; j = vv; k = pos; }
+                                if (m) { i = mirror_pos(v, 0, i); j = mirror_pos(v, 1, j); k = mirror_pos(v, 2, k); }
                                 const float tfv = (v.r_relay - ((float)(i - v.ic) * v.dx * s1 * c2 +
                                                                 (float)(j - v.jc) * v.dy * s1 * s2 +
                                                                 (float)(k - v.kc) * v.dz * c1)) / (aem::kC * v.dt);
@@ -670,6 +685,12 @@ void aem_sim::create(const aem_sim_options* opt) {
         // the histogram keeps one window of bins in LDS; larger windows bin
         // through global atomics
         if (ntff_mode == AEM_NTFF_HISTOGRAM && win_max > kHistMax) ntff_mode = AEM_NTFF_ATOMIC;
+        // an infinite ground plane's image currents are binned by the
+        // contributor lists only
+        if (v.mirror_type > 0 && (ntff_mode != AEM_NTFF_ORDERED || nl.n_waves == 0))
+            throw aem::Error(AEM_ERR_UNSUPPORTED,
+                             "the far field of an infinite ground plane deck needs the ordered binning from "
+                             "contributor lists (AEM_NTFF_ORDERED, ntff_row_scan 0)");
         if (ntff_mode == AEM_NTFF_ORDERED && nl.n_waves == 0)   // the row-scan kernel's limit
             for (int a = 0; a < 2; ++a)
                 if (!v.pbc[a] && 2 * (v.ff_hi[a] - v.ff_lo[a] + 1) > kExactCells)
@@ -1141,16 +1162,22 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             const dim3 gs(cdiv(sh[a].P, 32), cdiv(sh[a].Q, 8));
             k_h_sheets<<<gs, bb, 0, s>>>(g, sh[a], a, Hx, Hy, Hz, Ex, Ey, Ez, d_den[3], d_den[4], d_den[5], f);
         }
+    // the incident wave's faces, then (infinite ground plane) the mirrored
+    // wave's, as the reference orders them (f90:4627-4790, 5950-6111)
+    const int ntf = v.mirror_type > 0 ? 2 : 1;
     if ((parts & kPartHPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (fuse_cpml) {   // fused path: the three face pairs in one launch
-            if (nx > 0 && ny > 0 && nz > 0)
-                k_h_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, tf, Hx, Hy,
-                                                                                                        Hz, d_step);
-        } else {           // one launch per face pair, in the reference's order
-            if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Hx, Hy, Hz, d_step);
-            if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Hx, Hy, Hz, d_step);
-            if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Hx, Hy, Hz, d_step);
+        for (int w = 0; w < ntf; ++w) {
+            const Tfsf& t = w ? tfm : tf;
+            if (fuse_cpml) {   // fused path: the three face pairs in one launch
+                if (nx > 0 && ny > 0 && nz > 0)
+                    k_h_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, t, Hx, Hy,
+                                                                                                            Hz, d_step);
+            } else {           // one launch per face pair, in the reference's order
+                if (!v.pbc[1] && nx > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, t, 0, Hx, Hy, Hz, d_step);
+                if (!v.pbc[2] && nx > 0 && ny > 0) k_h_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, t, 1, Hx, Hy, Hz, d_step);
+                if (!v.pbc[0] && ny > 0 && nz > 0) k_h_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, t, 2, Hx, Hy, Hz, d_step);
+            }
         }
     }
     if ((parts & kPartHPort) && v.n_ports > 0) k_ports_h<<<1, 256, 0, s>>>(pa);
@@ -1192,14 +1219,17 @@ void aem_sim::enqueue_step_t(hipEvent_t* mark, hipStream_t s, unsigned parts) {
             if (db[c].nb > 0) k_drude<IdxT><<<cdiv(db[c].nb, 256), 256, 0, s>>>(g, db[c], c, v.n_poles, F[c], idx, d_tup);
     if ((parts & kPartEPre) && v.has_tfsf) {
         const int nx = v.tf_hi[0] - v.tf_lo[0] + 2, ny = v.tf_hi[1] - v.tf_lo[1] + 2, nz = v.tf_hi[2] - v.tf_lo[2] + 2;
-        if (fuse_cpml) {
-            if (nx > 0 && ny > 0 && nz > 0)
-                k_e_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, tf, Ex, Ey,
-                                                                                                        Ez, d_step);
-        } else {
-            if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 0, Ex, Ey, Ez, d_step);
-            if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, tf, 1, Ex, Ey, Ez, d_step);
-            if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, tf, 2, Ex, Ey, Ez, d_step);
+        for (int w = 0; w < ntf; ++w) {
+            const Tfsf& t = w ? tfm : tf;
+            if (fuse_cpml) {
+                if (nx > 0 && ny > 0 && nz > 0)
+                    k_e_tfsf3<<<dim3(cdiv(std::max(nx, ny), 32), cdiv(std::max(ny, nz), 8), 3), bb, 0, s>>>(g, t, Ex, Ey,
+                                                                                                            Ez, d_step);
+            } else {
+                if (!v.pbc[1] && nx > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(nz, 8)), bb, 0, s>>>(g, t, 0, Ex, Ey, Ez, d_step);
+                if (!v.pbc[2] && nx > 0 && ny > 0) k_e_tfsf<<<dim3(cdiv(nx, 32), cdiv(ny, 8)), bb, 0, s>>>(g, t, 1, Ex, Ey, Ez, d_step);
+                if (!v.pbc[0] && ny > 0 && nz > 0) k_e_tfsf<<<dim3(cdiv(ny, 32), cdiv(nz, 8)), bb, 0, s>>>(g, t, 2, Ex, Ey, Ez, d_step);
+            }
         }
     }
     if ((parts & kPartEPort) && v.n_ports > 0) k_ports_e<<<1, 256, 0, s>>>(pa);
@@ -1356,6 +1386,11 @@ void aem_sim::build_ntff_lists(const std::vector<int>& win, unsigned stride) {
             const int comp = arr % 3;
             const double shift = arr >= 3 ? -1.0 : -0.5;
             lists.assign((size_t)wlen, {});
+            // with a ground plane the image currents follow every face of the
+            // box, in the same order, their entries flagged with the top bit
+            // where the component's image sign is -1 (f90:7928-8240)
+            const int passes = v.mirror_type > 0 && v.pbc[0] + v.pbc[1] + v.pbc[2] < 2 ? 2 : 1;
+            for (int m = 0; m < passes; ++m)
             for (int a = 0; a < 3; ++a) {                     // face pairs in the reference's order
                 if (a == comp || v.pbc[a]) continue;
                 const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
@@ -1368,14 +1403,17 @@ void aem_sim::build_ntff_lists(const std::vector<int>& win, unsigned stride) {
                             if (a == 0) { i = pos; j = uu; k = vv; }
                             else if (a == 1) { i = uu; j = pos; k = vv; }
                             else { i = uu; j = vv; k = pos; }
+                            if (m) { i = mirror_pos(v, 0, i); j = mirror_pos(v, 1, j); k = mirror_pos(v, 2, k); }
                             const float tfv = (v.r_relay - ((float)(i - v.ic) * v.dx * s1 * c2 +
                                                             (float)(j - v.jc) * v.dy * s1 * s2 +
                                                             (float)(k - v.kc) * v.dz * c1)) / cdt;
                             const double x = (double)tfv + shift;
                             const int olo = (int)std::floor(x - eps), ohi = (int)std::floor(x + eps);
                             const int face = 2 * a + side;
-                            const unsigned idx = (unsigned)(2 * face + slot) * stride +
-                                                 (unsigned)(uu + nf.fd0[face] * vv);
+                            const float sg = m ? (arr >= 3 ? v.sgn_J[comp] : v.sgn_M[comp]) : 1.0f;
+                            const unsigned idx = ((unsigned)(2 * face + slot) * stride +
+                                                  (unsigned)(uu + nf.fd0[face] * vv)) |
+                                                 (sg < 0.0f ? 0x80000000u : 0u);
                             uint32_t tb;
                             std::memcpy(&tb, &tfv, 4);
                             for (int o = olo; o <= ohi + 1; ++o) {
@@ -2168,6 +2206,10 @@ aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
             aem::set_error("aem_batch_create: a member is a kmax (complex-field) simulation; run it alone");
             return AEM_ERR_UNSUPPORTED;
         }
+        if (sims[q]->plan->v.mirror_type > 0 && sims[q]->plan->v.n_angles > 0) {
+            aem::set_error("aem_batch_create: a member's far field has ground-plane images; run it alone");
+            return AEM_ERR_UNSUPPORTED;
+        }
     }
     aem_batch* b = new (std::nothrow) aem_batch();
     if (!b) {
@@ -2395,6 +2437,7 @@ const char* ports_unsupported(const aem_plan_view& v) {
 
 // Features whose update is not part of the transposed step.
 const char* adjoint_unsupported(const aem_plan_view& v) {
+    if (v.mirror_type > 0) return "an infinite ground plane";
     if (v.n_spice > 0) return "SPICE ports";
     if (const char* why = ports_unsupported(v)) return why;
     if (v.n_sheet_planes[0] + v.n_sheet_planes[1] + v.n_sheet_planes[2] > 0) return "thin sheets";
@@ -2480,6 +2523,10 @@ aem_status aem_objective_set(aem_sim* s, float theta_deg, float phi_deg, double 
         aem::set_error("aem_objective_set: needs a deck with a far-field box and a sim that has not stepped");
         return AEM_ERR_STATE;
     }
+    if (v.mirror_type > 0) {
+        aem::set_error("aem_objective_set: the frequency-domain far field has no image currents of a ground plane");
+        return AEM_ERR_UNSUPPORTED;
+    }
     return aem::guard([&]() {
         Objective& o = s->obj;
         for (int a = 0; a < 3; ++a) { o.lo[a] = v.ff_lo[a]; o.hi[a] = v.ff_hi[a]; o.pbc[a] = v.pbc[a]; }
@@ -2545,6 +2592,10 @@ aem_status aem_ffdft_set(aem_sim* s, int32_t n_ang, const float* theta_deg, cons
     if (!fresh(s) || v.n_angles <= 0 || s->ff_on) {
         aem::set_error("aem_ffdft_set: needs a deck with a far-field box and a sim that has not stepped (once)");
         return AEM_ERR_STATE;
+    }
+    if (v.mirror_type > 0) {
+        aem::set_error("aem_ffdft_set: the frequency-domain far field has no image currents of a ground plane");
+        return AEM_ERR_UNSUPPORTED;
     }
     return aem::guard([&]() {
         FfDft& p = s->ff;

@@ -154,10 +154,13 @@ class Deck:
     spice_ports: list = field(default_factory=list)
     netlist: str = "fdtd_netlist.cir"
     spice_extra_nodes: list = field(default_factory=list)
+    igp: Optional[Tuple[str, int]] = None            # infinite ground plane: ('-z', height) etc. (f90:746-765)
 
     def text(self) -> str:
         L = [_row(*self.cells), _row(*self.cell_size), _f(self.time_steps), _f(self.factor), _f(self.f_center),
-             _f(self.pulse_type), _row(*self.boundaries), "none", self.excitation]
+             _f(self.pulse_type), _row(*self.boundaries)]
+        L += ["IGP", self.igp[0], _f(self.igp[1])] if self.igp else ["none"]
+        L.append(self.excitation)
         if self.excitation == "plane wave":
             L += [_row(self.amplitude, self.delay), _row(*self.angles)]
         elif self.excitation == "antenna":

@@ -191,6 +191,21 @@ typedef struct aem_plan_view {
     const float *src_table_im;    /* imaginary parts of the complex src_table waveforms */
     const float *ksrc;            /* [4][time_steps]: the modal source's Gaussian factor and carrier phase
                                      for the H plane (time n-1), then for the E plane (time n-1/2) */
+
+    /* ---- infinite ground plane (IGP; deck f90:746-765, setup f90:1402-1445,
+     * 1571-1665): a PEC wall at a TF/SF face.  That face's TF/SF corrections
+     * and far-field currents are scaled by its wall factor 0, a mirrored
+     * plane wave is added after the incident one (f90:4712-4790,
+     * 6035-6111), and the far field gains the image currents
+     * (f90:7928-8240). */
+    int32_t mirror_type;          /* 0 none; 1..6 = -x,+x,-y,+y,-z,+z */
+    float   wall[6];              /* xlow_wall..zhigh_wall: 1, or 0 on the ground face */
+    int32_t delay_m[3];           /* x/y/z_delay_mirror */
+    float   WE_m[3], WH_m[3];     /* mirrored weights (f90:1590-1634) */
+    float   s_thm, c_thm, s_phm, c_phm;   /* sin/cos of theta_mirror and phi_mirror (radians, FP32) */
+    float   use_m[3];             /* use_x/y/z_mirror */
+    float   off_m[3];             /* x/y/z_mirror_offset */
+    float   sgn_M[3], sgn_J[3];   /* Mx..Mz_mirror, Jx..Jz_mirror */
 } aem_plan_view;
 
 /* Setup constants that the reference fixes in its source rather than reading

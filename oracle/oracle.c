@@ -146,6 +146,20 @@ static real inc_fn(const oracle_state* s, real w, real xf, real yf, real zf, rea
     return R(-1.0) * w * arg * EXP(R(0.5)) * EXP(R(-0.5) * (arg * arg));
 }
 
+/* Inc() of the incident (m = 0) or the ground plane's mirrored (m = 1)
+ * plane wave: the same function with theta_mirror, phi_mirror and the
+ * mirror delays (f90:4712-4790) */
+static real inc_w(const oracle_state* s, int m, real w, real xf, real yf, real zf, real cf) {
+    const aem_plan_view* p = &s->p;
+    if (!m) return inc_fn(s, w, xf, yf, zf, cf);
+    real tx = R(p->s_thm) * R(p->c_phm) * (xf - R(p->delay_m[0])) * R(p->dx);
+    real ty = R(p->s_thm) * R(p->s_phm) * (yf - R(p->delay_m[1])) * R(p->dy);
+    real tz = R(p->c_thm) * (zf - R(p->delay_m[2])) * R(p->dz);
+    real arg = (R(p->t_spread) - (cf * R(p->dt)) + R(p->inv_c) * (tx + ty + tz)) / R(p->spread);
+    if (p->pulse_type == 1) return w * EXP(R(-0.5) * (arg * arg));
+    return R(-1.0) * w * arg * EXP(R(0.5)) * EXP(R(-0.5) * (arg * arg));
+}
+
 /* E_or_V_1D() of f90:8844-8857; the kmax build's complex one (f90:8888-8900),
  * amp exp(-x^2/2) exp(i 2 pi f_adj ((tc-1) dt - pos/c)), gives this state's part */
 static real e_or_v(const oracle_state* s, real amp, real tc, real pos) {
@@ -374,50 +388,57 @@ static void h_sheets(oracle_state* s) {
 }
 
 /* TF/SF H faces, f90:4627-4710 (t = n-1) */
-static void h_tfsf(oracle_state* s, int n) {
+static void h_tfsf_wave(oracle_state* s, int n, int m) {
     const aem_plan_view* p = &s->p;
-    if (!p->has_tfsf) return;
     const int xl = p->tf_lo[0], xh = p->tf_hi[0], yl = p->tf_lo[1], yh = p->tf_hi[1], zl = p->tf_lo[2], zh = p->tf_hi[2];
     const int px = p->pbc[0], py = p->pbc[1], pz = p->pbc[2];
     const real cf = R(n) - R(1.0);
     const real kx = R(p->kHinc[0]), ky = R(p->kHinc[1]), kz = R(p->kHinc[2]);
-    const real WEx = R(p->WE[0]), WEy = R(p->WE[1]), WEz = R(p->WE[2]);
+    const float* W = m ? p->WE_m : p->WE;
+    const real WEx = R(W[0]), WEy = R(W[1]), WEz = R(W[2]);
     if (!py) {
         for (int k = zl; k <= zh; ++k)
             for (int i = xl; i <= xh - 1 + px; ++i) {
-                HZ(i, yl - 1, k) = HZ(i, yl - 1, k) - ky * inc_fn(s, WEx, R(i) + R(0.5), R(yl), R(k), cf) * R(1.0);
-                HZ(i, yh, k) = HZ(i, yh, k) + ky * inc_fn(s, WEx, R(i) + R(0.5), R(yh), R(k), cf) * R(1.0);
+                HZ(i, yl - 1, k) = HZ(i, yl - 1, k) - ky * inc_w(s, m, WEx, R(i) + R(0.5), R(yl), R(k), cf) * R(p->wall[2]);
+                HZ(i, yh, k) = HZ(i, yh, k) + ky * inc_w(s, m, WEx, R(i) + R(0.5), R(yh), R(k), cf) * R(p->wall[3]);
             }
         for (int k = zl; k <= zh - 1 + pz; ++k)
             for (int i = xl; i <= xh; ++i) {
-                HX(i, yl - 1, k) = HX(i, yl - 1, k) + ky * inc_fn(s, WEz, R(i), R(yl), R(k) + R(0.5), cf) * R(1.0);
-                HX(i, yh, k) = HX(i, yh, k) - ky * inc_fn(s, WEz, R(i), R(yh), R(k) + R(0.5), cf) * R(1.0);
+                HX(i, yl - 1, k) = HX(i, yl - 1, k) + ky * inc_w(s, m, WEz, R(i), R(yl), R(k) + R(0.5), cf) * R(p->wall[2]);
+                HX(i, yh, k) = HX(i, yh, k) - ky * inc_w(s, m, WEz, R(i), R(yh), R(k) + R(0.5), cf) * R(p->wall[3]);
             }
     }
     if (!pz) {
         for (int j = yl; j <= yh; ++j)
             for (int i = xl; i <= xh - 1 + px; ++i) {
-                HY(i, j, zl - 1) = HY(i, j, zl - 1) + kz * inc_fn(s, WEx, R(i) + R(0.5), R(j), R(zl), cf) * R(1.0);
-                HY(i, j, zh) = HY(i, j, zh) - kz * inc_fn(s, WEx, R(i) + R(0.5), R(j), R(zh), cf) * R(1.0);
+                HY(i, j, zl - 1) = HY(i, j, zl - 1) + kz * inc_w(s, m, WEx, R(i) + R(0.5), R(j), R(zl), cf) * R(p->wall[4]);
+                HY(i, j, zh) = HY(i, j, zh) - kz * inc_w(s, m, WEx, R(i) + R(0.5), R(j), R(zh), cf) * R(p->wall[5]);
             }
         for (int j = yl; j <= yh - 1 + py; ++j)
             for (int i = xl; i <= xh; ++i) {
-                HX(i, j, zl - 1) = HX(i, j, zl - 1) - kz * inc_fn(s, WEy, R(i), R(j) + R(0.5), R(zl), cf) * R(1.0);
-                HX(i, j, zh) = HX(i, j, zh) + kz * inc_fn(s, WEy, R(i), R(j) + R(0.5), R(zh), cf) * R(1.0);
+                HX(i, j, zl - 1) = HX(i, j, zl - 1) - kz * inc_w(s, m, WEy, R(i), R(j) + R(0.5), R(zl), cf) * R(p->wall[4]);
+                HX(i, j, zh) = HX(i, j, zh) + kz * inc_w(s, m, WEy, R(i), R(j) + R(0.5), R(zh), cf) * R(p->wall[5]);
             }
     }
     if (!px) {
         for (int k = zl; k <= zh; ++k)
             for (int j = yl; j <= yh - 1 + py; ++j) {
-                HZ(xl - 1, j, k) = HZ(xl - 1, j, k) + kx * inc_fn(s, WEy, R(xl), R(j) + R(0.5), R(k), cf) * R(1.0);
-                HZ(xh, j, k) = HZ(xh, j, k) - kx * inc_fn(s, WEy, R(xh), R(j) + R(0.5), R(k), cf) * R(1.0);
+                HZ(xl - 1, j, k) = HZ(xl - 1, j, k) + kx * inc_w(s, m, WEy, R(xl), R(j) + R(0.5), R(k), cf) * R(p->wall[0]);
+                HZ(xh, j, k) = HZ(xh, j, k) - kx * inc_w(s, m, WEy, R(xh), R(j) + R(0.5), R(k), cf) * R(p->wall[1]);
             }
         for (int k = zl; k <= zh - 1 + pz; ++k)
             for (int j = yl; j <= yh; ++j) {
-                HY(xl - 1, j, k) = HY(xl - 1, j, k) - kx * inc_fn(s, WEz, R(xl), R(j), R(k) + R(0.5), cf) * R(1.0);
-                HY(xh, j, k) = HY(xh, j, k) + kx * inc_fn(s, WEz, R(xh), R(j), R(k) + R(0.5), cf) * R(1.0);
+                HY(xl - 1, j, k) = HY(xl - 1, j, k) - kx * inc_w(s, m, WEz, R(xl), R(j), R(k) + R(0.5), cf) * R(p->wall[0]);
+                HY(xh, j, k) = HY(xh, j, k) + kx * inc_w(s, m, WEz, R(xh), R(j), R(k) + R(0.5), cf) * R(p->wall[1]);
             }
     }
+}
+
+static void h_tfsf(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    if (!p->has_tfsf) return;
+    h_tfsf_wave(s, n, 0);
+    if (p->mirror_type) h_tfsf_wave(s, n, 1);   /* after every face of the incident wave */
 }
 
 /* ports, H phase: loop integral, 1-D line current, source (f90:4800-5013) */
@@ -704,51 +725,58 @@ static void e_drude(oracle_state* s) {
 }
 
 /* TF/SF E faces, f90:5950-6033 (t = n-1/2) */
-static void e_tfsf(oracle_state* s, int n) {
+static void e_tfsf_wave(oracle_state* s, int n, int m) {
     const aem_plan_view* p = &s->p;
-    if (!p->has_tfsf) return;
     const int xl = p->tf_lo[0], xh = p->tf_hi[0], yl = p->tf_lo[1], yh = p->tf_hi[1], zl = p->tf_lo[2], zh = p->tf_hi[2];
     const int px = p->pbc[0], py = p->pbc[1], pz = p->pbc[2];
     const real cf = R(n) - R(0.5);
     const real kx = R(p->kEinc[0]), ky = R(p->kEinc[1]), kz = R(p->kEinc[2]);
-    const real WHx = R(p->WH[0]), WHy = R(p->WH[1]), WHz = R(p->WH[2]);
+    const float* W = m ? p->WH_m : p->WH;
+    const real WHx = R(W[0]), WHy = R(W[1]), WHz = R(W[2]);
     const real h = R(0.5);
     if (!py) {
         for (int k = zl; k <= zh - 1 + pz; ++k)
             for (int i = xl + px; i <= xh; ++i) {
-                EZ(i, yl, k) = EZ(i, yl, k) + ky * inc_fn(s, WHx, R(i), R(yl) - h, R(k) + h, cf) * R(1.0);
-                EZ(i, yh, k) = EZ(i, yh, k) - ky * inc_fn(s, WHx, R(i), R(yh) + h, R(k) + h, cf) * R(1.0);
+                EZ(i, yl, k) = EZ(i, yl, k) + ky * inc_w(s, m, WHx, R(i), R(yl) - h, R(k) + h, cf) * R(p->wall[2]);
+                EZ(i, yh, k) = EZ(i, yh, k) - ky * inc_w(s, m, WHx, R(i), R(yh) + h, R(k) + h, cf) * R(p->wall[3]);
             }
         for (int k = zl + pz; k <= zh; ++k)
             for (int i = xl; i <= xh - 1 + px; ++i) {
-                EX(i, yl, k) = EX(i, yl, k) - ky * inc_fn(s, WHz, R(i) + h, R(yl) - h, R(k), cf) * R(1.0);
-                EX(i, yh, k) = EX(i, yh, k) + ky * inc_fn(s, WHz, R(i) + h, R(yh) + h, R(k), cf) * R(1.0);
+                EX(i, yl, k) = EX(i, yl, k) - ky * inc_w(s, m, WHz, R(i) + h, R(yl) - h, R(k), cf) * R(p->wall[2]);
+                EX(i, yh, k) = EX(i, yh, k) + ky * inc_w(s, m, WHz, R(i) + h, R(yh) + h, R(k), cf) * R(p->wall[3]);
             }
     }
     if (!pz) {
         for (int j = yl; j <= yh - 1 + py; ++j)
             for (int i = xl + px; i <= xh; ++i) {
-                EY(i, j, zl) = EY(i, j, zl) - kz * inc_fn(s, WHx, R(i), R(j) + h, R(zl) - h, cf) * R(1.0);
-                EY(i, j, zh) = EY(i, j, zh) + kz * inc_fn(s, WHx, R(i), R(j) + h, R(zh) + h, cf) * R(1.0);
+                EY(i, j, zl) = EY(i, j, zl) - kz * inc_w(s, m, WHx, R(i), R(j) + h, R(zl) - h, cf) * R(p->wall[4]);
+                EY(i, j, zh) = EY(i, j, zh) + kz * inc_w(s, m, WHx, R(i), R(j) + h, R(zh) + h, cf) * R(p->wall[5]);
             }
         for (int j = yl + py; j <= yh; ++j)
             for (int i = xl; i <= xh - 1 + px; ++i) {
-                EX(i, j, zl) = EX(i, j, zl) + kz * inc_fn(s, WHy, R(i) + h, R(j), R(zl) - h, cf) * R(1.0);
-                EX(i, j, zh) = EX(i, j, zh) - kz * inc_fn(s, WHy, R(i) + h, R(j), R(zh) + h, cf) * R(1.0);
+                EX(i, j, zl) = EX(i, j, zl) + kz * inc_w(s, m, WHy, R(i) + h, R(j), R(zl) - h, cf) * R(p->wall[4]);
+                EX(i, j, zh) = EX(i, j, zh) - kz * inc_w(s, m, WHy, R(i) + h, R(j), R(zh) + h, cf) * R(p->wall[5]);
             }
     }
     if (!px) {
         for (int k = zl; k <= zh - 1 + pz; ++k)
             for (int j = yl + py; j <= yh; ++j) {
-                EZ(xl, j, k) = EZ(xl, j, k) - kx * inc_fn(s, WHy, R(xl) - h, R(j), R(k) + h, cf) * R(1.0);
-                EZ(xh, j, k) = EZ(xh, j, k) + kx * inc_fn(s, WHy, R(xh) + h, R(j), R(k) + h, cf) * R(1.0);
+                EZ(xl, j, k) = EZ(xl, j, k) - kx * inc_w(s, m, WHy, R(xl) - h, R(j), R(k) + h, cf) * R(p->wall[0]);
+                EZ(xh, j, k) = EZ(xh, j, k) + kx * inc_w(s, m, WHy, R(xh) + h, R(j), R(k) + h, cf) * R(p->wall[1]);
             }
         for (int k = zl + pz; k <= zh; ++k)
             for (int j = yl; j <= yh - 1 + py; ++j) {
-                EY(xl, j, k) = EY(xl, j, k) + kx * inc_fn(s, WHz, R(xl) - h, R(j) + h, R(k), cf) * R(1.0);
-                EY(xh, j, k) = EY(xh, j, k) - kx * inc_fn(s, WHz, R(xh) + h, R(j) + h, R(k), cf) * R(1.0);
+                EY(xl, j, k) = EY(xl, j, k) + kx * inc_w(s, m, WHz, R(xl) - h, R(j) + h, R(k), cf) * R(p->wall[0]);
+                EY(xh, j, k) = EY(xh, j, k) - kx * inc_w(s, m, WHz, R(xh) + h, R(j) + h, R(k), cf) * R(p->wall[1]);
             }
     }
+}
+
+static void e_tfsf(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    if (!p->has_tfsf) return;
+    e_tfsf_wave(s, n, 0);
+    if (p->mirror_type) e_tfsf_wave(s, n, 1);   /* after every face of the incident wave */
 }
 
 /* ports, E phase: 1-D line voltage, source, hard-set gap field (f90:6120-6283) */
@@ -1023,66 +1051,66 @@ static void out_ntff_currents(oracle_state* s) {
             for (int j = fyl; j <= fyh; ++j) {
                 int i = fxl;
                 FA(Mold, 0, 0, j, k) = FA(Mcur, 0, 0, j, k);
-                FA(Mcur, 0, 0, j, k) = R(-1.0) * (EZ(i, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(1.0);
+                FA(Mcur, 0, 0, j, k) = R(-1.0) * (EZ(i, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(p->wall[0]);
                 FA(Mold, 0, 1, j, k) = FA(Mcur, 0, 1, j, k);
-                FA(Mcur, 0, 1, j, k) = (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 0, 1, j, k) = (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(p->wall[0]);
                 FA(Jold, 0, 0, j, k) = FA(Jcur, 0, 0, j, k);
-                FA(Jcur, 0, 0, j, k) = (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(1.0);
+                FA(Jcur, 0, 0, j, k) = (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(p->wall[0]);
                 FA(Jold, 0, 1, j, k) = FA(Jcur, 0, 1, j, k);
-                FA(Jcur, 0, 1, j, k) = R(-1.0) * (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 0, 1, j, k) = R(-1.0) * (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(p->wall[0]);
                 i = fxh;
                 FA(Mold, 1, 0, j, k) = FA(Mcur, 1, 0, j, k);
-                FA(Mcur, 1, 0, j, k) = (EZ(i, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(1.0);
+                FA(Mcur, 1, 0, j, k) = (EZ(i, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(p->wall[1]);
                 FA(Mold, 1, 1, j, k) = FA(Mcur, 1, 1, j, k);
-                FA(Mcur, 1, 1, j, k) = R(-1.0) * (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 1, 1, j, k) = R(-1.0) * (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(p->wall[1]);
                 FA(Jold, 1, 0, j, k) = FA(Jcur, 1, 0, j, k);
-                FA(Jcur, 1, 0, j, k) = R(-1.0) * (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(1.0);
+                FA(Jcur, 1, 0, j, k) = R(-1.0) * (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(p->wall[1]);
                 FA(Jold, 1, 1, j, k) = FA(Jcur, 1, 1, j, k);
-                FA(Jcur, 1, 1, j, k) = (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 1, 1, j, k) = (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(p->wall[1]);
             }
     if (!p->pbc[1])
         for (int k = fzl; k <= fzh; ++k)
             for (int i = fxl; i <= fxh; ++i) {
                 int j = fyl;
                 FA(Mold, 2, 0, i, k) = FA(Mcur, 2, 0, i, k);
-                FA(Mcur, 2, 0, i, k) = (EZ(i, j, k) + EZ(i + 1, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(1.0);
+                FA(Mcur, 2, 0, i, k) = (EZ(i, j, k) + EZ(i + 1, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(p->wall[2]);
                 FA(Mold, 2, 1, i, k) = FA(Mcur, 2, 1, i, k);
-                FA(Mcur, 2, 1, i, k) = R(-1.0) * (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 2, 1, i, k) = R(-1.0) * (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(p->wall[2]);
                 FA(Jold, 2, 0, i, k) = FA(Jcur, 2, 0, i, k);
-                FA(Jcur, 2, 0, i, k) = R(-1.0) * (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(1.0);
+                FA(Jcur, 2, 0, i, k) = R(-1.0) * (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(p->wall[2]);
                 FA(Jold, 2, 1, i, k) = FA(Jcur, 2, 1, i, k);
-                FA(Jcur, 2, 1, i, k) = (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 2, 1, i, k) = (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(p->wall[2]);
                 j = fyh;
                 FA(Mold, 3, 0, i, k) = FA(Mcur, 3, 0, i, k);
-                FA(Mcur, 3, 0, i, k) = R(-1.0) * (EZ(i, j, k) + EZ(i + 1, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(1.0);
+                FA(Mcur, 3, 0, i, k) = R(-1.0) * (EZ(i, j, k) + EZ(i + 1, j, k) + EZ(i, j + 1, k) + EZ(i + 1, j + 1, k)) / R(4.0) * R(p->wall[3]);
                 FA(Mold, 3, 1, i, k) = FA(Mcur, 3, 1, i, k);
-                FA(Mcur, 3, 1, i, k) = (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 3, 1, i, k) = (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(p->wall[3]);
                 FA(Jold, 3, 0, i, k) = FA(Jcur, 3, 0, i, k);
-                FA(Jcur, 3, 0, i, k) = (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(1.0);
+                FA(Jcur, 3, 0, i, k) = (HZ(i, j, k) + HZ(i, j, k + 1)) / R(2.0) * R(p->wall[3]);
                 FA(Jold, 3, 1, i, k) = FA(Jcur, 3, 1, i, k);
-                FA(Jcur, 3, 1, i, k) = R(-1.0) * (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 3, 1, i, k) = R(-1.0) * (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(p->wall[3]);
             }
     if (!p->pbc[2])
         for (int j = fyl; j <= fyh; ++j)
             for (int i = fxl; i <= fxh; ++i) {
                 int k = fzl;
                 FA(Mold, 4, 0, i, j) = FA(Mcur, 4, 0, i, j);
-                FA(Mcur, 4, 0, i, j) = R(-1.0) * (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 4, 0, i, j) = R(-1.0) * (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(p->wall[4]);
                 FA(Mold, 4, 1, i, j) = FA(Mcur, 4, 1, i, j);
-                FA(Mcur, 4, 1, i, j) = (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 4, 1, i, j) = (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(p->wall[4]);
                 FA(Jold, 4, 0, i, j) = FA(Jcur, 4, 0, i, j);
-                FA(Jcur, 4, 0, i, j) = (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 4, 0, i, j) = (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(p->wall[4]);
                 FA(Jold, 4, 1, i, j) = FA(Jcur, 4, 1, i, j);
-                FA(Jcur, 4, 1, i, j) = R(-1.0) * (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 4, 1, i, j) = R(-1.0) * (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(p->wall[4]);
                 k = fzh;
                 FA(Mold, 5, 0, i, j) = FA(Mcur, 5, 0, i, j);
-                FA(Mcur, 5, 0, i, j) = (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 5, 0, i, j) = (EY(i, j, k) + EY(i + 1, j, k) + EY(i, j, k + 1) + EY(i + 1, j, k + 1)) / R(4.0) * R(p->wall[5]);
                 FA(Mold, 5, 1, i, j) = FA(Mcur, 5, 1, i, j);
-                FA(Mcur, 5, 1, i, j) = R(-1.0) * (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(1.0);
+                FA(Mcur, 5, 1, i, j) = R(-1.0) * (EX(i, j, k) + EX(i, j + 1, k) + EX(i, j, k + 1) + EX(i, j + 1, k + 1)) / R(4.0) * R(p->wall[5]);
                 FA(Jold, 5, 0, i, j) = FA(Jcur, 5, 0, i, j);
-                FA(Jcur, 5, 0, i, j) = R(-1.0) * (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 5, 0, i, j) = R(-1.0) * (HY(i, j, k) + HY(i, j + 1, k)) / R(2.0) * R(p->wall[5]);
                 FA(Jold, 5, 1, i, j) = FA(Jcur, 5, 1, i, j);
-                FA(Jcur, 5, 1, i, j) = (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(1.0);
+                FA(Jcur, 5, 1, i, j) = (HX(i, j, k) + HX(i + 1, j, k)) / R(2.0) * R(p->wall[5]);
             }
 }
 
@@ -1112,6 +1140,99 @@ static void bin_one(oracle_state* s, int kk, int n, int i, int j, int k, real K,
     Wa[kk * L + ii + 1] = Wa[kk * L + ii + 1] + wk * dJa;
     Wb[kk * L + ii] = Wb[kk * L + ii] + base * dJb;
     Wb[kk * L + ii + 1] = Wb[kk * L + ii + 1] + wk * dJb;
+}
+
+/* bin_one for an image current: each term times its component's sign
+ * (f90:7928-8240: (1-w) K dM sign) */
+static void bin_one_m(oracle_state* s, int kk, int n, int i, int j, int k, real K, real* Ua, real* Ub, real* Wa,
+                      real* Wb, real dMa, real dMb, real dJa, real dJb, real sMa, real sMb, real sJa, real sJb) {
+    const aem_plan_view* p = &s->p;
+    const float* a = p->ang + 6 * kk;
+    const float s1 = a[2], c1 = a[3], s2 = a[4], c2 = a[5];
+    float tfv = (p->r_relay - ((float)(i - p->ic) * p->dx * s1 * c2 + (float)(j - p->jc) * p->dy * s1 * s2 +
+                               (float)(k - p->kc) * p->dz * c1)) / (299792448.0f * p->dt);
+    const int64_t L = p->len_ff + 2;
+    float t = (float)n + 0.5f + tfv - 1.0f;
+    int ii = (int)t;
+    float w = t - (float)ii;
+    real base = (R(1.0) - R(w)) * K, wk = R(w) * K;
+    Ua[kk * L + ii] = Ua[kk * L + ii] + base * dMa * sMa;
+    Ua[kk * L + ii + 1] = Ua[kk * L + ii + 1] + wk * dMa * sMa;
+    Ub[kk * L + ii] = Ub[kk * L + ii] + base * dMb * sMb;
+    Ub[kk * L + ii + 1] = Ub[kk * L + ii + 1] + wk * dMb * sMb;
+    t = (float)n + tfv - 1.0f;
+    ii = (int)t;
+    w = t - (float)ii;
+    base = (R(1.0) - R(w)) * K; wk = R(w) * K;
+    Wa[kk * L + ii] = Wa[kk * L + ii] + base * dJa * sJa;
+    Wa[kk * L + ii + 1] = Wa[kk * L + ii + 1] + wk * dJa * sJa;
+    Wb[kk * L + ii] = Wb[kk * L + ii] + base * dJb * sJb;
+    Wb[kk * L + ii + 1] = Wb[kk * L + ii + 1] + wk * dJb * sJb;
+}
+
+/* image position of a face cell across the ground plane, in the
+ * reference's FP32 form i (1 - use) + (offset - i) use (f90:7957-7959) */
+static int mirror_pos(const aem_plan_view* p, int a, int i) {
+    return (int)((float)i * (1.0f - p->use_m[a]) + (p->off_m[a] - (float)i) * p->use_m[a]);
+}
+
+/* the image currents of a ground plane, after every face of the box, in the
+ * same loop order (f90:7928-8240) */
+static void out_ntff_bins_mirror(oracle_state* s, int n) {
+    const aem_plan_view* p = &s->p;
+    const int fxl = p->ff_lo[0], fxh = p->ff_hi[0], fyl = p->ff_lo[1], fyh = p->ff_hi[1], fzl = p->ff_lo[2], fzh = p->ff_hi[2];
+    real **Ux = &s->U[0], **Uy = &s->U[1], **Uz = &s->U[2], **Wx = &s->W[0], **Wy = &s->W[1], **Wz = &s->W[2];
+    const real mx = R(p->sgn_M[0]), my = R(p->sgn_M[1]), mz = R(p->sgn_M[2]);
+    const real jx = R(p->sgn_J[0]), jy = R(p->sgn_J[1]), jz = R(p->sgn_J[2]);
+    if (p->pbc[0] + p->pbc[1] + p->pbc[2] >= 2) return;
+    if (!p->pbc[0]) {
+        const real K = R(p->ff_K[0]);
+        for (int kk = 0; kk < p->n_angles; ++kk)
+            for (int k = fzl; k <= fzh; ++k)
+                for (int j = fyl; j <= fyh; ++j) {
+                    const int jm = mirror_pos(p, 1, j), km = mirror_pos(p, 2, k);
+                    bin_one_m(s, kk, n, mirror_pos(p, 0, fxl), jm, km, K, *Uy, *Uz, *Wy, *Wz,
+                              FA(Mcur, 0, 0, j, k) - FA(Mold, 0, 0, j, k), FA(Mcur, 0, 1, j, k) - FA(Mold, 0, 1, j, k),
+                              FA(Jcur, 0, 0, j, k) - FA(Jold, 0, 0, j, k), FA(Jcur, 0, 1, j, k) - FA(Jold, 0, 1, j, k),
+                              my, mz, jy, jz);
+                    bin_one_m(s, kk, n, mirror_pos(p, 0, fxh), jm, km, K, *Uy, *Uz, *Wy, *Wz,
+                              FA(Mcur, 1, 0, j, k) - FA(Mold, 1, 0, j, k), FA(Mcur, 1, 1, j, k) - FA(Mold, 1, 1, j, k),
+                              FA(Jcur, 1, 0, j, k) - FA(Jold, 1, 0, j, k), FA(Jcur, 1, 1, j, k) - FA(Jold, 1, 1, j, k),
+                              my, mz, jy, jz);
+                }
+    }
+    if (!p->pbc[1]) {
+        const real K = R(p->ff_K[1]);
+        for (int kk = 0; kk < p->n_angles; ++kk)
+            for (int k = fzl; k <= fzh; ++k)
+                for (int i = fxl; i <= fxh; ++i) {
+                    const int im = mirror_pos(p, 0, i), km = mirror_pos(p, 2, k);
+                    bin_one_m(s, kk, n, im, mirror_pos(p, 1, fyl), km, K, *Ux, *Uz, *Wx, *Wz,
+                              FA(Mcur, 2, 0, i, k) - FA(Mold, 2, 0, i, k), FA(Mcur, 2, 1, i, k) - FA(Mold, 2, 1, i, k),
+                              FA(Jcur, 2, 0, i, k) - FA(Jold, 2, 0, i, k), FA(Jcur, 2, 1, i, k) - FA(Jold, 2, 1, i, k),
+                              mx, mz, jx, jz);
+                    bin_one_m(s, kk, n, im, mirror_pos(p, 1, fyh), km, K, *Ux, *Uz, *Wx, *Wz,
+                              FA(Mcur, 3, 0, i, k) - FA(Mold, 3, 0, i, k), FA(Mcur, 3, 1, i, k) - FA(Mold, 3, 1, i, k),
+                              FA(Jcur, 3, 0, i, k) - FA(Jold, 3, 0, i, k), FA(Jcur, 3, 1, i, k) - FA(Jold, 3, 1, i, k),
+                              mx, mz, jx, jz);
+                }
+    }
+    if (!p->pbc[2]) {
+        const real K = R(p->ff_K[2]);
+        for (int kk = 0; kk < p->n_angles; ++kk)
+            for (int j = fyl; j <= fyh; ++j)
+                for (int i = fxl; i <= fxh; ++i) {
+                    const int im = mirror_pos(p, 0, i), jm = mirror_pos(p, 1, j);
+                    bin_one_m(s, kk, n, im, jm, mirror_pos(p, 2, fzl), K, *Uy, *Ux, *Wy, *Wx,
+                              FA(Mcur, 4, 1, i, j) - FA(Mold, 4, 1, i, j), FA(Mcur, 4, 0, i, j) - FA(Mold, 4, 0, i, j),
+                              FA(Jcur, 4, 1, i, j) - FA(Jold, 4, 1, i, j), FA(Jcur, 4, 0, i, j) - FA(Jold, 4, 0, i, j),
+                              my, mx, jy, jx);
+                    bin_one_m(s, kk, n, im, jm, mirror_pos(p, 2, fzh), K, *Uy, *Ux, *Wy, *Wx,
+                              FA(Mcur, 5, 1, i, j) - FA(Mold, 5, 1, i, j), FA(Mcur, 5, 0, i, j) - FA(Mold, 5, 0, i, j),
+                              FA(Jcur, 5, 1, i, j) - FA(Jold, 5, 1, i, j), FA(Jcur, 5, 0, i, j) - FA(Jold, 5, 0, i, j),
+                              my, mx, jy, jx);
+                }
+    }
 }
 
 /* NTFF retarded-time binning, f90:7643-7927 (sequential, serial-build order) */
@@ -1466,6 +1587,7 @@ static void step_from_exchange(oracle_state* s, int n) {
     if (p->n_angles > 0) {
         out_ntff_currents(s);
         out_ntff_bins(s, n);
+        if (p->mirror_type) out_ntff_bins_mirror(s, n);
     }
     for (int pi = 0; pi < p->n_ports; ++pi) s->Vout[(int64_t)pi * p->time_steps + (n - 1)] = s->V[(int64_t)pi * p->M + 1];
     out_phase_centre(s, n);

@@ -20,7 +20,7 @@ from tests.oracle_ffi import Oracle
 pytestmark = pytest.mark.gpu
 
 DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini",
-         "sheets_xyz"]
+         "sheets_xyz", "igp_mz", "igp_px"]
 TOL = 1e-5
 
 
@@ -333,3 +333,49 @@ def test_ntff_lists_equal_row_scan(golden_dir, name, angles):
     assert any(np.any(out[False][w] != 0) for w in range(6))
     for w in range(6):
         np.testing.assert_array_equal(out[False][w], out[True][w], err_msg=f"accumulator {w}")
+
+
+def test_ground_plane_refusals(golden_dir):
+    """An infinite ground plane deck's far field runs only in the ordered
+    binning from contributor lists; the other binnings, the frequency-domain
+    far field and the adjoint refuse it with AEM_ERR_UNSUPPORTED."""
+    from alpineem_amd._lib import AemError
+    deck = os.path.join(golden_dir, "decks", "igp_mz.txt")
+    for kw in (dict(ntff="histogram"), dict(ntff="atomic"), dict(ntff_lists=False)):
+        with pytest.raises(AemError, match="ground plane"):
+            Simulation(Plan.from_inputs(deck), **kw)
+    sim = Simulation(Plan.from_inputs(deck))
+    with pytest.raises(AemError, match="ground plane"):
+        sim.set_objective(0.0, 0.0, 10e9, "theta")
+    with pytest.raises(AemError, match="ground plane"):
+        sim.record_design((20, 20, 24), (21, 21, 25))
+    sim.close()
+
+
+@pytest.mark.parametrize("name", ["igp_mz", "igp_px"])
+def test_ground_plane_fields_close_to_oracle(golden_dir, name):
+    """Infinite ground planes (igp_mz: -z under a metal half sphere; igp_px:
+    +x beside a dielectric block, oblique incidence): one TF/SF face off, a
+    mirrored plane wave.  While the pulse enters and crosses the box the
+    fields stay within the exp() rounding of the incident and mirrored
+    waves from the oracle's (rel-L2 5e-5); after it has
+    left, within 2e-6 of the incident amplitude plus 1e-6 of the value (the
+    residual of one field is 1e-4 of the other's scale, so a rel-L2 of it
+    alone magnifies the same rounding)."""
+    plan = Plan.from_inputs(os.path.join(golden_dir, "decks", name + ".txt"))
+    sim = Simulation(plan)
+    orc = Oracle(plan)
+    done = 0
+    for n in (5, 120, 200, plan.time_steps):
+        sim.run(n - done, graph=n > 5)
+        orc.step(n - done)
+        done = n
+        for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+            a, b = sim.field(comp), orc.field(comp)
+            if n < plan.time_steps:
+                assert rel_l2(a, b) <= 5e-5, (n, comp, rel_l2(a, b))
+            else:
+                amp = float(plan.view.plane_wave_amp) * (1.0 if comp[0] == "E" else 1.0 / 376.73)
+                d = np.abs(a.astype(np.float64) - b)
+                assert np.all(d <= 2e-6 * amp + 1e-6 * np.abs(b)), (comp, d.max())
+    sim.close()

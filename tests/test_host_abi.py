@@ -176,10 +176,15 @@ def test_features_outside_the_path_are_refused(golden_dir):
     assert "gridded SPICE" in str(e.value)
     _expect_input_error(lines[:-1] + ["1"], "end of deck")    # a SPICE section cut short
     assert lines[7] == "none"                        # IGP flag
-    igp = lines[:7] + ["IGP", "-z", "3"] + lines[8:]
-    with pytest.raises(_lib.AemError) as e:
-        Plan.from_text("\n".join(igp) + "\n")
-    assert e.value.status == 5
+    # an infinite ground plane (f90:1402-1445): the -z face of the TF/SF box
+    # moves to the plane, loses its corrections and far-field currents, and
+    # that side's CPML shrinks to one surface
+    igp = lines[:7] + ["IGP", "-z", "14"] + lines[8:]
+    v = Plan.from_text("\n".join(igp) + "\n").view
+    assert v.mirror_type == 5 and list(v.wall) == [1, 1, 1, 1, 0, 1]
+    assert v.tf_lo[2] == 15 and v.npml[4] == 1 and v.ff_lo[2] == 16
+    assert list(v.use_m) == [0, 0, 1] and v.off_m[2] == 30.0
+    assert list(v.sgn_M) == [1, 1, -1] and list(v.sgn_J) == [-1, -1, 1]
 
 
 def test_missing_deck_file_is_an_io_error(tmp_path):

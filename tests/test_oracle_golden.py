@@ -18,7 +18,7 @@ from alpineem_amd.solver import Plan
 from tests.oracle_ffi import Oracle
 
 DECKS = ["dipole36", "master_ports", "drude_slab", "sphere_oblique", "antenna_x", "c1_mini", "c2_mini", "c3_mini",
-         "sheets_xyz", "pop4_m0", "pop4_m1", "pop4_m2", "pop4_m3", "export_small"]
+         "sheets_xyz", "pop4_m0", "pop4_m1", "pop4_m2", "pop4_m3", "export_small", "igp_mz", "igp_px"]
 
 
 def _run(golden_dir, name, fp64=False):
