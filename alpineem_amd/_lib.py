@@ -66,6 +66,7 @@ SIGNATURES = [
     ("aem_adjoint_design_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_adjoint", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     ("aem_adjoint_series", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    ("aem_adjoint_checkpoint", C.c_int, [C.c_void_p, C.c_int32]),
     ("aem_adjoint_step", C.c_int, [C.c_void_p, C.c_int64]),
     ("aem_adjoint_get_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("aem_adjoint_set_field", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),

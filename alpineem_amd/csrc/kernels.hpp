@@ -2861,7 +2861,8 @@ struct AdjRec {
     int lo[3];        // first edge (1-based) of the record box
     int n[3];         // extents
     long long nedge;  // n[0] * n[1] * n[2]
-    float* hist;      // [step][2 (E, S)][3 comps][nedge]
+    float* hist;      // [slot][2 (E, S)][3 comps][nedge]; step n in slot (n-1) % slots
+    int slots;        // time_steps, or the checkpoint interval
 };
 
 __global__ void k_adj_record(Grid g, AdjRec r, const float* Ex, const float* Ey, const float* Ez, const float* Hx,
@@ -2878,7 +2879,7 @@ __global__ void k_adj_record(Grid g, AdjRec r, const float* Ex, const float* Ey,
     const float s0 = (hz - Hz[o - g.PX]) * dy_j + (Hy[o - g.plane] - hy) * dz_k;
     const float s1 = (Hz[o - 1] - hz) * dx_i + (hx - Hx[o - g.plane]) * dz_k;
     const float s2 = (hy - Hy[o - 1]) * dx_i + (Hx[o - g.PX] - hx) * dy_j;
-    float* h = r.hist + (long long)(*step - 1) * 6 * r.nedge;
+    float* h = r.hist + (long long)((*step - 1) % r.slots) * 6 * r.nedge;
     h[e] = Ex[o];
     h[r.nedge + e] = Ey[o];
     h[2 * r.nedge + e] = Ez[o];
@@ -2896,7 +2897,7 @@ __global__ void k_adj_grad(Grid g, AdjRec r, const float* lEx, const float* lEy,
     const long long rr = e / r.n[0];
     const int j = r.lo[1] + (int)(rr % r.n[1]), k = r.lo[2] + (int)(rr / r.n[1]);
     const long long o = off(g, i, j, k);
-    const float* h = r.hist + (long long)(*step - 1) * 6 * r.nedge;
+    const float* h = r.hist + (long long)((*step - 1) % r.slots) * 6 * r.nedge;
     const float lam[3] = {lEx[o], lEy[o], lEz[o]};
     for (int c = 0; c < 3; ++c) {
         dga[c * r.nedge + e] += (double)lam[c] * (double)h[c * r.nedge + e];
@@ -3360,7 +3361,7 @@ __global__ void k_e_design(Grid g, DesignDev d, float* Ex, float* Ey, float* Ez,
     Ey[o] = c[2 * n + e] * ey0 + c[3 * n + e] * s1;
     Ez[o] = c[4 * n + e] * ez0 + c[5 * n + e] * s2;
     if (record) {
-        float* h = r.hist + (long long)(*step - 1) * 6 * n;
+        float* h = r.hist + (long long)((*step - 1) % r.slots) * 6 * n;
         h[e] = ex0;
         h[n + e] = ey0;
         h[2 * n + e] = ez0;

@@ -361,6 +361,16 @@ struct aem_sim {
     Cpml lpE{}, lpH{};             // adjoint CPML states
     DrudeBox adb[3]{};             // adjoint Drude states: J holds lambda_J, Eold lambda_Eold
     float *d_lV = nullptr, *d_lI = nullptr;   // adjoint lumped-port lines [np][M]
+    // adjoint checkpoints: with ck_K > 0 the design record keeps ck_K steps,
+    // the state is saved before every ck_K-th step, and the backward sweep
+    // recomputes each interval from its snapshot before transposing it
+    int ck_K = 0;
+    bool ck_last_ok = false;       // the record holds the last interval (true after the forward run)
+    size_t ck_bytes = 0;
+    std::vector<char*> ck_snap;    // one per interval, then the state at the end of the run
+    std::vector<std::pair<void*, size_t>> state_regions();
+    void snapshot(int q);
+    void restore(int q);
     int* d_bstep = nullptr;
     float* d_srcD = nullptr;       // adjoint source: [6][4][mv][mu] increments of a step
     float* d_desm = nullptr;       // design transpose: [3][nedge] gb lambda_E
@@ -840,6 +850,56 @@ void aem_sim::bind() {
 }
 
 // After a single-pass step: the other buffer set now holds the fields.
+// The device state one step depends on, as (address, bytes): the six fields
+// and the CPML psi of the current buffer set, the port lines, the Drude
+// currents and E_old, the step counter.  (Thin sheets and SPICE ports, which
+// the adjoint does not take, are not part of it.)
+std::vector<std::pair<void*, size_t>> aem_sim::state_regions() {
+    const aem_plan_view& v = plan->v;
+    std::vector<std::pair<void*, size_t>> r;
+    for (int q = 0; q < 6; ++q) r.push_back({F[q], ((size_t)v.ncell + v.plane) * 4});
+    for (int f = 0; f < 6; ++f) {
+        if (f == 1) continue;   // the two x faces share one allocation, face 0 first
+        const int a = f / 2;
+        size_t nt = (size_t)std::max(v.npml[f] - 1, 0);
+        if (f == 0) nt += (size_t)std::max(v.npml[1] - 1, 0);
+        const size_t trans = a == 0 ? (size_t)v.NY * v.NZ : (a == 1 ? (size_t)v.PX * v.NZ : (size_t)v.PX * v.NY);
+        for (int q = 0; q < 2; ++q)
+            if (nt * trans) {
+                r.push_back({cpH.psi[f][q], nt * trans * 4});
+                r.push_back({cpE.psi[f][q], nt * trans * 4});
+            }
+    }
+    if (v.n_ports > 0) {
+        r.push_back({d_V, (size_t)v.n_ports * v.M * 4});
+        r.push_back({d_I, (size_t)v.n_ports * v.M * 4});
+    }
+    if (v.has_drude)
+        for (int c = 0; c < 3; ++c)
+            if (db[c].nb > 0) {
+                r.push_back({db[c].J, (size_t)db[c].nb * v.n_poles * 4});
+                r.push_back({db[c].Eold, (size_t)db[c].nb * 4});
+            }
+    r.push_back({d_step, sizeof(int)});
+    return r;
+}
+
+void aem_sim::snapshot(int q) {
+    size_t o = 0;
+    for (const auto& reg : state_regions()) {
+        HIPCHK(hipMemcpyAsync(ck_snap[q] + o, reg.first, reg.second, hipMemcpyDeviceToDevice, st));
+        o += (reg.second + 255) & ~(size_t)255;
+    }
+}
+
+void aem_sim::restore(int q) {
+    size_t o = 0;
+    for (const auto& reg : state_regions()) {
+        HIPCHK(hipMemcpyAsync(reg.first, ck_snap[q] + o, reg.second, hipMemcpyDeviceToDevice, st));
+        o += (reg.second + 255) & ~(size_t)255;
+    }
+}
+
 void aem_sim::flip() {
     for (int q = 0; q < 6; ++q) std::swap(F[q], F2[q]);
     for (int f = 0; f < 2; ++f)
@@ -1568,6 +1628,7 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
         }
         HIPCHK(hipEventRecord(s->ev0, s->st));
         for (int64_t q = 0; q < n_steps; ++q) {
+            if (s->ck_K > 0 && (s->steps_done + q) % s->ck_K == 0) s->snapshot((int)((s->steps_done + q) / s->ck_K));
             if (use_graph) {
                 HIPCHK(hipGraphLaunch(s->par == s->gpar ? s->gexec : s->gexec2, s->st));
                 if (s->gsp) s->flip();
@@ -1587,6 +1648,7 @@ aem_status aem_run(aem_sim* s, int64_t n_steps, int32_t use_graph) {
         s->last_ms = ms / (float)n_steps;
         s->steps_done += n_steps;
         if (s->kim) s->kim->steps_done = s->steps_done;
+        s->ck_last_ok = true;
         s->check_watch();
     });
 }
@@ -1969,6 +2031,10 @@ aem_status aem_time_phases(aem_sim* s, int64_t n_steps, float* ms, int32_t n_ms)
         aem::set_error("aem_time_phases: not available for a kmax simulation");
         return AEM_ERR_UNSUPPORTED;
     }
+    if (s->ck_K > 0) {
+        aem::set_error("aem_time_phases: not available for a simulation with adjoint checkpoints");
+        return AEM_ERR_UNSUPPORTED;
+    }
     return aem::guard([&]() {
         hipEvent_t ev[6];
         for (int q = 0; q < 6; ++q) HIPCHK(hipEventCreate(&ev[q]));
@@ -2204,6 +2270,10 @@ aem_status aem_batch_create(aem_sim* const* sims, int32_t n, aem_batch** out) {
         }
         if (sims[q]->kim) {
             aem::set_error("aem_batch_create: a member is a kmax (complex-field) simulation; run it alone");
+            return AEM_ERR_UNSUPPORTED;
+        }
+        if (sims[q]->ck_K > 0) {
+            aem::set_error("aem_batch_create: a member keeps adjoint checkpoints; run it alone");
             return AEM_ERR_UNSUPPORTED;
         }
         if (sims[q]->plan->v.mirror_type > 0 && sims[q]->plan->v.n_angles > 0) {
@@ -2752,7 +2822,8 @@ aem_status aem_adjoint_record(aem_sim* s, const int32_t* lo, const int32_t* hi) 
             r.n[a] = hi[a] - lo[a] + 2;
         }
         r.nedge = (long long)r.n[0] * r.n[1] * r.n[2];
-        r.hist = s->zeros<float>((size_t)v.time_steps * 6 * r.nedge);
+        r.slots = s->ck_K > 0 ? std::min(s->ck_K, v.time_steps) : v.time_steps;
+        r.hist = s->zeros<float>((size_t)r.slots * 6 * r.nedge);
         s->rec_on = true;
         HIPCHK(hipStreamSynchronize(s->st));
     });
@@ -2915,9 +2986,37 @@ void backward_sweep(aem_sim* s, std::vector<double>& ha, std::vector<double>& hb
     HIPCHK(hipStreamEndCapture(s->st, &gr));
     HIPCHK(hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0));
     HIPCHK(hipEventRecord(s->ev0, s->st));
-    for (int n = 0; n < T; ++n) {
-        HIPCHK(hipGraphLaunch(gx, s->st));
-        s->ra.pace(s->st, n);
+    if (s->ck_K > 0) {
+        // intervals from the last: the record of the last one is still in
+        // place right after the forward run; every other one is recomputed
+        // from its snapshot (field update and record only, no outputs) and
+        // then transposed.  The state at the end of the run comes back at the
+        // end.
+        const int K = s->ck_K, S = (T + K - 1) / K;
+        constexpr unsigned kRecompute = kPartAll & ~(kPartOutPre | kPartNtff | kPartOutPost | kPartObjective);
+        s->snapshot(S);
+        int launched = 0;
+        for (int seg = S - 1; seg >= 0; --seg) {
+            const int n0 = seg * K, n1 = std::min(T, n0 + K);
+            if (seg < S - 1 || !s->ck_last_ok) {
+                s->restore(seg);
+                for (int n = n0; n < n1; ++n) {
+                    s->enqueue_parts(nullptr, s->st, kRecompute);
+                    s->ra.pace(s->st, launched++);
+                }
+            }
+            for (int n = n1; n > n0; --n) {
+                HIPCHK(hipGraphLaunch(gx, s->st));
+                s->ra.pace(s->st, launched++);
+            }
+        }
+        s->restore(S);
+        s->ck_last_ok = false;   // the record now holds the first interval
+    } else {
+        for (int n = 0; n < T; ++n) {
+            HIPCHK(hipGraphLaunch(gx, s->st));
+            s->ra.pace(s->st, n);
+        }
     }
     HIPCHK(hipEventRecord(s->ev1, s->st));
     HIPCHK(hipEventSynchronize(s->ev1));
@@ -3027,6 +3126,33 @@ aem_status aem_adjoint(aem_sim* s, const double* dJ_dF, double* grad, int64_t n)
                                                     (size_t)(v.NX - 1) * ((size_t)(j - 1) + (size_t)(v.NY - 1) * (k - 1))];
                     if (m >= 1 && m <= n) grad[m - 1] += cells[b];
                 }
+    });
+}
+
+aem_status aem_adjoint_checkpoint(aem_sim* s, int32_t every) {
+    if (!s || every < 1) {
+        aem::set_error("aem_adjoint_checkpoint: bad argument");
+        return AEM_ERR_STATE;
+    }
+    if (refuse_batched(s, "aem_adjoint_checkpoint")) return AEM_ERR_STATE;
+    if (!fresh(s) || s->rec_on || s->ck_K > 0) {
+        aem::set_error("aem_adjoint_checkpoint: call once, before aem_adjoint_record / aem_design_box and the first step");
+        return AEM_ERR_STATE;
+    }
+    if (const char* why = adjoint_unsupported(s->plan->v)) {
+        aem::set_error(std::string("aem_adjoint_checkpoint: the adjoint step does not include ") + why);
+        return AEM_ERR_UNSUPPORTED;
+    }
+    return aem::guard([&]() {
+        const int T = s->plan->v.time_steps;
+        const int K = std::min((int)every, T);
+        size_t bytes = 0;
+        for (const auto& reg : s->state_regions()) bytes += (reg.second + 255) & ~(size_t)255;
+        const int S = (T + K - 1) / K;
+        s->ck_bytes = bytes;
+        s->ck_snap.assign((size_t)S + 1, nullptr);
+        for (int q = 0; q <= S; ++q) s->ck_snap[q] = dalloc<char>(bytes, s->owned);
+        s->ck_K = K;
     });
 }
 

@@ -282,6 +282,12 @@ class Simulation:
         check(lib().aem_adjoint(self._h, g.ctypes.data, out.ctypes.data, n))
         return out.reshape(shape) if shape else out
 
+    def checkpoint(self, every: int) -> None:
+        """Keep the adjoint's design record for ``every`` steps and snapshots of
+        the state every ``every`` steps; gradients then recompute each interval
+        (aem_adjoint_checkpoint).  Call before record_design / design_box."""
+        check(lib().aem_adjoint_checkpoint(self._h, int(every)))
+
     def series_gradient(self, dJ_dV=None, dJ_dI=None, n_materials: int = 0) -> np.ndarray:
         """Gradient of a general real objective J of the lumped-port series
         (aem_adjoint_series): ``dJ_dV``, ``dJ_dI`` of shape (n_ports,

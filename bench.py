@@ -97,8 +97,10 @@ def population_bench(steps: int, count: int = 64) -> dict:
     to four dielectric blocks, 10-cell CPML, 50 ohm port, 19 far-field
     angles (decks.population).  Timed for ``steps`` steps: once as 64
     sequential solo runs, once as one batch; same options (FP64 histogram
-    far-field binning) in both.  Only stepping is timed (each solo run and
-    the batch include their one graph capture); setup is not."""
+    far-field binning) in both.  The batch's z chunk is the autotuner's
+    choice (alpineem_amd.tune) over throwaway members, untimed.  Only
+    stepping is timed (each solo run and the batch include their one graph
+    capture); setup is not."""
     from alpineem_amd import decks
     from alpineem_amd.solver import Batch, Plan, Simulation
 
@@ -112,14 +114,21 @@ def population_bench(steps: int, count: int = 64) -> dict:
         s.run(steps)
         seq += time.perf_counter() - t0
         s.close()
-    sims = [Simulation(p, ntff="histogram") for p in plans]
+    # the batch's z chunk chosen by the autotuner on throwaway members (not timed)
+    from alpineem_amd.tune import tune_batch_planes
+    t0 = time.perf_counter()
+    planes, tuned = tune_batch_planes(plans, candidates=(0, 2, 4, 8), steps=10, warmup=3, ntff="histogram")
+    t_tune = time.perf_counter() - t0
+    sims = [Simulation(p, ntff="histogram", planes=planes) for p in plans]
     b = Batch(sims)
     t0 = time.perf_counter()
     b.run(steps)
     bat = time.perf_counter() - t0
     res = {"members": count, "cells_per_step": cells, "steps": steps, "ntff": "histogram, 19 angles",
            "sequential_gcells": round(cells * steps / seq / 1e9, 3), "batched_gcells": round(cells * steps / bat / 1e9, 3),
-           "speedup": round(seq / bat, 3), "sequential_s": round(seq, 3), "batched_s": round(bat, 3)}
+           "speedup": round(seq / bat, 3), "sequential_s": round(seq, 3), "batched_s": round(bat, 3),
+           "tuned_planes": planes, "tuner_ms_per_step": {str(k): round(v, 4) for k, v in tuned.items()},
+           "tuner_s": round(t_tune, 2)}
     b.close()
     for s in sims:
         s.close()
@@ -158,12 +167,30 @@ def adjoint_bench(steps: int) -> dict:
     t_bwd = time.perf_counter() - t0
     J = abs(sim.objective()) ** 2
     sim.close()
+    # the same gradient with checkpoints every 200 steps (a 200-step record,
+    # each interval recomputed from its snapshot in the backward sweep)
+    ck = Simulation(plan, ntff="off")
+    ck.checkpoint(200)
+    ck.set_objective(30.0, 0.0, 10e9, "theta")
+    ck.design_box(lo, hi)
+    ck.set_design_eps(eps)
+    t0 = time.perf_counter()
+    ck.run()
+    t_fwd_ck = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g_ck = ck.design_gradient()
+    t_bwd_ck = time.perf_counter() - t0
+    ck.close()
     n = plan.view.nx * plan.view.ny * plan.view.nz
     return {"workload": f"{plan.view.nx}^3, {steps} steps, design box {shape[2]}x{shape[1]}x{shape[0]}",
             "forward_s": round(t_plain, 4), "recorded_forward_s": round(t_fwd, 4), "adjoint_s": round(t_bwd, 4),
             "gradient_over_forward": round((t_fwd + t_bwd) / t_plain, 3),
             "forward_gcells": round(n * steps / t_plain / 1e9, 2), "J": J,
-            "grad_norm": float(np.linalg.norm(g))}
+            "grad_norm": float(np.linalg.norm(g)),
+            "checkpoint_every": 200, "checkpointed_forward_s": round(t_fwd_ck, 4),
+            "checkpointed_adjoint_s": round(t_bwd_ck, 4),
+            "checkpointed_gradient_over_forward": round((t_fwd_ck + t_bwd_ck) / t_plain, 3),
+            "checkpointed_equal": bool(np.array_equal(g, g_ck))}
 
 
 def configs_bench(only: str = None) -> dict:
@@ -394,13 +421,17 @@ def main() -> int:
         pop = population_bench(args.population_steps)
         print(f"[bench] population: 64 members, {pop['cells_per_step'] / 1e6:.1f} Mcells per step, "
               f"sequential {pop['sequential_gcells']} Gcell/s, batched {pop['batched_gcells']} Gcell/s, "
-              f"speedup {pop['speedup']}x", file=sys.stderr, flush=True)
+              f"speedup {pop['speedup']}x (tuned z chunk {pop['tuned_planes']}: {pop['tuner_ms_per_step']} ms, "
+              f"{pop['tuner_s']} s)", file=sys.stderr, flush=True)
     adj = None
     if args.adjoint_steps > 0:
         adj = adjoint_bench(args.adjoint_steps)
         print(f"[bench] adjoint: {adj['workload']}: forward {adj['forward_s']} s, recorded forward "
               f"{adj['recorded_forward_s']} s, adjoint {adj['adjoint_s']} s, gradient/forward "
-              f"{adj['gradient_over_forward']}", file=sys.stderr, flush=True)
+              f"{adj['gradient_over_forward']}; with checkpoints every {adj['checkpoint_every']} steps: "
+              f"{adj['checkpointed_forward_s']} + {adj['checkpointed_adjoint_s']} s, gradient/forward "
+              f"{adj['checkpointed_gradient_over_forward']}, equal {adj['checkpointed_equal']}",
+              file=sys.stderr, flush=True)
     cfg = configs_bench() if args.configs else None
     cpu = None if args.no_cpu else cpu_baseline(n, args.cpu_steps)
     line = {

@@ -43,7 +43,7 @@ typedef enum aem_status {
     AEM_ERR_IO = 2,           /* file missing or unwritable */
     AEM_ERR_DEVICE = 3,       /* HIP runtime error */
     AEM_ERR_STATE = 4,        /* call out of order, bad handle or argument */
-    AEM_ERR_UNSUPPORTED = 5,  /* deck feature outside this library (gridded ports, IGP, kmax far field) */
+    AEM_ERR_UNSUPPORTED = 5,  /* deck feature outside this library (gridded ports, kmax far field) */
     AEM_ERR_NUMERIC = 6       /* non-finite field detected by the watchdog */
 } aem_status;
 
@@ -347,6 +347,20 @@ aem_status aem_adjoint(aem_sim* sim, const double* dJ_dF, double* grad, int64_t 
  * a cell clear of the CPML slabs, of the outer boundary and of Drude media
  * (AEM_ERR_UNSUPPORTED at aem_adjoint_record otherwise). */
 aem_status aem_adjoint_series(aem_sim* sim, const double* dJ_dV, const double* dJ_dI, double* grad, int64_t n);
+
+/* Checkpoints for the adjoint: the design record keeps `every` steps instead
+ * of all of them, the state the step depends on (fields, CPML psi, port
+ * lines, Drude currents, the step counter) is saved in device memory before
+ * every `every`-th step, and each gradient call recomputes each interval from
+ * its snapshot (field update and record only) before transposing it; the
+ * state at the end of the run is put back afterwards.  Gradients equal those
+ * without checkpoints bit for bit.  Memory: ceil(time_steps/every) + 1
+ * snapshots of the state instead of time_steps - every record steps; cost: one
+ * more forward pass of the field update per gradient.  No counterpart in the
+ * reference (its workflow keeps whole field slices, adjoint_optimization.py:
+ * 66-72).  Call once, before aem_adjoint_record / aem_design_box and the
+ * first step. */
+aem_status aem_adjoint_checkpoint(aem_sim* sim, int32_t every);
 
 /* Low-level adjoint access: apply the transposed step (no source) n_steps
  * times to the adjoint state, and read/write its six field components
