@@ -13,6 +13,7 @@ aem_adjoint_*) on the device.
   oracle runs to 1e-3 relative.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -277,22 +278,28 @@ def test_eps_gradient_matches_finite_difference(scatter_run):
 BOX_LO, BOX_HI = (18, 18, 19), (20, 20, 20)      # 3 x 3 x 2 design cells
 
 
-def design_eps(seed=5):
+# the 10 x 10 x 5 box of the sixteen-voxel finite-difference fixture
+# (tests/golden/make_adjoint_fd.py): inside the TF/SF box (14..27), over the
+# lossy block and the four single-cell voxels
+BOX16_LO, BOX16_HI = (16, 16, 18), (25, 25, 22)
+
+
+def design_eps(seed=5, lo=BOX_LO, hi=BOX_HI):
     """Per-cell permittivity of the design box, U[1, 4], shape (nz, ny, nx)."""
     rng = np.random.default_rng(seed)
-    shape = tuple(BOX_HI[a] - BOX_LO[a] + 1 for a in (2, 1, 0))
+    shape = tuple(hi[a] - lo[a] + 1 for a in (2, 1, 0))
     return rng.uniform(1.0, 4.0, shape).astype(np.float32)
 
 
-def design_deck(eps_cells=None):
+def design_deck(eps_cells=None, lo=BOX_LO, hi=BOX_HI):
     """The plane-wave scatterer with a sigma = 0.5 block (eps 2) filling the
     design box; with ``eps_cells`` every box cell is its own material
     (100, 101, ...) of that permittivity -- the per-voxel geometry the
     reference reaches through one material per voxel."""
     d = scatter_deck()
     d.materials.append(decks.Material(30, (2.0, 2.0, 2.0), (0.5, 0.5, 0.5)))
-    size = tuple(BOX_HI[a] - BOX_LO[a] + 1 for a in range(3))
-    d.objects.append(decks.Block(30, BOX_LO, size))
+    size = tuple(hi[a] - lo[a] + 1 for a in range(3))
+    d.objects.append(decks.Block(30, lo, size))
     if eps_cells is not None:
         q = 0
         for k in range(size[2]):
@@ -300,7 +307,7 @@ def design_deck(eps_cells=None):
                 for i in range(size[0]):
                     e = float(eps_cells[k, j, i])
                     d.materials.append(decks.Material(100 + q, (e, e, e), (0.5, 0.5, 0.5)))
-                    d.objects.append(decks.Block(100 + q, (BOX_LO[0] + i, BOX_LO[1] + j, BOX_LO[2] + k), (1, 1, 1)))
+                    d.objects.append(decks.Block(100 + q, (lo[0] + i, lo[1] + j, lo[2] + k), (1, 1, 1)))
                     q += 1
     return d
 
@@ -401,6 +408,38 @@ def test_design_gradient_matches_finite_difference():
         g_fd = (J[0] - J[1]) / (e32[0] - e32[1])
         print(f"cell {c}: adjoint {g_adj[c]:.9e} finite difference {g_fd:.9e}")
         assert abs(g_adj[c] - g_fd) <= 1e-3 * abs(g_fd)
+
+
+def test_design_gradient_matches_sixteen_voxel_finite_differences(golden_dir):
+    """d|F|^2/d eps at sixteen random voxels of a 10 x 10 x 5 design box (500
+    cells, eps U[1, 4]) against central differences of FP64 oracle runs with
+    one material per voxel (tests/golden/adjoint_fd16.npz, written by
+    tests/golden/make_adjoint_fd.py; h = 2e-2, whose differences agree with
+    h = 1e-2 to 4e-4) to 1e-3 relative; the device objective against the
+    FP64 oracle's to 1e-5."""
+    fx = np.load(os.path.join(golden_dir, "adjoint_fd16.npz"))
+    lo, hi = tuple(int(x) for x in fx["box_lo"]), tuple(int(x) for x in fx["box_hi"])
+    assert (lo, hi) == (BOX16_LO, BOX16_HI) and float(fx["h"][2]) == 2e-2
+    eps = fx["eps"]
+    np.testing.assert_array_equal(eps, design_eps(5, lo, hi))
+    sim = Simulation(Plan.from_text(design_deck(None, lo, hi).text()))
+    sim.set_objective(THETA, PHI, FREQ, "theta")
+    sim.design_box(lo, hi)
+    sim.set_design_eps(eps)
+    sim.run()
+    J = abs(sim.objective()) ** 2
+    print(f"device J={J:.9e} FP64 oracle J={float(fx['J0']):.9e}")
+    assert abs(J - float(fx["J0"])) <= 2e-5 * float(fx["J0"])
+    g_adj = sim.design_gradient()
+    g_fd = fx["g_fd"][2]
+    worst = 0.0
+    for q, c in enumerate(fx["cells"]):
+        c = tuple(int(x) for x in c)
+        rel = abs(g_adj[c] - g_fd[q]) / abs(g_fd[q])
+        worst = max(worst, rel)
+        print(f"cell {c}: adjoint {g_adj[c]:.9e} finite difference {g_fd[q]:.9e} rel {rel:.2e}")
+    print(f"worst relative difference over 16 voxels: {worst:.2e}")
+    assert worst <= 1e-3
 
 
 def test_autograd_wrapper():
