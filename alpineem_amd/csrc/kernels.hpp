@@ -2028,8 +2028,8 @@ __device__ __forceinline__ float list_term_nb(const NtffList& L, const NtffBins&
 constexpr int kListBatch = 8;
 constexpr int kLongBatch = 4;   // 64-entry chunks of a long list formed together
 
-__global__ void __launch_bounds__(256) k_ntff_list(NtffList L, NtffBins bn, const int* step) {
-    const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+__device__ __forceinline__ void ntff_list_body(const NtffList& L, const NtffBins& bn, const int* step, int blk) {
+    const int w = blk * 4 + (int)(threadIdx.x >> 6);
     if (w >= L.n_waves) return;
     const int lane = (int)(threadIdx.x & 63);
     const int4 d = L.waves[w];
@@ -2092,6 +2092,24 @@ __global__ void __launch_bounds__(256) k_ntff_list(NtffList L, NtffBins bn, cons
         }
     }
     if (lane == 0) A[bin] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_ntff_list(NtffList L, NtffBins bn, const int* step) {
+    ntff_list_body(L, bn, step, (int)blockIdx.x);
+}
+
+// Per-member arguments of the batched list binning: grid (blocks of the
+// member with the most waves, members); a member's surplus blocks exit.
+// Each member's bins take the same sums in the same order as its solo run.
+struct NtffListArgs {
+    NtffList L;
+    NtffBins bn;
+    const int* step;
+};
+
+__global__ void __launch_bounds__(256) k_ntff_list_batch(const NtffListArgs* __restrict__ args) {
+    const NtffListArgs& a = args[blockIdx.y];
+    ntff_list_body(a.L, a.bn, a.step, (int)blockIdx.x);
 }
 
 // NTFF retarded-time binning through an LDS histogram (f90:7643-7927 with

@@ -2085,8 +2085,12 @@ struct aem_batch {
     EArgs* d_eargs = nullptr;
     PortArgs* d_pargs = nullptr;
     int n_port = 0;
+    // batched far fields: the face currents of every member binned by the
+    // LDS histogram or by contributor lists, then each binning's own launch
     NtffArgs* d_nargs = nullptr;
-    int n_ntff = 0, max_angles = 0, ntff_mu = 0, ntff_mv = 0;
+    NtffArgs* d_hist_args = nullptr;
+    NtffListArgs* d_list_args = nullptr;
+    int n_ntff = 0, n_hist = 0, n_list = 0, max_angles = 0, max_list_blocks = 0, ntff_mu = 0, ntff_mv = 0;
     std::vector<int> ntff_alone;   // members whose far field bins outside the batch
     int** d_steps = nullptr;
     hipGraph_t graph = nullptr;
@@ -2161,18 +2165,27 @@ struct aem_batch {
         build(false, hg);
         build(true, eg);
         std::vector<PortArgs> pa;
-        std::vector<NtffArgs> na;
+        std::vector<NtffArgs> na, nh;
+        std::vector<NtffListArgs> nla;
         std::vector<int*> steps(n);
         for (int q = 0; q < n; ++q) {
             aem_sim* s = m[q];
             const aem_plan_view& v = s->plan->v;
             if (v.n_ports > 0) pa.push_back(s->pa);
             if (v.n_angles > 0 && s->ntff_mode != AEM_NTFF_OFF) {
-                if (s->ntff_mode == AEM_NTFF_HISTOGRAM) {
+                const bool hist = s->ntff_mode == AEM_NTFF_HISTOGRAM;
+                const bool list = s->ntff_mode == AEM_NTFF_ORDERED && s->nl.n_waves > 0;
+                if (hist || list) {
                     na.push_back(s->na);
-                    max_angles = std::max(max_angles, v.n_angles);
                     ntff_mu = std::max(ntff_mu, s->ntff_mu);
                     ntff_mv = std::max(ntff_mv, s->ntff_mv);
+                }
+                if (hist) {
+                    nh.push_back(s->na);
+                    max_angles = std::max(max_angles, v.n_angles);
+                } else if (list) {
+                    nla.push_back(NtffListArgs{s->nl, s->nb, s->d_step});
+                    max_list_blocks = std::max(max_list_blocks, (int)cdiv(s->nl.n_waves, 4));
                 } else {
                     ntff_alone.push_back(q);
                 }
@@ -2181,8 +2194,12 @@ struct aem_batch {
         }
         n_port = (int)pa.size();
         n_ntff = (int)na.size();
+        n_hist = (int)nh.size();
+        n_list = (int)nla.size();
         d_pargs = upload(pa);
         d_nargs = upload(na);
+        d_hist_args = upload(nh);
+        d_list_args = upload(nla);
         d_steps = upload(steps);
         HIPCHK(hipStreamSynchronize(st));
     }
@@ -2232,10 +2249,10 @@ struct aem_batch {
         if (n_port) k_ports_e_batch<<<n_port, 256, 0, s>>>(d_pargs);
         each(kPartEPost | kPartEOld);
         each(kPartOutPre);
-        if (n_ntff) {
+        if (n_ntff)
             k_ntff_currents_batch<<<dim3(cdiv(ntff_mu, 32), cdiv(ntff_mv, 8), 6 * n_ntff), bb, 0, s>>>(d_nargs);
-            k_ntff_hist_batch<<<dim3(max_angles, 6, n_ntff), 256, 0, s>>>(d_nargs);
-        }
+        if (n_hist) k_ntff_hist_batch<<<dim3(max_angles, 6, n_hist), 256, 0, s>>>(d_hist_args);
+        if (n_list) k_ntff_list_batch<<<dim3(max_list_blocks, n_list), 256, 0, s>>>(d_list_args);
         for (int q : ntff_alone) m[q]->enqueue_parts(nullptr, s, kPartNtff);
         each(kPartObjective | kPartOutPost);
         k_step_inc_batch<<<cdiv(n, 256), 256, 0, s>>>(d_steps, n);

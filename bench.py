@@ -91,16 +91,18 @@ def copy_probe(mib: int = 256, reps: int = 3) -> None:
 COPY_KIB = 256 * 1024      # --copy-probe: each copy reads and writes 256 MiB
 
 
-def population_bench(steps: int, count: int = 64) -> dict:
+def population_bench(steps: int, count: int = 64, ntff: str = "ordered") -> dict:
     """The batched-population workload: ``count`` independent dipole antennas
     with cube edges drawn from {80, 96, ..., 160} (seed 3), random arms and up
     to four dielectric blocks, 10-cell CPML, 50 ohm port, 19 far-field
     angles (decks.population).  Timed for ``steps`` steps: once as 64
-    sequential solo runs, once as one batch; same options (FP64 histogram
-    far-field binning) in both.  The batch's z chunk is the autotuner's
-    choice (alpineem_amd.tune) over throwaway members, untimed.  Only
-    stepping is timed (each solo run and the batch include their one graph
-    capture); setup is not."""
+    sequential solo runs, once as one batch; the same far-field binning in
+    both: "ordered" (the reference's FP32 sums in its order, from contributor
+    lists; deterministic and bit-exact) or "histogram" (FP64 LDS histograms,
+    whose atomic additions have no fixed order).  The batch's z chunk is the
+    autotuner's choice (alpineem_amd.tune) over throwaway members, untimed.
+    Only stepping is timed (each solo run and the batch include their one
+    graph capture); setup is not."""
     from alpineem_amd import decks
     from alpineem_amd.solver import Batch, Plan, Simulation
 
@@ -108,23 +110,26 @@ def population_bench(steps: int, count: int = 64) -> dict:
     plans = [Plan.from_text(d.text()) for d in pop]
     cells = sum(p.view.nx * p.view.ny * p.view.nz for p in plans)
     seq = 0.0
-    for p in plans:
-        s = Simulation(p, ntff="histogram")
+    for q, p in enumerate(plans):
+        s = Simulation(p, ntff=ntff)
         t0 = time.perf_counter()
         s.run(steps)
         seq += time.perf_counter() - t0
         s.close()
+        if q % 16 == 15:
+            print(f"[bench] population ({ntff}): {q + 1} of {len(plans)} solo runs", file=sys.stderr, flush=True)
     # the batch's z chunk chosen by the autotuner on throwaway members (not timed)
     from alpineem_amd.tune import tune_batch_planes
     t0 = time.perf_counter()
-    planes, tuned = tune_batch_planes(plans, candidates=(0, 2, 4, 8), steps=10, warmup=3, ntff="histogram")
+    planes, tuned = tune_batch_planes(plans, candidates=(0, 2, 4, 8), steps=10, warmup=3, ntff=ntff)
     t_tune = time.perf_counter() - t0
-    sims = [Simulation(p, ntff="histogram", planes=planes) for p in plans]
+    print(f"[bench] population ({ntff}): tuner chose planes={planes} in {t_tune:.1f} s", file=sys.stderr, flush=True)
+    sims = [Simulation(p, ntff=ntff, planes=planes) for p in plans]
     b = Batch(sims)
     t0 = time.perf_counter()
     b.run(steps)
     bat = time.perf_counter() - t0
-    res = {"members": count, "cells_per_step": cells, "steps": steps, "ntff": "histogram, 19 angles",
+    res = {"members": count, "cells_per_step": cells, "steps": steps, "ntff": f"{ntff}, 19 angles",
            "sequential_gcells": round(cells * steps / seq / 1e9, 3), "batched_gcells": round(cells * steps / bat / 1e9, 3),
            "speedup": round(seq / bat, 3), "sequential_s": round(seq, 3), "batched_s": round(bat, 3),
            "tuned_planes": planes, "tuner_ms_per_step": {str(k): round(v, 4) for k, v in tuned.items()},
@@ -416,13 +421,15 @@ def main() -> int:
     if args.copy_probe:
         copy_probe()
     sim.close()
-    pop = None
+    pop = pop_h = None
     if args.population_steps > 0:
-        pop = population_bench(args.population_steps)
-        print(f"[bench] population: 64 members, {pop['cells_per_step'] / 1e6:.1f} Mcells per step, "
-              f"sequential {pop['sequential_gcells']} Gcell/s, batched {pop['batched_gcells']} Gcell/s, "
-              f"speedup {pop['speedup']}x (tuned z chunk {pop['tuned_planes']}: {pop['tuner_ms_per_step']} ms, "
-              f"{pop['tuner_s']} s)", file=sys.stderr, flush=True)
+        pop = population_bench(args.population_steps, ntff="ordered")
+        pop_h = population_bench(args.population_steps, ntff="histogram")
+        for r in (pop, pop_h):
+            print(f"[bench] population ({r['ntff']}): 64 members, {r['cells_per_step'] / 1e6:.1f} Mcells per step, "
+                  f"sequential {r['sequential_gcells']} Gcell/s, batched {r['batched_gcells']} Gcell/s, "
+                  f"speedup {r['speedup']}x (tuned z chunk {r['tuned_planes']}: {r['tuner_ms_per_step']} ms, "
+                  f"{r['tuner_s']} s)", file=sys.stderr, flush=True)
     adj = None
     if args.adjoint_steps > 0:
         adj = adjoint_bench(args.adjoint_steps)
@@ -453,6 +460,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "population": pop,
+        "population_histogram": pop_h,
         "adjoint": adj,
         "configs": cfg,
     }

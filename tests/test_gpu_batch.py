@@ -4,7 +4,8 @@ computes when run alone.
 Members mix every path of a step: fused and separate CPML, thin sheets,
 TF/SF plane waves, multi-pole Drude media, periodic boundaries, plane sums,
 lumped ports, different grid sizes and planes per thread, and far-field
-binning that is batched (LDS histogram) or run per member (ordered, atomic).
+binning that is batched (LDS histogram, ordered contributor lists) or run
+per member (atomic).
 Fields and port series must be bit-identical to solo runs; far-field series
 binned through FP64 histograms (whose LDS additions have no fixed order)
 must agree to FP64 summation noise.
