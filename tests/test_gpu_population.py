@@ -6,8 +6,9 @@ angles, 300 steps) were run through the flang-built serial reference
 (tests/golden/make_golden.py).  Run as ONE batch on the device they must
 reproduce its outputs: bit for bit with the ordered FP32 far-field binning,
 within 1e-5 rel-L2 with the FP64 LDS-histogram binning that the population
-benchmark uses.  The full-size population (64 members, cubes 80-160, seed
-3) is checked batched against solo runs, field for field.
+benchmark also reports.  The full-size population (64 members, cubes
+80-160, seed 3) is checked batched against solo runs, field for field, in
+both binning modes, and far field for far field with the ordered one.
 """
 import os
 
@@ -90,25 +91,35 @@ def test_population_members_close_to_reference_histogram(golden_dir):
           f"strongest angle: {worst_null:.2e})")
 
 
-def test_full_population_batched_equals_solo():
-    """The benchmark's 64-member population (cubes 80-160, seed 3, 19 angles,
-    histogram binning) for 50 steps as one batch: every member's fields and
-    port series equal its solo run bit for bit."""
+@pytest.mark.parametrize("ntff", ["histogram", "ordered"])
+def test_full_population_batched_equals_solo(ntff):
+    """The benchmark's 64-member population (cubes 80-160, seed 3, 19 angles)
+    for 50 steps as one batch, in both of the benchmark's binning modes:
+    every member's fields and port series equal its solo run bit for bit,
+    and with the ordered binning (batched contributor lists) its far-field
+    series too."""
     steps = 50
     pop = decks.population(count=64, seed=3, steps=steps)
-    batched = [Simulation(Plan.from_text(d.text()), ntff="histogram") for d in pop]
+    batched = [Simulation(Plan.from_text(d.text()), ntff=ntff) for d in pop]
     Batch(batched).run(steps)
     for q, d in enumerate(pop):
-        solo = Simulation(Plan.from_text(d.text()), ntff="histogram")
+        solo = Simulation(Plan.from_text(d.text()), ntff=ntff)
         solo.run(steps)
         a = batched[q]
         for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
             np.testing.assert_array_equal(a.field(comp), solo.field(comp), err_msg=f"member {q} {comp}")
         np.testing.assert_array_equal(a.series("port_v", 0, steps), solo.series("port_v", 0, steps),
                                       err_msg=f"member {q}")
+        if ntff == "ordered":
+            a.finalize()
+            solo.finalize()
+            for ang in range(a.plan.view.n_angles):
+                for kind in ("e_theta", "e_phi"):
+                    np.testing.assert_array_equal(a.series(kind, ang), solo.series(kind, ang),
+                                                  err_msg=f"member {q} {kind}{ang}")
         solo.close()
         a.close()
-    print(f"\n  64 members x {steps} steps: batched == solo")
+    print(f"\n  64 members x {steps} steps ({ntff}): batched == solo")
 
 
 def test_single_pass_400cube_matches_split_launches():
