@@ -199,7 +199,7 @@ def adjoint_bench(steps: int) -> dict:
 
 
 def configs_bench(only: str = None) -> dict:
-    """The BASELINE configurations with every output of each, timed end to
+    """The benchmark configurations with every output of each, timed end to
     end on the device (wall clock around the stepping; setup excluded):
 
     C1  100^3 radiating dipole (arms 20 cells, 1-cell gap), 2000 steps,
